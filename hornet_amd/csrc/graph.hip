@@ -1,0 +1,607 @@
+// graph.hip -- storage: create from CSR/COO, export, degree queries, errors.
+//
+// Reference counterparts:
+//   Hornet(HornetInit&) / initialize  hornet/include/Core/HornetInitialize/HornetInitialize.i.cuh:86-146
+//   GraphStd::COOtoCSR                xlib/src/Graph/GraphStd.cpp:173-341
+//   Hornet::getCOO + flattenCOOKernel hornet/include/Core/HornetOperations/HornetQuery.i.cuh:46-65,111-151
+//   Hornet::max_degree_id             HornetQuery.i.cuh:10-36
+// The reference initialises with a serial host loop over every vertex (one
+// BlockArrayManager::insert each) and one H2D copy per 2^23-slot BlockArray;
+// here block sizes are a device scan over the degrees and the adjacency copy
+// is one load-balanced kernel.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+#include "graph.hpp"
+#include "prims.hpp"
+
+namespace hn {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+hn_status dmalloc(void** p, size_t bytes) {
+    *p = nullptr;
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? HN_E_OOM : HN_E_HIP;
+    }
+    return HN_OK;
+}
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int num_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+        cus = prop.multiProcessorCount;
+    }
+    return cus;
+}
+
+// ------------------------------------------------------------------ kernels
+namespace {
+
+struct BlockUnitsIn {
+    const int64_t* off;
+    __device__ __forceinline__ uint64_t operator()(int64_t v) const {
+        int64_t d = off[v + 1] - off[v];
+        int k = class_of((int32_t)d);
+        return k < 0 ? 0ull : (1ull << (k - MIN_CLASS));
+    }
+};
+
+__global__ void set_records_kernel(int32_t nv, const int64_t* __restrict__ off,
+                                   const uint64_t* __restrict__ uoff, hn_vrec* __restrict__ rec) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    int64_t d = off[v + 1] - off[v];
+    hn_vrec r;
+    r.off = (int64_t)uoff[v] * UNIT_SLOTS;
+    r.deg = (int32_t)d;
+    r.cap_log2 = class_of((int32_t)d);
+    rec[v] = r;
+}
+
+constexpr int COPY_IPT = 8;
+constexpr int COPY_TILE = BLOCK * COPY_IPT;
+
+// arena[rec[v].off + j] = adj[off[v] + j] for all edges, load balanced by edge
+__global__ __launch_bounds__(BLOCK) void copy_in_kernel(int32_t nv, int64_t ne,
+                                                        const int64_t* __restrict__ off,
+                                                        const int32_t* __restrict__ adj,
+                                                        const hn_vrec* __restrict__ rec,
+                                                        int32_t* __restrict__ arena) {
+    __shared__ LbShared<COPY_IPT, int64_t> sh;
+    for (int64_t w0 = (int64_t)blockIdx.x * COPY_TILE; w0 < ne; w0 += (int64_t)gridDim.x * COPY_TILE) {
+        lb_tile<COPY_IPT>(
+            off, (int64_t)nv, w0, ne, sh,
+            [&](int64_t s) { return rec[s].off - off[s]; },
+            [&](int64_t, int64_t w, int64_t base) { arena[base + w] = adj[w]; });
+    }
+}
+
+// COO/CSR flatten: out positions from the degree prefix eoff
+__global__ __launch_bounds__(BLOCK) void flatten_kernel(int32_t nv, int64_t ne,
+                                                        const int64_t* __restrict__ eoff,
+                                                        const hn_vrec* __restrict__ rec,
+                                                        const int32_t* __restrict__ arena,
+                                                        int32_t* __restrict__ src,
+                                                        int32_t* __restrict__ dst) {
+    __shared__ LbShared<COPY_IPT, int64_t> sh;
+    for (int64_t w0 = (int64_t)blockIdx.x * COPY_TILE; w0 < ne; w0 += (int64_t)gridDim.x * COPY_TILE) {
+        lb_tile<COPY_IPT>(
+            eoff, (int64_t)nv, w0, ne, sh,
+            [&](int64_t s) { return rec[s].off - eoff[s]; },
+            [&](int64_t s, int64_t w, int64_t base) {
+                dst[w] = arena[base + w];
+                if (src) src[w] = (int32_t)s;
+            });
+    }
+}
+
+struct DegIn {
+    const hn_vrec* rec;
+    __device__ __forceinline__ int64_t operator()(int64_t v) const { return rec[v].deg; }
+};
+
+__global__ void degrees_kernel(int32_t nv, const hn_vrec* __restrict__ rec, int32_t* __restrict__ deg) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nv) deg[v] = rec[v].deg;
+}
+
+__global__ void max_degree_kernel(int32_t nv, const hn_vrec* __restrict__ rec,
+                                  unsigned long long* __restrict__ best) {
+    unsigned long long mine = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        unsigned long long key = ((unsigned long long)(uint32_t)rec[v].deg << 32) |
+                                 (0xFFFFFFFFull - (unsigned long long)v);
+        mine = key > mine ? key : mine;
+    }
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        unsigned long long o = __shfl_xor(mine, d, WAVE);
+        mine = o > mine ? o : mine;
+    }
+    if (__lane_id() == 0) atomicMax(best, mine);
+}
+
+__global__ void init_alloc_kernel(AllocState* st, unsigned long long bump, unsigned long long cap) {
+    st->bump_units = bump;
+    st->cap_units = cap;
+    st->saved_bump = bump;
+    for (int k = 0; k < NCLASS; k++) {
+        st->free_count[k] = 0;
+        st->take[k] = 0;
+    }
+    st->error = 0;
+    st->pad = 0;
+}
+
+// ---- COO -> CSR on the device
+struct PackArgs {
+    const int32_t* src;
+    const int32_t* dst;
+    int64_t m;
+    int sbits;
+    int sym;
+    int drop;
+    int32_t nv;
+};
+constexpr uint64_t SENTINEL = ~0ull;
+
+__global__ void pack_keys_kernel(PackArgs a, uint64_t* __restrict__ keys, unsigned* __restrict__ bad) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.m) return;
+    int32_t s = a.src[i], d = a.dst[i];
+    if (s < 0 || s >= a.nv || d < 0 || d >= a.nv) {
+        atomicOr(bad, 1u);
+        keys[i] = SENTINEL;
+        if (a.sym) keys[a.m + i] = SENTINEL;
+        return;
+    }
+    bool self = s == d;
+    keys[i] = (a.drop && self) ? SENTINEL : (((uint64_t)s << a.sbits) | (uint64_t)d);
+    if (a.sym) keys[a.m + i] = self ? SENTINEL : (((uint64_t)d << a.sbits) | (uint64_t)s);
+}
+
+struct UniqueFlagIn {
+    const uint64_t* k;
+    uint64_t mask;   // sentinel pattern restricted to the sorted bits
+    __device__ __forceinline__ uint32_t operator()(int64_t i) const {
+        uint64_t x = k[i];
+        if (x == SENTINEL || (x & mask) == mask) return 0u;
+        return (i == 0 || k[i - 1] != x) ? 1u : 0u;
+    }
+};
+
+__global__ void unique_write_kernel(const uint64_t* __restrict__ k, int64_t n, uint64_t mask,
+                                    const uint32_t* __restrict__ pos, int sbits,
+                                    uint64_t* __restrict__ ukeys, int32_t* __restrict__ adj) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t x = k[i];
+    if (x == SENTINEL || (x & mask) == mask) return;
+    if (i > 0 && k[i - 1] == x) return;
+    uint32_t p = pos[i];
+    ukeys[p] = x;
+    adj[p] = (int32_t)(x & ((1ull << sbits) - 1));
+}
+
+__global__ void offsets_from_keys_kernel(const uint64_t* __restrict__ ukeys, int64_t u, int sbits,
+                                         int32_t nv, int64_t* __restrict__ off) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > nv) return;
+    // first compacted key with src >= v
+    uint64_t target = (uint64_t)v << sbits;
+    int64_t lo = 0, hi = u;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (ukeys[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    off[v] = lo;
+}
+
+}  // namespace
+
+hn_status read_back(hn_graph* g, const void* dptr, size_t bytes, void* host, hipStream_t st) {
+    if (bytes > 4096) {
+        HN_HIP(hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, st));
+        HN_HIP(hipStreamSynchronize(st));
+        return HN_OK;
+    }
+    if (!g || !g->h_pinned) {
+        HN_HIP(hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, st));
+        HN_HIP(hipStreamSynchronize(st));
+        return HN_OK;
+    }
+    HN_HIP(hipMemcpyAsync(g->h_pinned, dptr, bytes, hipMemcpyDeviceToHost, st));
+    HN_HIP(hipStreamSynchronize(st));
+    std::memcpy(host, g->h_pinned, bytes);
+    return HN_OK;
+}
+
+static void destroy_graph(hn_graph* g) {
+    if (!g) return;
+    dfree(g->rec);
+    dfree(g->arena);
+    dfree(g->alloc);
+    dfree(g->stacks);
+    dfree(g->stack_off);
+    g->s_bfs.release();
+    g->s_scan.release();
+    g->s_upd.release();
+    g->s_tc.release();
+    g->s_misc.release();
+    if (g->h_pinned) (void)hipHostFree(g->h_pinned);
+    delete g;
+}
+
+// allocate arena + allocator state for `used_units` occupied units
+static hn_status setup_arena(hn_graph* g, uint64_t used_units, double slack, hipStream_t st) {
+    if (slack < 1.0) slack = 2.0;
+    uint64_t cap = (uint64_t)((double)used_units * slack) + 4096;
+    HN_ARG(cap < (1ull << 32), "edge arena would exceed 2^32 units of 128 B");
+    g->cap_units = (int64_t)cap;
+    HN_TRY(dalloc(&g->arena, (size_t)cap * UNIT_SLOTS));
+    // class k stack holds at most cap >> (k - MIN_CLASS) blocks
+    int64_t tot = 0;
+    for (int k = 0; k < NCLASS; k++) {
+        g->h_stack_off[k] = tot;
+        if (k >= MIN_CLASS) tot += (int64_t)(cap >> (k - MIN_CLASS)) + 1;
+    }
+    HN_TRY(dalloc(&g->stacks, (size_t)tot));
+    HN_TRY(dalloc(&g->stack_off, NCLASS));
+    HN_HIP(hipMemcpyAsync(g->stack_off, g->h_stack_off, sizeof(g->h_stack_off), hipMemcpyHostToDevice, st));
+    HN_TRY(dalloc(&g->alloc, 1));
+    init_alloc_kernel<<<1, 1, 0, st>>>(g->alloc, used_units, cap);
+    HN_CHECK_LAUNCH();
+    return HN_OK;
+}
+
+static hn_status new_graph(int32_t nv, hn_graph** out) {
+    hn_graph* g = new hn_graph();
+    g->nv = nv;
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&g->h_pinned), 4096, 0);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        g->h_pinned = nullptr;
+        delete g;
+        set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        return e == hipErrorNoDevice ? HN_E_NO_DEVICE : HN_E_HIP;
+    }
+    *out = g;
+    return HN_OK;
+}
+
+hn_status graph_from_device_csr(int32_t nv, int64_t ne, const int64_t* d_off, const int32_t* d_adj,
+                                double slack, hipStream_t st, hn_graph** out) {
+    hn_graph* g = nullptr;
+    HN_TRY(new_graph(nv, &g));
+    auto fail = [&](hn_status s) { destroy_graph(g); return s; };
+    g->ne = ne;
+    hn_status s;
+    if ((s = dalloc(&g->rec, (size_t)(nv > 0 ? nv : 1))) != HN_OK) return fail(s);
+    uint64_t* uoff = nullptr;
+    if ((s = dalloc(&uoff, (size_t)nv + 1)) != HN_OK) return fail(s);
+    if ((s = g->s_scan.reserve(scan_temp_bytes(nv))) != HN_OK) { dfree(uoff); return fail(s); }
+    if ((s = exclusive_scan<uint64_t>(BlockUnitsIn{d_off}, uoff, nv, g->s_scan.ptr, st)) != HN_OK) {
+        dfree(uoff);
+        return fail(s);
+    }
+    uint64_t used = 0;
+    if ((s = read_back(g, uoff + nv, 8, &used, st)) != HN_OK) { dfree(uoff); return fail(s); }
+    if ((s = setup_arena(g, used, slack, st)) != HN_OK) { dfree(uoff); return fail(s); }
+    if (nv > 0) {
+        set_records_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(nv, d_off, uoff, g->rec);
+    }
+    if (ne > 0) {
+        int64_t tiles = ceil_div(ne, COPY_TILE);
+        int64_t grid = std::min<int64_t>(tiles, (int64_t)num_cus() * 16);
+        copy_in_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(nv, ne, d_off, d_adj, g->rec, g->arena);
+    }
+    hipError_t e = hipStreamSynchronize(st);
+    dfree(uoff);
+    if (e != hipSuccess) {
+        set_error(std::string("graph build: ") + hipGetErrorString(e));
+        return fail(HN_E_HIP);
+    }
+    *out = g;
+    return HN_OK;
+}
+
+hn_status csr_from_device_coo(int32_t nv, int64_t m, const int32_t* src, const int32_t* dst,
+                              uint32_t flags, hipStream_t st, int64_t** d_off_out, int32_t** d_adj_out,
+                              int64_t* ne_out) {
+    const int sym = (flags & HN_SYMMETRIZE) ? 1 : 0;
+    const int drop = (flags & HN_DROP_SELF_LOOPS) ? 1 : 0;
+    const int64_t n = m * (sym ? 2 : 1);
+    int sbits = ceil_log2_host((int64_t)nv + 1);
+    if (sbits < 1) sbits = 1;
+    const int key_bits = 2 * sbits;
+    const uint64_t mask = key_bits >= 64 ? ~0ull : ((1ull << key_bits) - 1);
+    uint64_t* ka = nullptr;
+    uint64_t* kb = nullptr;
+    uint32_t* pos = nullptr;
+    unsigned* bad = nullptr;
+    void* tmp = nullptr;
+    int64_t* d_off = nullptr;
+    int32_t* d_adj = nullptr;
+    hn_status s = HN_OK;
+    auto cleanup = [&]() {
+        dfree(ka); dfree(kb); dfree(pos); dfree(bad); dfree(tmp);
+    };
+    size_t tmpb = std::max(sort_temp_bytes(n), scan_temp_bytes(n));
+    if ((s = dalloc(&ka, (size_t)n + 1)) != HN_OK || (s = dalloc(&kb, (size_t)n + 1)) != HN_OK ||
+        (s = dalloc(&pos, (size_t)n + 1)) != HN_OK || (s = dalloc(&bad, 1)) != HN_OK ||
+        (s = dmalloc(&tmp, tmpb)) != HN_OK) {
+        cleanup();
+        return s;
+    }
+    HN_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned), st));
+    if (m > 0) {
+        PackArgs a{src, dst, m, sbits, sym, drop, nv};
+        pack_keys_kernel<<<(unsigned)ceil_div(m, BLOCK), BLOCK, 0, st>>>(a, ka, bad);
+        HN_CHECK_LAUNCH();
+    }
+    if ((s = radix_sort_u64(&ka, &kb, n, key_bits, tmp, st)) != HN_OK) { cleanup(); return s; }
+    if ((s = exclusive_scan<uint32_t>(UniqueFlagIn{ka, mask}, pos, n, tmp, st)) != HN_OK) {
+        cleanup();
+        return s;
+    }
+    uint32_t u32 = 0;
+    unsigned hbad = 0;
+    HN_HIP(hipMemcpyAsync(&u32, pos + n, 4, hipMemcpyDeviceToHost, st));
+    HN_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, st));
+    HN_HIP(hipStreamSynchronize(st));
+    if (hbad) {
+        cleanup();
+        set_error("COO contains a vertex id outside [0, nv)");
+        return HN_E_INVALID_ARG;
+    }
+    int64_t u = u32;
+    if ((s = dalloc(&d_off, (size_t)nv + 1)) != HN_OK || (s = dalloc(&d_adj, (size_t)(u > 0 ? u : 1))) != HN_OK) {
+        dfree(d_off); dfree(d_adj);
+        cleanup();
+        return s;
+    }
+    // compacted keys reuse kb
+    if (n > 0) {
+        unique_write_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(ka, n, mask, pos, sbits, kb, d_adj);
+        HN_CHECK_LAUNCH();
+    }
+    offsets_from_keys_kernel<<<(unsigned)ceil_div((int64_t)nv + 1, BLOCK), BLOCK, 0, st>>>(kb, u, sbits, nv, d_off);
+    HN_CHECK_LAUNCH();
+    hipError_t e = hipStreamSynchronize(st);
+    cleanup();
+    if (e != hipSuccess) {
+        dfree(d_off); dfree(d_adj);
+        set_error(std::string("coo->csr: ") + hipGetErrorString(e));
+        return HN_E_HIP;
+    }
+    *d_off_out = d_off;
+    *d_adj_out = d_adj;
+    *ne_out = u;
+    return HN_OK;
+}
+
+hn_status degree_prefix(hn_graph* g, int64_t* d_eoff, hipStream_t st) {
+    HN_TRY(g->s_scan.reserve(scan_temp_bytes(g->nv)));
+    return exclusive_scan<int64_t>(DegIn{g->rec}, d_eoff, g->nv, g->s_scan.ptr, st);
+}
+
+}  // namespace hn
+
+using namespace hn;
+
+// ===================================================================== C ABI
+extern "C" {
+
+const char* hn_status_string(hn_status s) {
+    switch (s) {
+        case HN_OK: return "ok";
+        case HN_E_INVALID_ARG: return "invalid argument";
+        case HN_E_OOM: return "out of device memory";
+        case HN_E_HIP: return "HIP runtime error";
+        case HN_E_NO_DEVICE: return "no device";
+        case HN_E_CAPACITY: return "output capacity too small";
+        default: return "unknown status";
+    }
+}
+
+const char* hn_last_error(void) { return g_last_error.c_str(); }
+
+const char* hn_version(void) { return "hornet-hip 0.1 (gfx950)"; }
+
+hn_status hn_device_info(int32_t* cus, int64_t* lds, int64_t* hbm, int32_t* wave) {
+    int dev = 0;
+    HN_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HN_HIP(hipGetDeviceProperties(&p, dev));
+    if (cus) *cus = p.multiProcessorCount;
+    if (lds) *lds = (int64_t)p.maxSharedMemoryPerMultiProcessor;
+    if (hbm) *hbm = (int64_t)p.totalGlobalMem;
+    if (wave) *wave = p.warpSize;
+    return HN_OK;
+}
+
+hn_status hn_graph_create_csr(int32_t nv, int64_t ne, const void* offsets, const int32_t* edges,
+                              uint32_t flags, double slack, void* stream, hn_graph** out) {
+    HN_ARG(out != nullptr, "out is null");
+    *out = nullptr;
+    HN_ARG(nv >= 0 && ne >= 0, "negative size");
+    HN_ARG(offsets != nullptr, "offsets is null");
+    HN_ARG(ne == 0 || edges != nullptr, "edges is null");
+    HN_ARG(ne < (int64_t)0xFFFFFFFF, "more than 2^32-1 edges");
+    hipStream_t st = as_stream(stream);
+    const bool dev = flags & HN_DEVICE_PTRS;
+    const bool o32 = flags & HN_OFFSETS_INT32;
+    // validate on the host when the input is on the host
+    std::vector<int64_t> hoff;
+    if (!dev) {
+        hoff.resize((size_t)nv + 1);
+        for (int64_t v = 0; v <= nv; v++)
+            hoff[v] = o32 ? (int64_t)((const int32_t*)offsets)[v] : ((const int64_t*)offsets)[v];
+        HN_ARG(hoff[0] == 0 && hoff[nv] == ne, "offsets[0] must be 0 and offsets[nv] == ne");
+        for (int64_t v = 0; v < nv; v++) HN_ARG(hoff[v] <= hoff[v + 1], "offsets not monotone");
+        for (int64_t v = 0; v < nv; v++) {
+            for (int64_t j = hoff[v]; j < hoff[v + 1]; j++) {
+                HN_ARG(edges[j] >= 0 && edges[j] < nv, "edge endpoint outside [0, nv)");
+                HN_ARG(j == hoff[v] || edges[j - 1] <= edges[j], "adjacency not sorted ascending");
+            }
+        }
+    }
+    int64_t* d_off = nullptr;
+    int32_t* d_adj = nullptr;
+    hn_status s;
+    if ((s = dalloc(&d_off, (size_t)nv + 1)) != HN_OK) return s;
+    if (dev && !o32) {
+        HN_HIP(hipMemcpyAsync(d_off, offsets, sizeof(int64_t) * ((size_t)nv + 1), hipMemcpyDeviceToDevice, st));
+    } else if (dev && o32) {
+        std::vector<int32_t> t((size_t)nv + 1);
+        HN_HIP(hipMemcpyAsync(t.data(), offsets, sizeof(int32_t) * t.size(), hipMemcpyDeviceToHost, st));
+        HN_HIP(hipStreamSynchronize(st));
+        hoff.assign(t.begin(), t.end());
+        HN_HIP(hipMemcpyAsync(d_off, hoff.data(), sizeof(int64_t) * hoff.size(), hipMemcpyHostToDevice, st));
+    } else {
+        HN_HIP(hipMemcpyAsync(d_off, hoff.data(), sizeof(int64_t) * hoff.size(), hipMemcpyHostToDevice, st));
+    }
+    const int32_t* adj_in = edges;
+    if (!dev) {
+        if ((s = dalloc(&d_adj, (size_t)(ne > 0 ? ne : 1))) != HN_OK) { dfree(d_off); return s; }
+        if (ne > 0) HN_HIP(hipMemcpyAsync(d_adj, edges, sizeof(int32_t) * (size_t)ne, hipMemcpyHostToDevice, st));
+        adj_in = d_adj;
+    }
+    s = graph_from_device_csr(nv, ne, d_off, adj_in, slack, st, out);
+    dfree(d_off);
+    dfree(d_adj);
+    return s;
+}
+
+hn_status hn_graph_create_coo(int32_t nv, int64_t m, const int32_t* src, const int32_t* dst,
+                              uint32_t flags, double slack, void* stream, hn_graph** out) {
+    HN_ARG(out != nullptr, "out is null");
+    *out = nullptr;
+    HN_ARG(nv >= 0 && m >= 0, "negative size");
+    HN_ARG(m == 0 || (src && dst), "null COO");
+    HN_ARG(flags & HN_DEVICE_PTRS, "hn_graph_create_coo takes device pointers");
+    HN_ARG(m * ((flags & HN_SYMMETRIZE) ? 2 : 1) < (int64_t)0xFFFFFFFF, "COO too large");
+    hipStream_t st = as_stream(stream);
+    int64_t* d_off = nullptr;
+    int32_t* d_adj = nullptr;
+    int64_t ne = 0;
+    HN_TRY(csr_from_device_coo(nv, m, src, dst, flags, st, &d_off, &d_adj, &ne));
+    hn_status s = graph_from_device_csr(nv, ne, d_off, d_adj, slack, st, out);
+    dfree(d_off);
+    dfree(d_adj);
+    return s;
+}
+
+hn_status hn_graph_destroy(hn_graph* g) {
+    if (g) {
+        (void)hipDeviceSynchronize();
+        destroy_graph(g);
+    }
+    return HN_OK;
+}
+
+hn_status hn_graph_nv(const hn_graph* g, int32_t* nv) {
+    HN_ARG(g && nv, "null argument");
+    *nv = g->nv;
+    return HN_OK;
+}
+
+hn_status hn_graph_ne(const hn_graph* g, int64_t* ne) {
+    HN_ARG(g && ne, "null argument");
+    *ne = g->ne;
+    return HN_OK;
+}
+
+hn_status hn_graph_device_view(const hn_graph* g, hn_dview* view) {
+    HN_ARG(g && view, "null argument");
+    *view = g->view();
+    return HN_OK;
+}
+
+hn_status hn_graph_memory(hn_graph* g, int64_t* used_slots, int64_t* capacity_slots, void* stream) {
+    HN_ARG(g, "null graph");
+    AllocState a;
+    HN_TRY(read_back(g, g->alloc, sizeof(AllocState), &a, as_stream(stream)));
+    int64_t freed = 0;
+    for (int k = MIN_CLASS; k < NCLASS; k++) freed += (int64_t)a.free_count[k] << k;
+    if (used_slots) *used_slots = (int64_t)a.bump_units * UNIT_SLOTS - freed;
+    if (capacity_slots) *capacity_slots = (int64_t)a.cap_units * UNIT_SLOTS;
+    return HN_OK;
+}
+
+static hn_status flatten(hn_graph* g, int32_t* src, int32_t* dst, int64_t* eoff_out, hipStream_t st) {
+    int64_t* eoff = eoff_out;
+    if (!eoff) HN_TRY(g->s_misc.reserve(sizeof(int64_t) * ((size_t)g->nv + 1)));
+    if (!eoff) eoff = g->s_misc.as<int64_t>();
+    HN_TRY(degree_prefix(g, eoff, st));
+    if (g->ne > 0) {
+        int64_t tiles = ceil_div(g->ne, COPY_TILE);
+        int64_t grid = std::min<int64_t>(tiles, (int64_t)num_cus() * 16);
+        flatten_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(g->nv, g->ne, eoff, g->rec, g->arena, src, dst);
+        HN_CHECK_LAUNCH();
+    }
+    HN_HIP(hipStreamSynchronize(st));
+    return HN_OK;
+}
+
+hn_status hn_graph_export_coo(hn_graph* g, int32_t* src, int32_t* dst, int64_t cap, int64_t* n_out,
+                              void* stream) {
+    HN_ARG(g && src && dst, "null argument");
+    if (n_out) *n_out = g->ne;
+    if (cap < g->ne) {
+        set_error("export_coo: capacity " + std::to_string(cap) + " < ne " + std::to_string(g->ne));
+        return HN_E_CAPACITY;
+    }
+    return flatten(g, src, dst, nullptr, as_stream(stream));
+}
+
+hn_status hn_graph_export_csr(hn_graph* g, int64_t* offsets, int32_t* adj, int64_t cap, void* stream) {
+    HN_ARG(g && offsets && adj, "null argument");
+    if (cap < g->ne) {
+        set_error("export_csr: capacity too small");
+        return HN_E_CAPACITY;
+    }
+    return flatten(g, nullptr, adj, offsets, as_stream(stream));
+}
+
+hn_status hn_graph_degrees(hn_graph* g, int32_t* deg, void* stream) {
+    HN_ARG(g && deg, "null argument");
+    if (g->nv > 0) {
+        degrees_kernel<<<(unsigned)ceil_div(g->nv, BLOCK), BLOCK, 0, as_stream(stream)>>>(g->nv, g->rec, deg);
+        HN_CHECK_LAUNCH();
+    }
+    return HN_OK;
+}
+
+hn_status hn_graph_max_degree(hn_graph* g, int32_t* id, int32_t* deg, void* stream) {
+    HN_ARG(g, "null graph");
+    hipStream_t st = as_stream(stream);
+    HN_TRY(g->s_misc.reserve(64));
+    unsigned long long* best = g->s_misc.as<unsigned long long>();
+    HN_HIP(hipMemsetAsync(best, 0, 8, st));
+    if (g->nv > 0) {
+        int64_t grid = std::min<int64_t>(ceil_div(g->nv, BLOCK), (int64_t)num_cus() * 8);
+        max_degree_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(g->nv, g->rec, best);
+        HN_CHECK_LAUNCH();
+    }
+    unsigned long long b = 0;
+    HN_TRY(read_back(g, best, 8, &b, st));
+    if (id) *id = g->nv > 0 ? (int32_t)(0xFFFFFFFFull - (b & 0xFFFFFFFFull)) : -1;
+    if (deg) *deg = (int32_t)(b >> 32);
+    return HN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// graph.hpp -- internal layout of an hn_graph (vertex records + edge arena +
+// device-side block allocator).
+//
+// Reference: hornet::gpu::Hornet (hornet/include/Core/Hornet.cuh:54-131) keeps
+// a vertex SoA {degree, edge_block_ptr, vertex_offset, edges_per_block} and a
+// host-side BlockArrayManager (hornet/include/Core/MemoryManager/BlockArray/
+// BlockArray.i.cuh:111-173) that finds free power-of-two blocks with a host
+// bit tree and one cudaMalloc per 2^23-slot BlockArray.
+//
+// Here the edge storage is ONE device arena of int32 slots, carved into
+// power-of-two blocks of at least 32 slots (128 B, so every block start is
+// 128-byte aligned).  Block offsets are kept in 128-byte units (uint32), so an
+// arena may span up to 2^32 * 128 B = 512 GiB -- more than the 288 GB of an
+// MI355X.  Allocation and release happen on the device:
+//   * one free stack (LIFO of unit offsets) per size class k (block = 2^k slots);
+//   * a bump pointer for fresh blocks.
+// A batch allocates in one kernel (every requester takes a ticket per class;
+// tickets below the stack depth pop, the rest bump), then a fixup kernel
+// commits the stack depths, and frees are pushed after the data has moved.
+// No host round trip per vertex.
+#pragma once
+#include "common.hpp"
+
+namespace hn {
+
+constexpr int MIN_CLASS = 5;          // 32 slots = 128 bytes
+constexpr int NCLASS = 32;
+constexpr int64_t UNIT_SLOTS = 32;    // slots per 128-byte unit
+
+struct AllocState {
+    unsigned long long bump_units;    // next never-used unit
+    unsigned long long cap_units;     // arena capacity in units
+    unsigned long long saved_bump;    // bump at the start of the current batch
+    unsigned int free_count[NCLASS];  // stack depth per class
+    unsigned int take[NCLASS];        // tickets handed out in the current batch
+    unsigned int error;               // bit 0: arena exhausted
+    unsigned int pad;
+};
+
+struct AllocView {
+    AllocState* st;
+    uint32_t* stacks;                 // all class stacks, concatenated
+    const int64_t* stack_off;         // start of class k's stack (entries)
+};
+
+__device__ __forceinline__ int class_of(int32_t deg) {
+    if (deg <= 0) return -1;
+    int k = 32 - __clz(deg - 1);      // ceil(log2(deg)) for deg >= 1 (deg=1 -> 0)
+    return k < MIN_CLASS ? MIN_CLASS : k;
+}
+
+// Pop a block of class k, or carve a fresh one; -1 when the arena is full.
+__device__ __forceinline__ int64_t alloc_block(const AllocView& A, int k) {
+    unsigned t = atomicAdd(&A.st->take[k], 1u);
+    unsigned fc = A.st->free_count[k];
+    if (t < fc) return (int64_t)A.stacks[A.stack_off[k] + (fc - 1 - t)] * UNIT_SLOTS;
+    unsigned long long units = 1ull << (k - MIN_CLASS);
+    unsigned long long u = atomicAdd(&A.st->bump_units, units);
+    if (u + units > A.st->cap_units) {
+        atomicOr(&A.st->error, 1u);
+        return -1;
+    }
+    return (int64_t)u * UNIT_SLOTS;
+}
+
+__device__ __forceinline__ void free_block(const AllocView& A, int k, int64_t slot_off) {
+    unsigned idx = atomicAdd(&A.st->free_count[k], 1u);
+    A.stacks[A.stack_off[k] + idx] = (uint32_t)(slot_off / UNIT_SLOTS);
+}
+
+}  // namespace hn
+
+struct hn_graph {
+    int32_t nv = 0;
+    int64_t ne = 0;
+    hn_vrec* rec = nullptr;           // device, nv records
+    int32_t* arena = nullptr;         // device, cap_units * 32 slots
+    int64_t cap_units = 0;
+    hn::AllocState* alloc = nullptr;  // device
+    uint32_t* stacks = nullptr;       // device
+    int64_t* stack_off = nullptr;     // device, NCLASS entries
+    int64_t h_stack_off[hn::NCLASS] = {};
+    // scratch owned by the handle (grown on demand)
+    hn::Scratch s_bfs;                // BFS queues, bitmap, work prefix
+    hn::Scratch s_scan;               // scan partials
+    hn::Scratch s_upd;                // batch update buffers
+    hn::Scratch s_tc;                 // triangle counting
+    hn::Scratch s_misc;
+    uint64_t* h_pinned = nullptr;     // small pinned readback area
+
+    hn::AllocView alloc_view() const { return hn::AllocView{alloc, stacks, stack_off}; }
+    hn_dview view() const {
+        hn_dview v;
+        v.nv = nv;
+        v.pad_ = 0;
+        v.ne = ne;
+        v.rec = rec;
+        v.arena = arena;
+        return v;
+    }
+};
+
+namespace hn {
+// build a graph from device CSR (offsets int64[nv+1], adj int32[ne]); adjacency
+// must already be sorted per vertex.  Does not take ownership of the inputs.
+hn_status graph_from_device_csr(int32_t nv, int64_t ne, const int64_t* d_off, const int32_t* d_adj,
+                                double slack, hipStream_t st, hn_graph** out);
+// device CSR from device COO: sort + dedup (+ symmetrize / drop self loops);
+// returns buffers allocated with dalloc (caller frees).
+hn_status csr_from_device_coo(int32_t nv, int64_t m, const int32_t* src, const int32_t* dst,
+                              uint32_t flags, hipStream_t st, int64_t** d_off, int32_t** d_adj,
+                              int64_t* ne_out);
+// exclusive prefix of the record degrees into d_eoff[nv+1] (int64)
+hn_status degree_prefix(hn_graph* g, int64_t* d_eoff, hipStream_t st);
+// synchronous readback of a few bytes via the handle's pinned buffer
+hn_status read_back(hn_graph* g, const void* dptr, size_t bytes, void* host, hipStream_t st);
+}  // namespace hn

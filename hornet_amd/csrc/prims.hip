@@ -1,0 +1,142 @@
+// prims.hip -- wave64 LSD radix sort of 64-bit keys.
+//
+// Replaces the lexicographic (src,dst) batch sort of the reference, two stable
+// thrust::sort_by_key calls (hornet/include/Core/SoA/impl/SoAPtr.i.cuh:560-571),
+// with one sort of packed (src << bits | dst) keys.  Each pass: per-tile digit
+// histogram in LDS, a scan of the digit-major histogram, then a stable scatter
+// that ranks keys with 8 ballots per digit (one per digit bit, 64-lane
+// ballots) and stages the tile in LDS in digit order so that global writes
+// come out as per-bucket runs.
+#include "prims.hpp"
+
+namespace hn {
+
+// ------------------------------------------------------------------ radix sort
+namespace {
+
+constexpr int RS_BITS = 8;
+constexpr int RS_BUCKETS = 1 << RS_BITS;
+constexpr int RS_IPT = 16;                      // keys per lane
+constexpr int RS_TILE = BLOCK * RS_IPT;         // 4096 keys
+constexpr int RS_WAVE_KEYS = WAVE * RS_IPT;     // 1024 keys per wave
+
+__global__ __launch_bounds__(BLOCK) void rs_hist_kernel(const uint64_t* __restrict__ keys, int64_t n,
+                                                        int shift, int64_t ntiles,
+                                                        uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[RS_BUCKETS];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int j = 0; j < RS_IPT; j++) {
+        int64_t i = base + (int64_t)j * BLOCK + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (RS_BUCKETS - 1)], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(BLOCK) void rs_scatter_kernel(const uint64_t* __restrict__ keys, int64_t n,
+                                                           int shift, int64_t ntiles,
+                                                           const uint32_t* __restrict__ hoff,
+                                                           uint64_t* __restrict__ out) {
+    __shared__ uint32_t wcnt[NWAVES][RS_BUCKETS];   // per-wave digit counts
+    __shared__ uint32_t dstart[RS_BUCKETS];         // tile-local start of each digit
+    __shared__ uint32_t gbase[RS_BUCKETS];          // global start of this tile's digit run
+    __shared__ uint64_t stage[RS_TILE];
+    __shared__ uint32_t sc[NWAVES + 1];
+
+    const int w = threadIdx.x / WAVE;
+    const int l = __lane_id();
+    for (int d = threadIdx.x; d < NWAVES * RS_BUCKETS; d += BLOCK) (&wcnt[0][0])[d] = 0;
+    __syncthreads();
+
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * RS_WAVE_KEYS;
+    uint64_t k[RS_IPT];
+    uint32_t rank[RS_IPT];
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < RS_IPT; j++) {
+        int64_t i = base + (int64_t)j * WAVE + l;
+        bool valid = i < n;
+        k[j] = valid ? keys[i] : 0;
+        uint32_t d = (uint32_t)(k[j] >> shift) & (RS_BUCKETS - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RS_BITS; b++) {
+            uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        uint32_t before = (uint32_t)__popcll(peers & lt);
+        uint32_t cntp = (uint32_t)__popcll(peers);
+        uint32_t basec = valid ? wcnt[w][d] : 0;
+        rank[j] = basec + before;
+        // the lowest peer lane publishes the new count after every peer has read it
+        if (valid && before == 0) wcnt[w][d] = basec + cntp;
+    }
+    __syncthreads();
+    // per digit: totals over waves, exclusive across waves
+    {
+        const int d = threadIdx.x;
+        uint32_t run = 0;
+#pragma unroll
+        for (int q = 0; q < NWAVES; q++) {
+            uint32_t c = wcnt[q][d];
+            wcnt[q][d] = run;
+            run += c;
+        }
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan(run, sc, &tot);
+        dstart[d] = ex;
+        gbase[d] = hoff[(int64_t)d * ntiles + blockIdx.x];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_IPT; j++) {
+        int64_t i = base + (int64_t)j * WAVE + l;
+        if (i < n) {
+            uint32_t d = (uint32_t)(k[j] >> shift) & (RS_BUCKETS - 1);
+            stage[dstart[d] + wcnt[w][d] + rank[j]] = k[j];
+        }
+    }
+    __syncthreads();
+    const int64_t tile_n = min((int64_t)RS_TILE, n - (int64_t)blockIdx.x * RS_TILE);
+    for (int p = threadIdx.x; p < tile_n; p += BLOCK) {
+        uint64_t key = stage[p];
+        uint32_t d = (uint32_t)(key >> shift) & (RS_BUCKETS - 1);
+        out[(int64_t)gbase[d] + (p - dstart[d])] = key;
+    }
+}
+
+}  // namespace
+
+size_t sort_temp_bytes(int64_t n) {
+    int64_t tiles = ceil_div(n > 0 ? n : 1, RS_TILE);
+    int64_t hn_ = tiles * RS_BUCKETS;
+    return (size_t)(hn_ + 1) * 4 * 2 + scan_temp_bytes(hn_) + 512;
+}
+
+hn_status radix_sort_u64(uint64_t** keys, uint64_t** alt, int64_t n, int key_bits, void* tmp,
+                         hipStream_t st) {
+    if (n <= 1) return HN_OK;
+    int64_t tiles = ceil_div(n, RS_TILE);
+    int64_t hn_ = tiles * RS_BUCKETS;
+    HN_ARG(hn_ < (int64_t)0xFFFFFFFF && n < (int64_t)0xFFFFFFFF, "radix_sort_u64: too many keys");
+    uint32_t* hist = reinterpret_cast<uint32_t*>(tmp);
+    uint32_t* hoff = hist + hn_ + 1;
+    void* stmp = reinterpret_cast<char*>(tmp) + (size_t)(hn_ + 1) * 4 * 2 + 256;
+    stmp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(stmp) + 255) & ~uintptr_t(255));
+    for (int shift = 0; shift < key_bits; shift += RS_BITS) {
+        rs_hist_kernel<<<(unsigned)tiles, BLOCK, 0, st>>>(*keys, n, shift, tiles, hist);
+        HN_CHECK_LAUNCH();
+        HN_TRY((exclusive_scan<uint32_t>(ArrayIn<uint32_t>{hist}, hoff, hn_, stmp, st)));
+        rs_scatter_kernel<<<(unsigned)tiles, BLOCK, 0, st>>>(*keys, n, shift, tiles, hoff, *alt);
+        HN_CHECK_LAUNCH();
+        uint64_t* t = *keys;
+        *keys = *alt;
+        *alt = t;
+    }
+    return HN_OK;
+}
+
+}  // namespace hn

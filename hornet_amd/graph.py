@@ -1,0 +1,333 @@
+"""Host-side mirror of the reference's Hornet API over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference:
+
+* ``HornetInit(nV, nE, offsets, edges)``  -- hornet/include/Core/HornetInitialize/HornetInit.cuh:70-73
+* ``Hornet(init)``                        -- hornet/include/Core/Hornet.cuh:105-129
+  (``nV``, ``nE``, ``insert``, ``erase``, ``getCOO``, ``max_degree_id``,
+  ``max_degree``, ``device``)
+* ``BatchUpdatePtr(n, src, dst)`` / ``BatchUpdate`` -- hornet/include/Core/BatchUpdate/BatchUpdate.cuh:95-197
+* ``BfsTopDown``                          -- hornetsnest/include/Static/BreadthFirstSearch/TopDown.cuh:52-70
+* ``TriangleCounting2``                   -- hornetsnest/include/Static/TriangleCounting/triangle2.cuh:18-37
+
+Device memory is held in torch tensors (plumbing only); every computation runs
+in libhornet_hip.so.  Where the reference exits the process on an error, these
+raise ``HornetError``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import HornetError, call
+
+INF = np.iinfo(np.int32).max
+
+
+def _dev() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("hornet_amd needs a HIP device (no CPU fallback)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t: torch.Tensor):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _as_device_i32(x) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        t = x
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(x, dtype=np.int32))
+    return t.to(device=_dev(), dtype=torch.int32).contiguous()
+
+
+class HornetInit:
+    """Host CSR (offsets[nV+1], edges[nE]) used to build a Hornet."""
+
+    def __init__(self, nV: int, nE: int, offsets, edges):
+        self.nV = int(nV)
+        self.nE = int(nE)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.edges = np.ascontiguousarray(edges, dtype=np.int32)
+        if self.offsets.size != self.nV + 1 or self.edges.size != self.nE:
+            raise HornetError(_lib.HN_E_INVALID_ARG, "HornetInit", "size mismatch")
+
+
+class BatchUpdatePtr:
+    """(n, src, dst) batch; host arrays or device tensors."""
+
+    def __init__(self, n: int, src, dst):
+        self.n = int(n)
+        self.src = src
+        self.dst = dst
+
+
+class BatchUpdate:
+    """A batch copied to the device (BatchUpdate::reset copies it,
+    hornet/include/Core/BatchUpdate/BatchUpdate.i.cuh:154-184)."""
+
+    def __init__(self, ptr_or_src, dst=None):
+        if isinstance(ptr_or_src, BatchUpdatePtr):
+            src, dst, n = ptr_or_src.src, ptr_or_src.dst, ptr_or_src.n
+        else:
+            src, n = ptr_or_src, len(ptr_or_src)
+        self.src = _as_device_i32(src)[:n]
+        self.dst = _as_device_i32(dst)[:n]
+        if self.src.numel() != self.dst.numel():
+            raise HornetError(_lib.HN_E_INVALID_ARG, "BatchUpdate", "src/dst length mismatch")
+
+    def nE(self) -> int:
+        return int(self.src.numel())
+
+    size = nE
+
+
+class Hornet:
+    """hornet::gpu::Hornet<int> over an hn_graph handle."""
+
+    def __init__(self, init: HornetInit | None = None, *, slack: float = 0.0, _handle=None):
+        self._h = ctypes.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        elif init is not None:
+            call("hn_graph_create_csr", init.nV, init.nE, init.offsets.ctypes.data_as(ctypes.c_void_p),
+                 init.edges.ctypes.data_as(ctypes.c_void_p), 0, float(slack), _stream(),
+                 ctypes.byref(self._h))
+        else:
+            raise HornetError(_lib.HN_E_INVALID_ARG, "Hornet", "need a HornetInit")
+
+    # ---------------------------------------------------------------- factories
+    @classmethod
+    def from_csr(cls, offsets, edges, slack: float = 0.0) -> "Hornet":
+        offsets = np.asarray(offsets)
+        return cls(HornetInit(offsets.size - 1, int(offsets[-1]), offsets, edges), slack=slack)
+
+    @classmethod
+    def from_coo_device(cls, nv: int, src: torch.Tensor, dst: torch.Tensor, symmetrize=True,
+                        drop_self_loops=True, slack: float = 0.0) -> "Hornet":
+        h = ctypes.c_void_p()
+        flags = _lib.HN_DEVICE_PTRS | (_lib.HN_SYMMETRIZE if symmetrize else 0) | \
+            (_lib.HN_DROP_SELF_LOOPS if drop_self_loops else 0)
+        call("hn_graph_create_coo", int(nv), int(src.numel()), _ptr(src), _ptr(dst), flags,
+             float(slack), _stream(), ctypes.byref(h))
+        return cls(_handle=h)
+
+    @classmethod
+    def rmat(cls, scale: int, seed: int, edge_factor: int = 16, slack: float = 0.0) -> "Hornet":
+        h = ctypes.c_void_p()
+        call("hn_gen_rmat_graph", int(scale), int(edge_factor), int(seed), float(slack), _stream(),
+             ctypes.byref(h))
+        return cls(_handle=h)
+
+    # ------------------------------------------------------------------ queries
+    @property
+    def handle(self):
+        return self._h
+
+    def nV(self) -> int:
+        v = ctypes.c_int32()
+        call("hn_graph_nv", self._h, ctypes.byref(v))
+        return v.value
+
+    def nE(self) -> int:
+        v = ctypes.c_int64()
+        call("hn_graph_ne", self._h, ctypes.byref(v))
+        return v.value
+
+    def device(self) -> _lib.HnDview:
+        v = _lib.HnDview()
+        call("hn_graph_device_view", self._h, ctypes.byref(v))
+        return v
+
+    def memory(self):
+        used, cap = ctypes.c_int64(), ctypes.c_int64()
+        call("hn_graph_memory", self._h, ctypes.byref(used), ctypes.byref(cap), _stream())
+        return used.value, cap.value
+
+    def max_degree_id(self) -> int:
+        i, d = ctypes.c_int32(), ctypes.c_int32()
+        call("hn_graph_max_degree", self._h, ctypes.byref(i), ctypes.byref(d), _stream())
+        return i.value
+
+    def max_degree(self) -> int:
+        i, d = ctypes.c_int32(), ctypes.c_int32()
+        call("hn_graph_max_degree", self._h, ctypes.byref(i), ctypes.byref(d), _stream())
+        return d.value
+
+    def degrees(self) -> torch.Tensor:
+        out = torch.empty(max(self.nV(), 1), dtype=torch.int32, device=_dev())
+        call("hn_graph_degrees", self._h, _ptr(out), _stream())
+        return out[: self.nV()]
+
+    def getCOO(self, sort: bool = True):
+        """(src, dst) device tensors sorted by (src, dst) (adjacency is kept sorted)."""
+        ne = self.nE()
+        src = torch.empty(max(ne, 1), dtype=torch.int32, device=_dev())
+        dst = torch.empty(max(ne, 1), dtype=torch.int32, device=_dev())
+        n = ctypes.c_int64()
+        call("hn_graph_export_coo", self._h, _ptr(src), _ptr(dst), ne, ctypes.byref(n), _stream())
+        return src[:ne], dst[:ne]
+
+    def getCSR(self):
+        nv, ne = self.nV(), self.nE()
+        off = torch.empty(nv + 1, dtype=torch.int64, device=_dev())
+        adj = torch.empty(max(ne, 1), dtype=torch.int32, device=_dev())
+        call("hn_graph_export_csr", self._h, _ptr(off), _ptr(adj), ne, _stream())
+        return off, adj[:ne]
+
+    # ------------------------------------------------------------------ updates
+    def insert(self, batch: BatchUpdate, removeBatchDuplicates: bool = False,
+               removeGraphDuplicates: bool = False) -> int:
+        flags = _lib.HN_DEVICE_PTRS
+        if removeBatchDuplicates:
+            flags |= _lib.HN_DEDUP_BATCH
+        if removeGraphDuplicates:
+            flags |= _lib.HN_DEDUP_GRAPH
+        n = ctypes.c_int64()
+        call("hn_graph_insert", self._h, _ptr(batch.src), _ptr(batch.dst), batch.nE(), flags,
+             ctypes.byref(n), _stream())
+        return n.value
+
+    def erase(self, batch: BatchUpdate, removeBatchDuplicates: bool = False) -> int:
+        flags = _lib.HN_DEVICE_PTRS | (_lib.HN_DEDUP_BATCH if removeBatchDuplicates else 0)
+        n = ctypes.c_int64()
+        call("hn_graph_erase", self._h, _ptr(batch.src), _ptr(batch.dst), batch.nE(), flags,
+             ctypes.byref(n), _stream())
+        return n.value
+
+    # ------------------------------------------------------------------ cleanup
+    def release(self):
+        if self._h:
+            _lib.load().hn_graph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class StaticAlgorithm:
+    """primitives/HornetAlg.cuh:51-150: run/reset/release/validate."""
+
+    def __init__(self, hornet: Hornet):
+        self.hornet = hornet
+
+    def run(self):
+        raise NotImplementedError
+
+    def reset(self):
+        raise NotImplementedError
+
+    def release(self):
+        raise NotImplementedError
+
+    def validate(self) -> bool:
+        raise NotImplementedError
+
+
+class BfsTopDown(StaticAlgorithm):
+    """HornetsNest BfsTopDown (TopDown.cu:68-128).  ``validate`` needs a host
+    reference, which the caller supplies (the reference's is a stub returning
+    true, TopDown.cu:113-128)."""
+
+    def __init__(self, hornet: Hornet):
+        super().__init__(hornet)
+        self.d_distances = torch.empty(max(hornet.nV(), 1), dtype=torch.int32, device=_dev())
+        self.bfs_source = 0
+        self.stats = _lib.HnBfsStats()
+
+    def reset(self):
+        self.d_distances.fill_(INF)
+
+    def set_parameters(self, source: int):
+        self.bfs_source = int(source)
+
+    def run(self):
+        call("hn_bfs_topdown", self.hornet.handle, self.bfs_source, _ptr(self.d_distances),
+             ctypes.byref(self.stats), _stream())
+
+    def release(self):
+        self.d_distances = None
+
+    def distances(self) -> torch.Tensor:
+        return self.d_distances[: self.hornet.nV()]
+
+    def validate(self, host_distances=None) -> bool:
+        if host_distances is None:
+            return True
+        return bool(np.array_equal(self.distances().cpu().numpy(), np.asarray(host_distances)))
+
+
+class TriangleCounting2(StaticAlgorithm):
+    """HornetsNest TriangleCounting2 (triangle2.cu:24-175).  ``run`` counts on the
+    graph as stored (the reference's driver stores the DIRECTED_BY_DEGREE
+    orientation, TriangleTest2.cu:86-87); ``run_undirected`` orients the
+    symmetric storage on the device first."""
+
+    def __init__(self, hornet: Hornet):
+        super().__init__(hornet)
+        self.triPerVertex = None
+        self.total = 0
+
+    def init(self):
+        self.triPerVertex = torch.zeros(max(self.hornet.nV(), 1), dtype=torch.int64, device=_dev())
+
+    def reset(self):
+        if self.triPerVertex is not None:
+            self.triPerVertex.zero_()
+        self.total = 0
+
+    def run(self, work_factor: int = 1):
+        if self.triPerVertex is None:
+            self.init()
+        t = ctypes.c_uint64()
+        call("hn_tc2_stored", self.hornet.handle, ctypes.byref(t), _ptr(self.triPerVertex), _stream())
+        self.total = t.value
+
+    def run_undirected(self):
+        if self.triPerVertex is None:
+            self.init()
+        t = ctypes.c_uint64()
+        call("hn_tc_count", self.hornet.handle, ctypes.byref(t), _ptr(self.triPerVertex), _stream())
+        self.total = t.value
+
+    def countTriangles(self) -> int:
+        return int(self.total)
+
+    def copyTCToHost(self) -> np.ndarray:
+        return self.triPerVertex[: self.hornet.nV()].cpu().numpy().view(np.uint64)
+
+    def release(self):
+        self.triPerVertex = None
+
+    def validate(self) -> bool:
+        return True
+
+
+def rmat_pairs_device(scale: int, seed: int, edge_factor: int = 16, first: int = 0, count=None):
+    m = edge_factor << scale
+    if count is None:
+        count = m - first
+    src = torch.empty(max(count, 1), dtype=torch.int32, device=_dev())
+    dst = torch.empty(max(count, 1), dtype=torch.int32, device=_dev())
+    call("hn_gen_rmat_pairs", int(scale), int(first), int(count), int(seed), _ptr(src), _ptr(dst),
+         _stream())
+    return src[:count], dst[:count]
+
+
+def uniform_pairs_device(n: int, count: int, seed: int):
+    src = torch.empty(max(count, 1), dtype=torch.int32, device=_dev())
+    dst = torch.empty(max(count, 1), dtype=torch.int32, device=_dev())
+    call("hn_gen_uniform_pairs", int(n), int(count), int(seed), _ptr(src), _ptr(dst), _stream())
+    return src[:count], dst[:count]

@@ -1,0 +1,191 @@
+/*
+ * hornet_hip.h -- C ABI of libhornet_hip.so, the MI355X (gfx950) implementation
+ * of Hornet's dynamic-graph hot path.
+ *
+ * The reference has no shared-library boundary: everything is header-only
+ * C++/CUDA instantiated in the caller's translation unit
+ * (hornetsnest/include/HornetAlg.hpp:44-53).  This header is the boundary that
+ * the C++ facade (the headers under include/hornet/) and any FFI binding call; each entry
+ * point names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every call returns an hn_status; nothing here exits or throws (the
+ *     reference prints and calls exit(EXIT_FAILURE) on any CUDA error,
+ *     xlib/src/Device/Util/SafeCudaAPI.cpp:71-94).  hn_last_error() gives a
+ *     thread-local message for the last failure.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Calls that return a host-visible scalar synchronise that stream; all
+ *     others are asynchronous on it.
+ *   - Input pointers are borrowed for the duration of the call.  Output
+ *     buffers are caller-owned, with explicit capacity where sizes vary.
+ *   - A graph handle is not internally locked: one host thread (or one
+ *     external lock) per handle.  Different handles are independent.
+ *   - Vertex ids and degrees are int32 (VID_T = DEGREE_T = int,
+ *     hornet/include/Core/Conf/HornetConf.cuh:41-42); edge counts and offsets
+ *     are int64.
+ */
+#ifndef HORNET_HIP_H
+#define HORNET_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t hn_status;
+enum {
+    HN_OK = 0,
+    HN_E_INVALID_ARG = 1, /* bad size, id out of range, null pointer        */
+    HN_E_OOM = 2,         /* device memory or edge arena exhausted           */
+    HN_E_HIP = 3,         /* a HIP runtime call failed                       */
+    HN_E_NO_DEVICE = 4,   /* no usable gfx950 device                         */
+    HN_E_CAPACITY = 5     /* caller's output buffer too small                */
+};
+
+/* flags */
+#define HN_DEVICE_PTRS      0x1u  /* pointer arguments are device memory      */
+#define HN_OFFSETS_INT32    0x2u  /* CSR offsets are int32 (HornetInit form)  */
+#define HN_DEDUP_BATCH      0x4u  /* insert/erase: removeBatchDuplicates      */
+#define HN_DEDUP_GRAPH      0x8u  /* insert: removeGraphDuplicates            */
+#define HN_DROP_SELF_LOOPS  0x10u /* graph build: drop (v,v)                  */
+#define HN_SYMMETRIZE       0x20u /* graph build: add (d,s) for every (s,d)   */
+
+typedef struct hn_graph hn_graph;
+
+/* Per-vertex record in HBM: 16 bytes, one 128-bit load.  Replaces the
+ * reference's SoA {degree, edge_block_ptr, vertex_offset, edges_per_block}
+ * (hornet/include/Core/HornetDevice/impl/Vertex.i.cuh:59-146).  The adjacency
+ * of v is arena[off .. off+deg), sorted ascending; its block holds
+ * 2^cap_log2 slots (cap_log2 = -1 for a vertex without a block). */
+typedef struct hn_vrec {
+    int64_t off;
+    int32_t deg;
+    int32_t cap_log2;
+} hn_vrec;
+
+/* POD device view passed by value into kernels (HornetDevice,
+ * hornet/include/Core/HornetDevice/HornetDevice.cuh:52-90).  Valid until the
+ * next mutating call on the graph. */
+typedef struct hn_dview {
+    int32_t nv;
+    int32_t pad_;
+    int64_t ne;
+    hn_vrec* rec;
+    int32_t* arena;
+} hn_dview;
+
+/* BFS statistics for one traversal. */
+typedef struct hn_bfs_stats {
+    int32_t levels;        /* number of non-empty frontiers                   */
+    int32_t pad_;
+    int64_t visited;       /* reached vertices, source included               */
+    int64_t edges_scanned; /* sum of degrees of reached vertices (m_C)        */
+} hn_bfs_stats;
+
+/* ---------------------------------------------------------------- misc */
+const char* hn_status_string(hn_status s);
+const char* hn_last_error(void);
+/* Library version string. */
+const char* hn_version(void);
+/* Device properties the kernels size themselves from. */
+hn_status hn_device_info(int32_t* num_cus, int64_t* lds_per_cu, int64_t* hbm_bytes,
+                         int32_t* wave_size);
+
+/* ----------------------------------------------------------- storage */
+/* Hornet(HornetInit&) -- hornet/include/Core/HornetInitialize/HornetInitialize.i.cuh:86-146
+ * with HornetInit(nV, nE, offsets, edges) -- HornetInit.cuh:70-73.
+ * offsets: nv+1 entries (int64, or int32 with HN_OFFSETS_INT32), host memory
+ * unless HN_DEVICE_PTRS.  Each adjacency is stored sorted ascending.
+ * `arena_slack` >= 1.0 sizes the edge arena as slack x the pow2 blocks the
+ * input needs (0 = default 2.0). */
+hn_status hn_graph_create_csr(int32_t nv, int64_t ne, const void* offsets,
+                              const int32_t* edges, uint32_t flags, double arena_slack,
+                              void* stream, hn_graph** out);
+/* Graph from a device COO (src,dst) of m entries: optional symmetrize /
+ * self-loop drop (GraphStd::COOtoCSR, xlib/src/Graph/GraphStd.cpp:173-341),
+ * always sorted and deduplicated.  Device pointers only. */
+hn_status hn_graph_create_coo(int32_t nv, int64_t m, const int32_t* src, const int32_t* dst,
+                              uint32_t flags, double arena_slack, void* stream, hn_graph** out);
+/* ~Hornet() */
+hn_status hn_graph_destroy(hn_graph* g);
+/* Hornet::nV() / nE() -- Hornet.cuh:113-117 */
+hn_status hn_graph_nv(const hn_graph* g, int32_t* nv);
+hn_status hn_graph_ne(const hn_graph* g, int64_t* ne);
+/* Hornet::device() -- HornetInsert.i.cuh:4-10 */
+hn_status hn_graph_device_view(const hn_graph* g, hn_dview* view);
+/* Arena bookkeeping: slots in use by blocks, slots reserved, free-list blocks. */
+hn_status hn_graph_memory(hn_graph* g, int64_t* used_slots, int64_t* capacity_slots,
+                          void* stream);
+/* Hornet::getCOO(sort=true) -- HornetQuery.i.cuh:111-151 (flattenCOOKernel
+ * :46-65).  Writes ne (src,dst) pairs sorted by (src,dst) into device buffers
+ * of capacity `cap`; *n_out = ne.  Synchronises. */
+hn_status hn_graph_export_coo(hn_graph* g, int32_t* src, int32_t* dst, int64_t cap,
+                              int64_t* n_out, void* stream);
+/* CSR export (the working replacement for the reference's broken getCSR,
+ * hornet/include/Core/Static/Static.i.cuh:100-103): offsets[nv+1] int64,
+ * adj[cap] int32, device buffers.  Synchronises. */
+hn_status hn_graph_export_csr(hn_graph* g, int64_t* offsets, int32_t* adj, int64_t cap,
+                              void* stream);
+/* Per-vertex degrees into a device int32[nv]. */
+hn_status hn_graph_degrees(hn_graph* g, int32_t* deg, void* stream);
+/* Hornet::max_degree_id / max_degree -- HornetQuery.i.cuh:10-36 (lowest id
+ * on ties).  Synchronises. */
+hn_status hn_graph_max_degree(hn_graph* g, int32_t* id, int32_t* deg, void* stream);
+
+/* ------------------------------------------------------------ updates */
+/* Hornet::insert(batch, removeBatchDuplicates, removeGraphDuplicates) --
+ * hornet/include/Core/HornetOperations/HornetInsert.i.cuh:12-27 with
+ * BatchUpdate::preprocess, BatchUpdate.i.cuh:485-502.  Flags: HN_DEDUP_BATCH,
+ * HN_DEDUP_GRAPH, HN_DEVICE_PTRS.  Adjacency stays sorted.  *inserted (may
+ * be NULL) receives the number of entries added.  Synchronises. */
+hn_status hn_graph_insert(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n,
+                          uint32_t flags, int64_t* inserted, void* stream);
+/* Hornet::erase(batch, removeBatchDuplicates) -- HornetInsert.i.cuh:71-89
+ * with BatchUpdate::preprocess_erase, BatchUpdate.i.cuh:504-524: multiset
+ * difference, absent edges ignored.  Adjacency order is preserved.
+ * *erased (may be NULL) receives the number of entries removed.
+ * Synchronises. */
+hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n,
+                         uint32_t flags, int64_t* erased, void* stream);
+
+/* --------------------------------------------------------- algorithms */
+/* BfsTopDown{set_parameters(source); run()} --
+ * hornetsnest/src/Static/BreadthFirstSearch/TopDown.cu:88-106.
+ * dist: device int32[nv]; source gets 0, unreached INT32_MAX.  Level loop on
+ * the host with one 4-byte readback per level.  stats may be NULL. */
+hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats,
+                         void* stream);
+/* TriangleCounting2{init(); run(); countTriangles()} on the undirected simple
+ * graph the symmetric storage represents --
+ * hornetsnest/src/Static/TriangleCounting/triangle2.cu:130-160 applied to the
+ * DIRECTED_BY_DEGREE orientation (hornetsnest/test/TriangleTest2.cu:86-87,
+ * xlib/src/Graph/GraphStd.cpp:247-304).  per_vertex (device uint64[nv], may be
+ * NULL) receives the count credited to the lower-ranked endpoint of each
+ * oriented edge.  Synchronises. */
+hn_status hn_tc_count(hn_graph* g, uint64_t* total, uint64_t* per_vertex, void* stream);
+/* TriangleCounting2::run() on the graph exactly as stored (no orientation):
+ * for every stored (s,d) with s != d and deg(s) >= 2, |N(s) ∩ N(d)| credited
+ * to the smaller-degree endpoint (primitives/Operator++.i.cuh:97-246,
+ * triangle2.cu:75-124).  On a graph built DIRECTED_BY_DEGREE this is the
+ * triangle count.  per_vertex may be NULL.  Synchronises. */
+hn_status hn_tc2_stored(hn_graph* g, uint64_t* total, uint64_t* per_vertex, void* stream);
+
+/* ---------------------------------------------------------- generators */
+/* Seeded R-MAT(a=.57,b=.19,c=.19,d=.05) pairs on the device: pair i, level l
+ * uses draw i*64+l of SplitMix64 stream `seed` (specified by this library; the
+ * reference has no generator).  Writes pairs [first, first+count). */
+hn_status hn_gen_rmat_pairs(int32_t scale, int64_t first, int64_t count, uint64_t seed,
+                            int32_t* src, int32_t* dst, void* stream);
+/* R-MAT graph: edge_factor * 2^scale pairs, self loops dropped, symmetrized,
+ * deduplicated, sorted adjacency. */
+hn_status hn_gen_rmat_graph(int32_t scale, int32_t edge_factor, uint64_t seed,
+                            double arena_slack, void* stream, hn_graph** out);
+/* Uniform update pairs over [0,n) with ~1 % in-batch repeats (stream `seed`). */
+hn_status hn_gen_uniform_pairs(int32_t n, int64_t count, uint64_t seed, int32_t* src,
+                               int32_t* dst, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HORNET_HIP_H */

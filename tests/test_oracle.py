@@ -1,0 +1,271 @@
+"""Pins for the CPU oracle (no GPU).
+
+Each oracle function is checked against something that does not restate it:
+the golden fixtures built from the reference's example graphs, networkx, dense
+linear algebra, and the identities the reference's own update tests assert
+(hornet/test/HornetInsertTest.cu:52-75, hornet/test/HornetDeleteTest.cu:97-120).
+"""
+import collections
+import json
+import os
+
+import networkx as nx
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+INF = 2**31 - 1
+
+
+def to_nx(off, adj, directed=False):
+    n = off.size - 1
+    G = nx.DiGraph() if directed else nx.Graph()
+    G.add_nodes_from(range(n))
+    s = np.repeat(np.arange(n), np.diff(off))
+    G.add_edges_from(zip(s.tolist(), adj.tolist()))
+    return G
+
+
+def coo(off, adj):
+    s = np.repeat(np.arange(off.size - 1), np.diff(off)).astype(np.int32)
+    return s, adj.astype(np.int32)
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(os.path.join(GOLD, "golden.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="module")
+def rmat10():
+    return O.rmat_graph(10, seed=7)
+
+
+# ---------------------------------------------------------------- fixtures
+@pytest.mark.parametrize("name", ["G.mtx", "G_cc2.mtx", "rome99.gr"])
+def test_fixture_bfs_and_triangles(golden, name):
+    g = golden[name]
+    path = os.path.join(GOLD, name)
+    if name.endswith(".mtx"):
+        n, s, d, _ = O.read_market(path)
+        off, adj = O.coo_to_csr(n, s, d, symmetrize=True, drop_self_loops=False, dedup=True)
+    else:
+        n, s, d = O.read_dimacs9(path)
+        off, adj = O.coo_to_csr(n, s, d, symmetrize=False, drop_self_loops=False, dedup=False, sort=True)
+    assert off.size - 1 == g["nv"] and off[-1] == g["ne"]
+    assert O.max_degree_id(off) == g["max_degree_id"]
+    for tag in ("bfs_maxdeg", "bfs_0"):
+        dist, _ = O.bfs(off, adj, g[tag]["source"])
+        assert dist.tolist() == g[tag]["dist"]
+        # independent check: networkx shortest path lengths on the stored arcs
+        ref = nx.single_source_shortest_path_length(to_nx(off, adj, directed=True), g[tag]["source"])
+        exp = np.full(off.size - 1, INF)
+        for k, v in ref.items():
+            exp[k] = v
+        assert np.array_equal(dist, exp)
+    so, sa = O.coo_to_csr(off.size - 1, *coo(off, adj), symmetrize=True, drop_self_loops=True)
+    assert O.triangles(so, sa) == g["triangles"]
+
+
+def test_gmtx_csr_matches_reader_semantics(golden):
+    # G.mtx has 12 arcs, one reciprocal pair (3,4)/(4,3): 24 - 2 = 22 after sort+unique
+    g = golden["G.mtx"]
+    assert g["ne"] == 22
+    off, adj = np.array(g["offsets"]), np.array(g["adj"])
+    for v in range(off.size - 1):
+        seg = adj[off[v]:off[v + 1]]
+        assert np.all(np.diff(seg) > 0)
+
+
+# ---------------------------------------------------------------- BFS
+def test_bfs_vs_networkx_rmat(rmat10):
+    off, adj = rmat10
+    G = to_nx(off, adj)
+    for src in [O.max_degree_id(off), 3, 100, 1023]:
+        dist, scanned = O.bfs(off, adj, src)
+        ref = nx.single_source_shortest_path_length(G, src)
+        exp = np.full(off.size - 1, INF)
+        for k, v in ref.items():
+            exp[k] = v
+        assert np.array_equal(dist, exp)
+        deg = np.diff(off)
+        assert scanned == int(deg[dist != INF].sum())
+
+
+def test_bfs_invariants(rmat10):
+    off, adj = rmat10
+    src = O.max_degree_id(off)
+    dist, _ = O.bfs(off, adj, src)
+    s, d = coo(off, adj)
+    ds, dd = dist[s].astype(np.int64), dist[d].astype(np.int64)
+    reached = (ds != INF) & (dd != INF)
+    assert np.all(np.abs(ds[reached] - dd[reached]) <= 1)
+    # reached and unreached never share an edge (symmetric graph)
+    assert np.all((ds == INF) == (dd == INF))
+    # every reached non-source vertex has a neighbour one level closer
+    has_parent = np.zeros(off.size - 1, bool)
+    ok = reached & (dd == ds + 1)
+    has_parent[d[ok]] = True
+    rv = np.flatnonzero(dist != INF)
+    assert np.all(has_parent[rv[rv != src]])
+    assert dist[src] == 0
+
+
+# ---------------------------------------------------------------- triangles
+def test_triangles_vs_networkx_and_dense(rmat10):
+    off, adj = rmat10
+    tri = O.triangles(off, adj)
+    assert tri == sum(nx.triangles(to_nx(off, adj)).values()) // 3
+    n = off.size - 1
+    A = np.zeros((n, n), np.float64)
+    s, d = coo(off, adj)
+    A[s, d] = 1
+    assert tri == int(round(np.trace(A @ A @ A) / 6))
+
+
+def test_tc2_semantics_on_oriented_graph(rmat10):
+    off, adj = rmat10
+    do, da = O.directed_by_degree(off, adj)
+    # each undirected edge oriented exactly once
+    assert da.size * 2 == adj.size
+    per = O.tc2_per_vertex(do, da)
+    assert int(per.sum()) == O.host_tc(do, da) == O.triangles(off, adj)
+    # on the symmetric storage every triangle is seen from all six arcs, and
+    # the smaller-degree endpoint credit rule still sums over all arcs
+    assert int(O.tc2_per_vertex(off, adj).sum()) == 6 * O.triangles(off, adj)
+
+
+def test_directed_by_degree_rule():
+    # path 0-1-2 plus 1-3: deg(1)=3 is the largest, so every edge points to 1
+    off = np.array([0, 1, 4, 5, 6], np.int64)
+    adj = np.array([1, 0, 2, 3, 1, 1], np.int32)
+    do, da = O.directed_by_degree(off, adj)
+    assert do.tolist() == [0, 1, 1, 2, 3] and da.tolist() == [1, 1, 1]
+
+
+# ---------------------------------------------------------------- updates
+def multiset(off, adj):
+    return collections.Counter(zip(*[x.tolist() for x in coo(off, adj)]))
+
+
+def test_insert_identity_reference(rmat10):
+    # HornetInsertTest.cu:52-75: without dedup, sorted(after) == sorted(before ++ batch)
+    off, adj = rmat10
+    n = off.size - 1
+    bs, bd = O.uniform_pairs(n, 500, seed=11)
+    o2, a2, added = O.insert(off, adj, bs, bd)
+    assert added == 500
+    s0, d0 = coo(off, adj)
+    before = np.concatenate([(s0.astype(np.int64) << 32) | d0, (bs.astype(np.int64) << 32) | bd])
+    s1, d1 = coo(o2, a2)
+    after = (s1.astype(np.int64) << 32) | d1
+    assert np.array_equal(np.sort(before), after)  # after is already sorted
+
+
+def test_erase_identity_reference(rmat10):
+    # HornetDeleteTest.cu:97-120: erase distinct stored edges, then
+    # sorted(after ++ batch) == before
+    off, adj = rmat10
+    s0, d0 = coo(off, adj)
+    rng = np.random.default_rng(5)
+    pick = rng.choice(adj.size, 300, replace=False)
+    o2, a2, removed = O.erase(off, adj, s0[pick], d0[pick])
+    assert removed == 300
+    s1, d1 = coo(o2, a2)
+    merged = np.concatenate([(s1.astype(np.int64) << 32) | d1,
+                             (s0[pick].astype(np.int64) << 32) | d0[pick]])
+    assert np.array_equal(np.sort(merged), (s0.astype(np.int64) << 32) | d0)
+
+
+def test_insert_dedup_flags_set_semantics(rmat10):
+    off, adj = rmat10
+    n = off.size - 1
+    s0, d0 = coo(off, adj)
+    bs, bd = O.uniform_pairs(n, 400, seed=3)
+    # add copies of existing edges and in-batch repeats
+    bs = np.concatenate([bs, s0[:50], bs[:20]]).astype(np.int32)
+    bd = np.concatenate([bd, d0[:50], bd[:20]]).astype(np.int32)
+    o2, a2, added = O.insert(off, adj, bs, bd, dedup_batch=True, dedup_graph=True)
+    exp = set(zip(s0.tolist(), d0.tolist())) | set(zip(bs.tolist(), bd.tolist()))
+    got = multiset(o2, a2)
+    assert set(got) == exp and max(got.values()) == 1
+    assert added == len(exp) - adj.size
+    # dedup_graph only: the FIRST batch copy of an existing edge is dropped
+    o3, a3, added3 = O.insert(off, adj, np.array([s0[0]] * 3, np.int32), np.array([d0[0]] * 3, np.int32),
+                              dedup_graph=True)
+    assert added3 == 2 and multiset(o3, a3)[(int(s0[0]), int(d0[0]))] == 3
+
+
+def test_erase_multiset_semantics():
+    # vertex 0 stores 1 twice (multigraph after an insert without dedup)
+    off = np.array([0, 3, 4, 4], np.int64)
+    adj = np.array([1, 1, 2, 0], np.int32)
+    o2, a2, rem = O.erase(off, adj, np.array([0, 0, 0, 1], np.int32), np.array([1, 1, 1, 2], np.int32))
+    # three copies requested, two stored -> both removed; (1,2) absent -> ignored
+    assert rem == 2 and o2.tolist() == [0, 1, 2, 2] and a2.tolist() == [2, 0]
+    o3, a3, rem3 = O.erase(off, adj, np.array([0, 0], np.int32), np.array([1, 1], np.int32),
+                           dedup_batch=True)
+    assert rem3 == 1 and a3.tolist() == [1, 2, 0]
+
+
+def test_insert_self_loops_and_empty_batch(rmat10):
+    off, adj = rmat10
+    o2, a2, added = O.insert(off, adj, np.array([5], np.int32), np.array([5], np.int32))
+    assert added == 1 and 5 in a2[o2[5]:o2[6]]
+    o3, a3, added3 = O.insert(off, adj, np.zeros(0, np.int32), np.zeros(0, np.int32))
+    assert added3 == 0 and np.array_equal(o3, off) and np.array_equal(a3, adj)
+
+
+# ---------------------------------------------------------------- generators
+def test_rmat_generator_invariants():
+    s, d = O.rmat_pairs(12, seed=1)
+    assert s.size == 16 << 12 and s.min() >= 0 and s.max() < 4096 and d.max() < 4096
+    # slicing is consistent (counter-based)
+    s2, d2 = O.rmat_pairs(12, seed=1, first=1000, count=77)
+    assert np.array_equal(s2, s[1000:1077]) and np.array_equal(d2, d[1000:1077])
+    # the top bit follows the quadrant probabilities: P(src top) = c+d = .24, P(dst top) = b+d = .24
+    top_s = (s >> 11).mean()
+    top_d = (d >> 11).mean()
+    assert abs(top_s - 0.24) < 0.01 and abs(top_d - 0.24) < 0.01
+    both = ((s >> 11) & (d >> 11)).mean()
+    assert abs(both - 0.05) < 0.005
+    # different seeds differ
+    s3, _ = O.rmat_pairs(12, seed=2)
+    assert not np.array_equal(s, s3)
+
+
+def test_rmat_graph_shape():
+    off, adj = O.rmat_graph(16, seed=1)
+    assert off.size == 65537
+    assert int(off[-1]) == 1819292         # fixed by the generator spec
+    s, d = coo(off, adj)
+    assert not np.any(s == d)
+    key = (s.astype(np.int64) << 32) | d
+    assert np.all(np.diff(key) > 0)        # sorted, no duplicates
+    rk = (d.astype(np.int64) << 32) | s
+    assert np.array_equal(np.sort(rk), key)  # symmetric
+
+
+def test_uniform_pairs_repeat_rate():
+    s, d = O.uniform_pairs(1 << 20, 200000, seed=4)
+    keys = (s.astype(np.int64) << 32) | d
+    dup = keys.size - np.unique(keys).size
+    assert 0.007 < dup / keys.size < 0.013
+
+
+def test_pick_sources():
+    off, adj = O.rmat_graph(10, seed=7)
+    src = O.pick_sources(off, 16, seed=3)
+    deg = np.diff(off)
+    assert src.size == 16 and np.all(deg[src] > 0)
+
+
+def test_two_hop_vs_networkx(rmat10):
+    off, adj = rmat10
+    G = to_nx(off, adj)
+    for v in [0, 5, O.max_degree_id(off)]:
+        assert O.two_hop(off, adj, v) == len(nx.ego_graph(G, v, radius=2))
