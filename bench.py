@@ -1,0 +1,189 @@
+"""Benchmark of the Hornet hot path on one MI355X.
+
+Workload (one "step" = one pass of the hot path over one batch of input):
+  R-MAT scale 24, edge factor 16 (a=.57 b=.19 c=.19), seed 24, int32 ids,
+  symmetrized, self loops and duplicates removed, sorted adjacency, built on
+  the device.  A step is BFS TopDown from 16 seeded non-isolated sources
+  (seed 3) -- the BFS half of the headline metric.  GTEPS counts directed
+  adjacency entries scanned (sum of degrees of reached vertices, m_C) per
+  second; Graph500's convention (m_C / 2) is reported beside it.
+
+Printed: ONE JSON line with the metric, the roofline of the dominant kernel
+(bfs_expand_kernel, HIP events around every launch in the timed region) and
+a CPU baseline (the oracle's sequential host BFS, restated from
+xlib/src/Graph/BFS.cpp:54-76, on a bounded sample of the same sources).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "BFS GTEPS + edge-updates/s at R-MAT s24; served p50/p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE for the
+    same workload (committed summary), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=24)
+    ap.add_argument("--sources", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="budget for the CPU baseline sample (0 disables)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    from hornet_amd import rng
+    from hornet_amd.graph import BfsTopDown, Hornet
+
+    torch.cuda.set_device(0)
+    t_build = time.perf_counter()
+    h = Hornet.rmat(args.scale, seed=args.seed, edge_factor=args.edge_factor)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
+    nv, ne = h.nV(), h.nE()
+    deg = h.degrees().cpu().numpy()
+    sources = rng.pick_sources(deg, args.sources, seed=3)
+    print(f"# graph R-MAT s{args.scale} ef{args.edge_factor}: nv={nv} ne={ne} built in {t_build:.2f}s",
+          file=sys.stderr, flush=True)
+
+    bfs = BfsTopDown(h)
+
+    def step(timed: bool):
+        tot_m = tot_n = kns = launches = 0
+        for s in sources:
+            bfs.set_parameters(int(s))
+            bfs.stats.time_kernels = 1 if timed else 0
+            bfs.run()
+            tot_m += bfs.stats.edges_scanned
+            tot_n += bfs.stats.visited
+            kns += bfs.stats.expand_ns
+            launches += bfs.stats.expand_launches
+        return tot_m, tot_n, kns, launches
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    agg_m = agg_n = agg_ns = agg_l = 0
+    for _ in range(args.steps):
+        m, n, kns, ln = step(True)
+        agg_m += m
+        agg_n += n
+        agg_ns += kns
+        agg_l += ln
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    print(f"# timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
+
+    gteps = agg_m / elapsed / 1e9
+    # bfs_expand_kernel algorithmic bytes: 4 B per adjacency entry scanned,
+    # 16 B vertex record + 4 B queue read per frontier vertex, 4 B distance +
+    # 4 B queue write per discovered vertex -> 4 m_C + 28 n_C
+    alg_bytes = 4 * agg_m + 28 * agg_n
+    kernel_s = agg_ns * 1e-9
+    achieved = alg_bytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
+    traffic = load_traffic("bfs_expand_traffic.json")
+    roofline = {
+        "kernel": "bfs_expand_kernel",
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": (traffic or {}).get("bytes_per_launch"),
+        "alg_bytes_per_launch": round(alg_bytes / max(agg_l, 1)),
+        "avg_launch_us": round(agg_ns / max(agg_l, 1) / 1e3, 2),
+        "launches": agg_l,
+    }
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        import oracle as O
+        off, adj = h.getCSR()
+        off = off.cpu().numpy()
+        adj = adj.cpu().numpy()
+        cm = 0
+        used = 0
+        t_cpu = 0.0
+        for s in sources:
+            t = time.perf_counter()
+            _, scanned = O.bfs(off, adj, int(s))
+            t_cpu += time.perf_counter() - t
+            cm += scanned
+            used += 1
+            if t_cpu >= args.cpu_seconds:
+                break
+        cpu = {"value": round(cm / t_cpu / 1e9, 4), "unit": "GTEPS", "cores": 1, "kind": "port",
+               "sample": f"sequential host BFS (oracle restatement of xlib/src/Graph/BFS.cpp:54-76) "
+                         f"from the first {used} of the {len(sources)} sources on the same graph",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+
+    out = {
+        "metric": METRIC,
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"BFS TopDown x{len(sources)} sources (seed 3) on R-MAT s{args.scale} "
+                        f"ef{args.edge_factor} seed {args.seed} (a=.57,b=.19,c=.19), symmetrized, deduped",
+            "scale": args.scale, "nv": nv, "ne": ne, "sources": len(sources),
+            "load_balancing": "BinarySearch (wave64 LDS-staged)",
+        },
+        "gteps_graph500": round(gteps / 2, 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -43,8 +43,10 @@ class HnDview(ctypes.Structure):
 
 
 class HnBfsStats(ctypes.Structure):
-    _fields_ = [("levels", ctypes.c_int32), ("pad_", ctypes.c_int32),
-                ("visited", ctypes.c_int64), ("edges_scanned", ctypes.c_int64)]
+    _fields_ = [("levels", ctypes.c_int32), ("time_kernels", ctypes.c_int32),
+                ("visited", ctypes.c_int64), ("edges_scanned", ctypes.c_int64),
+                ("expand_ns", ctypes.c_int64), ("expand_launches", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
 
 
 P = ctypes.c_void_p

@@ -9,14 +9,24 @@
 // may be enqueued more than once; the distances are still exact.
 //
 // Here, per level:
-//   1. a scan of deg(q[i]) read straight from the vertex records (no separate
-//      work array pass);
-//   2. one expansion kernel: the wave64 LB tile maps each edge of the frontier
-//      to (vertex, offset), the neighbour is tested against a visited bitmap
-//      (n/8 bytes, L2/MALL resident) and claimed with atomicOr, so each vertex
-//      enters the next frontier exactly once; appends are aggregated per wave
-//      with one 64-lane ballot and one atomicAdd;
-//   3. one 8-byte readback (next frontier size + this level's edge count).
+//   1. a scan of deg(q[i]) read straight from the vertex records; its
+//      down-sweep also writes, per frontier vertex, the adjacency base
+//      (record offset minus prefix, so an edge's address is base + work index)
+//      and, per 2048-edge tile, the frontier index holding the tile's first
+//      edge -- the partition that BinarySearchLB recomputes with two global
+//      binary searches per block (BinarySearchLB.i.cuh:254-264);
+//   2. one expansion kernel: each tile stages its frontier slice in LDS, maps
+//      every edge to (vertex, offset) with an LDS search, tests the neighbour
+//      against the visited bitmap as of the start of the level (n/8 bytes,
+//      read-only during the kernel, so it stays cache resident) and marks an
+//      unvisited neighbour with an idempotent byte store.  No atomics: with
+//      eight XCDs, each holding its own L2, device-scope atomics cannot be
+//      served from cache, and claiming vertices with atomicOr measured 7x
+//      slower;
+//   3. one collect kernel: a lane owns 32 vertices (one bitmap word), turns
+//      their marks into visited bits, distances and next-frontier entries
+//      (one atomicAdd per wave), so every vertex enters a frontier once;
+//   4. one 8-byte readback (next frontier size + this level's edge count).
 // Distances: source 0, its neighbours 1, unreached INT32_MAX (TopDown.cu:42,81).
 #include "graph.hpp"
 #include "prims.hpp"
@@ -34,6 +44,24 @@ struct FrontierDegIn {
     __device__ __forceinline__ uint32_t operator()(int64_t i) const { return (uint32_t)rec[q[i]].deg; }
 };
 
+// per frontier vertex i with exclusive edge prefix `ex` and degree d
+struct FrontierOut {
+    const int32_t* q;
+    const hn_vrec* rec;
+    uint32_t* P;
+    int64_t* base;
+    uint32_t* tile_lo;
+    __device__ __forceinline__ void operator()(int64_t i, uint32_t ex, uint32_t d) const {
+        P[i] = ex;
+        base[i] = rec[q[i]].off - (int64_t)ex;
+        if (d) {
+            uint32_t t0 = (uint32_t)(((uint64_t)ex + EXP_TILE - 1) / EXP_TILE);
+            uint32_t t1 = (uint32_t)(((uint64_t)ex + d + EXP_TILE - 1) / EXP_TILE);
+            for (uint32_t t = t0; t < t1; t++) tile_lo[t] = (uint32_t)i;
+        }
+    }
+};
+
 __global__ void bfs_init_kernel(int32_t nv, int32_t* __restrict__ dist, uint32_t* __restrict__ bitmap,
                                 int32_t source, int32_t* __restrict__ q) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -43,40 +71,76 @@ __global__ void bfs_init_kernel(int32_t nv, int32_t* __restrict__ dist, uint32_t
     if (i == 0) q[0] = source;
 }
 
-__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const int32_t* __restrict__ q, int64_t F,
-                                                           const uint32_t* __restrict__ P,
-                                                           const hn_vrec* __restrict__ rec,
+// Phase 1: every edge of the frontier tests the visited bitmap as of the
+// start of the level (read-only in this kernel, so cacheable) and marks an
+// unvisited neighbour in the byte array `next`.  The mark is an idempotent
+// byte store -- the reference's check-then-set without the queue insert --
+// so no atomics are needed and no update can be lost.
+__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(int64_t F, const uint32_t* __restrict__ P,
+                                                           const int64_t* __restrict__ segbase,
+                                                           const uint32_t* __restrict__ tile_lo,
                                                            const int32_t* __restrict__ arena,
-                                                           uint32_t* __restrict__ bitmap,
-                                                           int32_t* __restrict__ dist, int32_t level,
-                                                           int32_t* __restrict__ qn,
-                                                           uint32_t* __restrict__ qn_count) {
+                                                           const uint32_t* __restrict__ bitmap,
+                                                           uint8_t* __restrict__ next) {
     __shared__ LbShared<EXP_IPT, uint32_t> sh;
     const int64_t W = P[F];
-    const uint64_t lt = lanemask_lt();
-    for (int64_t w0 = (int64_t)blockIdx.x * EXP_TILE; w0 < W; w0 += (int64_t)gridDim.x * EXP_TILE) {
-        lb_tile<EXP_IPT>(
-            P, F, w0, W, sh,
-            [&](int64_t s) { return rec[q[s]].off - (int64_t)P[s]; },
+    const int64_t ntiles = (W + EXP_TILE - 1) / EXP_TILE;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t lo = tile_lo[t];
+        const int64_t hi = (t + 1 < ntiles) ? (int64_t)tile_lo[t + 1] : F - 1;
+        lb_tile_range<EXP_IPT>(
+            P, lo, hi, t * EXP_TILE, W, sh, [&](int64_t s) { return segbase[s]; },
             [&](int64_t, int64_t w, int64_t base) {
                 const int32_t v = arena[base + w];
-                const uint32_t bit = 1u << (v & 31);
-                uint32_t* word = bitmap + (v >> 5);
-                bool won = false;
-                if (!(*word & bit)) {
-                    uint32_t old = atomicOr(word, bit);
-                    won = !(old & bit);
-                }
-                if (won) dist[v] = level;
-                const uint64_t m = __ballot(won);
-                if (m) {
-                    const int leader = __ffsll((unsigned long long)m) - 1;
-                    uint32_t b = 0;
-                    if (__lane_id() == leader) b = atomicAdd(qn_count, (uint32_t)__popcll(m));
-                    b = __shfl(b, leader, WAVE);
-                    if (won) qn[b + (uint32_t)__popcll(m & lt)] = v;
-                }
+                if (!(bitmap[v >> 5] & (1u << (v & 31)))) next[v] = 1;
             });
+    }
+}
+
+// Phase 2: one lane owns 32 vertices (one bitmap word and 32 mark bytes):
+// marked vertices get their distance, their visited bit and a slot in the
+// next frontier (one atomicAdd per wave); marks are cleared for the next level.
+__global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, uint32_t* __restrict__ bitmap,
+                                                            uint32_t* __restrict__ next4,
+                                                            int32_t* __restrict__ dist, int32_t level,
+                                                            int32_t* __restrict__ qn,
+                                                            uint32_t* __restrict__ qn_count) {
+    const int64_t wi = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t found = 0;
+    if (wi < words) {
+        const uint4* p = reinterpret_cast<const uint4*>(next4 + wi * 8);
+        uint4 a = p[0], b = p[1];
+        uint32_t m[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            uint32_t x = m[k];
+            // byte j of word k is vertex 32*wi + 4k + j
+            found |= ((x & 0xFFu) ? 1u : 0u) << (4 * k);
+            found |= ((x & 0xFF00u) ? 1u : 0u) << (4 * k + 1);
+            found |= ((x & 0xFF0000u) ? 1u : 0u) << (4 * k + 2);
+            found |= ((x & 0xFF000000u) ? 1u : 0u) << (4 * k + 3);
+        }
+        if (found) {
+            uint4 z = make_uint4(0, 0, 0, 0);
+            uint4* pw = reinterpret_cast<uint4*>(next4 + wi * 8);
+            pw[0] = z;
+            pw[1] = z;
+            bitmap[wi] |= found;
+        }
+    }
+    const uint32_t cnt = (uint32_t)__popc(found);
+    const uint32_t inc = wave_inclusive_scan(cnt);
+    const uint32_t tot = __shfl(inc, WAVE - 1, WAVE);
+    if (tot == 0) return;
+    uint32_t b = 0;
+    if (__lane_id() == WAVE - 1) b = atomicAdd(qn_count, tot);
+    b = __shfl(b, WAVE - 1, WAVE) + inc - cnt;
+    while (found) {
+        const int j = __ffs(found) - 1;
+        found &= found - 1;
+        const int32_t v = (int32_t)(wi * 32 + j);
+        dist[v] = level;
+        qn[b++] = v;
     }
 }
 
@@ -88,18 +152,25 @@ struct BfsBufs {
     int32_t* qa;
     int32_t* qb;
     uint32_t* P;
+    int64_t* base;
+    uint32_t* tile_lo;
+    uint8_t* next;   // one mark byte per vertex, padded to whole bitmap words
     uint32_t* cnt;   // [0] next frontier size
 };
 
 static hn_status bfs_buffers(hn_graph* g, BfsBufs* b) {
     const int64_t nv = g->nv > 0 ? g->nv : 1;
     const int64_t words = (nv + 31) / 32;
-    auto al = [](int64_t x) { return (x + 63) & ~int64_t(63); };
+    const int64_t tiles = g->ne / EXP_TILE + 2;
+    auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
     int64_t o_bitmap = 0;
     int64_t o_qa = al(o_bitmap + words * 4);
     int64_t o_qb = al(o_qa + nv * 4);
     int64_t o_P = al(o_qb + nv * 4);
-    int64_t o_cnt = al(o_P + (nv + 1) * 4);
+    int64_t o_base = al(o_P + (nv + 1) * 4);
+    int64_t o_tile = al(o_base + nv * 8);
+    int64_t o_next = al(o_tile + tiles * 4);
+    int64_t o_cnt = al(o_next + words * 32);
     int64_t total = al(o_cnt + 64);
     HN_TRY(g->s_bfs.reserve((size_t)total));
     char* base = g->s_bfs.as<char>();
@@ -107,6 +178,9 @@ static hn_status bfs_buffers(hn_graph* g, BfsBufs* b) {
     b->qa = reinterpret_cast<int32_t*>(base + o_qa);
     b->qb = reinterpret_cast<int32_t*>(base + o_qb);
     b->P = reinterpret_cast<uint32_t*>(base + o_P);
+    b->base = reinterpret_cast<int64_t*>(base + o_base);
+    b->tile_lo = reinterpret_cast<uint32_t*>(base + o_tile);
+    b->next = reinterpret_cast<uint8_t*>(base + o_next);
     b->cnt = reinterpret_cast<uint32_t*>(base + o_cnt);
     HN_TRY(g->s_scan.reserve(scan_temp_bytes(nv)));
     return HN_OK;
@@ -125,35 +199,73 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
     BfsBufs b;
     HN_TRY(bfs_buffers(g, &b));
     const int64_t nv = g->nv;
+    const int64_t words = (nv + 31) / 32;
     bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(g->nv, dist, b.bitmap, source, b.qa);
     HN_CHECK_LAUNCH();
+    HN_HIP(hipMemsetAsync(b.next, 0, (size_t)words * 32, st));
     const int64_t grid_cap = (int64_t)num_cus() * 8;
+    const bool timed = stats && stats->time_kernels;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (timed) {
+        HN_HIP(hipEventCreate(&ev0));
+        HN_HIP(hipEventCreate(&ev1));
+    }
     int64_t F = 1;
     int32_t level = 1;
-    int64_t visited = 0, scanned = 0;
+    int64_t visited = 0, scanned = 0, kns = 0;
     int32_t nlevels = 0;
     uint32_t* host = reinterpret_cast<uint32_t*>(g->h_pinned);
+    hn_status rc = HN_OK;
     while (F > 0) {
         visited += F;
         nlevels++;
-        HN_TRY((exclusive_scan<uint32_t>(FrontierDegIn{b.qa, g->rec}, b.P, F, g->s_scan.ptr, st)));
-        HN_HIP(hipMemsetAsync(b.cnt, 0, 4, st));
-        bfs_expand_kernel<<<(unsigned)grid_cap, BLOCK, 0, st>>>(b.qa, F, b.P, g->rec, g->arena, b.bitmap,
-                                                                  dist, level, b.qb, b.cnt);
-        HN_CHECK_LAUNCH();
-        HN_HIP(hipMemcpyAsync(host, b.cnt, 4, hipMemcpyDeviceToHost, st));
-        HN_HIP(hipMemcpyAsync(host + 1, b.P + F, 4, hipMemcpyDeviceToHost, st));
-        HN_HIP(hipStreamSynchronize(st));
+        rc = exclusive_scan_fn<uint32_t>(FrontierDegIn{b.qa, g->rec},
+                                         FrontierOut{b.qa, g->rec, b.P, b.base, b.tile_lo}, b.P + F, F,
+                                         g->s_scan.ptr, st);
+        if (rc != HN_OK) break;
+        hipError_t e = hipMemsetAsync(b.cnt, 0, 4, st);
+        if (timed && e == hipSuccess) e = hipEventRecord(ev0, st);
+        if (e == hipSuccess) {
+            bfs_expand_kernel<<<(unsigned)grid_cap, BLOCK, 0, st>>>(F, b.P, b.base, b.tile_lo, g->arena,
+                                                                      b.bitmap, b.next);
+            e = hipGetLastError();
+        }
+        if (timed && e == hipSuccess) e = hipEventRecord(ev1, st);
+        if (e == hipSuccess) {
+            bfs_collect_kernel<<<(unsigned)ceil_div(words, BLOCK), BLOCK, 0, st>>>(
+                words, b.bitmap, reinterpret_cast<uint32_t*>(b.next), dist, level, b.qb, b.cnt);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(host, b.cnt, 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(host + 1, b.P + F, 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (timed && e == hipSuccess) {
+            float ms = 0.f;
+            e = hipEventElapsedTime(&ms, ev0, ev1);
+            kns += (int64_t)((double)ms * 1e6);
+        }
+        if (e != hipSuccess) {
+            set_error(std::string("bfs level: ") + hipGetErrorString(e));
+            rc = HN_E_HIP;
+            break;
+        }
         scanned += host[1];
         F = host[0];
         std::swap(b.qa, b.qb);
         level++;
     }
+    if (timed) {
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+    }
+    if (rc != HN_OK) return rc;
     if (stats) {
         stats->levels = nlevels;
-        stats->pad_ = 0;
         stats->visited = visited;
         stats->edges_scanned = scanned;
+        stats->expand_ns = kns;
+        stats->expand_launches = nlevels;
+        stats->pad_ = 0;
     }
     return HN_OK;
 }

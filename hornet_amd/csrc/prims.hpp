@@ -97,6 +97,56 @@ __global__ __launch_bounds__(BLOCK) void scan_down_kernel(InFn in, int64_t n, co
 template <typename Out>
 __global__ void scan_zero_kernel(Out* out) { *out = 0; }
 
+// down-sweep variant that hands (index, exclusive prefix, value) to a functor
+template <typename Out, typename InFn, typename OutFn>
+__global__ __launch_bounds__(BLOCK) void scan_down_fn_kernel(InFn in, int64_t n, const Out* __restrict__ partial,
+                                                             OutFn out) {
+    __shared__ Out wtot[NWAVES];
+    const int w = threadIdx.x / WAVE;
+    const int l = __lane_id();
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)w * SCAN_WAVE_CHUNK;
+    Out pre[SCAN_IPT];
+    Out val[SCAN_IPT];
+    Out carry = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; j++) {
+        int64_t i = base + (int64_t)j * WAVE + l;
+        Out v = i < n ? (Out)in(i) : (Out)0;
+        Out inc = wave_inclusive_scan(v);
+        pre[j] = carry + inc - v;
+        val[j] = v;
+        carry += __shfl(inc, WAVE - 1, WAVE);
+    }
+    if (l == 0) wtot[w] = carry;
+    __syncthreads();
+    Out off = partial[blockIdx.x];
+    for (int k = 0; k < w; k++) off += wtot[k];
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; j++) {
+        int64_t i = base + (int64_t)j * WAVE + l;
+        if (i < n) out(i, (Out)(off + pre[j]), val[j]);
+    }
+}
+
+// Exclusive scan of in(0..n) delivered to out(i, prefix, value); *total = sum.
+template <typename Out, typename InFn, typename OutFn>
+hn_status exclusive_scan_fn(InFn in, OutFn out, Out* total, int64_t n, void* tmp, hipStream_t st) {
+    if (n <= 0) {
+        scan_zero_kernel<Out><<<1, 1, 0, st>>>(total);
+        HN_CHECK_LAUNCH();
+        return HN_OK;
+    }
+    int64_t tiles = ceil_div(n, SCAN_TILE);
+    Out* partial = reinterpret_cast<Out*>(tmp);
+    scan_reduce_kernel<Out, InFn><<<(unsigned)tiles, BLOCK, 0, st>>>(in, n, partial);
+    HN_CHECK_LAUNCH();
+    scan_partials_kernel<Out><<<1, BLOCK, 0, st>>>(partial, tiles, total);
+    HN_CHECK_LAUNCH();
+    scan_down_fn_kernel<Out, InFn, OutFn><<<(unsigned)tiles, BLOCK, 0, st>>>(in, n, partial, out);
+    HN_CHECK_LAUNCH();
+    return HN_OK;
+}
+
 // out[0..n) = exclusive prefix of in(0..n), out[n] = total.
 template <typename Out, typename InFn>
 hn_status exclusive_scan(InFn in, Out* out, int64_t n, void* tmp, hipStream_t st) {
@@ -130,9 +180,31 @@ struct LbShared {
     static constexpr int CAP = BLOCK * IPT + 1;   // segment entries staged per tile
     PrefT pref[CAP];
     int64_t base[CAP];
-    int64_t lo;
-    int count;
 };
+
+// Largest s in [lo, hi) with P[s] <= x (P[lo] <= x), found by the whole
+// workgroup: each round samples BLOCK evenly spaced entries with one parallel
+// load and counts the samples <= x, shrinking the range BLOCK-fold, instead of
+// one lane walking ~log2(n) dependent global loads.
+template <typename PrefT>
+__device__ __forceinline__ int64_t block_upper_bound_m1(const PrefT* __restrict__ P, int64_t lo, int64_t hi,
+                                                        int64_t x) {
+    while (hi - lo > 1) {
+        const int64_t len = hi - lo;
+        const int64_t step = (len + BLOCK - 1) / BLOCK;
+        const int64_t idx = lo + (int64_t)threadIdx.x * step;
+        const int pred = idx < hi && (int64_t)P[idx] <= x;
+        const int c = __syncthreads_count(pred);   // samples <= x, >= 1 since P[lo] <= x
+        lo = lo + (int64_t)(c - 1) * step;
+        hi = min(hi, lo + step);
+    }
+    return lo;
+}
+
+template <int IPT, typename PrefT, typename SegFn, typename ItemFn>
+__device__ __forceinline__ void lb_tile_range(const PrefT* __restrict__ P, int64_t lo, int64_t hi, int64_t w0,
+                                              int64_t total, LbShared<IPT, PrefT>& sh, SegFn segbase,
+                                              ItemFn item);
 
 // For items w in [w0, min(w0+BLOCK*IPT, total)): item(s, w, base_s) where s is
 // the segment containing w (P exclusive prefix over nseg segments, P[nseg] =
@@ -143,15 +215,21 @@ __device__ __forceinline__ void lb_tile(const PrefT* __restrict__ P, int64_t nse
                                         ItemFn item) {
     constexpr int TILE = BLOCK * IPT;
     const int64_t wend = min(w0 + (int64_t)TILE, total);
-    if (threadIdx.x == 0) {
-        int64_t lo = upper_bound_m1<int64_t>(P, 0, nseg + 1, w0);
-        int64_t hi = upper_bound_m1<int64_t>(P, lo, nseg + 1, wend - 1);
-        sh.lo = lo;
-        sh.count = (hi - lo + 1 <= LbShared<IPT, PrefT>::CAP) ? (int)(hi - lo + 1) : -(int)min(hi - lo + 1, (int64_t)0x7fffffff);
-    }
-    __syncthreads();
-    const int64_t lo = sh.lo;
-    const int count = sh.count;
+    const int64_t lo = block_upper_bound_m1(P, 0, nseg + 1, w0);
+    const int64_t hi = block_upper_bound_m1(P, lo, nseg + 1, wend - 1);
+    lb_tile_range<IPT>(P, lo, hi, w0, total, sh, segbase, item);
+}
+
+// Same, with the tile's segment range [lo, hi] (inclusive) already known: lo
+// holds item w0 and hi is at or past the segment holding the tile's last item.
+template <int IPT, typename PrefT, typename SegFn, typename ItemFn>
+__device__ __forceinline__ void lb_tile_range(const PrefT* __restrict__ P, int64_t lo, int64_t hi, int64_t w0,
+                                              int64_t total, LbShared<IPT, PrefT>& sh, SegFn segbase,
+                                              ItemFn item) {
+    constexpr int TILE = BLOCK * IPT;
+    const int64_t wend = min(w0 + (int64_t)TILE, total);
+    const int count = (hi - lo + 1 <= LbShared<IPT, PrefT>::CAP) ? (int)(hi - lo + 1)
+                                                                 : -(int)min(hi - lo + 1, (int64_t)0x7fffffff);
     if (count > 0) {
         for (int i = threadIdx.x; i < count; i += BLOCK) {
             sh.pref[i] = P[lo + i];

@@ -77,10 +77,13 @@ typedef struct hn_dview {
 
 /* BFS statistics for one traversal. */
 typedef struct hn_bfs_stats {
-    int32_t levels;        /* number of non-empty frontiers                   */
+    int32_t levels;          /* number of non-empty frontiers                 */
+    int32_t time_kernels;    /* in: nonzero -> time the expansion kernel      */
+    int64_t visited;         /* reached vertices, source included             */
+    int64_t edges_scanned;   /* sum of degrees of reached vertices (m_C)      */
+    int64_t expand_ns;       /* out: summed expansion-kernel time (HIP events) */
+    int32_t expand_launches; /* out: expansion-kernel launches                */
     int32_t pad_;
-    int64_t visited;       /* reached vertices, source included               */
-    int64_t edges_scanned; /* sum of degrees of reached vertices (m_C)        */
 } hn_bfs_stats;
 
 /* ---------------------------------------------------------------- misc */
