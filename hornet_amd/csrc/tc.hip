@@ -1,0 +1,298 @@
+// tc.hip -- triangle counting (HornetsNest TriangleCounting2).
+//
+// Reference: TriangleCounting2::run -> forAllAdjUnions
+// (primitives/Operator++.i.cuh:369-475) bins every stored edge by estimated
+// work into 2,048 bins with 64-bit atomics, copies the bin sizes to the host,
+// and launches up to 39 merge-path / binary-search intersection kernels
+// (:97-246), each followed by a device sync; the operator
+// (hornetsnest/src/Static/TriangleCounting/triangle2.cu:75-124) skips
+// src == dst and deg(src) < 2, intersects N(u) with N(v) where u is the
+// endpoint with the smaller degree (ties -> the destination), and credits u.
+// The driver stores the DIRECTED_BY_DEGREE orientation
+// (hornetsnest/test/TriangleTest2.cu:86-87, xlib/src/Graph/GraphStd.cpp:247-304),
+// so the sum is the triangle count.
+//
+// Here:
+//   * orientation of the symmetric storage is a flag + scan compaction (no
+//     atomics): keep (u,v) iff (deg u, u) < (deg v, v); the oriented offsets
+//     are the scan positions at the symmetric offsets;
+//   * intersection work min(d(u), d(v)) splits every edge into one of three
+//     wave64 classes: one lane per edge (<= 16 probes), eight lanes per edge
+//     (<= 512), or a whole wave per edge; each lane binary-searches elements
+//     of the shorter list in the longer one (equal to the merge count on
+//     simple graphs);
+//   * counts are reduced per wave; the total takes one 64-bit atomicAdd per
+//     wave, per-vertex credits (optional) one per edge with a non-zero count.
+#include "graph.hpp"
+#include "prims.hpp"
+
+namespace hn {
+namespace {
+
+constexpr int TC_IPT = 4;
+constexpr int TC_TILE = BLOCK * TC_IPT;
+constexpr int LIGHT_MAX = 16;
+constexpr int MEDIUM_MAX = 512;
+constexpr int MEDIUM_LANES = 8;
+
+__device__ __forceinline__ bool bsearch_has(const int32_t* __restrict__ a, int64_t n, int32_t x) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        int32_t y = a[mid];
+        if (y < x) lo = mid + 1;
+        else if (y > x) hi = mid;
+        else return true;
+    }
+    return false;
+}
+
+struct TcCsr {
+    const int64_t* off;
+    const int32_t* adj;
+};
+
+struct TcOut {
+    unsigned long long* total;
+    unsigned long long* per_vertex;   // may be null
+    uint32_t* med_q;                  // edge ids for the medium class
+    uint32_t* heavy_q;                // edge ids for the heavy class
+    uint32_t* cnt;                    // [0] medium, [1] heavy
+    int32_t* src_of;                  // edge -> source (written by the light pass)
+};
+
+// light pass over all stored edges; defers medium/heavy edges to queues
+__global__ __launch_bounds__(BLOCK) void tc_light_kernel(int32_t nv, int64_t ne, TcCsr g, TcOut o) {
+    __shared__ LbShared<TC_IPT, int64_t> sh;
+    unsigned long long acc = 0;
+    const uint64_t lt = lanemask_lt();
+    for (int64_t w0 = (int64_t)blockIdx.x * TC_TILE; w0 < ne; w0 += (int64_t)gridDim.x * TC_TILE) {
+        lb_tile<TC_IPT>(
+            g.off, (int64_t)nv, w0, ne, sh, [&](int64_t s) { return (int64_t)0; },
+            [&](int64_t s, int64_t e, int64_t) {
+                const int32_t u0 = (int32_t)s;
+                const int32_t v0 = g.adj[e];
+                const int64_t su = g.off[u0], du = g.off[u0 + 1] - su;
+                const int64_t sv = g.off[v0], dv = g.off[v0 + 1] - sv;
+                int cls = 0;   // 0 done here, 1 medium, 2 heavy
+                uint32_t c = 0;
+                o.src_of[e] = u0;
+                if (u0 != v0 && du >= 2) {
+                    const bool src_smaller = du < dv;
+                    const int32_t* sp = g.adj + (src_smaller ? su : sv);
+                    const int64_t sn = src_smaller ? du : dv;
+                    const int32_t* lp = g.adj + (src_smaller ? sv : su);
+                    const int64_t ln = src_smaller ? dv : du;
+                    if (sn <= LIGHT_MAX) {
+                        for (int64_t i = 0; i < sn; i++) c += bsearch_has(lp, ln, sp[i]) ? 1u : 0u;
+                        if (c && o.per_vertex) atomicAdd(o.per_vertex + (src_smaller ? u0 : v0), (unsigned long long)c);
+                    } else {
+                        cls = sn <= MEDIUM_MAX ? 1 : 2;
+                    }
+                }
+                acc += c;
+                // wave-aggregated queue appends for deferred edges
+                for (int q = 1; q <= 2; q++) {
+                    const uint64_t m = __ballot(cls == q);
+                    if (m) {
+                        const int leader = __ffsll((unsigned long long)m) - 1;
+                        uint32_t b = 0;
+                        if (__lane_id() == leader) b = atomicAdd(o.cnt + (q - 1), (uint32_t)__popcll(m));
+                        b = __shfl(b, leader, WAVE);
+                        if (cls == q) (q == 1 ? o.med_q : o.heavy_q)[b + (uint32_t)__popcll(m & lt)] = (uint32_t)e;
+                    }
+                }
+            });
+    }
+    acc = wave_reduce_sum(acc);
+    if (__lane_id() == 0 && acc) atomicAdd(o.total, acc);
+}
+
+// `lanes` lanes cooperate on one deferred edge
+template <int LANES>
+__global__ __launch_bounds__(BLOCK) void tc_group_kernel(TcCsr g, const uint32_t* __restrict__ q,
+                                                         const uint32_t* __restrict__ qcount,
+                                                         const int32_t* __restrict__ src_of, TcOut o) {
+    const uint32_t n = *qcount;
+    const int64_t gid = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) / LANES;
+    const int sub = threadIdx.x % LANES;
+    const int64_t ngroups = (int64_t)gridDim.x * BLOCK / LANES;
+    unsigned long long acc = 0;
+    // the LANES lanes of a group share k, so a group is either wholly active or
+    // wholly done and the group shuffles below never read an exited lane
+    for (int64_t k = gid; k < (int64_t)n; k += ngroups) {
+        uint32_t c = 0;
+        int32_t credit = -1;
+        if (k < (int64_t)n) {
+            const int64_t e = q[k];
+            const int32_t u0 = src_of[e];
+            const int32_t v0 = g.adj[e];
+            const int64_t su = g.off[u0], du = g.off[u0 + 1] - su;
+            const int64_t sv = g.off[v0], dv = g.off[v0 + 1] - sv;
+            const bool src_smaller = du < dv;
+            const int32_t* sp = g.adj + (src_smaller ? su : sv);
+            const int64_t sn = src_smaller ? du : dv;
+            const int32_t* lp = g.adj + (src_smaller ? sv : su);
+            const int64_t ln = src_smaller ? dv : du;
+            for (int64_t i = sub; i < sn; i += LANES) c += bsearch_has(lp, ln, sp[i]) ? 1u : 0u;
+            credit = src_smaller ? u0 : v0;
+        }
+        // reduce over the group
+#pragma unroll
+        for (int d = LANES / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, WAVE);
+        if (sub == 0 && c && o.per_vertex && credit >= 0) atomicAdd(o.per_vertex + credit, (unsigned long long)c);
+        if (sub == 0) acc += c;
+    }
+    acc = wave_reduce_sum(acc);
+    if (__lane_id() == 0 && acc) atomicAdd(o.total, acc);
+}
+
+// ---- orientation of the symmetric storage
+constexpr int OR_IPT = 8;
+constexpr int OR_TILE = BLOCK * OR_IPT;
+
+__global__ __launch_bounds__(BLOCK) void orient_flags_kernel(int32_t nv, int64_t ne, const int64_t* __restrict__ eoff,
+                                                             const hn_vrec* __restrict__ rec,
+                                                             const int32_t* __restrict__ arena,
+                                                             int32_t* __restrict__ adj_tmp,
+                                                             uint8_t* __restrict__ keep) {
+    __shared__ LbShared<OR_IPT, int64_t> sh;
+    for (int64_t w0 = (int64_t)blockIdx.x * OR_TILE; w0 < ne; w0 += (int64_t)gridDim.x * OR_TILE) {
+        lb_tile<OR_IPT>(
+            eoff, (int64_t)nv, w0, ne, sh, [&](int64_t s) { return rec[s].off - eoff[s]; },
+            [&](int64_t s, int64_t w, int64_t base) {
+                const int32_t v = arena[base + w];
+                const int32_t u = (int32_t)s;
+                const int32_t du = (int32_t)(eoff[u + 1] - eoff[u]);
+                const int32_t dv = rec[v].deg;
+                adj_tmp[w] = v;
+                keep[w] = (du < dv || (du == dv && u < v)) ? 1 : 0;
+            });
+    }
+}
+
+__global__ void orient_compact_kernel(int64_t ne, const uint8_t* __restrict__ keep, const uint32_t* __restrict__ pos,
+                                      const int32_t* __restrict__ adj_tmp, int32_t* __restrict__ adjp) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < ne && keep[e]) adjp[pos[e]] = adj_tmp[e];
+}
+
+__global__ void orient_offsets_kernel(int32_t nv, const int64_t* __restrict__ eoff, const uint32_t* __restrict__ pos,
+                                      int64_t* __restrict__ offp) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v <= nv) offp[v] = pos[eoff[v]];
+}
+
+struct U8In {
+    const uint8_t* a;
+    __device__ __forceinline__ uint32_t operator()(int64_t i) const { return a[i]; }
+};
+
+}  // namespace
+
+// Triangle counting core over a device CSR (sorted adjacency).
+static hn_status tc_core(hn_graph* g, int32_t nv, int64_t ne, const int64_t* off, const int32_t* adj,
+                         uint64_t* total_out, uint64_t* per_vertex, hipStream_t st) {
+    const int64_t ne1 = ne > 0 ? ne : 1;
+    auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
+    int64_t o_tot = 0;
+    int64_t o_cnt = 256;
+    int64_t o_src = 512;
+    int64_t o_med = al(o_src + ne1 * 4);
+    int64_t o_heavy = al(o_med + ne1 * 4);
+    int64_t total = al(o_heavy + ne1 * 4);
+    HN_TRY(g->s_tc.reserve((size_t)total));
+    char* base = g->s_tc.as<char>();
+    TcOut o;
+    o.total = reinterpret_cast<unsigned long long*>(base + o_tot);
+    o.cnt = reinterpret_cast<uint32_t*>(base + o_cnt);
+    o.src_of = reinterpret_cast<int32_t*>(base + o_src);
+    o.med_q = reinterpret_cast<uint32_t*>(base + o_med);
+    o.heavy_q = reinterpret_cast<uint32_t*>(base + o_heavy);
+    o.per_vertex = reinterpret_cast<unsigned long long*>(per_vertex);
+    HN_HIP(hipMemsetAsync(base, 0, 512, st));
+    if (per_vertex && nv > 0) HN_HIP(hipMemsetAsync(per_vertex, 0, sizeof(uint64_t) * (size_t)nv, st));
+    TcCsr c{off, adj};
+    const int64_t cap = (int64_t)num_cus() * 8;
+    if (ne > 0) {
+        int64_t grid = std::min<int64_t>(ceil_div(ne, TC_TILE), cap);
+        tc_light_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(nv, ne, c, o);
+        HN_CHECK_LAUNCH();
+        tc_group_kernel<MEDIUM_LANES><<<(unsigned)cap, BLOCK, 0, st>>>(c, o.med_q, o.cnt, o.src_of, o);
+        HN_CHECK_LAUNCH();
+        tc_group_kernel<WAVE><<<(unsigned)cap, BLOCK, 0, st>>>(c, o.heavy_q, o.cnt + 1, o.src_of, o);
+        HN_CHECK_LAUNCH();
+    }
+    unsigned long long t = 0;
+    HN_TRY(read_back(g, o.total, 8, &t, st));
+    *total_out = t;
+    return HN_OK;
+}
+
+}  // namespace hn
+
+using namespace hn;
+
+extern "C" {
+
+hn_status hn_tc2_stored(hn_graph* g, uint64_t* total, uint64_t* per_vertex, void* stream) {
+    HN_ARG(g && total, "null argument");
+    HN_ARG(g->ne < (int64_t)0xFFFFFFFF, "edge count exceeds 32-bit queue ids");
+    hipStream_t st = as_stream(stream);
+    const int64_t nv = g->nv, ne = g->ne;
+    int64_t* off = nullptr;
+    int32_t* adj = nullptr;
+    HN_TRY(dalloc(&off, (size_t)nv + 1));
+    hn_status s = dalloc(&adj, (size_t)(ne > 0 ? ne : 1));
+    if (s == HN_OK) s = hn_graph_export_csr(g, off, adj, ne, stream);
+    if (s == HN_OK) s = tc_core(g, g->nv, ne, off, adj, total, per_vertex, st);
+    (void)hipStreamSynchronize(st);
+    dfree(off);
+    dfree(adj);
+    return s;
+}
+
+hn_status hn_tc_count(hn_graph* g, uint64_t* total, uint64_t* per_vertex, void* stream) {
+    HN_ARG(g && total, "null argument");
+    HN_ARG(g->ne < (int64_t)0xFFFFFFFF, "edge count exceeds the 32-bit scan");
+    hipStream_t st = as_stream(stream);
+    const int32_t nv = g->nv;
+    const int64_t ne = g->ne, ne1 = ne > 0 ? ne : 1;
+    int64_t *eoff = nullptr, *offp = nullptr;
+    int32_t *adj_tmp = nullptr, *adjp = nullptr;
+    uint8_t* keep = nullptr;
+    uint32_t* pos = nullptr;
+    void* tmp = nullptr;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(st);
+        dfree(eoff); dfree(offp); dfree(adj_tmp); dfree(adjp); dfree(keep); dfree(pos); dfree(tmp);
+    };
+    hn_status s = HN_OK;
+    if ((s = dalloc(&eoff, (size_t)nv + 1)) != HN_OK || (s = dalloc(&offp, (size_t)nv + 1)) != HN_OK ||
+        (s = dalloc(&adj_tmp, (size_t)ne1)) != HN_OK || (s = dalloc(&adjp, (size_t)ne1)) != HN_OK ||
+        (s = dalloc(&keep, (size_t)ne1)) != HN_OK || (s = dalloc(&pos, (size_t)ne1 + 1)) != HN_OK ||
+        (s = dmalloc(&tmp, scan_temp_bytes(ne1))) != HN_OK) {
+        cleanup();
+        return s;
+    }
+    if ((s = degree_prefix(g, eoff, st)) != HN_OK) { cleanup(); return s; }
+    if (ne > 0) {
+        int64_t grid = std::min<int64_t>(ceil_div(ne, OR_TILE), (int64_t)num_cus() * 16);
+        orient_flags_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(nv, ne, eoff, g->rec, g->arena, adj_tmp, keep);
+        HN_CHECK_LAUNCH();
+    }
+    if ((s = exclusive_scan<uint32_t>(U8In{keep}, pos, ne, tmp, st)) != HN_OK) { cleanup(); return s; }
+    if (ne > 0) {
+        orient_compact_kernel<<<(unsigned)ceil_div(ne, BLOCK), BLOCK, 0, st>>>(ne, keep, pos, adj_tmp, adjp);
+        HN_CHECK_LAUNCH();
+    }
+    orient_offsets_kernel<<<(unsigned)ceil_div((int64_t)nv + 1, BLOCK), BLOCK, 0, st>>>(nv, eoff, pos, offp);
+    HN_CHECK_LAUNCH();
+    uint32_t nep = 0;
+    if ((s = read_back(g, pos + ne, 4, &nep, st)) != HN_OK) { cleanup(); return s; }
+    s = tc_core(g, nv, (int64_t)nep, offp, adjp, total, per_vertex, st);
+    cleanup();
+    return s;
+}
+
+}  // extern "C"
