@@ -1,0 +1,728 @@
+// update.hip -- BatchUpdate insert / erase on the device.
+//
+// Reference pipeline (hornet/include/Core/BatchUpdate/BatchUpdate.i.cuh and
+// BatchUpdateKernels.cuh):
+//   insert  sort (2x thrust sort_by_key) -> unique_copy -> graph dedup
+//           (RLE, scans, mark_duplicate_edges_kernel) -> realloc queue ->
+//           HOST allocator loop -> move_adjacency_lists -> append at old_deg+k
+//           (adjacency order not kept), ~8 host syncs per batch;
+//   erase   sort -> unique -> multiplicities (markUniqueOffsets) ->
+//           locate_erased_edges (atomicSub) -> copy_if -> tail fill of the
+//           holes (order destroyed) -> shrink realloc.
+// Semantics kept: insert without dedup is a multiset union; removeBatchDuplicates
+// drops in-batch repeats; removeGraphDuplicates drops the FIRST batch copy of
+// every (u,v) already stored (lower_bound_left, BatchUpdateKernels.cuh:27-58);
+// erase removes min(multiplicity in batch, stored copies) and ignores absent
+// edges (BatchUpdateKernels.cuh:263-300).  Self loops are not filtered.
+//
+// Redesign for the device: one 64-bit LSD radix sort of packed (src,dst) keys,
+// flag + scan compactions, and every touched vertex rewritten in place (when
+// its block still fits) or into a block from the device allocator:
+//   insert  each element of old list A and batch segment B finds its place in
+//           merge(A, B) with one binary search in the other list (stored
+//           copies first on ties), so adjacency stays sorted;
+//   erase   survivors are compacted in order; a survivor's shift is the number
+//           of deleted slots before it, found by a binary search over the
+//           vertex's deleted ranges.
+// A device error word (bad vertex id, arena exhausted) turns every later
+// kernel into a no-op and rolls the allocator back, so a failed batch leaves
+// the graph unchanged.  Two host synchronisations per batch (staging size,
+// final status).
+#include "graph.hpp"
+#include "prims.hpp"
+
+namespace hn {
+namespace {
+
+constexpr uint32_t ERR_BAD_ID = 1u;
+constexpr uint32_t ERR_OOM = 2u;     // AllocState::error bit 0 is mirrored here
+constexpr int UPD_IPT = 8;
+constexpr int UPD_TILE = BLOCK * UPD_IPT;
+
+struct KeyFmt {
+    int sbits;
+    __device__ __forceinline__ int32_t src(uint64_t k) const { return (int32_t)(k >> sbits); }
+    __device__ __forceinline__ int32_t dst(uint64_t k) const {
+        return (int32_t)(k & ((1ull << sbits) - 1));
+    }
+};
+
+struct UpdScal {
+    uint32_t err;
+    uint32_t pad;
+    unsigned long long changed;   // entries inserted / erased
+};
+
+__global__ void pack_batch_kernel(int64_t n, const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
+                                  int32_t nv, int sbits, uint64_t* __restrict__ keys, UpdScal* sc) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t s = src[i], d = dst[i];
+    if (s < 0 || s >= nv || d < 0 || d >= nv) {
+        atomicOr(&sc->err, ERR_BAD_ID);
+        s = 0;
+        d = 0;
+    }
+    keys[i] = ((uint64_t)s << sbits) | (uint64_t)d;
+}
+
+__device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t n, int32_t x) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ int64_t upper_bound_i32(const int32_t* a, int64_t n, int32_t x) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// keep flag for the sorted batch (insert): in-batch dedup and graph dedup
+struct InsertKeepIn {
+    const uint64_t* k;
+    int64_t n;
+    KeyFmt f;
+    int dedup_batch, dedup_graph;
+    const hn_vrec* rec;
+    const int32_t* arena;
+    __device__ __forceinline__ uint32_t operator()(int64_t i) const {
+        const uint64_t x = k[i];
+        const bool first = (i == 0) || k[i - 1] != x;
+        if (dedup_batch && !first) return 0u;
+        if (dedup_graph && first) {
+            const hn_vrec r = rec[f.src(x)];
+            const int32_t d = f.dst(x);
+            const int64_t p = lower_bound_i32(arena + r.off, r.deg, d);
+            if (p < r.deg && arena[r.off + p] == d) return 0u;
+        }
+        return 1u;
+    }
+};
+
+// keep flag for the sorted batch (erase): optional in-batch dedup
+struct FirstKeepIn {
+    const uint64_t* k;
+    int dedup;
+    __device__ __forceinline__ uint32_t operator()(int64_t i) const {
+        return (!dedup || i == 0 || k[i - 1] != k[i]) ? 1u : 0u;
+    }
+};
+
+__global__ void compact_keys_kernel(int64_t n, const uint64_t* __restrict__ in, const uint32_t* __restrict__ pos,
+                                    uint64_t* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && pos[i + 1] != pos[i]) out[pos[i]] = in[i];
+}
+
+// segment starts over the first `*cnt` keys: key i starts a segment when its
+// group (given by `shift`: the source, or the whole key) differs from key i-1
+struct SegStartIn {
+    const uint64_t* k;
+    const uint32_t* cnt;
+    int shift;   // 64 -> whole key compared
+    __device__ __forceinline__ uint32_t operator()(int64_t i) const {
+        if (i >= (int64_t)*cnt) return 0u;
+        if (i == 0) return 1u;
+        return shift >= 64 ? (k[i] != k[i - 1] ? 1u : 0u) : ((k[i] >> shift) != (k[i - 1] >> shift) ? 1u : 0u);
+    }
+};
+
+__global__ void seg_write_kernel(int64_t n, const uint64_t* __restrict__ k, const uint32_t* __restrict__ cnt,
+                                 const uint32_t* __restrict__ spos, uint32_t* __restrict__ seg_start,
+                                 uint64_t* __restrict__ seg_key) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t m = *cnt;
+    if (i > m || i >= n + 1) return;
+    if (i == m) {
+        seg_start[spos[n]] = (uint32_t)m;
+        return;
+    }
+    if (spos[i + 1] != spos[i]) {
+        seg_start[spos[i]] = (uint32_t)i;
+        seg_key[spos[i]] = k[i];
+    }
+}
+
+// Per touched vertex.  A vertex is either updated in place (its block still
+// fits: insert new_deg <= capacity, the reference's `new_degree > limit`
+// test inverted, BatchUpdateKernels.cuh:180-186; erase keeps the size class)
+// or relocated into a block from the device allocator.  In place, only the
+// suffix of the old list that moves -- from `p` on -- is first staged in
+// scratch, so the rewrite never reads a slot it has already overwritten.
+struct TouchInfo {
+    int64_t old_off;
+    int64_t new_off;   // == old_off in place
+    int64_t soff;      // staging offset of A[p..old_deg) (in place only)
+    int32_t old_deg;
+    int32_t new_deg;
+    int32_t old_cap;
+    int32_t new_cap;
+    int32_t p;         // first old slot that moves (0 when relocating)
+    int32_t inplace;
+};
+
+// A[i] of a touched vertex during the rewrite: the unchanged prefix from the
+// block, the moving suffix from the staging copy (or the old block)
+struct AView {
+    const int32_t* lo;   // i < p
+    const int32_t* hi;   // i >= p, indexed by i
+    int64_t p;
+    __device__ __forceinline__ int32_t operator[](int64_t i) const { return i < p ? lo[i] : hi[i]; }
+};
+
+__device__ __forceinline__ AView a_view(const TouchInfo& t, const int32_t* arena, const int32_t* stage) {
+    AView a;
+    a.lo = arena + t.old_off;
+    a.hi = t.inplace ? stage + t.soff - t.p : arena + t.old_off;
+    a.p = t.p;
+    return a;
+}
+
+// insert: per touched source j -- new degree, in place or a new block
+__global__ void insert_alloc_kernel(const uint32_t* __restrict__ ucnt, const uint32_t* __restrict__ seg_start,
+                                    const uint64_t* __restrict__ seg_key, const uint64_t* __restrict__ keys, KeyFmt f,
+                                    const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena, AllocView A,
+                                    TouchInfo* __restrict__ ti, const UpdScal* sc) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (int64_t)*ucnt || sc->err) return;
+    const int32_t v = f.src(seg_key[j]);
+    const hn_vrec r = rec[v];
+    const int32_t kj = (int32_t)(seg_start[j + 1] - seg_start[j]);
+    TouchInfo t;
+    t.old_off = r.off;
+    t.old_deg = r.deg;
+    t.old_cap = r.cap_log2;
+    t.new_deg = r.deg + kj;
+    t.soff = 0;
+    t.inplace = (r.cap_log2 >= 0 && (int64_t)t.new_deg <= (1ll << r.cap_log2)) ? 1 : 0;
+    if (t.inplace) {
+        t.new_cap = r.cap_log2;
+        t.new_off = r.off;
+        // old entries below the smallest batch value keep their slots
+        t.p = (int32_t)lower_bound_i32(arena + r.off, r.deg, f.dst(keys[seg_start[j]]));
+    } else {
+        t.new_cap = class_of(t.new_deg);
+        t.new_off = alloc_block(A, t.new_cap);
+        t.p = 0;
+    }
+    ti[j] = t;
+}
+
+// commit or roll back the tickets of this batch
+__global__ void alloc_fix_kernel(AllocState* st, UpdScal* sc) {
+    if (st->error) {
+        st->bump_units = st->saved_bump;
+        for (int k = 0; k < NCLASS; k++) st->take[k] = 0;
+        st->error = 0;
+        sc->err |= ERR_OOM;
+        return;
+    }
+    if (sc->err) {   // bad input: nothing was allocated
+        for (int k = 0; k < NCLASS; k++) st->take[k] = 0;
+        st->bump_units = st->saved_bump;
+        return;
+    }
+    for (int k = 0; k < NCLASS; k++) {
+        unsigned fc = st->free_count[k], t = st->take[k];
+        st->free_count[k] = t >= fc ? 0u : fc - t;
+        st->take[k] = 0;
+    }
+    st->saved_bump = st->bump_units;
+}
+
+// staged length of touched vertex j (moving suffix of an in-place rewrite)
+struct StageLenIn {
+    const TouchInfo* ti;
+    const uint32_t* ucnt;
+    const UpdScal* sc;
+    const uint8_t* skip;
+    __device__ __forceinline__ int64_t operator()(int64_t j) const {
+        if (j >= (int64_t)*ucnt || sc->err || (skip && skip[j])) return 0;
+        const TouchInfo& t = ti[j];
+        return t.inplace ? (int64_t)(t.old_deg - t.p) : 0;
+    }
+};
+
+__global__ void set_soff_kernel(const uint32_t* __restrict__ ucnt, const int64_t* __restrict__ spre,
+                                TouchInfo* __restrict__ ti, const UpdScal* sc) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (int64_t)*ucnt || sc->err) return;
+    ti[j].soff = spre[j];
+}
+
+// stage[soff_j + i] = A_j[p_j + i]
+__global__ __launch_bounds__(BLOCK) void stage_kernel(int64_t nseg, const int64_t* __restrict__ P,
+                                                      const TouchInfo* __restrict__ ti,
+                                                      const int32_t* __restrict__ arena, int32_t* __restrict__ stage,
+                                                      const UpdScal* sc) {
+    __shared__ LbShared<UPD_IPT, int64_t> sh;
+    if (sc->err) return;
+    const int64_t W = P[nseg];
+    for (int64_t w0 = (int64_t)blockIdx.x * UPD_TILE; w0 < W; w0 += (int64_t)gridDim.x * UPD_TILE) {
+        lb_tile<UPD_IPT>(
+            P, nseg, w0, W, sh, [&](int64_t s) { return P[s]; },
+            [&](int64_t j, int64_t w, int64_t pj) {
+                const TouchInfo t = ti[j];
+                stage[w] = arena[t.old_off + t.p + (w - pj)];
+            });
+    }
+}
+
+// items of the insert rewrite: moving old entries and the batch segment
+struct InsertWorkIn {
+    const TouchInfo* ti;
+    const uint32_t* ucnt;
+    const UpdScal* sc;
+    __device__ __forceinline__ int64_t operator()(int64_t j) const {
+        if (j >= (int64_t)*ucnt || sc->err) return 0;
+        return (int64_t)(ti[j].new_deg - ti[j].p);
+    }
+};
+
+// merge(A, B): A[i] -> i + #(B < A[i]); B[q] -> q + #(A <= B[q]) (stored
+// copies first on ties); only A[p..) and B are moved
+__global__ __launch_bounds__(BLOCK) void insert_merge_kernel(int64_t nseg, const int64_t* __restrict__ P,
+                                                             const TouchInfo* __restrict__ ti,
+                                                             const uint32_t* __restrict__ seg_start,
+                                                             const uint64_t* __restrict__ keys, KeyFmt f,
+                                                             int32_t* __restrict__ arena,
+                                                             const int32_t* __restrict__ stage, const UpdScal* sc) {
+    __shared__ LbShared<UPD_IPT, int64_t> sh;
+    if (sc->err) return;
+    const int64_t W = P[nseg];
+    for (int64_t w0 = (int64_t)blockIdx.x * UPD_TILE; w0 < W; w0 += (int64_t)gridDim.x * UPD_TILE) {
+        lb_tile<UPD_IPT>(
+            P, nseg, w0, W, sh, [&](int64_t s) { return P[s]; },
+            [&](int64_t j, int64_t w, int64_t pj) {
+                const TouchInfo t = ti[j];
+                const AView A = a_view(t, arena, stage);
+                const int64_t na = t.old_deg;
+                const uint64_t* B = keys + seg_start[j];
+                const int64_t nb = t.new_deg - t.old_deg;
+                const int64_t r = t.p + (w - pj);
+                int64_t dest;
+                int32_t val;
+                if (r < na) {
+                    val = A[r];
+                    int64_t lo = 0, hi = nb;   // #(B < val)
+                    while (lo < hi) {
+                        int64_t mid = (lo + hi) >> 1;
+                        if (f.dst(B[mid]) < val) lo = mid + 1; else hi = mid;
+                    }
+                    dest = r + lo;
+                } else {
+                    const int64_t q = r - na;
+                    val = f.dst(B[q]);
+                    int64_t lo = 0, hi = na;   // #(A <= val)
+                    while (lo < hi) {
+                        int64_t mid = (lo + hi) >> 1;
+                        if (A[mid] <= val) lo = mid + 1; else hi = mid;
+                    }
+                    dest = q + lo;
+                }
+                arena[t.new_off + dest] = val;
+            });
+    }
+}
+
+// publish the new records and release the blocks that were left
+__global__ void commit_kernel(const uint32_t* __restrict__ ucnt, const uint64_t* __restrict__ seg_key, KeyFmt f,
+                              const TouchInfo* __restrict__ ti, const uint8_t* __restrict__ skip,
+                              hn_vrec* __restrict__ rec, AllocView A, const UpdScal* sc) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (int64_t)*ucnt || sc->err) return;
+    if (skip && skip[j]) return;
+    const TouchInfo t = ti[j];
+    hn_vrec r;
+    r.off = t.new_cap >= 0 ? t.new_off : 0;
+    r.deg = t.new_deg;
+    r.cap_log2 = t.new_cap;
+    rec[f.src(seg_key[j])] = r;
+    if (!t.inplace && t.old_cap >= 0) free_block(A, t.old_cap, t.old_off);
+}
+
+// ---------------------------------------------------------------- erase
+// per distinct (u,v,c): stored range [lb, lb+e), deleted count min(c, e)
+__global__ void erase_locate_kernel(const uint32_t* __restrict__ dcnt, const uint32_t* __restrict__ dstart,
+                                    const uint64_t* __restrict__ dkey, KeyFmt f, const hn_vrec* __restrict__ rec,
+                                    const int32_t* __restrict__ arena, uint32_t* __restrict__ dpos,
+                                    uint32_t* __restrict__ dlen, const UpdScal* sc) {
+    int64_t jj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (jj >= (int64_t)*dcnt) return;
+    if (sc->err) {
+        dlen[jj] = 0;
+        dpos[jj] = 0;
+        return;
+    }
+    const uint64_t x = dkey[jj];
+    const hn_vrec r = rec[f.src(x)];
+    const int32_t d = f.dst(x);
+    const int64_t lb = lower_bound_i32(arena + r.off, r.deg, d);
+    const int64_t ub = upper_bound_i32(arena + r.off, r.deg, d);
+    const int64_t c = dstart[jj + 1] - dstart[jj];
+    dpos[jj] = (uint32_t)lb;
+    dlen[jj] = (uint32_t)(c < ub - lb ? c : ub - lb);
+}
+
+struct DLenIn {
+    const uint32_t* dlen;
+    const uint32_t* dcnt;
+    __device__ __forceinline__ uint32_t operator()(int64_t i) const { return i < (int64_t)*dcnt ? dlen[i] : 0u; }
+};
+
+// per touched source u (segments of the distinct keys): deletions; in place
+// unless the list drops to a smaller size class (the reference shrinks when
+// new_degree <= limit/2, BatchUpdateKernels.cuh:180-186) or empties
+__global__ void erase_alloc_kernel(const uint32_t* __restrict__ ucnt, const uint32_t* __restrict__ ustart,
+                                   const uint64_t* __restrict__ ukey, KeyFmt f, const uint32_t* __restrict__ dpos,
+                                   const uint32_t* __restrict__ dlen, const uint32_t* __restrict__ dlp,
+                                   const hn_vrec* __restrict__ rec, AllocView A, TouchInfo* __restrict__ ti,
+                                   uint8_t* __restrict__ skip, const UpdScal* sc) {
+    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= (int64_t)*ucnt || sc->err) return;
+    const int32_t v = f.src(ukey[u]);
+    const hn_vrec r = rec[v];
+    const uint32_t del = dlp[ustart[u + 1]] - dlp[ustart[u]];
+    TouchInfo t;
+    t.old_off = r.off;
+    t.old_deg = r.deg;
+    t.old_cap = r.cap_log2;
+    t.new_deg = r.deg - (int32_t)del;
+    t.soff = 0;
+    t.p = 0;
+    t.inplace = 0;
+    t.new_off = 0;
+    t.new_cap = class_of(t.new_deg);
+    skip[u] = del == 0 ? 1 : 0;
+    if (del > 0 && t.new_cap >= 0) {
+        if (t.new_cap == r.cap_log2) {
+            t.inplace = 1;
+            t.new_off = r.off;
+            // first deleted slot: the first distinct key of u that removes entries
+            for (uint32_t jj = ustart[u]; jj < ustart[u + 1]; jj++) {
+                if (dlen[jj]) {
+                    t.p = (int32_t)dpos[jj];
+                    break;
+                }
+            }
+        } else {
+            t.new_off = alloc_block(A, t.new_cap);
+        }
+    }
+    ti[u] = t;
+}
+
+struct EraseWorkIn {
+    const TouchInfo* ti;
+    const uint8_t* skip;
+    const uint32_t* ucnt;
+    const UpdScal* sc;
+    __device__ __forceinline__ int64_t operator()(int64_t u) const {
+        if (u >= (int64_t)*ucnt || sc->err || skip[u]) return 0;
+        const TouchInfo& t = ti[u];
+        return t.new_deg > 0 ? (int64_t)(t.old_deg - t.p) : 0;
+    }
+};
+
+// survivors of source u's old list from slot p on, in order, into the block
+__global__ __launch_bounds__(BLOCK) void erase_compact_kernel(int64_t nseg, const int64_t* __restrict__ P,
+                                                              const TouchInfo* __restrict__ ti,
+                                                              const uint32_t* __restrict__ ustart,
+                                                              const uint32_t* __restrict__ dpos,
+                                                              const uint32_t* __restrict__ dlen,
+                                                              const uint32_t* __restrict__ dlp,
+                                                              int32_t* __restrict__ arena,
+                                                              const int32_t* __restrict__ stage, const UpdScal* sc) {
+    __shared__ LbShared<UPD_IPT, int64_t> sh;
+    if (sc->err) return;
+    const int64_t W = P[nseg];
+    for (int64_t w0 = (int64_t)blockIdx.x * UPD_TILE; w0 < W; w0 += (int64_t)gridDim.x * UPD_TILE) {
+        lb_tile<UPD_IPT>(
+            P, nseg, w0, W, sh, [&](int64_t s) { return P[s]; },
+            [&](int64_t u, int64_t w, int64_t pu) {
+                const TouchInfo t = ti[u];
+                const AView A = a_view(t, arena, stage);
+                const int64_t x = t.p + (w - pu);
+                const int64_t a = ustart[u], b = ustart[u + 1];
+                // largest jj in [a, b) with dpos[jj] <= x
+                int64_t lo = a - 1, hi = b;
+                while (hi - lo > 1) {
+                    int64_t mid = (lo + hi) >> 1;
+                    if ((int64_t)dpos[mid] <= x) lo = mid; else hi = mid;
+                }
+                int64_t before = 0;
+                if (lo >= a) {
+                    if (x < (int64_t)dpos[lo] + (int64_t)dlen[lo]) return;   // deleted
+                    before = (int64_t)(dlp[lo] - dlp[a]) + dlen[lo];
+                }
+                arena[t.new_off + (x - before)] = A[x];
+            });
+    }
+}
+
+}  // namespace
+
+// scratch for a batch of n entries
+struct UpdBufs {
+    uint64_t* ka;
+    uint64_t* kb;
+    uint32_t* pos;       // n+1
+    uint32_t* spos;      // n+1
+    uint32_t* seg_start; // n+1
+    uint64_t* seg_key;   // n+1
+    uint32_t* seg2_start;
+    uint64_t* seg2_key;
+    uint32_t* tpos;      // n+1
+    uint32_t* dpos;      // n
+    uint32_t* dlen;      // n
+    uint32_t* dlp;       // n+1
+    int64_t* wpos;       // n+1
+    int64_t* stpos;      // n+1
+    TouchInfo* ti;       // n
+    uint8_t* skip;       // n
+    UpdScal* sc;
+    void* tmp;
+};
+
+static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
+    auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
+    const int64_t n1 = n + 1;
+    int64_t o = 0;
+    int64_t o_ka = o; o = al(o + n1 * 8);
+    int64_t o_kb = o; o = al(o + n1 * 8);
+    int64_t o_pos = o; o = al(o + n1 * 4);
+    int64_t o_spos = o; o = al(o + n1 * 4);
+    int64_t o_ss = o; o = al(o + n1 * 4);
+    int64_t o_sk = o; o = al(o + n1 * 8);
+    int64_t o_ss2 = o; o = al(o + n1 * 4);
+    int64_t o_sk2 = o; o = al(o + n1 * 8);
+    int64_t o_tpos = o; o = al(o + n1 * 4);
+    int64_t o_dpos = o; o = al(o + n1 * 4);
+    int64_t o_dlen = o; o = al(o + n1 * 4);
+    int64_t o_dlp = o; o = al(o + n1 * 4);
+    int64_t o_wpos = o; o = al(o + n1 * 8);
+    int64_t o_stpos = o; o = al(o + n1 * 8);
+    int64_t o_ti = o; o = al(o + n1 * (int64_t)sizeof(TouchInfo));
+    int64_t o_skip = o; o = al(o + n1);
+    int64_t o_sc = o; o = al(o + 64);
+    int64_t o_tmp = o; o = al(o + (int64_t)std::max(sort_temp_bytes(n1), scan_temp_bytes(n1)));
+    HN_TRY(g->s_upd.reserve((size_t)o));
+    char* base = g->s_upd.as<char>();
+    b->ka = reinterpret_cast<uint64_t*>(base + o_ka);
+    b->kb = reinterpret_cast<uint64_t*>(base + o_kb);
+    b->pos = reinterpret_cast<uint32_t*>(base + o_pos);
+    b->spos = reinterpret_cast<uint32_t*>(base + o_spos);
+    b->seg_start = reinterpret_cast<uint32_t*>(base + o_ss);
+    b->seg_key = reinterpret_cast<uint64_t*>(base + o_sk);
+    b->seg2_start = reinterpret_cast<uint32_t*>(base + o_ss2);
+    b->seg2_key = reinterpret_cast<uint64_t*>(base + o_sk2);
+    b->tpos = reinterpret_cast<uint32_t*>(base + o_tpos);
+    b->dpos = reinterpret_cast<uint32_t*>(base + o_dpos);
+    b->dlen = reinterpret_cast<uint32_t*>(base + o_dlen);
+    b->dlp = reinterpret_cast<uint32_t*>(base + o_dlp);
+    b->wpos = reinterpret_cast<int64_t*>(base + o_wpos);
+    b->stpos = reinterpret_cast<int64_t*>(base + o_stpos);
+    b->ti = reinterpret_cast<TouchInfo*>(base + o_ti);
+    b->skip = reinterpret_cast<uint8_t*>(base + o_skip);
+    b->sc = reinterpret_cast<UpdScal*>(base + o_sc);
+    b->tmp = base + o_tmp;
+    return HN_OK;
+}
+
+// copy a host or device batch into the key buffer (packed, validated) and sort it
+static hn_status load_and_sort(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n, uint32_t flags,
+                               UpdBufs& b, KeyFmt f, hipStream_t st, uint64_t** sorted) {
+    const int32_t* ds = src;
+    const int32_t* dd = dst;
+    int32_t* staged = nullptr;
+    if (!(flags & HN_DEVICE_PTRS)) {
+        HN_TRY(dalloc(&staged, (size_t)(2 * n)));
+        HN_HIP(hipMemcpyAsync(staged, src, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+        HN_HIP(hipMemcpyAsync(staged + n, dst, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+        ds = staged;
+        dd = staged + n;
+    }
+    HN_HIP(hipMemsetAsync(b.sc, 0, sizeof(UpdScal), st));
+    pack_batch_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(n, ds, dd, g->nv, f.sbits, b.ka, b.sc);
+    HN_CHECK_LAUNCH();
+    if (staged) {
+        HN_HIP(hipStreamSynchronize(st));
+        dfree(staged);
+    }
+    uint64_t* keys = b.ka;
+    uint64_t* alt = b.kb;
+    HN_TRY(radix_sort_u64(&keys, &alt, n, 2 * f.sbits, b.tmp, st));
+    if (keys != b.ka) {
+        // keep the sorted keys in ka so kb is free for the compaction
+        HN_HIP(hipMemcpyAsync(b.ka, keys, sizeof(uint64_t) * (size_t)n, hipMemcpyDeviceToDevice, st));
+    }
+    *sorted = b.ka;
+    return HN_OK;
+}
+
+static hn_status check_scal(hn_graph* g, UpdBufs& b, hipStream_t st, UpdScal* h) {
+    HN_TRY(read_back(g, b.sc, sizeof(UpdScal), h, st));
+    if (h->err & ERR_BAD_ID) {
+        set_error("batch holds a vertex id outside [0, nv)");
+        return HN_E_INVALID_ARG;
+    }
+    if (h->err & ERR_OOM) {
+        set_error("edge arena exhausted (graph unchanged); create the graph with a larger arena_slack");
+        return HN_E_OOM;
+    }
+    return HN_OK;
+}
+
+__global__ void store_changed_kernel(const uint32_t* __restrict__ total, UpdScal* sc) {
+    if (!sc->err) sc->changed = *total;
+}
+
+// size the staging buffer (one readback) and copy the moving suffixes
+static hn_status stage_suffixes(hn_graph* g, UpdBufs& b, int64_t n, const uint32_t* ucnt, const uint8_t* skip,
+                                hipStream_t st, int32_t** stage_out) {
+    const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
+    HN_TRY(exclusive_scan<int64_t>(StageLenIn{b.ti, ucnt, b.sc, skip}, b.stpos, n, b.tmp, st));
+    set_soff_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.stpos, b.ti, b.sc);
+    HN_CHECK_LAUNCH();
+    int64_t total = 0;
+    UpdScal h;
+    HN_TRY(read_back(g, b.stpos + n, 8, &total, st));
+    HN_TRY(check_scal(g, b, st, &h));
+    HN_TRY(g->s_misc.reserve(sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
+    int32_t* stage = g->s_misc.as<int32_t>();
+    if (total > 0) {
+        const int64_t grid = std::min<int64_t>(ceil_div(total, UPD_TILE), (int64_t)num_cus() * 8);
+        stage_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.stpos, b.ti, g->arena, stage, b.sc);
+        HN_CHECK_LAUNCH();
+    }
+    *stage_out = stage;
+    return HN_OK;
+}
+
+}  // namespace hn
+
+using namespace hn;
+
+extern "C" {
+
+hn_status hn_graph_insert(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n, uint32_t flags,
+                          int64_t* inserted, void* stream) {
+    HN_ARG(g, "null graph");
+    HN_ARG(n >= 0, "negative batch size");
+    if (inserted) *inserted = 0;
+    if (n == 0) return HN_OK;
+    HN_ARG(src && dst, "null batch");
+    HN_ARG(n < (int64_t)0x7FFFFFFF, "batch larger than 2^31-1 entries");
+    HN_ARG(g->ne + n < (int64_t)0xFFFFFFFF, "edge count would exceed 2^32-1");
+    hipStream_t st = as_stream(stream);
+    UpdBufs b;
+    HN_TRY(upd_buffers(g, n, &b));
+    KeyFmt f{std::max(1, ceil_log2_host((int64_t)g->nv + 1))};
+    uint64_t* keys = nullptr;
+    HN_TRY(load_and_sort(g, src, dst, n, flags, b, f, st, &keys));
+    const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
+    // 1. keep flags (batch / graph dedup) -> compact into kb
+    InsertKeepIn kin{keys, n, f, (flags & HN_DEDUP_BATCH) ? 1 : 0, (flags & HN_DEDUP_GRAPH) ? 1 : 0, g->rec, g->arena};
+    HN_TRY(exclusive_scan<uint32_t>(kin, b.pos, n, b.tmp, st));
+    compact_keys_kernel<<<gridn, BLOCK, 0, st>>>(n, keys, b.pos, b.kb);
+    HN_CHECK_LAUNCH();
+    uint32_t* nb = b.pos + n;   // entries kept
+    // 2. segments by source
+    HN_TRY(exclusive_scan<uint32_t>(SegStartIn{b.kb, nb, f.sbits}, b.spos, n, b.tmp, st));
+    seg_write_kernel<<<gridn, BLOCK, 0, st>>>(n, b.kb, nb, b.spos, b.seg_start, b.seg_key);
+    HN_CHECK_LAUNCH();
+    uint32_t* ucnt = b.spos + n;
+    // 3. in place or new blocks
+    insert_alloc_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg_start, b.seg_key, b.kb, f, g->rec, g->arena,
+                                                 g->alloc_view(), b.ti, b.sc);
+    HN_CHECK_LAUNCH();
+    alloc_fix_kernel<<<1, 1, 0, st>>>(g->alloc, b.sc);
+    HN_CHECK_LAUNCH();
+    int32_t* stage = nullptr;
+    HN_TRY(stage_suffixes(g, b, n, ucnt, nullptr, st, &stage));
+    // 4. merge the moving old entries and the batch segment
+    HN_TRY(exclusive_scan<int64_t>(InsertWorkIn{b.ti, ucnt, b.sc}, b.wpos, n, b.tmp, st));
+    const int64_t grid = (int64_t)num_cus() * 8;
+    insert_merge_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.wpos, b.ti, b.seg_start, b.kb, f, g->arena, stage,
+                                                          b.sc);
+    HN_CHECK_LAUNCH();
+    // 5. publish
+    commit_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg_key, f, b.ti, nullptr, g->rec, g->alloc_view(), b.sc);
+    HN_CHECK_LAUNCH();
+    store_changed_kernel<<<1, 1, 0, st>>>(nb, b.sc);
+    HN_CHECK_LAUNCH();
+    UpdScal h;
+    HN_TRY(check_scal(g, b, st, &h));
+    g->ne += (int64_t)h.changed;
+    if (inserted) *inserted = (int64_t)h.changed;
+    return HN_OK;
+}
+
+hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n, uint32_t flags,
+                         int64_t* erased, void* stream) {
+    HN_ARG(g, "null graph");
+    HN_ARG(n >= 0, "negative batch size");
+    if (erased) *erased = 0;
+    if (n == 0) return HN_OK;
+    HN_ARG(src && dst, "null batch");
+    HN_ARG(n < (int64_t)0x7FFFFFFF, "batch larger than 2^31-1 entries");
+    hipStream_t st = as_stream(stream);
+    UpdBufs b;
+    HN_TRY(upd_buffers(g, n, &b));
+    KeyFmt f{std::max(1, ceil_log2_host((int64_t)g->nv + 1))};
+    uint64_t* keys = nullptr;
+    HN_TRY(load_and_sort(g, src, dst, n, flags, b, f, st, &keys));
+    const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
+    // 1. optional in-batch dedup -> kb
+    HN_TRY(exclusive_scan<uint32_t>(FirstKeepIn{keys, (flags & HN_DEDUP_BATCH) ? 1 : 0}, b.pos, n, b.tmp, st));
+    compact_keys_kernel<<<gridn, BLOCK, 0, st>>>(n, keys, b.pos, b.kb);
+    HN_CHECK_LAUNCH();
+    uint32_t* nb = b.pos + n;
+    // 2. distinct keys with multiplicities
+    HN_TRY(exclusive_scan<uint32_t>(SegStartIn{b.kb, nb, 64}, b.spos, n, b.tmp, st));
+    seg_write_kernel<<<gridn, BLOCK, 0, st>>>(n, b.kb, nb, b.spos, b.seg_start, b.seg_key);
+    HN_CHECK_LAUNCH();
+    uint32_t* dcnt = b.spos + n;
+    // 3. stored ranges and deletion counts per distinct key
+    erase_locate_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, b.seg_start, b.seg_key, f, g->rec, g->arena, b.dpos, b.dlen,
+                                                 b.sc);
+    HN_CHECK_LAUNCH();
+    HN_TRY(exclusive_scan<uint32_t>(DLenIn{b.dlen, dcnt}, b.dlp, n, b.tmp, st));
+    // 4. group the distinct keys by source
+    HN_TRY(exclusive_scan<uint32_t>(SegStartIn{b.seg_key, dcnt, f.sbits}, b.tpos, n, b.tmp, st));
+    seg_write_kernel<<<gridn, BLOCK, 0, st>>>(n, b.seg_key, dcnt, b.tpos, b.seg2_start, b.seg2_key);
+    HN_CHECK_LAUNCH();
+    uint32_t* ucnt = b.tpos + n;
+    // 5. in place, smaller block, or no block
+    erase_alloc_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg2_start, b.seg2_key, f, b.dpos, b.dlen, b.dlp, g->rec,
+                                                g->alloc_view(), b.ti, b.skip, b.sc);
+    HN_CHECK_LAUNCH();
+    alloc_fix_kernel<<<1, 1, 0, st>>>(g->alloc, b.sc);
+    HN_CHECK_LAUNCH();
+    int32_t* stage = nullptr;
+    HN_TRY(stage_suffixes(g, b, n, ucnt, b.skip, st, &stage));
+    // 6. ordered compaction of the survivors
+    HN_TRY(exclusive_scan<int64_t>(EraseWorkIn{b.ti, b.skip, ucnt, b.sc}, b.wpos, n, b.tmp, st));
+    const int64_t grid = (int64_t)num_cus() * 8;
+    erase_compact_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.wpos, b.ti, b.seg2_start, b.dpos, b.dlen, b.dlp,
+                                                           g->arena, stage, b.sc);
+    HN_CHECK_LAUNCH();
+    // 7. publish
+    commit_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg2_key, f, b.ti, b.skip, g->rec, g->alloc_view(), b.sc);
+    HN_CHECK_LAUNCH();
+    store_changed_kernel<<<1, 1, 0, st>>>(b.dlp + n, b.sc);
+    HN_CHECK_LAUNCH();
+    UpdScal h;
+    HN_TRY(check_scal(g, b, st, &h));
+    g->ne -= (int64_t)h.changed;
+    if (erased) *erased = (int64_t)h.changed;
+    return HN_OK;
+}
+
+}  // extern "C"
