@@ -51,6 +51,65 @@ def load_traffic(name: str):
         return None
 
 
+def update_rounds(h, args, torch):
+    """Dynamic updates on the same graph: per round r, insert `batch` uniform
+    pairs (seed 4+r, ~1 % in-batch repeats, self loops dropped, symmetrized,
+    removeBatchDuplicates + removeGraphDuplicates) and erase `batch` entries
+    sampled uniformly from the current edge set (symmetrized,
+    removeBatchDuplicates); recount triangles after every round."""
+    from hornet_amd.graph import BatchUpdate, TriangleCounting2, uniform_pairs_device
+
+    nv = h.nV()
+    ins_t = era_t = 0.0
+    ins_n = era_n = 0
+    ins_added = era_removed = 0
+    tc_t = 0.0
+    tris = []
+    tc = TriangleCounting2(h)
+    tc.init()
+    for r in range(args.rounds):
+        us, ud = uniform_pairs_device(nv, args.batch, seed=4 + r)
+        keep = us != ud
+        bi = BatchUpdate(torch.cat([us[keep], ud[keep]]), torch.cat([ud[keep], us[keep]]))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ins_added += h.insert(bi, True, True)
+        torch.cuda.synchronize()
+        ins_t += time.perf_counter() - t
+        ins_n += bi.nE()
+        # deletions: positions uniform over the current directed entries
+        src, dst = h.getCOO()
+        pos, _ = uniform_pairs_device(src.numel(), args.batch, seed=100 + r)
+        ps, pd = src[pos.long()], dst[pos.long()]
+        be = BatchUpdate(torch.cat([ps, pd]), torch.cat([pd, ps]))
+        del src, dst
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        era_removed += h.erase(be, True)
+        torch.cuda.synchronize()
+        era_t += time.perf_counter() - t
+        era_n += be.nE()
+        t = time.perf_counter()
+        tc.run_undirected()
+        torch.cuda.synchronize()
+        tc_t += time.perf_counter() - t
+        tris.append(tc.countTriangles())
+        print(f"# round {r}: ne={h.nE()} triangles={tris[-1]}", file=sys.stderr, flush=True)
+    updates = {
+        "rounds": args.rounds, "pairs_per_round": args.batch,
+        "edge_updates_per_s": round((ins_n + era_n) / (ins_t + era_t), 1),
+        "insert_updates_per_s": round(ins_n / ins_t, 1),
+        "erase_updates_per_s": round(era_n / era_t, 1),
+        "insert_ms_avg": round(ins_t / args.rounds * 1e3, 3),
+        "erase_ms_avg": round(era_t / args.rounds * 1e3, 3),
+        "inserted": ins_added, "erased": era_removed, "final_ne": h.nE(),
+        "unit": "directed edge-updates/s (submitted entries)",
+    }
+    tc_info = {"triangles_per_round": tris, "ms_avg": round(tc_t / args.rounds * 1e3, 3),
+               "triangles_per_s": round(sum(tris) / tc_t, 1)}
+    return updates, tc_info
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +121,8 @@ def main() -> int:
     ap.add_argument("--sources", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget for the CPU baseline sample (0 disables)")
+    ap.add_argument("--rounds", type=int, default=10, help="dynamic-update rounds after the BFS steps")
+    ap.add_argument("--batch", type=int, default=1 << 20, help="pairs inserted / deleted per round")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -136,12 +197,31 @@ def main() -> int:
         "launches": agg_l,
     }
 
+    updates = tc_info = None
+    if args.rounds > 0:
+        updates, tc_info = update_rounds(h, args, torch)
+
     cpu = None
     if args.cpu_seconds > 0:
         import oracle as O
         off, adj = h.getCSR()
         off = off.cpu().numpy()
         adj = adj.cpu().numpy()
+        if updates is not None:
+            # host set model on the same graph: one insert batch of the same size
+            from hornet_amd.graph import uniform_pairs_device
+            us, ud = uniform_pairs_device(nv, args.batch, seed=1000)
+            us, ud = us.cpu().numpy(), ud.cpu().numpy()
+            keep = us != ud
+            bs = np.concatenate([us[keep], ud[keep]])
+            bd = np.concatenate([ud[keep], us[keep]])
+            t = time.perf_counter()
+            O.insert(off, adj, bs, bd, True, True)
+            dt = time.perf_counter() - t
+            updates["cpu_baseline"] = {
+                "value": round(bs.size / dt, 1), "unit": "edge-updates/s", "cores": 1, "kind": "port",
+                "sample": f"one host set-model insert (oracle restatement of BatchUpdate semantics) of "
+                          f"{bs.size} directed entries into the same graph"}
         cm = 0
         used = 0
         t_cpu = 0.0
@@ -180,6 +260,8 @@ def main() -> int:
         "gteps_graph500": round(gteps / 2, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "updates": updates,
+        "triangle_counting": tc_info,
     }
     print(json.dumps(out), flush=True)
     return 0
