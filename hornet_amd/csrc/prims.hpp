@@ -1,260 +1,26 @@
-// prims.hpp -- header-only wave64 scan and load-balancing building blocks.
-//
-// exclusive_scan(): reduce-then-scan over an input functor, three launches.
-//   Replaces cub::DeviceScan::ExclusiveSum
-//   (xlib/src/Device/Primitives/CubWrapper.cu:583-617) as used by
-//   BinarySearch::apply (primitives/LoadBalancing/BinarySearch.i.cuh:82) and
-//   the BatchUpdate pipeline.  The functor form lets a caller scan a gathered
-//   quantity (e.g. the degree of each frontier vertex) without materialising it.
-//
-// lb_tile(): edge-granular load balancing for one workgroup tile.  Restates
-//   xlib::binarySearchLB (xlib/include/Device/Primitives/impl/BinarySearchLB.i.cuh:227-332):
-//   map work item w to (segment s, offset w - P[s]) with s the largest index
-//   with P[s] <= w, skipping empty segments.  Redesigned for wave64: the tile's
-//   segment boundaries and per-segment bases are staged once in LDS, items are
-//   striped across lanes so that consecutive lanes touch consecutive edges of
-//   the same segment, and a tile whose segment range overflows LDS falls back
-//   to a global binary search.
+// prims.hpp -- status-returning wrappers of the shared wave64 scan
+// (include/hornet/detail/wave.hpp) for the library.
 #pragma once
 #include "common.hpp"
 
 namespace hn {
 
-constexpr int SCAN_IPT = 16;                   // values per lane per tile
-constexpr int SCAN_TILE = BLOCK * SCAN_IPT;    // 4096
-constexpr int SCAN_WAVE_CHUNK = WAVE * SCAN_IPT;
-
-inline size_t scan_temp_bytes(int64_t n) {
-    int64_t tiles = ceil_div(n > 0 ? n : 1, SCAN_TILE);
-    return (size_t)(tiles + 1) * 8 + 256;
-}
-
-template <typename Out, typename InFn>
-__global__ __launch_bounds__(BLOCK) void scan_reduce_kernel(InFn in, int64_t n, Out* __restrict__ partial) {
-    __shared__ Out lds[NWAVES];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
-    Out acc = 0;
-#pragma unroll
-    for (int j = 0; j < SCAN_IPT; j++) {
-        int64_t i = base + (int64_t)j * BLOCK + threadIdx.x;
-        if (i < n) acc += (Out)in(i);
-    }
-    acc = wave_reduce_sum(acc);
-    if (__lane_id() == 0) lds[threadIdx.x / WAVE] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Out t = 0;
-        for (int w = 0; w < NWAVES; w++) t += lds[w];
-        partial[blockIdx.x] = t;
-    }
-}
-
-// one workgroup: exclusive scan of `count` partials in place, total -> *total
-template <typename Out>
-__global__ __launch_bounds__(BLOCK) void scan_partials_kernel(Out* __restrict__ partial, int64_t count,
-                                                              Out* __restrict__ total) {
-    __shared__ Out lds[NWAVES + 1];
-    Out carry = 0;
-    for (int64_t b = 0; b < count; b += BLOCK) {
-        int64_t i = b + threadIdx.x;
-        Out v = i < count ? partial[i] : (Out)0;
-        Out tot;
-        Out ex = block_exclusive_scan(v, lds, &tot);
-        if (i < count) partial[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) *total = carry;
-}
-
-template <typename Out, typename InFn>
-__global__ __launch_bounds__(BLOCK) void scan_down_kernel(InFn in, int64_t n, const Out* __restrict__ partial,
-                                                          Out* __restrict__ out) {
-    __shared__ Out wtot[NWAVES];
-    const int w = threadIdx.x / WAVE;
-    const int l = __lane_id();
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)w * SCAN_WAVE_CHUNK;
-    Out pre[SCAN_IPT];
-    Out carry = 0;
-#pragma unroll
-    for (int j = 0; j < SCAN_IPT; j++) {
-        int64_t i = base + (int64_t)j * WAVE + l;
-        Out v = i < n ? (Out)in(i) : (Out)0;
-        Out inc = wave_inclusive_scan(v);
-        pre[j] = carry + inc - v;
-        carry += __shfl(inc, WAVE - 1, WAVE);
-    }
-    if (l == 0) wtot[w] = carry;
-    __syncthreads();
-    Out off = partial[blockIdx.x];
-    for (int k = 0; k < w; k++) off += wtot[k];
-#pragma unroll
-    for (int j = 0; j < SCAN_IPT; j++) {
-        int64_t i = base + (int64_t)j * WAVE + l;
-        if (i < n) out[i] = off + pre[j];
-    }
-}
-
-template <typename Out>
-__global__ void scan_zero_kernel(Out* out) { *out = 0; }
-
-// down-sweep variant that hands (index, exclusive prefix, value) to a functor
-template <typename Out, typename InFn, typename OutFn>
-__global__ __launch_bounds__(BLOCK) void scan_down_fn_kernel(InFn in, int64_t n, const Out* __restrict__ partial,
-                                                             OutFn out) {
-    __shared__ Out wtot[NWAVES];
-    const int w = threadIdx.x / WAVE;
-    const int l = __lane_id();
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)w * SCAN_WAVE_CHUNK;
-    Out pre[SCAN_IPT];
-    Out val[SCAN_IPT];
-    Out carry = 0;
-#pragma unroll
-    for (int j = 0; j < SCAN_IPT; j++) {
-        int64_t i = base + (int64_t)j * WAVE + l;
-        Out v = i < n ? (Out)in(i) : (Out)0;
-        Out inc = wave_inclusive_scan(v);
-        pre[j] = carry + inc - v;
-        val[j] = v;
-        carry += __shfl(inc, WAVE - 1, WAVE);
-    }
-    if (l == 0) wtot[w] = carry;
-    __syncthreads();
-    Out off = partial[blockIdx.x];
-    for (int k = 0; k < w; k++) off += wtot[k];
-#pragma unroll
-    for (int j = 0; j < SCAN_IPT; j++) {
-        int64_t i = base + (int64_t)j * WAVE + l;
-        if (i < n) out(i, (Out)(off + pre[j]), val[j]);
-    }
+inline hn_status scan_status(hipError_t e) {
+    if (e == hipSuccess) return HN_OK;
+    set_error(std::string("scan launch: ") + hipGetErrorString(e));
+    return HN_E_HIP;
 }
 
 // Exclusive scan of in(0..n) delivered to out(i, prefix, value); *total = sum.
 template <typename Out, typename InFn, typename OutFn>
 hn_status exclusive_scan_fn(InFn in, OutFn out, Out* total, int64_t n, void* tmp, hipStream_t st) {
-    if (n <= 0) {
-        scan_zero_kernel<Out><<<1, 1, 0, st>>>(total);
-        HN_CHECK_LAUNCH();
-        return HN_OK;
-    }
-    int64_t tiles = ceil_div(n, SCAN_TILE);
-    Out* partial = reinterpret_cast<Out*>(tmp);
-    scan_reduce_kernel<Out, InFn><<<(unsigned)tiles, BLOCK, 0, st>>>(in, n, partial);
-    HN_CHECK_LAUNCH();
-    scan_partials_kernel<Out><<<1, BLOCK, 0, st>>>(partial, tiles, total);
-    HN_CHECK_LAUNCH();
-    scan_down_fn_kernel<Out, InFn, OutFn><<<(unsigned)tiles, BLOCK, 0, st>>>(in, n, partial, out);
-    HN_CHECK_LAUNCH();
-    return HN_OK;
+    return scan_status(exclusive_scan_dev<Out>(in, out, total, n, tmp, st));
 }
 
 // out[0..n) = exclusive prefix of in(0..n), out[n] = total.
 template <typename Out, typename InFn>
 hn_status exclusive_scan(InFn in, Out* out, int64_t n, void* tmp, hipStream_t st) {
-    if (n <= 0) {
-        scan_zero_kernel<Out><<<1, 1, 0, st>>>(out);
-        HN_CHECK_LAUNCH();
-        return HN_OK;
-    }
-    int64_t tiles = ceil_div(n, SCAN_TILE);
-    Out* partial = reinterpret_cast<Out*>(tmp);
-    scan_reduce_kernel<Out, InFn><<<(unsigned)tiles, BLOCK, 0, st>>>(in, n, partial);
-    HN_CHECK_LAUNCH();
-    scan_partials_kernel<Out><<<1, BLOCK, 0, st>>>(partial, tiles, out + n);
-    HN_CHECK_LAUNCH();
-    scan_down_kernel<Out, InFn><<<(unsigned)tiles, BLOCK, 0, st>>>(in, n, partial, out);
-    HN_CHECK_LAUNCH();
-    return HN_OK;
-}
-
-template <typename T>
-struct ArrayIn {
-    const T* a;
-    __device__ __forceinline__ T operator()(int64_t i) const { return a[i]; }
-};
-
-// --------------------------------------------------------------------------
-// load balancing over one tile of TILE = BLOCK*IPT work items
-// --------------------------------------------------------------------------
-template <int IPT, typename PrefT>
-struct LbShared {
-    static constexpr int CAP = BLOCK * IPT + 1;   // segment entries staged per tile
-    PrefT pref[CAP];
-    int64_t base[CAP];
-};
-
-// Largest s in [lo, hi) with P[s] <= x (P[lo] <= x), found by the whole
-// workgroup: each round samples BLOCK evenly spaced entries with one parallel
-// load and counts the samples <= x, shrinking the range BLOCK-fold, instead of
-// one lane walking ~log2(n) dependent global loads.
-template <typename PrefT>
-__device__ __forceinline__ int64_t block_upper_bound_m1(const PrefT* __restrict__ P, int64_t lo, int64_t hi,
-                                                        int64_t x) {
-    while (hi - lo > 1) {
-        const int64_t len = hi - lo;
-        const int64_t step = (len + BLOCK - 1) / BLOCK;
-        const int64_t idx = lo + (int64_t)threadIdx.x * step;
-        const int pred = idx < hi && (int64_t)P[idx] <= x;
-        const int c = __syncthreads_count(pred);   // samples <= x, >= 1 since P[lo] <= x
-        lo = lo + (int64_t)(c - 1) * step;
-        hi = min(hi, lo + step);
-    }
-    return lo;
-}
-
-template <int IPT, typename PrefT, typename SegFn, typename ItemFn>
-__device__ __forceinline__ void lb_tile_range(const PrefT* __restrict__ P, int64_t lo, int64_t hi, int64_t w0,
-                                              int64_t total, LbShared<IPT, PrefT>& sh, SegFn segbase,
-                                              ItemFn item);
-
-// For items w in [w0, min(w0+BLOCK*IPT, total)): item(s, w, base_s) where s is
-// the segment containing w (P exclusive prefix over nseg segments, P[nseg] =
-// total) and base_s = segbase(s).  Items are striped: item w0 + j*BLOCK + tid.
-template <int IPT, typename PrefT, typename SegFn, typename ItemFn>
-__device__ __forceinline__ void lb_tile(const PrefT* __restrict__ P, int64_t nseg, int64_t w0,
-                                        int64_t total, LbShared<IPT, PrefT>& sh, SegFn segbase,
-                                        ItemFn item) {
-    constexpr int TILE = BLOCK * IPT;
-    const int64_t wend = min(w0 + (int64_t)TILE, total);
-    const int64_t lo = block_upper_bound_m1(P, 0, nseg + 1, w0);
-    const int64_t hi = block_upper_bound_m1(P, lo, nseg + 1, wend - 1);
-    lb_tile_range<IPT>(P, lo, hi, w0, total, sh, segbase, item);
-}
-
-// Same, with the tile's segment range [lo, hi] (inclusive) already known: lo
-// holds item w0 and hi is at or past the segment holding the tile's last item.
-template <int IPT, typename PrefT, typename SegFn, typename ItemFn>
-__device__ __forceinline__ void lb_tile_range(const PrefT* __restrict__ P, int64_t lo, int64_t hi, int64_t w0,
-                                              int64_t total, LbShared<IPT, PrefT>& sh, SegFn segbase,
-                                              ItemFn item) {
-    constexpr int TILE = BLOCK * IPT;
-    const int64_t wend = min(w0 + (int64_t)TILE, total);
-    const int count = (hi - lo + 1 <= LbShared<IPT, PrefT>::CAP) ? (int)(hi - lo + 1)
-                                                                 : -(int)min(hi - lo + 1, (int64_t)0x7fffffff);
-    if (count > 0) {
-        for (int i = threadIdx.x; i < count; i += BLOCK) {
-            sh.pref[i] = P[lo + i];
-            sh.base[i] = segbase(lo + i);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < IPT; j++) {
-            int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
-            if (w < wend) {
-                int s = (int)upper_bound_m1<int64_t>(sh.pref, 0, count, w);
-                item(lo + s, w, sh.base[s]);
-            }
-        }
-    } else {
-        const int64_t hi_excl = lo + (int64_t)(-count);
-        for (int j = 0; j < IPT; j++) {
-            int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
-            if (w < wend) {
-                int64_t s = upper_bound_m1<int64_t>(P, lo, hi_excl, w);
-                item(s, w, segbase(s));
-            }
-        }
-    }
-    __syncthreads();
+    return scan_status(exclusive_scan_dev<Out>(in, ArrayOut<Out>{out}, out + n, n, tmp, st));
 }
 
 }  // namespace hn

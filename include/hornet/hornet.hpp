@@ -1,0 +1,243 @@
+// hornet/hornet.hpp -- the dynamic graph object and its update batches.
+//
+// Header-only C++ over the C ABI of libhornet_hip.so (hornet_hip.h), with the
+// names, arguments and defaults of the reference:
+//   hornet::HornetInit<vid_t>        hornet/include/Core/HornetInitialize/HornetInit.cuh:54-97
+//   hornet::gpu::Hornet<vid_t>       hornet/include/Core/Hornet.cuh:54-131
+//   hornet::BatchUpdatePtr / gpu::BatchUpdate
+//                                    hornet/include/Core/BatchUpdate/BatchUpdate.cuh:77-272
+//   hornet::COO                      hornet/include/Core/Static/Static.cuh:19-130
+// Vertex / edge meta types are not supported (vid_t = degree_t = int, EMPTY
+// meta); the extra template parameters are accepted and must be EMPTY.
+// Errors: the reference prints and exits (xlib/src/Device/Util/SafeCudaAPI.cpp:71-94);
+// here a failed call throws hornet::Error carrying the hn_status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "hornet_hip.h"
+#include "hornet/device.hpp"
+
+namespace hornet {
+
+struct EMPTY {};
+enum class DeviceType { HOST, DEVICE };
+
+class Error : public std::runtime_error {
+public:
+    // detail defaults to the library's message for the failed call
+    Error(hn_status s, const std::string& where, const std::string& detail = std::string())
+        : std::runtime_error(where + ": " + hn_status_string(s) + ": " + (detail.empty() ? hn_last_error() : detail)),
+          status(s) {}
+    hn_status status;
+};
+
+inline void check(hn_status s, const char* where) {
+    if (s != HN_OK) throw Error(s, where);
+}
+
+// Host CSR handed to the Hornet constructor (pointers are borrowed)
+template <typename VID = vid_t, typename VMeta = EMPTY, typename EMeta = EMPTY, typename DEG = degree_t>
+class HornetInit {
+    static_assert(std::is_same<VID, int>::value && std::is_same<DEG, int>::value, "int ids and degrees");
+    static_assert(std::is_same<VMeta, EMPTY>::value && std::is_same<EMeta, EMPTY>::value, "no meta data");
+
+public:
+    HornetInit(VID nV, DEG nE, const DEG* csr_offsets, const VID* csr_edges) noexcept
+        : _nV(nV), _nE(nE), _off(csr_offsets), _edges(csr_edges) {}
+    VID nV() const noexcept { return _nV; }
+    DEG nE() const noexcept { return _nE; }
+    const DEG* csr_offsets() const noexcept { return _off; }
+    const VID* csr_edges() const noexcept { return _edges; }
+
+private:
+    VID _nV;
+    DEG _nE;
+    const DEG* _off;
+    const VID* _edges;
+};
+
+// (n, src, dst) batch on the host or the device (BatchUpdate.cuh:95-99)
+template <typename VID = vid_t, typename EMeta = EMPTY, DeviceType device = DeviceType::DEVICE,
+          typename DEG = degree_t>
+class BatchUpdatePtr {
+public:
+    BatchUpdatePtr(DEG n, VID* src, VID* dst) noexcept : _n(n), _src(src), _dst(dst) {}
+    DEG nE() const noexcept { return _n; }
+    VID* src() const noexcept { return _src; }
+    VID* dst() const noexcept { return _dst; }
+    static constexpr bool on_device = device == DeviceType::DEVICE;
+
+private:
+    DEG _n;
+    VID* _src;
+    VID* _dst;
+};
+
+// Flat (src, dst) edge list; device storage owned by the object
+template <DeviceType device = DeviceType::DEVICE, typename VID = vid_t, typename EMeta = EMPTY,
+          typename DEG = degree_t>
+class COO {
+public:
+    COO() = default;
+    explicit COO(int64_t n) : _n(n) {
+        if (n > 0) {
+            check(hipMalloc(&_src, sizeof(VID) * (size_t)n) == hipSuccess ? HN_OK : HN_E_OOM, "COO");
+            check(hipMalloc(&_dst, sizeof(VID) * (size_t)n) == hipSuccess ? HN_OK : HN_E_OOM, "COO");
+        }
+    }
+    COO(const COO&) = delete;
+    COO& operator=(const COO&) = delete;
+    COO(COO&& o) noexcept : _n(o._n), _src(o._src), _dst(o._dst) { o._src = o._dst = nullptr; o._n = 0; }
+    ~COO() {
+        if (_src) (void)hipFree(_src);
+        if (_dst) (void)hipFree(_dst);
+    }
+    int64_t size() const noexcept { return _n; }
+    VID* srcPtr() const noexcept { return _src; }
+    VID* dstPtr() const noexcept { return _dst; }
+    // copy to host vectors
+    void toHost(std::vector<VID>& s, std::vector<VID>& d) const {
+        s.resize((size_t)_n);
+        d.resize((size_t)_n);
+        if (_n > 0) {
+            (void)hipMemcpy(s.data(), _src, sizeof(VID) * (size_t)_n, hipMemcpyDeviceToHost);
+            (void)hipMemcpy(d.data(), _dst, sizeof(VID) * (size_t)_n, hipMemcpyDeviceToHost);
+        }
+    }
+
+private:
+    int64_t _n = 0;
+    VID* _src = nullptr;
+    VID* _dst = nullptr;
+};
+
+namespace gpu {
+
+// A batch as seen by insert/erase: the library sorts and deduplicates it in
+// its own scratch, so the caller's arrays are only read
+template <typename VID = vid_t, typename EMeta = EMPTY, typename DEG = degree_t>
+class BatchUpdate {
+public:
+    template <DeviceType D>
+    explicit BatchUpdate(const BatchUpdatePtr<VID, EMeta, D, DEG>& p) noexcept
+        : _n(p.nE()), _src(p.src()), _dst(p.dst()), _device(D == DeviceType::DEVICE) {}
+    template <DeviceType D>
+    explicit BatchUpdate(COO<D, VID, EMeta, DEG>& coo) noexcept
+        : _n((DEG)coo.size()), _src(coo.srcPtr()), _dst(coo.dstPtr()), _device(D == DeviceType::DEVICE) {}
+
+    DEG nE() const noexcept { return _n; }
+    DEG size() const noexcept { return _n; }
+    const VID* src() const noexcept { return _src; }
+    const VID* dst() const noexcept { return _dst; }
+    uint32_t flags() const noexcept { return _device ? HN_DEVICE_PTRS : 0u; }
+    // entries that took effect in the last insert / erase
+    int64_t applied() const noexcept { return _applied; }
+    void set_applied(int64_t a) noexcept { _applied = a; }
+
+private:
+    DEG _n;
+    const VID* _src;
+    const VID* _dst;
+    bool _device;
+    int64_t _applied = 0;
+};
+
+template <typename VID = vid_t, typename VMeta = EMPTY, typename EMeta = EMPTY, typename DEG = degree_t>
+class Hornet {
+    static_assert(std::is_same<VID, int>::value && std::is_same<DEG, int>::value, "int ids and degrees");
+    static_assert(std::is_same<VMeta, EMPTY>::value && std::is_same<EMeta, EMPTY>::value, "no meta data");
+
+public:
+    using VertexType = VID;
+    using DegreeType = DEG;
+    using HornetDeviceT = HornetDevice;
+    using HInitT = HornetInit<VID, VMeta, EMeta, DEG>;
+
+    // Hornet(HornetInit&) (HornetInitialize.i.cuh:86-146); `arena_slack` sizes
+    // the edge arena as a multiple of the blocks the input needs
+    explicit Hornet(HInitT& init, double arena_slack = 0.0, hipStream_t stream = nullptr) : _stream(stream) {
+        check(hn_graph_create_csr(init.nV(), init.nE(), init.csr_offsets(), init.csr_edges(), HN_OFFSETS_INT32,
+                                  arena_slack, stream, &_g),
+              "Hornet(HornetInit&)");
+    }
+    // adopt a handle created through the C ABI
+    explicit Hornet(hn_graph* g, hipStream_t stream = nullptr) noexcept : _g(g), _stream(stream) {}
+    Hornet(const Hornet&) = delete;
+    Hornet& operator=(const Hornet&) = delete;
+    ~Hornet() { hn_graph_destroy(_g); }
+
+    DEG nV() const {
+        int32_t v = 0;
+        check(hn_graph_nv(_g, &v), "Hornet::nV");
+        return v;
+    }
+    DEG nE() const {
+        int64_t e = 0;
+        check(hn_graph_ne(_g, &e), "Hornet::nE");
+        return (DEG)e;
+    }
+    int64_t nE64() const {
+        int64_t e = 0;
+        check(hn_graph_ne(_g, &e), "Hornet::nE");
+        return e;
+    }
+    HornetDevice device() const {
+        hn_dview v;
+        check(hn_graph_device_view(_g, &v), "Hornet::device");
+        return HornetDevice(v);
+    }
+    // Hornet::insert (HornetInsert.i.cuh:12-27)
+    void insert(BatchUpdate<VID, EMeta, DEG>& batch, bool removeBatchDuplicates = false,
+                bool removeGraphDuplicates = false) {
+        uint32_t f = batch.flags() | (removeBatchDuplicates ? HN_DEDUP_BATCH : 0u) |
+                     (removeGraphDuplicates ? HN_DEDUP_GRAPH : 0u);
+        int64_t n = 0;
+        check(hn_graph_insert(_g, batch.src(), batch.dst(), batch.nE(), f, &n, _stream), "Hornet::insert");
+        batch.set_applied(n);
+    }
+    // Hornet::erase (HornetInsert.i.cuh:71-89)
+    void erase(BatchUpdate<VID, EMeta, DEG>& batch, bool removeBatchDuplicates = false) {
+        uint32_t f = batch.flags() | (removeBatchDuplicates ? HN_DEDUP_BATCH : 0u);
+        int64_t n = 0;
+        check(hn_graph_erase(_g, batch.src(), batch.dst(), batch.nE(), f, &n, _stream), "Hornet::erase");
+        batch.set_applied(n);
+    }
+    // HornetQuery.i.cuh:10-36 (lowest id on ties)
+    VID max_degree_id() const {
+        int32_t id = 0, d = 0;
+        check(hn_graph_max_degree(_g, &id, &d, _stream), "Hornet::max_degree_id");
+        return id;
+    }
+    DEG max_degree() const {
+        int32_t id = 0, d = 0;
+        check(hn_graph_max_degree(_g, &id, &d, _stream), "Hornet::max_degree");
+        return d;
+    }
+    // getCOO (HornetQuery.i.cuh:111-151): adjacency is kept sorted, so the
+    // result is always sorted by (src, dst)
+    COO<DeviceType::DEVICE, VID, EMeta, DEG> getCOO(bool sortAdjacencyList = true) const {
+        (void)sortAdjacencyList;
+        int64_t ne = nE64();
+        COO<DeviceType::DEVICE, VID, EMeta, DEG> coo(ne);
+        int64_t n = 0;
+        if (ne > 0) check(hn_graph_export_coo(_g, coo.srcPtr(), coo.dstPtr(), ne, &n, _stream), "Hornet::getCOO");
+        return coo;
+    }
+    hn_graph* handle() const noexcept { return _g; }
+    hipStream_t stream() const noexcept { return _stream; }
+
+private:
+    hn_graph* _g = nullptr;
+    hipStream_t _stream = nullptr;
+};
+
+}  // namespace gpu
+}  // namespace hornet

@@ -1,0 +1,117 @@
+"""GPU parity of the C++ drop-in headers: a hipcc-built driver that uses
+hornet::gpu::Hornet, BatchUpdate, BfsTopDown, TriangleCounting2 and the
+forAll* operators, checked against the oracle."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "cpp", "_build", "hornet_driver")
+INF = 2**31 - 1
+
+
+def write_graph(path, off, adj):
+    with open(path, "wb") as f:
+        np.array([off.size - 1, adj.size], np.int64).tofile(f)
+        off.astype(np.int64).tofile(f)
+        adj.astype(np.int32).tofile(f)
+
+
+def run(*args):
+    assert os.path.exists(DRIVER), "build with __graft_entry__.build()"
+    print("case driver", " ".join(args[:1]), flush=True)
+    out = subprocess.run([DRIVER, *args], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr + out.stdout
+    return out.stdout
+
+
+def field(text, key):
+    return int(re.search(rf"\b{key}=(-?\d+)", text).group(1))
+
+
+@pytest.fixture(scope="module")
+def graph(tmp_path_factory):
+    off, adj = O.rmat_graph(12, seed=6)
+    p = str(tmp_path_factory.mktemp("g") / "g.bin")
+    write_graph(p, off, adj)
+    return off, adj, p
+
+
+def test_bfs_topdown_operator_path(graph, tmp_path):
+    off, adj, p = graph
+    src = O.max_degree_id(off)
+    out = run("bfs", p, str(src), str(tmp_path / "d.bin"))
+    assert field(out, "max_degree_id") == src
+    assert field(out, "validate") == 1 and field(out, "native_equal") == 1
+    got = np.fromfile(tmp_path / "d.bin", np.int32)
+    exp, _ = O.bfs(off, adj, src)
+    assert np.array_equal(got, exp)
+    assert field(out, "levels") == int(exp[exp != INF].max()) + 1
+
+
+def test_triangle_counting2_oriented(graph, tmp_path):
+    off, adj, _ = graph
+    do, da = O.directed_by_degree(off, adj)
+    p = str(tmp_path / "o.bin")
+    write_graph(p, do, da)
+    out = run("tc", p)
+    assert field(out, "total") == field(out, "sum") == O.triangles(off, adj)
+
+
+def test_operators_checksums(graph):
+    off, adj, p = graph
+    out = run("ops", p)
+    s = np.repeat(np.arange(off.size - 1), np.diff(off))
+    deg = np.diff(off)
+    for lb in ("binary_search", "vertex_based4", "vertex_based64"):
+        line = next(x for x in out.splitlines() if x.startswith(f"ops {lb} "))
+        assert field(line, "edges") == adj.size
+        assert field(line, "dst_sum") == int(adj.sum())
+        assert field(line, "src_sum") == int(s.sum())
+    higher = int((deg[s] > deg[adj]).sum())
+    for lb in ("pairs_binary_search", "pairs_vertex_based1"):
+        line = next(x for x in out.splitlines() if x.startswith(f"ops {lb} "))
+        assert field(line, "higher") == higher
+    assert field(out, "degree_sum") == adj.size
+    even = s % 2 == 0
+    line = next(x for x in out.splitlines() if x.startswith("ops queue_even "))
+    assert field(line, "edges") == int(even.sum())
+    assert field(line, "dst_sum") == int(adj[even].sum())
+    assert field(out, "id_sum") == (off.size - 1) * (off.size - 2) // 2
+
+
+def test_batch_update_through_headers(graph, tmp_path):
+    off, adj, p = graph
+    n = off.size - 1
+    bs, bd = O.uniform_pairs(n, 2000, seed=31)
+    s0 = np.repeat(np.arange(n), np.diff(off)).astype(np.int32)
+    bs = np.concatenate([bs, s0[:40]]).astype(np.int32)
+    bd = np.concatenate([bd, adj[:40]]).astype(np.int32)
+    bp = tmp_path / "b.bin"
+    with open(bp, "wb") as f:
+        np.array([bs.size], np.int64).tofile(f)
+        bs.tofile(f)
+        bd.tofile(f)
+    out = run("update", p, str(bp), str(tmp_path / "coo.bin"))
+    o1, a1, added = O.insert(off, adj, bs, bd, True, True)
+    half = bs.size // 2
+    o2, a2, removed = O.erase(o1, a1, bs[:half], bd[:half])
+    assert field(out, "inserted") == added and field(out, "erased") == removed
+    raw = np.fromfile(tmp_path / "coo.bin", np.int8)
+    pos = 0
+    for (oo, aa) in ((o1, a1), (o2, a2)):
+        m = int(raw[pos:pos + 8].view(np.int64)[0])
+        pos += 8
+        src = raw[pos:pos + 4 * m].view(np.int32)
+        pos += 4 * m
+        dst = raw[pos:pos + 4 * m].view(np.int32)
+        pos += 4 * m
+        es = np.repeat(np.arange(oo.size - 1), np.diff(oo))
+        assert np.array_equal(src, es) and np.array_equal(dst, aa)
