@@ -10,8 +10,8 @@ from ._lib import HornetError, LIB_PATH, load  # noqa: F401
 
 def __getattr__(name):
     # torch is imported lazily so that the ABI can be inspected without it
-    if name in ("Hornet", "HornetInit", "BatchUpdate", "BatchUpdatePtr", "BfsTopDown",
-                "TriangleCounting2", "StaticAlgorithm", "rmat_pairs_device",
+    if name in ("Hornet", "HornetInit", "BatchUpdate", "BatchUpdatePtr", "BfsTopDown", "BfsMulti",
+                "TriangleCounting2", "StaticAlgorithm", "rmat_pairs_device", "chung_lu_pairs_device",
                 "uniform_pairs_device", "INF"):
         from . import graph
         return getattr(graph, name)

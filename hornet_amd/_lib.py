@@ -49,6 +49,11 @@ class HnBfsStats(ctypes.Structure):
                 ("pad_", ctypes.c_int32)]
 
 
+class HnMsbfsStats(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int32), ("push_levels", ctypes.c_int32),
+                ("pull_levels", ctypes.c_int32), ("pad_", ctypes.c_int32), ("edges", ctypes.c_int64)]
+
+
 P = ctypes.c_void_p
 I32 = ctypes.c_int32
 I64 = ctypes.c_int64
@@ -80,11 +85,17 @@ SIGNATURES = {
     "hn_graph_insert": [P, P, P, I64, U32, PI64, P],
     "hn_graph_erase": [P, P, P, I64, U32, PI64, P],
     "hn_bfs_topdown": [P, I32, P, ctypes.POINTER(HnBfsStats), P],
+    "hn_bfs_multi": [P, P, I32, P, ctypes.POINTER(HnMsbfsStats), P],
+    "hn_bfs_multi_depth": [P, P, P, I32, P, ctypes.POINTER(HnMsbfsStats), P],
+    "hn_bfs_multi_params": [P, I32, I32],
+    "hn_bfs_multi_tune": [P, P, I32, P, PI32, PI32, P],
     "hn_tc_count": [P, PU64, P, P],
     "hn_tc2_stored": [P, PU64, P, P],
     "hn_gen_rmat_pairs": [I32, I64, I64, U64, P, P, P],
     "hn_gen_rmat_graph": [I32, I32, U64, D, P, PP],
     "hn_gen_uniform_pairs": [I32, I64, U64, P, P, P],
+    "hn_gen_chung_lu_pairs": [I32, D, I64, I64, U64, P, P, P],
+    "hn_gen_chung_lu_graph": [I32, I32, D, U64, D, P, PP],
 }
 _RESTYPES = {"hn_status_string": ctypes.c_char_p, "hn_last_error": ctypes.c_char_p,
              "hn_version": ctypes.c_char_p}

@@ -231,6 +231,8 @@ hn_status read_back(hn_graph* g, const void* dptr, size_t bytes, void* host, hip
 
 static void destroy_graph(hn_graph* g) {
     if (!g) return;
+    if (g->ms && g->ms_release) g->ms_release(g->ms);
+    g->ms = nullptr;
     dfree(g->rec);
     dfree(g->arena);
     dfree(g->alloc);

@@ -73,6 +73,7 @@ __device__ __forceinline__ void free_block(const AllocView& A, int k, int64_t sl
 struct hn_graph {
     int32_t nv = 0;
     int64_t ne = 0;
+    uint64_t version = 0;             // bumped by every applied insert / erase
     hn_vrec* rec = nullptr;           // device, nv records
     int32_t* arena = nullptr;         // device, cap_units * 32 slots
     int64_t cap_units = 0;
@@ -87,6 +88,8 @@ struct hn_graph {
     hn::Scratch s_tc;                 // triangle counting
     hn::Scratch s_misc;
     uint64_t* h_pinned = nullptr;     // small pinned readback area
+    void* ms = nullptr;               // multi-source BFS engine (msbfs.hip)
+    void (*ms_release)(void*) = nullptr;
 
     hn::AllocView alloc_view() const { return hn::AllocView{alloc, stacks, stack_off}; }
     hn_dview view() const {
@@ -114,4 +117,8 @@ hn_status csr_from_device_coo(int32_t nv, int64_t m, const int32_t* src, const i
 hn_status degree_prefix(hn_graph* g, int64_t* d_eoff, hipStream_t st);
 // synchronous readback of a few bytes via the handle's pinned buffer
 hn_status read_back(hn_graph* g, const void* dptr, size_t bytes, void* host, hipStream_t st);
+// batched BFS of k <= 64 sources (host array), optional per-source depth
+// limits (0 = none); dist is device int32[k * nv], source-major
+hn_status ms_run(hn_graph* g, const int32_t* sources, int32_t k, const int32_t* depth_limit, int32_t* dist,
+                 hn_msbfs_stats* stats, hipStream_t caller);
 }  // namespace hn

@@ -660,6 +660,7 @@ hn_status hn_graph_insert(hn_graph* g, const int32_t* src, const int32_t* dst, i
     UpdScal h;
     HN_TRY(check_scal(g, b, st, &h));
     g->ne += (int64_t)h.changed;
+    g->version++;
     if (inserted) *inserted = (int64_t)h.changed;
     return HN_OK;
 }
@@ -721,6 +722,7 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     UpdScal h;
     HN_TRY(check_scal(g, b, st, &h));
     g->ne -= (int64_t)h.changed;
+    g->version++;
     if (erased) *erased = (int64_t)h.changed;
     return HN_OK;
 }

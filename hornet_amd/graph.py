@@ -128,6 +128,13 @@ class Hornet:
              ctypes.byref(h))
         return cls(_handle=h)
 
+    @classmethod
+    def chung_lu(cls, n: int, avg_degree: int, gamma: float, seed: int, slack: float = 0.0) -> "Hornet":
+        h = ctypes.c_void_p()
+        call("hn_gen_chung_lu_graph", int(n), int(avg_degree), float(gamma), int(seed), float(slack), _stream(),
+             ctypes.byref(h))
+        return cls(_handle=h)
+
     # ------------------------------------------------------------------ queries
     @property
     def handle(self):
@@ -315,6 +322,49 @@ class TriangleCounting2(StaticAlgorithm):
         return True
 
 
+class BfsMulti:
+    """Batched BFS of up to 64 sources per launch (bit-packed source masks,
+    captured level loop).  ``run`` returns int32 distances of shape (k, nV)."""
+
+    MAX_SOURCES = 64
+
+    def __init__(self, hornet: Hornet):
+        self.hornet = hornet
+        self.stats = _lib.HnMsbfsStats()
+        self._dist = None
+
+    def _buf(self, k: int) -> torch.Tensor:
+        n = k * max(self.hornet.nV(), 1)
+        if self._dist is None or self._dist.numel() < n:
+            self._dist = torch.empty(n, dtype=torch.int32, device=_dev())
+        return self._dist[:n]
+
+    def run(self, sources, depth_limits=None, out: torch.Tensor | None = None) -> torch.Tensor:
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        k = src.size
+        if not 1 <= k <= self.MAX_SOURCES:
+            raise HornetError(_lib.HN_E_INVALID_ARG, "BfsMulti.run", "1..64 sources per launch")
+        dist = out if out is not None else self._buf(k)
+        if depth_limits is None:
+            call("hn_bfs_multi", self.hornet.handle, src.ctypes.data_as(ctypes.c_void_p), k, _ptr(dist),
+                 ctypes.byref(self.stats), _stream())
+        else:
+            dl = np.ascontiguousarray(depth_limits, dtype=np.int32)
+            call("hn_bfs_multi_depth", self.hornet.handle, src.ctypes.data_as(ctypes.c_void_p),
+                 dl.ctypes.data_as(ctypes.c_void_p), k, _ptr(dist), ctypes.byref(self.stats), _stream())
+        return dist.view(k, -1)[:, : self.hornet.nV()]
+
+    def set_params(self, degree_threshold: int = 0, push_alpha: int = 0):
+        call("hn_bfs_multi_params", self.hornet.handle, int(degree_threshold), int(push_alpha))
+
+    def tune(self, sources):
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        t, a = ctypes.c_int32(), ctypes.c_int32()
+        call("hn_bfs_multi_tune", self.hornet.handle, src.ctypes.data_as(ctypes.c_void_p), src.size,
+             _ptr(self._buf(src.size)), ctypes.byref(t), ctypes.byref(a), _stream())
+        return t.value, a.value
+
+
 def rmat_pairs_device(scale: int, seed: int, edge_factor: int = 16, first: int = 0, count=None):
     m = edge_factor << scale
     if count is None:
@@ -323,6 +373,13 @@ def rmat_pairs_device(scale: int, seed: int, edge_factor: int = 16, first: int =
     dst = torch.empty(max(count, 1), dtype=torch.int32, device=_dev())
     call("hn_gen_rmat_pairs", int(scale), int(first), int(count), int(seed), _ptr(src), _ptr(dst),
          _stream())
+    return src[:count], dst[:count]
+
+
+def chung_lu_pairs_device(n: int, count: int, seed: int, gamma: float = 2.1):
+    src = torch.empty(max(count, 1), dtype=torch.int32, device=_dev())
+    dst = torch.empty(max(count, 1), dtype=torch.int32, device=_dev())
+    call("hn_gen_chung_lu_pairs", int(n), float(gamma), 0, int(count), int(seed), _ptr(src), _ptr(dst), _stream())
     return src[:count], dst[:count]
 
 

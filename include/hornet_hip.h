@@ -86,6 +86,15 @@ typedef struct hn_bfs_stats {
     int32_t pad_;
 } hn_bfs_stats;
 
+/* Multi-source BFS statistics for one launch. */
+typedef struct hn_msbfs_stats {
+    int32_t levels;       /* level steps run                                  */
+    int32_t push_levels;  /* of which top-down (push)                         */
+    int32_t pull_levels;  /* of which bottom-up (pull)                        */
+    int32_t pad_;
+    int64_t edges;        /* sum over sources of degrees of reached vertices  */
+} hn_msbfs_stats;
+
 /* ---------------------------------------------------------------- misc */
 const char* hn_status_string(hn_status s);
 const char* hn_last_error(void);
@@ -159,6 +168,27 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
  * the host with one 4-byte readback per level.  stats may be NULL. */
 hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats,
                          void* stream);
+/* Batched BFS: k <= 64 sources (host array) in one traversal with 64-bit
+ * per-vertex source masks; no reference counterpart (the reference runs one
+ * BfsTopDown per source, TopDown.cu:94-106).  dist: device int32[k * nv],
+ * source-major, same convention as hn_bfs_topdown.  The level step is a
+ * captured hipGraph replayed until the frontier empties.  stats may be NULL. */
+hn_status hn_bfs_multi(hn_graph* g, const int32_t* sources, int32_t k, int32_t* dist,
+                       hn_msbfs_stats* stats, void* stream);
+/* Same with a per-source depth limit (0 = none, else <= 63): vertices farther
+ * than the limit stay INT32_MAX (depth 2 gives the 2-hop neighbourhood). */
+hn_status hn_bfs_multi_depth(hn_graph* g, const int32_t* sources, const int32_t* depth_limit,
+                             int32_t k, int32_t* dist, hn_msbfs_stats* stats, void* stream);
+/* Load-balancing parameters of the batched BFS: pull-side degree threshold
+ * between one lane and one wave per vertex, and the push/pull switch
+ * (push while frontier degree * alpha < unvisited degree).  0 keeps a value. */
+hn_status hn_bfs_multi_params(hn_graph* g, int32_t degree_threshold, int32_t push_alpha);
+/* Autotuner: times the batched BFS of `sources` over a grid of thresholds and
+ * switch points and keeps the fastest for this graph.  dist is scratch of
+ * k * nv int32.  Synchronises. */
+hn_status hn_bfs_multi_tune(hn_graph* g, const int32_t* sources, int32_t k, int32_t* dist,
+                            int32_t* best_threshold, int32_t* best_alpha, void* stream);
+
 /* TriangleCounting2{init(); run(); countTriangles()} on the undirected simple
  * graph the symmetric storage represents --
  * hornetsnest/src/Static/TriangleCounting/triangle2.cu:130-160 applied to the
@@ -184,6 +214,15 @@ hn_status hn_gen_rmat_pairs(int32_t scale, int64_t first, int64_t count, uint64_
  * deduplicated, sorted adjacency. */
 hn_status hn_gen_rmat_graph(int32_t scale, int32_t edge_factor, uint64_t seed,
                             double arena_slack, void* stream, hn_graph** out);
+/* Chung-Lu pairs over [0,n): endpoints drawn ∝ w_i = (i+1)^(-1/(gamma-1))
+ * from a cumulative weight table (floor(2^24 w_i) per vertex, built on the
+ * host); pair i uses draws 2i and 2i+1 of stream `seed`. */
+hn_status hn_gen_chung_lu_pairs(int32_t n, double gamma, int64_t first, int64_t count, uint64_t seed,
+                                int32_t* src, int32_t* dst, void* stream);
+/* Chung-Lu graph: n*avg_degree/2 pairs, self loops dropped, symmetrized,
+ * deduplicated, sorted adjacency. */
+hn_status hn_gen_chung_lu_graph(int32_t n, int32_t avg_degree, double gamma, uint64_t seed,
+                                double arena_slack, void* stream, hn_graph** out);
 /* Uniform update pairs over [0,n) with ~1 % in-batch repeats (stream `seed`). */
 hn_status hn_gen_uniform_pairs(int32_t n, int64_t count, uint64_t seed, int32_t* src,
                                int32_t* dst, void* stream);

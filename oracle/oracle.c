@@ -470,10 +470,11 @@ int64_t orc_two_hop(int32_t n, const int64_t* off, const int32_t* adj, int32_t v
 }
 
 /*
- * Chung-Lu graph: expected degrees w_i = (i+1)^(-1/(gamma-1)) scaled to mean
- * `avg`.  Pairs are drawn ∝ w_i w_j by inverse-CDF sampling of both endpoints
- * over an integer cumulative weight table (cum, n+1 entries, 2^40 total, built
- * by the caller with orc_chung_lu_table).  Pair i uses draws 2i and 2i+1.
+ * Chung-Lu graph: expected degrees ∝ w_i = (i+1)^(-1/(gamma-1)); with
+ * avg*n/2 pairs the mean degree is avg.  Pairs are drawn ∝ w_i w_j by
+ * inverse-CDF sampling of both endpoints over an integer cumulative weight
+ * table (cum, n+1 entries, built with orc_chung_lu_table).  Pair i uses draws
+ * 2i and 2i+1; an endpoint is the largest v with cum[v] <= high64(r * cum[n]).
  */
 static int32_t cdf_pick(const uint64_t* cum, int32_t n, uint64_t r) {
     /* r in [0, cum[n]) -> largest v with cum[v] <= r */
@@ -498,9 +499,9 @@ void orc_chung_lu_pairs(int32_t n, const uint64_t* cum, int64_t first, int64_t c
     uint64_t total = cum[n];
     for (int64_t k = 0; k < count; k++) {
         uint64_t i = (uint64_t)(first + k);
-        uint64_t r0 = draw(key, 2 * i) >> 24;      /* 40 random bits */
-        uint64_t r1 = draw(key, 2 * i + 1) >> 24;
-        src[k] = cdf_pick(cum, n, (r0 * total) >> 40);
-        dst[k] = cdf_pick(cum, n, (r1 * total) >> 40);
+        uint64_t r0 = draw(key, 2 * i);
+        uint64_t r1 = draw(key, 2 * i + 1);
+        src[k] = cdf_pick(cum, n, (uint64_t)(((unsigned __int128)r0 * total) >> 64));
+        dst[k] = cdf_pick(cum, n, (uint64_t)(((unsigned __int128)r1 * total) >> 64));
     }
 }

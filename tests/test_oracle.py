@@ -257,6 +257,24 @@ def test_uniform_pairs_repeat_rate():
     assert 0.007 < dup / keys.size < 0.013
 
 
+def test_chung_lu_generator_invariants():
+    n = 1 << 14
+    cum = O.chung_lu_table(n, 2.1)
+    w = np.diff(cum).astype(np.float64)
+    # weights follow (i+1)^(-1/1.1) up to the floor
+    i = np.array([0, 9, 99, 999])
+    assert np.allclose(w[i], np.floor(2**24 * (i + 1.0) ** (-1 / 1.1)))
+    s, d = O.chung_lu_pairs(n, 16 * n, seed=9)
+    assert s.min() >= 0 and s.max() < n
+    # endpoint frequency tracks the weights: vertex 0 vs the expected share
+    share = w[0] / w.sum()
+    f0 = ((s == 0).sum() + (d == 0).sum()) / (2 * s.size)
+    assert abs(f0 - share) < 0.1 * share
+    # slicing is consistent
+    s2, _ = O.chung_lu_pairs(n, 16 * n, seed=9)
+    assert np.array_equal(s, s2)
+
+
 def test_pick_sources():
     off, adj = O.rmat_graph(10, seed=7)
     src = O.pick_sources(off, 16, seed=3)
