@@ -110,6 +110,63 @@ def update_rounds(h, args, torch):
     return updates, tc_info
 
 
+def batched_bfs(args, torch):
+    """C4: 4,096 seeded sources in launches of 64 (bit-packed masks, captured
+    level loop, autotuned load balancing) on R-MAT s23 and a Chung-Lu graph
+    (8 M vertices, average degree 32, exponent 2.1), against one source per
+    launch (hn_bfs_topdown) on the same GPU."""
+    from hornet_amd import rng
+    from hornet_amd.graph import BfsMulti, BfsTopDown, Hornet
+
+    out = {}
+    makers = [("rmat_s23", lambda: Hornet.rmat(23, seed=23)),
+              ("chung_lu_8M_avg32_g2.1", lambda: Hornet.chung_lu(1 << 23, 32, 2.1, seed=8))]
+    for name, make in makers:
+        h = make()
+        nv = h.nV()
+        srcs = rng.pick_sources(h.degrees().cpu().numpy(), args.ms_sources, seed=3)
+        ms = BfsMulti(h)
+        T, A = ms.tune(srcs[:64])
+        dist = torch.empty(64 * nv, dtype=torch.int32, device="cuda")
+        ms.run(srcs[:64], out=dist)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        edges = levels = 0
+        for i in range(0, srcs.size, 64):
+            ms.run(srcs[i:i + 64], out=dist[: 64 * nv] if srcs[i:i + 64].size == 64 else None)
+            edges += ms.stats.edges
+            levels += ms.stats.levels
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        del dist
+        bfs = BfsTopDown(h)
+        plain_n = min(args.ms_plain, srcs.size)
+        bfs.set_parameters(int(srcs[0]))
+        bfs.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pe = 0
+        for s in srcs[:plain_n]:
+            bfs.set_parameters(int(s))
+            bfs.run()
+            pe += bfs.stats.edges_scanned
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+        out[name] = {
+            "nv": nv, "ne": h.nE(), "sources": int(srcs.size), "launches": int((srcs.size + 63) // 64),
+            "batched_gteps": round(edges / tb / 1e9, 2), "batched_s": round(tb, 4),
+            "plain_gteps": round(pe / tp / 1e9, 2), "plain_sources_timed": plain_n,
+            "speedup_vs_plain": round((edges / tb) / (pe / tp), 2),
+            "tuned_degree_threshold": T, "tuned_push_alpha": A,
+            "avg_levels_per_launch": round(levels / ((srcs.size + 63) // 64), 2),
+        }
+        print(f"# batched BFS {name}: {out[name]}", file=sys.stderr, flush=True)
+        del bfs, ms
+        h.release()
+        torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +180,8 @@ def main() -> int:
                     help="budget for the CPU baseline sample (0 disables)")
     ap.add_argument("--rounds", type=int, default=10, help="dynamic-update rounds after the BFS steps")
     ap.add_argument("--batch", type=int, default=1 << 20, help="pairs inserted / deleted per round")
+    ap.add_argument("--ms-sources", type=int, default=4096, help="batched-BFS sources (0 disables)")
+    ap.add_argument("--ms-plain", type=int, default=64, help="sources timed one per launch for the ratio")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -197,7 +256,7 @@ def main() -> int:
         "launches": agg_l,
     }
 
-    updates = tc_info = None
+    updates = tc_info = ms_info = None
     if args.rounds > 0:
         updates, tc_info = update_rounds(h, args, torch)
 
@@ -238,6 +297,12 @@ def main() -> int:
                          f"from the first {used} of the {len(sources)} sources on the same graph",
                "cpu_model": cpu_model(), "nproc": os.cpu_count()}
 
+    if args.ms_sources > 0:
+        del bfs
+        h.release()
+        torch.cuda.empty_cache()
+        ms_info = batched_bfs(args, torch)
+
     out = {
         "metric": METRIC,
         "value": round(gteps, 3),
@@ -262,6 +327,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "updates": updates,
         "triangle_counting": tc_info,
+        "batched_bfs": ms_info,
     }
     print(json.dumps(out), flush=True)
     return 0
