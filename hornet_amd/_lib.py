@@ -54,6 +54,25 @@ class HnMsbfsStats(ctypes.Structure):
                 ("pull_levels", ctypes.c_int32), ("pad_", ctypes.c_int32), ("edges", ctypes.c_int64)]
 
 
+class HnRequest(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("source", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("src", ctypes.c_void_p), ("dst", ctypes.c_void_p)]
+
+
+class HnResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("kind", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("t_arrive_ns", ctypes.c_int64), ("t_start_ns", ctypes.c_int64), ("t_done_ns", ctypes.c_int64),
+                ("reached", ctypes.c_int64), ("dist_sum", ctypes.c_int64), ("max_depth", ctypes.c_int32),
+                ("batch", ctypes.c_int32), ("applied", ctypes.c_int64), ("version", ctypes.c_uint64)]
+
+
+class HnServeConfig(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_int32), ("slots", ctypes.c_int32), ("mem_budget_bytes", ctypes.c_int64),
+                ("plain", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+HN_REQ_BFS, HN_REQ_TWO_HOP, HN_REQ_INSERT, HN_REQ_ERASE = 0, 1, 2, 3
+
 P = ctypes.c_void_p
 I32 = ctypes.c_int32
 I64 = ctypes.c_int64
@@ -94,6 +113,13 @@ SIGNATURES = {
     "hn_gen_rmat_pairs": [I32, I64, I64, U64, P, P, P],
     "hn_gen_rmat_graph": [I32, I32, U64, D, P, PP],
     "hn_gen_uniform_pairs": [I32, I64, U64, P, P, P],
+    "hn_server_create": [P, ctypes.POINTER(HnServeConfig), PP],
+    "hn_server_submit": [P, ctypes.POINTER(HnRequest), PI64],
+    "hn_server_replay": [P, ctypes.POINTER(HnRequest), PI64, I64],
+    "hn_server_drain": [P],
+    "hn_server_results": [P, ctypes.POINTER(HnResult), I64, PI64],
+    "hn_server_slots": [P, PI32],
+    "hn_server_destroy": [P],
     "hn_gen_chung_lu_pairs": [I32, D, I64, I64, U64, P, P, P],
     "hn_gen_chung_lu_graph": [I32, I32, D, U64, D, P, PP],
 }

@@ -292,7 +292,10 @@ struct MsEngine {
     MsParams params{};
     hipStream_t own = nullptr;        // non-blocking stream the level graph is captured on
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    int32_t* pinned = nullptr;        // engine-private readback buffer
     void release() {
+        if (pinned) (void)hipHostFree(pinned);
+        pinned = nullptr;
         if (exec) (void)hipGraphExecDestroy(exec);
         exec = nullptr;
         if (own) (void)hipStreamDestroy(own);
@@ -308,38 +311,59 @@ struct MsEngine {
     }
 };
 
+// a standalone engine: masks, state, its own stream and pinned readback
+// buffer, so several engines may run on one graph from different threads
+hn_status ms_engine_create(hn_graph* g, MsEngine** out) {
+    MsEngine* e = new MsEngine();
+    e->g = g;
+    e->params.T = 32;
+    e->params.alpha = 16;
+    const size_t nv = (size_t)(g->nv > 0 ? g->nv : 1);
+    hn_status s;
+    if ((s = dalloc(&e->visited, nv)) != HN_OK || (s = dalloc(&e->frontier, nv)) != HN_OK ||
+        (s = dalloc(&e->next, nv)) != HN_OK || (s = dalloc(&e->S, 1)) != HN_OK ||
+        (s = dalloc(&e->d_src, 64)) != HN_OK) {
+        e->release();
+        delete e;
+        return s;
+    }
+    if (hipStreamCreateWithFlags(&e->own, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_out, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&e->pinned), 4096, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        e->release();
+        delete e;
+        set_error("multi-source BFS: stream / event / pinned buffer creation failed");
+        return HN_E_HIP;
+    }
+    *out = e;
+    return HN_OK;
+}
+
+void ms_engine_destroy(MsEngine* e) {
+    if (!e) return;
+    e->release();
+    delete e;
+}
+
+// the graph's default engine, created on first use
 static hn_status ms_engine(hn_graph* g, MsEngine** out) {
     if (!g->ms) {
-        MsEngine* e = new MsEngine();
-        e->g = g;
-        e->params.T = 32;
-        e->params.alpha = 16;
-        const size_t nv = (size_t)(g->nv > 0 ? g->nv : 1);
-        hn_status s;
-        if ((s = dalloc(&e->visited, nv)) != HN_OK || (s = dalloc(&e->frontier, nv)) != HN_OK ||
-            (s = dalloc(&e->next, nv)) != HN_OK || (s = dalloc(&e->S, 1)) != HN_OK ||
-            (s = dalloc(&e->d_src, 64)) != HN_OK) {
-            e->release();
-            delete e;
-            return s;
-        }
-        if (hipStreamCreateWithFlags(&e->own, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&e->ev_in, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&e->ev_out, hipEventDisableTiming) != hipSuccess) {
-            (void)hipGetLastError();
-            e->release();
-            delete e;
-            set_error("multi-source BFS: stream / event creation failed");
-            return HN_E_HIP;
-        }
+        MsEngine* e = nullptr;
+        HN_TRY(ms_engine_create(g, &e));
         g->ms = e;
-        g->ms_release = [](void* p) {
-            MsEngine* m = static_cast<MsEngine*>(p);
-            m->release();
-            delete m;
-        };
+        g->ms_release = [](void* p) { ms_engine_destroy(static_cast<MsEngine*>(p)); };
     }
     *out = static_cast<MsEngine*>(g->ms);
+    return HN_OK;
+}
+
+// synchronous readback through the engine's own pinned buffer
+static hn_status ms_read(MsEngine* e, const void* dptr, size_t bytes, void* host, hipStream_t st) {
+    HN_HIP(hipMemcpyAsync(e->pinned, dptr, bytes, hipMemcpyDeviceToHost, st));
+    HN_HIP(hipStreamSynchronize(st));
+    std::memcpy(host, e->pinned, bytes);
     return HN_OK;
 }
 
@@ -358,7 +382,7 @@ static hn_status ms_heavy_list(MsEngine* e, hipStream_t st) {
     if (nv > 0) heavy_flag_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(nv, g->rec, e->params.T, flag);
     s = exclusive_scan<uint32_t>(ArrayIn<uint32_t>{flag}, pos, nv, tmp, st);
     uint32_t cnt = 0;
-    if (s == HN_OK) s = read_back(g, pos + nv, 4, &cnt, st);
+    if (s == HN_OK) s = ms_read(e, pos + nv, 4, &cnt, st);
     if (s == HN_OK) {
         dfree(e->heavy);
         e->heavy = nullptr;
@@ -403,6 +427,11 @@ hn_status ms_run(hn_graph* g, const int32_t* sources, int32_t k, const int32_t* 
                  hn_msbfs_stats* stats, hipStream_t caller) {
     MsEngine* e = nullptr;
     HN_TRY(ms_engine(g, &e));
+    return ms_run_engine(e, sources, k, depth_limit, dist, stats, caller);
+}
+
+hn_status ms_run_engine(MsEngine* e, const int32_t* sources, int32_t k, const int32_t* depth_limit, int32_t* dist,
+                        hn_msbfs_stats* stats, hipStream_t caller) {
     // all work runs on the engine's own stream, ordered after the caller's
     const hipStream_t st = e->own;
     HN_HIP(hipEventRecord(e->ev_in, caller));
@@ -454,7 +483,7 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
     }
     // replay; the done flag is read every `chunk` levels
     const int chunk = 4;
-    int32_t* host = reinterpret_cast<int32_t*>(g->h_pinned);
+    int32_t* host = e->pinned;
     for (int64_t rep = 0; rep <= nv + 1; rep += chunk) {
         for (int c = 0; c < chunk; c++) HN_HIP(hipGraphLaunch(e->exec, st));
         HN_HIP(hipMemcpyAsync(host, &e->S->done, 4, hipMemcpyDeviceToHost, st));
@@ -463,7 +492,8 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
     }
     if (stats) {
         MsState s;
-        HN_TRY(read_back(g, e->S, sizeof(MsState), &s, st));
+        static_assert(sizeof(MsState) <= 4096, "state readback fits the pinned buffer");
+        HN_TRY(ms_read(e, e->S, sizeof(MsState), &s, st));
         stats->levels = s.levels_run;
         stats->push_levels = (int32_t)s.push_levels;
         stats->pull_levels = (int32_t)s.pull_levels;

@@ -1,0 +1,419 @@
+// serve.hip -- continuous-batching request loop over a resident graph.
+//
+// No reference counterpart (the reference has no serving mode); built on the
+// multi-source BFS engines (msbfs.hip) and the update pipeline (update.hip).
+//
+//   client threads --submit--> arrival-ordered queue --dispatcher--+--> slot 0 (engine, stream, thread)
+//                                                                  +--> slot 1 ...
+//                                                                  +--> updates, applied in place
+// * Queries (BFS, 2-hop) that arrive between two updates are batched, up to
+//   max_batch (<= 64), into one multi-source launch; a free slot takes the
+//   batch at once, so launches admitted while others run overlap on the
+//   device (one stream per slot).
+// * An update is applied once every launch before it has finished, and the
+//   queries after it wait for it: each query sees exactly the updates that
+//   arrived before it.  Its graph version is reported with the result.
+// * Slots are admitted against a device-memory budget: one slot holds an
+//   engine (3 x 8 B per vertex) and a distance block (max_batch x 4 B per
+//   vertex); an update is refused (HN_E_OOM result) when its scratch estimate
+//   does not fit what the slots leave.
+// * Results are digests computed on the device (reached count, distance sum,
+//   eccentricity), so a batch returns 24 B per query instead of its vectors.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "graph.hpp"
+
+namespace hn {
+namespace {
+
+constexpr int32_t INF = 0x7fffffff;
+
+int64_t now_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+// digest[b] = {reached, dist_sum, max_depth} of row b of a k x nv distance block
+__global__ __launch_bounds__(BLOCK) void digest_kernel(int64_t nv, const int32_t* __restrict__ dist,
+                                                       unsigned long long* __restrict__ digest) {
+    const int b = blockIdx.y;
+    const int32_t* row = dist + (int64_t)b * nv;
+    unsigned long long cnt = 0, sum = 0;
+    unsigned int mx = 0;
+    for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < nv; v += (int64_t)gridDim.x * BLOCK) {
+        const int32_t d = row[v];
+        if (d != INF) {
+            cnt++;
+            sum += (unsigned long long)d;
+            mx = d > (int32_t)mx ? (unsigned)d : mx;
+        }
+    }
+    cnt = wave_reduce_sum(cnt);
+    sum = wave_reduce_sum(sum);
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        unsigned o = __shfl_xor(mx, d, WAVE);
+        mx = o > mx ? o : mx;
+    }
+    if (__lane_id() == 0 && cnt) {
+        atomicAdd(digest + 3 * b, cnt);
+        atomicAdd(digest + 3 * b + 1, sum);
+        atomicMax(digest + 3 * b + 2, (unsigned long long)mx);
+    }
+}
+
+}  // namespace
+
+struct ServeReq {
+    int64_t id = 0;
+    int32_t kind = 0;
+    int32_t source = 0;
+    std::vector<int32_t> src, dst;
+};
+
+struct ServeSlot {
+    MsEngine* eng = nullptr;
+    int32_t* dist = nullptr;                 // max_batch x nv
+    unsigned long long* digest = nullptr;    // 64 x 3
+    unsigned long long* h_digest = nullptr;  // pinned
+    hipStream_t stream = nullptr;
+    std::thread th;
+    std::vector<ServeReq*> batch;
+    bool has_work = false;
+    bool busy = false;
+    std::condition_variable cv;
+};
+
+}  // namespace hn
+
+struct hn_server {
+    hn_graph* g = nullptr;
+    hn_serve_config cfg{};
+    int64_t slot_bytes = 0;
+    int64_t update_budget = 0;
+    std::mutex mu;
+    std::condition_variable cv_disp, cv_done;
+    std::deque<hn::ServeReq*> queue;
+    std::vector<std::unique_ptr<hn::ServeReq>> reqs;
+    std::vector<hn_result> results;
+    std::vector<std::unique_ptr<hn::ServeSlot>> slots;
+    std::thread dispatcher;
+    hipStream_t upd_stream = nullptr;
+    int inflight = 0;
+    int64_t completed = 0;
+    bool stop = false;
+};
+
+namespace hn {
+
+static void finish_query(hn_server* s, ServeReq* r, const unsigned long long* dg, int32_t k, uint64_t ver,
+                         hn_status st, int64_t t_done) {
+    hn_result& res = s->results[(size_t)r->id];
+    res.status = st;
+    res.t_done_ns = t_done;
+    res.reached = st == HN_OK ? (int64_t)dg[0] : 0;
+    res.dist_sum = st == HN_OK ? (int64_t)dg[1] : 0;
+    res.max_depth = st == HN_OK ? (int32_t)dg[2] : 0;
+    res.batch = k;
+    res.version = ver;
+}
+
+static void worker_loop(hn_server* s, ServeSlot* slot) {
+    hn_graph* g = s->g;
+    const int64_t nv = g->nv;
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        slot->cv.wait(lk, [&] { return s->stop || slot->has_work; });
+        if (!slot->has_work) break;
+        std::vector<ServeReq*> batch;
+        batch.swap(slot->batch);
+        slot->has_work = false;
+        const uint64_t ver = g->version;   // stable: updates wait for in-flight launches
+        lk.unlock();
+
+        const int32_t k = (int32_t)batch.size();
+        int32_t srcs[64], limits[64];
+        for (int32_t i = 0; i < k; i++) {
+            srcs[i] = batch[i]->source;
+            limits[i] = batch[i]->kind == HN_REQ_TWO_HOP ? 2 : 0;
+        }
+        hn_status st = HN_OK;
+        if (s->cfg.plain && k == 1 && batch[0]->kind == HN_REQ_BFS) {
+            st = hn_bfs_topdown(g, srcs[0], slot->dist, nullptr, slot->stream);
+        } else {
+            st = ms_run_engine(slot->eng, srcs, k, limits, slot->dist, nullptr, slot->stream);
+        }
+        if (st == HN_OK) {
+            hipError_t e = hipMemsetAsync(slot->digest, 0, sizeof(unsigned long long) * 3 * 64, slot->stream);
+            if (e == hipSuccess && nv > 0) {
+                dim3 grid((unsigned)std::min<int64_t>(ceil_div(nv, BLOCK), 1024), (unsigned)k);
+                digest_kernel<<<grid, BLOCK, 0, slot->stream>>>(nv, slot->dist, slot->digest);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(slot->h_digest, slot->digest, sizeof(unsigned long long) * 3 * (size_t)k,
+                                   hipMemcpyDeviceToHost, slot->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(slot->stream);
+            if (e != hipSuccess) st = HN_E_HIP;
+        }
+        const int64_t t_done = now_ns();
+        lk.lock();
+        for (int32_t i = 0; i < k; i++) finish_query(s, batch[i], slot->h_digest + 3 * i, k, ver, st, t_done);
+        s->completed += k;
+        s->inflight--;
+        slot->busy = false;
+        s->cv_disp.notify_one();
+        s->cv_done.notify_all();
+    }
+}
+
+static bool is_query(const ServeReq* r) { return r->kind == HN_REQ_BFS || r->kind == HN_REQ_TWO_HOP; }
+
+static ServeSlot* free_slot(hn_server* s) {
+    for (auto& sl : s->slots)
+        if (!sl->busy) return sl.get();
+    return nullptr;
+}
+
+static void dispatcher_loop(hn_server* s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        s->cv_disp.wait(lk, [&] {
+            if (s->stop) return true;
+            if (s->queue.empty()) return false;
+            return is_query(s->queue.front()) ? free_slot(s) != nullptr : s->inflight == 0;
+        });
+        if (s->stop) break;
+        ServeReq* head = s->queue.front();
+        if (!is_query(head)) {
+            s->queue.pop_front();
+            // index, never hold a reference: submissions may grow `results`
+            // while the lock is released below
+            s->results[(size_t)head->id].t_start_ns = now_ns();
+            lk.unlock();
+            int64_t applied = 0;
+            hn_status st;
+            const int64_t n = (int64_t)head->src.size();
+            if (n * 160 > s->update_budget) {
+                hn::set_error("update batch exceeds the server's memory budget");
+                st = HN_E_OOM;
+            } else if (head->kind == HN_REQ_INSERT) {
+                st = hn_graph_insert(s->g, head->src.data(), head->dst.data(), n, HN_DEDUP_BATCH | HN_DEDUP_GRAPH,
+                                     &applied, s->upd_stream);
+            } else {
+                st = hn_graph_erase(s->g, head->src.data(), head->dst.data(), n, HN_DEDUP_BATCH, &applied,
+                                    s->upd_stream);
+            }
+            const int64_t t_done = now_ns();
+            lk.lock();
+            hn_result& res = s->results[(size_t)head->id];
+            res.status = st;
+            res.applied = applied;
+            res.t_done_ns = t_done;
+            res.version = s->g->version;
+            res.batch = 1;
+            s->completed++;
+            s->cv_done.notify_all();
+            continue;
+        }
+        ServeSlot* slot = free_slot(s);
+        const int maxb = s->cfg.plain ? 1 : s->cfg.max_batch;
+        const int64_t t0 = now_ns();
+        while (!s->queue.empty() && is_query(s->queue.front()) && (int)slot->batch.size() < maxb) {
+            ServeReq* r = s->queue.front();
+            s->queue.pop_front();
+            s->results[(size_t)r->id].t_start_ns = t0;
+            slot->batch.push_back(r);
+        }
+        slot->busy = true;
+        slot->has_work = true;
+        s->inflight++;
+        slot->cv.notify_one();
+        // in plain mode queries run one at a time, with nothing overlapping
+        if (s->cfg.plain) s->cv_disp.wait(lk, [&] { return s->stop || s->inflight == 0; });
+    }
+}
+
+static void destroy_server(hn_server* s) {
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_disp.notify_all();
+    for (auto& sl : s->slots) sl->cv.notify_all();
+    if (s->dispatcher.joinable()) s->dispatcher.join();
+    for (auto& sl : s->slots) {
+        if (sl->th.joinable()) sl->th.join();
+        ms_engine_destroy(sl->eng);
+        dfree(sl->dist);
+        dfree(sl->digest);
+        if (sl->h_digest) (void)hipHostFree(sl->h_digest);
+        if (sl->stream) (void)hipStreamDestroy(sl->stream);
+    }
+    if (s->upd_stream) (void)hipStreamDestroy(s->upd_stream);
+    delete s;
+}
+
+}  // namespace hn
+
+using namespace hn;
+
+extern "C" {
+
+hn_status hn_server_create(hn_graph* g, const hn_serve_config* cfg, hn_server** out) {
+    HN_ARG(g && cfg && out, "null argument");
+    *out = nullptr;
+    HN_ARG(cfg->max_batch >= 1 && cfg->max_batch <= 64, "max_batch must be in [1, 64]");
+    HN_ARG(cfg->slots >= 1, "slots must be >= 1");
+    hn_server* s = new hn_server();
+    s->g = g;
+    s->cfg = *cfg;
+    const int64_t nv = g->nv > 0 ? g->nv : 1;
+    const int64_t batch = cfg->plain ? 1 : cfg->max_batch;
+    s->slot_bytes = ms_engine_bytes(nv) + batch * nv * 4 + 64 * 24;
+    int64_t budget = cfg->mem_budget_bytes;
+    if (budget <= 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            (void)hipGetLastError();
+            fr = (size_t)1 << 34;
+        }
+        budget = (int64_t)(fr / 4);
+    }
+    int nslots = cfg->plain ? 1 : (int)std::min<int64_t>(cfg->slots, budget / s->slot_bytes);
+    if (nslots < 1) {
+        delete s;
+        set_error("memory budget does not fit one launch slot");
+        return HN_E_OOM;
+    }
+    s->update_budget = std::max<int64_t>(budget - nslots * s->slot_bytes, 0);
+    s->cfg.slots = nslots;
+    if (hipStreamCreateWithFlags(&s->upd_stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete s;
+        set_error("server: stream creation failed");
+        return HN_E_HIP;
+    }
+    for (int i = 0; i < nslots; i++) {
+        auto sl = std::make_unique<ServeSlot>();
+        hn_status st = ms_engine_create(g, &sl->eng);
+        if (st == HN_OK) st = dalloc(&sl->dist, (size_t)(batch * nv));
+        if (st == HN_OK) st = dalloc(&sl->digest, 64 * 3);
+        if (st == HN_OK && (hipHostMalloc(reinterpret_cast<void**>(&sl->h_digest), 64 * 3 * 8, 0) != hipSuccess ||
+                            hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess)) {
+            (void)hipGetLastError();
+            set_error("server: slot resources");
+            st = HN_E_HIP;
+        }
+        s->slots.push_back(std::move(sl));
+        if (st != HN_OK) {
+            destroy_server(s);
+            return st;
+        }
+    }
+    for (auto& sl : s->slots) {
+        ServeSlot* p = sl.get();
+        p->th = std::thread(worker_loop, s, p);
+    }
+    s->dispatcher = std::thread(dispatcher_loop, s);
+    *out = s;
+    return HN_OK;
+}
+
+hn_status hn_server_submit(hn_server* s, const hn_request* req, int64_t* id) {
+    HN_ARG(s && req, "null argument");
+    const bool q = req->kind == HN_REQ_BFS || req->kind == HN_REQ_TWO_HOP;
+    HN_ARG(q || req->kind == HN_REQ_INSERT || req->kind == HN_REQ_ERASE, "unknown request kind");
+    if (q) HN_ARG(req->source >= 0 && req->source < s->g->nv, "query source outside [0, nv)");
+    else HN_ARG(req->n >= 0 && (req->n == 0 || (req->src && req->dst)), "bad update batch");
+    auto r = std::make_unique<ServeReq>();
+    r->kind = req->kind;
+    r->source = req->source;
+    if (!q) {
+        r->src.assign(req->src, req->src + req->n);
+        r->dst.assign(req->dst, req->dst + req->n);
+    }
+    const int64_t t = now_ns();
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->id = (int64_t)s->reqs.size();
+    hn_result res{};
+    res.id = r->id;
+    res.kind = r->kind;
+    res.status = -1;
+    res.t_arrive_ns = t;
+    s->results.push_back(res);
+    s->queue.push_back(r.get());
+    if (id) *id = r->id;
+    s->reqs.push_back(std::move(r));
+    s->cv_disp.notify_one();
+    return HN_OK;
+}
+
+hn_status hn_server_drain(hn_server* s) {
+    HN_ARG(s, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->completed >= (int64_t)s->reqs.size(); });
+    return HN_OK;
+}
+
+hn_status hn_server_replay(hn_server* s, const hn_request* reqs, const int64_t* arrival_ns, int64_t n) {
+    HN_ARG(s && (n == 0 || (reqs && arrival_ns)), "null argument");
+    for (int64_t i = 0; i < n; i++) {
+        const hn_request& r = reqs[i];
+        const bool q = r.kind == HN_REQ_BFS || r.kind == HN_REQ_TWO_HOP;
+        if (q) HN_ARG(r.source >= 0 && r.source < s->g->nv, "query source outside [0, nv)");
+        HN_ARG(i == 0 || arrival_ns[i] >= arrival_ns[i - 1], "arrival times must be non-decreasing");
+    }
+    hn_status rc = HN_OK;
+    std::thread client([&] {
+        const int64_t t0 = now_ns();
+        for (int64_t i = 0; i < n; i++) {
+            const int64_t target = t0 + arrival_ns[i];
+            int64_t t = now_ns();
+            if (target - t > 200000) std::this_thread::sleep_for(std::chrono::nanoseconds(target - t - 100000));
+            while (now_ns() < target) {
+            }
+            hn_status st = hn_server_submit(s, &reqs[i], nullptr);
+            if (st != HN_OK) rc = st;
+        }
+    });
+    client.join();
+    HN_TRY(rc);
+    return hn_server_drain(s);
+}
+
+hn_status hn_server_results(hn_server* s, hn_result* out, int64_t cap, int64_t* n) {
+    HN_ARG(s, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int64_t m = (int64_t)s->results.size();
+    if (n) *n = m;
+    if (out) {
+        if (cap < m) {
+            set_error("server results: capacity too small");
+            return HN_E_CAPACITY;
+        }
+        std::memcpy(out, s->results.data(), sizeof(hn_result) * (size_t)m);
+    }
+    return HN_OK;
+}
+
+hn_status hn_server_slots(hn_server* s, int32_t* slots) {
+    HN_ARG(s && slots, "null argument");
+    *slots = s->cfg.slots;
+    return HN_OK;
+}
+
+hn_status hn_server_destroy(hn_server* s) {
+    if (s) destroy_server(s);
+    return HN_OK;
+}
+
+}  // extern "C"

@@ -376,6 +376,78 @@ def rmat_pairs_device(scale: int, seed: int, edge_factor: int = 16, first: int =
     return src[:count], dst[:count]
 
 
+class Server:
+    """Continuous-batching request loop over a resident graph (hn_server_*).
+
+    ``replay(trace)`` plays a list of requests at their arrival offsets and
+    returns one result row per request (a numpy structured array)."""
+
+    BFS, TWO_HOP, INSERT, ERASE = (_lib.HN_REQ_BFS, _lib.HN_REQ_TWO_HOP, _lib.HN_REQ_INSERT,
+                                   _lib.HN_REQ_ERASE)
+
+    def __init__(self, hornet: Hornet, max_batch: int = 64, slots: int = 2, mem_budget_bytes: int = 0,
+                 plain: bool = False):
+        self.hornet = hornet
+        cfg = _lib.HnServeConfig(max_batch, slots, mem_budget_bytes, 1 if plain else 0, 0)
+        self._s = ctypes.c_void_p()
+        call("hn_server_create", hornet.handle, ctypes.byref(cfg), ctypes.byref(self._s))
+
+    def slots(self) -> int:
+        v = ctypes.c_int32()
+        call("hn_server_slots", self._s, ctypes.byref(v))
+        return v.value
+
+    @staticmethod
+    def _requests(trace):
+        """trace: list of (kind, payload) with payload = source (queries) or
+        (src, dst) int32 arrays (updates)."""
+        keep = []
+        reqs = (_lib.HnRequest * len(trace))()
+        for i, (kind, payload) in enumerate(trace):
+            r = reqs[i]
+            r.kind = kind
+            if kind in (_lib.HN_REQ_BFS, _lib.HN_REQ_TWO_HOP):
+                r.source = int(payload)
+            else:
+                s = np.ascontiguousarray(payload[0], np.int32)
+                d = np.ascontiguousarray(payload[1], np.int32)
+                keep += [s, d]
+                r.n = s.size
+                r.src = s.ctypes.data
+                r.dst = d.ctypes.data
+        return reqs, keep
+
+    def replay(self, trace, arrival_ns):
+        reqs, keep = self._requests(trace)
+        arr = np.ascontiguousarray(arrival_ns, np.int64)
+        call("hn_server_replay", self._s, reqs, arr.ctypes.data_as(_lib.PI64), len(trace))
+        del keep
+        return self.results()
+
+    def results(self) -> np.ndarray:
+        n = ctypes.c_int64()
+        call("hn_server_results", self._s, None, 0, ctypes.byref(n))
+        buf = (_lib.HnResult * max(n.value, 1))()
+        call("hn_server_results", self._s, buf, n.value, ctypes.byref(n))
+        dt = np.dtype([(name, np.dtype(ct)) for name, ct in
+                       [("id", np.int64), ("kind", np.int32), ("status", np.int32), ("t_arrive_ns", np.int64),
+                        ("t_start_ns", np.int64), ("t_done_ns", np.int64), ("reached", np.int64),
+                        ("dist_sum", np.int64), ("max_depth", np.int32), ("batch", np.int32),
+                        ("applied", np.int64), ("version", np.uint64)]])
+        return np.frombuffer(bytes(buf), dtype=dt, count=n.value).copy()
+
+    def close(self):
+        if self._s:
+            _lib.load().hn_server_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def chung_lu_pairs_device(n: int, count: int, seed: int, gamma: float = 2.1):
     src = torch.empty(max(count, 1), dtype=torch.int32, device=_dev())
     dst = torch.empty(max(count, 1), dtype=torch.int32, device=_dev())

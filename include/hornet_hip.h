@@ -204,6 +204,63 @@ hn_status hn_tc_count(hn_graph* g, uint64_t* total, uint64_t* per_vertex, void* 
  * triangle count.  per_vertex may be NULL.  Synchronises. */
 hn_status hn_tc2_stored(hn_graph* g, uint64_t* total, uint64_t* per_vertex, void* stream);
 
+/* ------------------------------------------------------------- serving */
+/* A request loop that keeps the graph resident and serves a stream of BFS
+ * queries, 2-hop neighbourhood-size queries and update batches (no reference
+ * counterpart: the reference has no serving mode).  Requests are taken in
+ * arrival order; consecutive queries (no update between them) are batched, up
+ * to 64, into one multi-source launch (2-hop = depth limit 2), and up to
+ * `slots` launches run at once on their own streams; an update waits for the
+ * launches before it and blocks the queries after it, so every query sees
+ * exactly the updates that arrived before it (snapshot-consistent order).
+ * The number of slots admitted is bounded by the device-memory budget. */
+typedef struct hn_server hn_server;
+enum { HN_REQ_BFS = 0, HN_REQ_TWO_HOP = 1, HN_REQ_INSERT = 2, HN_REQ_ERASE = 3 };
+
+typedef struct hn_request {
+    int32_t kind;          /* HN_REQ_*                                         */
+    int32_t source;        /* queries                                          */
+    int64_t n;             /* updates: entries                                 */
+    const int32_t* src;    /* updates: host arrays, copied at submission       */
+    const int32_t* dst;
+} hn_request;
+
+typedef struct hn_result {
+    int64_t id;
+    int32_t kind;
+    int32_t status;        /* hn_status of the request                         */
+    int64_t t_arrive_ns;   /* steady-clock times                               */
+    int64_t t_start_ns;
+    int64_t t_done_ns;
+    int64_t reached;       /* BFS: reached vertices; 2-hop: |{v} u N(v) u N(N(v))| */
+    int64_t dist_sum;      /* BFS: sum of the distances of reached vertices    */
+    int32_t max_depth;     /* BFS: largest finite distance                     */
+    int32_t batch;         /* queries that shared the launch                   */
+    int64_t applied;       /* updates: entries inserted / erased               */
+    uint64_t version;      /* graph version the request saw (update: produced) */
+} hn_result;
+
+typedef struct hn_serve_config {
+    int32_t max_batch;        /* queries per launch, 1..64                     */
+    int32_t slots;            /* concurrent launches wanted                    */
+    int64_t mem_budget_bytes; /* device memory for launches (0: 1/4 of free)   */
+    int32_t plain;            /* 1: one query per launch, single-source BFS    */
+    int32_t pad_;
+} hn_serve_config;
+
+hn_status hn_server_create(hn_graph* g, const hn_serve_config* cfg, hn_server** out);
+/* Thread-safe; *id (may be NULL) receives the request id (0, 1, ...). */
+hn_status hn_server_submit(hn_server* s, const hn_request* req, int64_t* id);
+/* Submit reqs[i] at arrival_ns[i] after the call starts (a client thread),
+ * then wait until every request is done. */
+hn_status hn_server_replay(hn_server* s, const hn_request* reqs, const int64_t* arrival_ns, int64_t n);
+hn_status hn_server_drain(hn_server* s);
+/* Results of all requests so far, indexed by id; *n = number of requests. */
+hn_status hn_server_results(hn_server* s, hn_result* out, int64_t cap, int64_t* n);
+/* Slots admitted after the memory budget. */
+hn_status hn_server_slots(hn_server* s, int32_t* slots);
+hn_status hn_server_destroy(hn_server* s);
+
 /* ---------------------------------------------------------- generators */
 /* Seeded R-MAT(a=.57,b=.19,c=.19,d=.05) pairs on the device: pair i, level l
  * uses draw i*64+l of SplitMix64 stream `seed` (specified by this library; the
