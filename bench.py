@@ -167,6 +167,98 @@ def batched_bfs(args, torch):
     return out
 
 
+def serve_trace(h, nreq, seed, torch):
+    """C5 mix: 70 % single-source BFS, 20 % 2-hop counts, 10 % update batches
+    of floor(10^U(3,6)) pairs (half inserts of uniform pairs, half erases of
+    pairs sampled from the current edge set), symmetrized.  Sources are
+    uniform over non-isolated vertices."""
+    import numpy as np
+    from hornet_amd.graph import Server, uniform_pairs_device
+
+    rng = np.random.default_rng(seed)
+    nv = h.nV()
+    deg = h.degrees().cpu().numpy()
+    nz = np.flatnonzero(deg > 0)
+    kinds = rng.choice(4, nreq, p=[0.7, 0.2, 0.05, 0.05])
+    src, dst = h.getCOO()
+    trace = []
+    for i, k in enumerate(kinds):
+        if k == Server.BFS or k == Server.TWO_HOP:
+            trace.append((int(k), int(nz[rng.integers(0, nz.size)])))
+            continue
+        m = int(10 ** rng.uniform(3, 6))
+        if k == Server.INSERT:
+            a, b = uniform_pairs_device(nv, m, seed=10_000 + i)
+            a, b = a.cpu().numpy(), b.cpu().numpy()
+            keep = a != b
+            a, b = a[keep], b[keep]
+        else:
+            pos = torch.from_numpy(rng.integers(0, src.numel(), m)).cuda()
+            a, b = src[pos].cpu().numpy(), dst[pos].cpu().numpy()
+        trace.append((int(k), (np.concatenate([a, b]), np.concatenate([b, a]))))
+    del src, dst
+    return trace
+
+
+def latency_stats(res, t_total):
+    import numpy as np
+    out = {"requests": int(res.size), "throughput_rps": round(res.size / t_total, 1)}
+    lat = (res["t_done_ns"] - res["t_arrive_ns"]) / 1e6
+    for name, kind in (("bfs", 0), ("two_hop", 1), ("update", -1)):
+        m = res["kind"] >= 2 if kind < 0 else res["kind"] == kind
+        if m.any():
+            out[name] = {"n": int(m.sum()), "p50_ms": round(float(np.percentile(lat[m], 50)), 3),
+                         "p99_ms": round(float(np.percentile(lat[m], 99)), 3)}
+    q = res["kind"] <= 1
+    if q.any():
+        out["mean_batch"] = round(float(res["batch"][q].mean()), 2)
+    out["failed"] = int((res["status"] != 0).sum())
+    return out
+
+
+def served(args, torch):
+    """C5: R-MAT s25 resident; the trace is first played closed-loop (all at
+    t=0) to find the saturating throughput, then with Poisson arrivals (seed 5)
+    at 70 % of it; the plain form (one query per launch, no overlap) is run
+    the same way on a prefix of the same trace."""
+    import numpy as np
+    from hornet_amd.graph import Hornet, Server
+
+    def fresh():
+        return Hornet.rmat(args.serve_scale, seed=args.serve_scale)
+
+    h = fresh()
+    trace = serve_trace(h, args.serve_requests, seed=5, torch=torch)
+    out = {"graph": f"R-MAT s{args.serve_scale} ef16", "nv": h.nV(), "ne": h.nE()}
+    h.release()
+    for label, plain, n in (("served", False, len(trace)), ("plain", True, min(args.serve_plain, len(trace)))):
+        # every replay starts from the same graph: the trace's updates change it
+        sub = trace[:n]
+        h = fresh()
+        srv = Server(h, max_batch=64, slots=args.serve_slots, plain=plain)
+        t = time.perf_counter()
+        srv.replay(sub, np.zeros(n, np.int64))
+        sat = n / (time.perf_counter() - t)
+        srv.close()
+        h.release()
+        rate = 0.7 * sat
+        arrivals = np.cumsum(np.random.default_rng(5).exponential(1e9 / rate, n)).astype(np.int64)
+        h = fresh()
+        srv = Server(h, max_batch=64, slots=args.serve_slots, plain=plain)
+        t = time.perf_counter()
+        res = srv.replay(sub, arrivals)
+        el = time.perf_counter() - t
+        out[label] = {"saturating_rps": round(sat, 1), "offered_rps": round(rate, 1), "slots": srv.slots(),
+                      **latency_stats(res, el)}
+        srv.close()
+        h.release()
+        torch.cuda.empty_cache()
+        print(f"# serve {label}: {out[label]}", file=sys.stderr, flush=True)
+    out["saturating_speedup_vs_plain"] = round(out["served"]["saturating_rps"] / out["plain"]["saturating_rps"], 2)
+    out["plain_requests"] = min(args.serve_plain, len(trace))
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +274,10 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=1 << 20, help="pairs inserted / deleted per round")
     ap.add_argument("--ms-sources", type=int, default=4096, help="batched-BFS sources (0 disables)")
     ap.add_argument("--ms-plain", type=int, default=64, help="sources timed one per launch for the ratio")
+    ap.add_argument("--serve-requests", type=int, default=2000, help="served-stream requests (0 disables)")
+    ap.add_argument("--serve-plain", type=int, default=300, help="requests replayed in the plain form")
+    ap.add_argument("--serve-scale", type=int, default=25)
+    ap.add_argument("--serve-slots", type=int, default=2)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -302,6 +398,7 @@ def main() -> int:
         h.release()
         torch.cuda.empty_cache()
         ms_info = batched_bfs(args, torch)
+    serve_info = served(args, torch) if args.serve_requests > 0 else None
 
     out = {
         "metric": METRIC,
@@ -328,6 +425,7 @@ def main() -> int:
         "updates": updates,
         "triangle_counting": tc_info,
         "batched_bfs": ms_info,
+        "served": serve_info,
     }
     print(json.dumps(out), flush=True)
     return 0
