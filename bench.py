@@ -278,6 +278,9 @@ def main() -> int:
     ap.add_argument("--serve-plain", type=int, default=300, help="requests replayed in the plain form")
     ap.add_argument("--serve-scale", type=int, default=25)
     ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (a known-size reference "
+                         "for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -292,6 +295,12 @@ def main() -> int:
     from hornet_amd.graph import BfsTopDown, Hornet
 
     torch.cuda.set_device(0)
+    if args.calibrate_copy_mib > 0:
+        a = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     t_build = time.perf_counter()
     h = Hornet.rmat(args.scale, seed=args.seed, edge_factor=args.edge_factor)
     torch.cuda.synchronize()

@@ -38,22 +38,23 @@ constexpr int EXP_IPT = 8;
 constexpr int EXP_TILE = BLOCK * EXP_IPT;
 constexpr int32_t INF = 0x7fffffff;
 
+// The frontier is kept as (degree, adjacency offset) pairs written by the
+// collect kernel, so the scan reads contiguous arrays instead of gathering
+// one 16-byte vertex record per frontier vertex twice.
 struct FrontierDegIn {
-    const int32_t* q;
-    const hn_vrec* rec;
-    __device__ __forceinline__ uint32_t operator()(int64_t i) const { return (uint32_t)rec[q[i]].deg; }
+    const int32_t* qdeg;
+    __device__ __forceinline__ uint32_t operator()(int64_t i) const { return (uint32_t)qdeg[i]; }
 };
 
 // per frontier vertex i with exclusive edge prefix `ex` and degree d
 struct FrontierOut {
-    const int32_t* q;
-    const hn_vrec* rec;
+    const int64_t* qoff;
     uint32_t* P;
     int64_t* base;
     uint32_t* tile_lo;
     __device__ __forceinline__ void operator()(int64_t i, uint32_t ex, uint32_t d) const {
         P[i] = ex;
-        base[i] = rec[q[i]].off - (int64_t)ex;
+        base[i] = qoff[i] - (int64_t)ex;
         if (d) {
             uint32_t t0 = (uint32_t)(((uint64_t)ex + EXP_TILE - 1) / EXP_TILE);
             uint32_t t1 = (uint32_t)(((uint64_t)ex + d + EXP_TILE - 1) / EXP_TILE);
@@ -88,13 +89,48 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(int64_t F, const uint
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int64_t lo = tile_lo[t];
         const int64_t hi = (t + 1 < ntiles) ? (int64_t)tile_lo[t + 1] : F - 1;
-        lb_tile_range<EXP_IPT>(
-            P, lo, hi, t * EXP_TILE, W, sh, [&](int64_t s) { return segbase[s]; },
-            [&](int64_t, int64_t w, int64_t base) {
-                const int32_t v = arena[base + w];
-                if (!(bitmap[v >> 5] & (1u << (v & 31)))) next[v] = 1;
-            });
+        const int64_t w0 = t * EXP_TILE;
+        const int count = lb_stage<EXP_IPT>(P, lo, hi, sh, [&](int64_t s) { return segbase[s]; });
+        // phase 1: every item's adjacency slot (independent LDS searches)
+        int64_t addr[EXP_IPT];
+        if (count > 0) {
+#pragma unroll
+            for (int j = 0; j < EXP_IPT; j++) {
+                const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+                const int64_t a = sh.base[lb_locate<EXP_IPT>(sh, count, w)] + w;
+                addr[j] = w < W ? a : -1;
+            }
+        } else {   // segment range wider than LDS (zero-degree frontier vertices)
+            for (int j = 0; j < EXP_IPT; j++) {
+                const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+                addr[j] = w < W ? segbase[upper_bound_m1<int64_t>(P, lo, lo + (int64_t)(-count), w)] + w : -1;
+            }
+        }
+        // phase 2: all neighbour ids in flight at once (streamed, not kept in L2)
+        int32_t v[EXP_IPT];
+#pragma unroll
+        for (int j = 0; j < EXP_IPT; j++) v[j] = addr[j] >= 0 ? __builtin_nontemporal_load(arena + addr[j]) : -1;
+        // phase 3: all visited words in flight at once
+        uint32_t word[EXP_IPT];
+#pragma unroll
+        for (int j = 0; j < EXP_IPT; j++) word[j] = v[j] >= 0 ? bitmap[v[j] >> 5] : 0xFFFFFFFFu;
+        // phase 4: idempotent marks (testing the mark first cut L2 write-backs
+        // 10x but added a dependent read and measured slower)
+#pragma unroll
+        for (int j = 0; j < EXP_IPT; j++)
+            if (!(word[j] & (1u << (v[j] & 31)))) next[v[j]] = 1;
+        __syncthreads();
     }
+}
+
+// (degree, adjacency offset) of every frontier vertex, one record gather each
+__global__ void bfs_gather_kernel(int64_t F, const int32_t* __restrict__ q, const hn_vrec* __restrict__ rec,
+                                  int32_t* __restrict__ qdeg, int64_t* __restrict__ qoff) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= F) return;
+    const hn_vrec r = rec[q[i]];
+    qdeg[i] = r.deg;
+    qoff[i] = r.off;
 }
 
 // Phase 2: one lane owns 32 vertices (one bitmap word and 32 mark bytes):
@@ -147,10 +183,13 @@ __global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, uint3
 }  // namespace
 
 // scratch layout inside g->s_bfs
+// The frontier needs no ping-pong: q is gathered into (qdeg, qoff) and
+// scanned into P / base / tile_lo before collect overwrites q with level L+1.
 struct BfsBufs {
     uint32_t* bitmap;
-    int32_t* qa;
-    int32_t* qb;
+    int32_t* q;
+    int32_t* qdeg;
+    int64_t* qoff;
     uint32_t* P;
     int64_t* base;
     uint32_t* tile_lo;
@@ -164,9 +203,10 @@ static hn_status bfs_buffers(hn_graph* g, BfsBufs* b) {
     const int64_t tiles = g->ne / EXP_TILE + 2;
     auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
     int64_t o_bitmap = 0;
-    int64_t o_qa = al(o_bitmap + words * 4);
-    int64_t o_qb = al(o_qa + nv * 4);
-    int64_t o_P = al(o_qb + nv * 4);
+    int64_t o_q = al(o_bitmap + words * 4);
+    int64_t o_qdeg = al(o_q + nv * 4);
+    int64_t o_qoff = al(o_qdeg + nv * 4);
+    int64_t o_P = al(o_qoff + nv * 8);
     int64_t o_base = al(o_P + (nv + 1) * 4);
     int64_t o_tile = al(o_base + nv * 8);
     int64_t o_next = al(o_tile + tiles * 4);
@@ -175,8 +215,9 @@ static hn_status bfs_buffers(hn_graph* g, BfsBufs* b) {
     HN_TRY(g->s_bfs.reserve((size_t)total));
     char* base = g->s_bfs.as<char>();
     b->bitmap = reinterpret_cast<uint32_t*>(base + o_bitmap);
-    b->qa = reinterpret_cast<int32_t*>(base + o_qa);
-    b->qb = reinterpret_cast<int32_t*>(base + o_qb);
+    b->q = reinterpret_cast<int32_t*>(base + o_q);
+    b->qdeg = reinterpret_cast<int32_t*>(base + o_qdeg);
+    b->qoff = reinterpret_cast<int64_t*>(base + o_qoff);
     b->P = reinterpret_cast<uint32_t*>(base + o_P);
     b->base = reinterpret_cast<int64_t*>(base + o_base);
     b->tile_lo = reinterpret_cast<uint32_t*>(base + o_tile);
@@ -200,7 +241,7 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
     HN_TRY(bfs_buffers(g, &b));
     const int64_t nv = g->nv;
     const int64_t words = (nv + 31) / 32;
-    bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(g->nv, dist, b.bitmap, source, b.qa);
+    bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(g->nv, dist, b.bitmap, source, b.q);
     HN_CHECK_LAUNCH();
     HN_HIP(hipMemsetAsync(b.next, 0, (size_t)words * 32, st));
     const int64_t grid_cap = (int64_t)num_cus() * 8;
@@ -219,9 +260,9 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
     while (F > 0) {
         visited += F;
         nlevels++;
-        rc = exclusive_scan_fn<uint32_t>(FrontierDegIn{b.qa, g->rec},
-                                         FrontierOut{b.qa, g->rec, b.P, b.base, b.tile_lo}, b.P + F, F,
-                                         g->s_scan.ptr, st);
+        bfs_gather_kernel<<<(unsigned)ceil_div(F, BLOCK), BLOCK, 0, st>>>(F, b.q, g->rec, b.qdeg, b.qoff);
+        rc = exclusive_scan_fn<uint32_t>(FrontierDegIn{b.qdeg}, FrontierOut{b.qoff, b.P, b.base, b.tile_lo},
+                                         b.P + F, F, g->s_scan.ptr, st);
         if (rc != HN_OK) break;
         hipError_t e = hipMemsetAsync(b.cnt, 0, 4, st);
         if (timed && e == hipSuccess) e = hipEventRecord(ev0, st);
@@ -233,7 +274,7 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
         if (timed && e == hipSuccess) e = hipEventRecord(ev1, st);
         if (e == hipSuccess) {
             bfs_collect_kernel<<<(unsigned)ceil_div(words, BLOCK), BLOCK, 0, st>>>(
-                words, b.bitmap, reinterpret_cast<uint32_t*>(b.next), dist, level, b.qb, b.cnt);
+                words, b.bitmap, reinterpret_cast<uint32_t*>(b.next), dist, level, b.q, b.cnt);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipMemcpyAsync(host, b.cnt, 4, hipMemcpyDeviceToHost, st);
@@ -251,7 +292,6 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
         }
         scanned += host[1];
         F = host[0];
-        std::swap(b.qa, b.qb);
         level++;
     }
     if (timed) {

@@ -261,6 +261,42 @@ __device__ __forceinline__ void lb_tile_range(const PrefT* __restrict__ P, int64
     __syncthreads();
 }
 
+// Phase-split form for kernels that want all of a lane's items in flight at
+// once: lb_stage() stages [lo, hi] and returns the staged count (> 0), or
+// -(count) when the range overflowed LDS; lb_locate() is then a fixed-step,
+// branch-free search (independent per item, so the compiler can interleave
+// the IPT searches and the loads that follow).
+template <int IPT, typename PrefT, typename SegFn>
+__device__ __forceinline__ int lb_stage(const PrefT* __restrict__ P, int64_t lo, int64_t hi,
+                                        LbShared<IPT, PrefT>& sh, SegFn segbase) {
+    const int64_t span = hi - lo + 1;
+    const int count = span <= LbShared<IPT, PrefT>::CAP ? (int)span : -(int)(span < 0x7fffffff ? span : 0x7fffffff);
+    if (count > 0) {
+        for (int i = threadIdx.x; i < count; i += BLOCK) {
+            sh.pref[i] = P[lo + i];
+            sh.base[i] = segbase(lo + i);
+        }
+    }
+    __syncthreads();
+    return count;
+}
+
+constexpr int pow2_floor(int x) { return x <= 1 ? 1 : 2 * pow2_floor(x / 2); }
+
+// largest i in [0, count) with pref[i] <= w (pref[0] <= w), log2(CAP) fixed steps
+template <int IPT, typename PrefT>
+__device__ __forceinline__ int lb_locate(const LbShared<IPT, PrefT>& sh, int count, int64_t w) {
+    constexpr int TOP = pow2_floor(LbShared<IPT, PrefT>::CAP - 1);
+    int pos = 0;
+#pragma unroll
+    for (int step = TOP; step >= 1; step >>= 1) {
+        const int cand = pos + step;
+        const int idx = cand < count ? cand : count - 1;   // always a staged slot
+        pos = (cand < count && (int64_t)sh.pref[idx] <= w) ? cand : pos;
+    }
+    return pos;
+}
+
 // Items w in [w0, min(w0+BLOCK*IPT, total)): item(s, w, segbase(s)) where s is
 // the segment containing w (P exclusive prefix over nseg segments, P[nseg] =
 // total).  Items are striped: item w0 + j*BLOCK + tid.
