@@ -9,24 +9,33 @@
 // may be enqueued more than once; the distances are still exact.
 //
 // Here, per level:
-//   1. a scan of deg(q[i]) read straight from the vertex records; its
-//      down-sweep also writes, per frontier vertex, the adjacency base
-//      (record offset minus prefix, so an edge's address is base + work index)
-//      and, per 2048-edge tile, the frontier index holding the tile's first
-//      edge -- the partition that BinarySearchLB recomputes with two global
-//      binary searches per block (BinarySearchLB.i.cuh:254-264);
-//   2. one expansion kernel: each tile stages its frontier slice in LDS, maps
-//      every edge to (vertex, offset) with an LDS search, tests the neighbour
-//      against the visited bitmap as of the start of the level (n/8 bytes,
-//      read-only during the kernel, so it stays cache resident) and marks an
-//      unvisited neighbour with an idempotent byte store.  No atomics: with
-//      eight XCDs, each holding its own L2, device-scope atomics cannot be
-//      served from cache, and claiming vertices with atomicOr measured 7x
-//      slower;
-//   3. one collect kernel: a lane owns 32 vertices (one bitmap word), turns
-//      their marks into visited bits, distances and next-frontier entries
-//      (one atomicAdd per wave), so every vertex enters a frontier once;
-//   4. one 8-byte readback (next frontier size + this level's edge count).
+//   1. gather: (degree, adjacency offset) of every frontier vertex, one
+//      16-byte record read each;
+//   2. one scan whose values pack (degree << 32 | groups), a group being four
+//      adjacency slots (16 bytes).  Its down-sweep writes the group prefix P
+//      and, per tile, the frontier index holding the tile's first group -- the
+//      partition that BinarySearchLB recomputes with two global binary
+//      searches per block (BinarySearchLB.i.cuh:254-264); its total carries
+//      the level's edge count beside the group count;
+//   3. expand: every edge block is a power of two of at least 32 slots and
+//      128-byte aligned, so a vertex's ceil(deg/4) groups lie inside its block
+//      and each is one aligned 16-byte load (slots past deg in the last group
+//      are masked).  Each tile stages its frontier slice (prefix, offset,
+//      degree) in LDS and maps a group to its vertex with a fixed-step LDS
+//      search.  Every neighbour probes the byte-per-vertex state array
+//      (0 = never reached) and an unreached one is marked with an idempotent
+//      byte store of 1: probe and mark touch the same cache line, and marks
+//      made earlier in the level suppress later ones.  No atomics: with eight
+//      XCDs, each holding its own L2, device-scope atomics are not served from
+//      cache (claiming vertices with atomicOr measured 7x slower, and even
+//      XCD-private mark bitmaps with workgroup-scope atomics measured 1.5x
+//      slower than byte stores);
+//   4. collect: a lane owns 32 vertices (32 state bytes, one word of the
+//      visited bitmap as of the start of the level); a set state byte whose
+//      bitmap bit is clear is a newly reached vertex: it gets its distance, its
+//      bitmap bit and one next-frontier slot (one atomicAdd per wave), so every
+//      vertex enters a frontier once;
+//   5. one 12-byte readback (level totals + next frontier size).
 // Distances: source 0, its neighbours 1, unreached INT32_MAX (TopDown.cu:42,81).
 #include "graph.hpp"
 #include "prims.hpp"
@@ -34,92 +43,44 @@
 namespace hn {
 namespace {
 
-constexpr int EXP_IPT = 8;
-constexpr int EXP_TILE = BLOCK * EXP_IPT;
+constexpr int EXP_G = 4;                       // adjacency slots per work unit
+constexpr int EXP_IPT = 4;                     // work units per lane per tile
+constexpr int EXP_TILE = BLOCK * EXP_IPT;      // work units per tile
 constexpr int32_t INF = 0x7fffffff;
+static_assert(UNIT_SLOTS % EXP_G == 0, "groups must not straddle a block start");
 
-// The frontier is kept as (degree, adjacency offset) pairs written by the
-// collect kernel, so the scan reads contiguous arrays instead of gathering
-// one 16-byte vertex record per frontier vertex twice.
-struct FrontierDegIn {
+struct FrontierIn {
     const int32_t* qdeg;
-    __device__ __forceinline__ uint32_t operator()(int64_t i) const { return (uint32_t)qdeg[i]; }
+    __device__ __forceinline__ uint64_t operator()(int64_t i) const {
+        const uint32_t d = (uint32_t)qdeg[i];
+        return ((uint64_t)d << 32) | ((d + EXP_G - 1) / EXP_G);
+    }
 };
 
-// per frontier vertex i with exclusive edge prefix `ex` and degree d
+// per frontier vertex i with exclusive packed prefix `ex` and value `x`
 struct FrontierOut {
-    const int64_t* qoff;
     uint32_t* P;
-    int64_t* base;
     uint32_t* tile_lo;
-    __device__ __forceinline__ void operator()(int64_t i, uint32_t ex, uint32_t d) const {
-        P[i] = ex;
-        base[i] = qoff[i] - (int64_t)ex;
-        if (d) {
-            uint32_t t0 = (uint32_t)(((uint64_t)ex + EXP_TILE - 1) / EXP_TILE);
-            uint32_t t1 = (uint32_t)(((uint64_t)ex + d + EXP_TILE - 1) / EXP_TILE);
+    __device__ __forceinline__ void operator()(int64_t i, uint64_t ex, uint64_t x) const {
+        const uint32_t g0 = (uint32_t)ex, ng = (uint32_t)x;
+        P[i] = g0;
+        if (ng) {
+            uint32_t t0 = (uint32_t)(((uint64_t)g0 + EXP_TILE - 1) / EXP_TILE);
+            uint32_t t1 = (uint32_t)(((uint64_t)g0 + ng + EXP_TILE - 1) / EXP_TILE);
             for (uint32_t t = t0; t < t1; t++) tile_lo[t] = (uint32_t)i;
         }
     }
 };
 
 __global__ void bfs_init_kernel(int32_t nv, int32_t* __restrict__ dist, uint32_t* __restrict__ bitmap,
-                                int32_t source, int32_t* __restrict__ q) {
+                                uint8_t* __restrict__ state, int32_t source, int32_t* __restrict__ q) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t words = ((int64_t)nv + 31) / 32;
     if (i < nv) dist[i] = (i == source) ? 0 : INF;
     if (i < words) bitmap[i] = (i == (source >> 5)) ? (1u << (source & 31)) : 0u;
-    if (i == 0) q[0] = source;
-}
-
-// Phase 1: every edge of the frontier tests the visited bitmap as of the
-// start of the level (read-only in this kernel, so cacheable) and marks an
-// unvisited neighbour in the byte array `next`.  The mark is an idempotent
-// byte store -- the reference's check-then-set without the queue insert --
-// so no atomics are needed and no update can be lost.
-__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(int64_t F, const uint32_t* __restrict__ P,
-                                                           const int64_t* __restrict__ segbase,
-                                                           const uint32_t* __restrict__ tile_lo,
-                                                           const int32_t* __restrict__ arena,
-                                                           const uint32_t* __restrict__ bitmap,
-                                                           uint8_t* __restrict__ next) {
-    __shared__ LbShared<EXP_IPT, uint32_t> sh;
-    const int64_t W = P[F];
-    const int64_t ntiles = (W + EXP_TILE - 1) / EXP_TILE;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int64_t lo = tile_lo[t];
-        const int64_t hi = (t + 1 < ntiles) ? (int64_t)tile_lo[t + 1] : F - 1;
-        const int64_t w0 = t * EXP_TILE;
-        const int count = lb_stage<EXP_IPT>(P, lo, hi, sh, [&](int64_t s) { return segbase[s]; });
-        // phase 1: every item's adjacency slot (independent LDS searches)
-        int64_t addr[EXP_IPT];
-        if (count > 0) {
-#pragma unroll
-            for (int j = 0; j < EXP_IPT; j++) {
-                const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
-                const int64_t a = sh.base[lb_locate<EXP_IPT>(sh, count, w)] + w;
-                addr[j] = w < W ? a : -1;
-            }
-        } else {   // segment range wider than LDS (zero-degree frontier vertices)
-            for (int j = 0; j < EXP_IPT; j++) {
-                const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
-                addr[j] = w < W ? segbase[upper_bound_m1<int64_t>(P, lo, lo + (int64_t)(-count), w)] + w : -1;
-            }
-        }
-        // phase 2: all neighbour ids in flight at once (streamed, not kept in L2)
-        int32_t v[EXP_IPT];
-#pragma unroll
-        for (int j = 0; j < EXP_IPT; j++) v[j] = addr[j] >= 0 ? __builtin_nontemporal_load(arena + addr[j]) : -1;
-        // phase 3: all visited words in flight at once
-        uint32_t word[EXP_IPT];
-#pragma unroll
-        for (int j = 0; j < EXP_IPT; j++) word[j] = v[j] >= 0 ? bitmap[v[j] >> 5] : 0xFFFFFFFFu;
-        // phase 4: idempotent marks (testing the mark first cut L2 write-backs
-        // 10x but added a dependent read and measured slower)
-#pragma unroll
-        for (int j = 0; j < EXP_IPT; j++)
-            if (!(word[j] & (1u << (v[j] & 31)))) next[v[j]] = 1;
-        __syncthreads();
+    if (i == 0) {
+        q[0] = source;
+        state[source] = 1;
     }
 }
 
@@ -133,35 +94,124 @@ __global__ void bfs_gather_kernel(int64_t F, const int32_t* __restrict__ q, cons
     qoff[i] = r.off;
 }
 
-// Phase 2: one lane owns 32 vertices (one bitmap word and 32 mark bytes):
-// marked vertices get their distance, their visited bit and a slot in the
-// next frontier (one atomicAdd per wave); marks are cleared for the next level.
-__global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, uint32_t* __restrict__ bitmap,
-                                                            uint32_t* __restrict__ next4,
+struct ExpShared {
+    LbShared<EXP_IPT, uint32_t> lb;   // pref = group prefix, base = adjacency offset
+    int32_t deg[LbShared<EXP_IPT, uint32_t>::CAP];
+};
+
+// Every neighbour of the frontier probes state[] and marks the unreached
+// ones; state[] is only ever set to 1 here, so concurrent stores agree.
+template <int G>
+__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __restrict__ tot,
+                                                           const uint32_t* __restrict__ P,
+                                                           const uint32_t* __restrict__ tile_lo,
+                                                           const int32_t* __restrict__ qdeg,
+                                                           const int64_t* __restrict__ qoff, int64_t F,
+                                                           const int32_t* __restrict__ arena,
+                                                           uint8_t* __restrict__ state) {
+    typedef int32_t Vec __attribute__((ext_vector_type(G)));
+    constexpr int CAP = LbShared<EXP_IPT, uint32_t>::CAP;
+    __shared__ ExpShared sh;
+    const int64_t W = (uint32_t)tot[0];
+    const int64_t ntiles = (W + EXP_TILE - 1) / EXP_TILE;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t lo = tile_lo[t];
+        const int64_t hi = (t + 1 < ntiles) ? (int64_t)tile_lo[t + 1] : F - 1;
+        const int64_t w0 = t * EXP_TILE;
+        const int64_t span = hi - lo + 1;
+        const bool staged = span <= CAP;
+        if (staged) {
+            for (int i = threadIdx.x; i < (int)span; i += BLOCK) {
+                sh.lb.pref[i] = P[lo + i];
+                sh.lb.base[i] = qoff[lo + i];
+                sh.deg[i] = qdeg[lo + i];
+            }
+        }
+        __syncthreads();
+        // phase 1: every unit's slot address and valid slot count
+        int64_t addr[EXP_IPT];
+        int nval[EXP_IPT];
+        if (staged) {
+#pragma unroll
+            for (int j = 0; j < EXP_IPT; j++) {
+                const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+                const int s = lb_locate<EXP_IPT>(sh.lb, (int)span, w);
+                const int64_t k = (w - (int64_t)sh.lb.pref[s]) * G;
+                addr[j] = sh.lb.base[s] + k;
+                nval[j] = w < W ? (int)min((int64_t)G, (int64_t)sh.deg[s] - k) : 0;
+            }
+        } else {   // slice wider than LDS (runs of zero-degree frontier vertices)
+            for (int j = 0; j < EXP_IPT; j++) {
+                const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+                addr[j] = 0;
+                nval[j] = 0;
+                if (w < W) {
+                    const int64_t s = upper_bound_m1<int64_t>(P, lo, hi + 1, w);
+                    const int64_t k = (w - (int64_t)P[s]) * G;
+                    addr[j] = qoff[s] + k;
+                    nval[j] = (int)min((int64_t)G, (int64_t)qdeg[s] - k);
+                }
+            }
+        }
+        // Phases 2 and 3 load unconditionally (a unit without work reads
+        // arena[0..G), a masked slot probes state[0]) so that no load sits
+        // in a branch of its own, where the compiler waits for it at once.
+        // phase 2: all adjacency groups in flight at once (streamed, not kept in L2)
+        Vec v[EXP_IPT];
+#pragma unroll
+        for (int j = 0; j < EXP_IPT; j++)
+            v[j] = __builtin_nontemporal_load(reinterpret_cast<const Vec*>(arena + (nval[j] > 0 ? addr[j] : 0)));
+        // phase 3: all state probes in flight at once, folded branch-free
+        // into one bit per slot that needs a mark
+        uint8_t s[EXP_IPT][G];
+#pragma unroll
+        for (int j = 0; j < EXP_IPT; j++)
+#pragma unroll
+            for (int e = 0; e < G; e++) s[j][e] = state[e < nval[j] ? v[j][e] : 0];
+        uint32_t need = 0;
+#pragma unroll
+        for (int j = 0; j < EXP_IPT; j++)
+#pragma unroll
+            for (int e = 0; e < G; e++) need |= (uint32_t)((e < nval[j]) & (s[j][e] == 0)) << (j * G + e);
+        // phase 4: idempotent marks
+#pragma unroll
+        for (int j = 0; j < EXP_IPT; j++)
+#pragma unroll
+            for (int e = 0; e < G; e++)
+                if (need & (1u << (j * G + e))) state[v[j][e]] = 1;
+        __syncthreads();
+    }
+}
+
+// One lane owns 32 vertices (32 state bytes and one visited-bitmap word):
+// set state bytes not yet in the bitmap are this level's discoveries; they
+// get their distance, their bitmap bit and a slot in the next frontier (one
+// atomicAdd per wave).
+__global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, const uint32_t* __restrict__ state4,
+                                                            uint32_t* __restrict__ bitmap,
                                                             int32_t* __restrict__ dist, int32_t level,
                                                             int32_t* __restrict__ qn,
                                                             uint32_t* __restrict__ qn_count) {
     const int64_t wi = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     uint32_t found = 0;
     if (wi < words) {
-        const uint4* p = reinterpret_cast<const uint4*>(next4 + wi * 8);
-        uint4 a = p[0], b = p[1];
-        uint32_t m[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        const uint4* p = reinterpret_cast<const uint4*>(state4 + wi * 8);
+        const uint4 a = p[0], b = p[1];
+        const uint32_t m[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        uint32_t set = 0;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            uint32_t x = m[k];
+            const uint32_t x = m[k];
             // byte j of word k is vertex 32*wi + 4k + j
-            found |= ((x & 0xFFu) ? 1u : 0u) << (4 * k);
-            found |= ((x & 0xFF00u) ? 1u : 0u) << (4 * k + 1);
-            found |= ((x & 0xFF0000u) ? 1u : 0u) << (4 * k + 2);
-            found |= ((x & 0xFF000000u) ? 1u : 0u) << (4 * k + 3);
+            set |= ((x & 0xFFu) ? 1u : 0u) << (4 * k);
+            set |= ((x & 0xFF00u) ? 1u : 0u) << (4 * k + 1);
+            set |= ((x & 0xFF0000u) ? 1u : 0u) << (4 * k + 2);
+            set |= ((x & 0xFF000000u) ? 1u : 0u) << (4 * k + 3);
         }
-        if (found) {
-            uint4 z = make_uint4(0, 0, 0, 0);
-            uint4* pw = reinterpret_cast<uint4*>(next4 + wi * 8);
-            pw[0] = z;
-            pw[1] = z;
-            bitmap[wi] |= found;
+        if (set) {
+            const uint32_t old = bitmap[wi];
+            found = set & ~old;
+            if (found) bitmap[wi] = old | found;
         }
     }
     const uint32_t cnt = (uint32_t)__popc(found);
@@ -184,33 +234,33 @@ __global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, uint3
 
 // scratch layout inside g->s_bfs
 // The frontier needs no ping-pong: q is gathered into (qdeg, qoff) and
-// scanned into P / base / tile_lo before collect overwrites q with level L+1.
+// scanned into P / tile_lo before collect overwrites q with level L+1.
 struct BfsBufs {
-    uint32_t* bitmap;
+    uint32_t* bitmap;   // visited as of the start of the level
     int32_t* q;
     int32_t* qdeg;
     int64_t* qoff;
     uint32_t* P;
-    int64_t* base;
     uint32_t* tile_lo;
-    uint8_t* next;   // one mark byte per vertex, padded to whole bitmap words
-    uint32_t* cnt;   // [0] next frontier size
+    uint8_t* state;     // one byte per vertex, padded to whole bitmap words
+    uint64_t* tot;      // [0] level totals (edges << 32 | groups)
+    uint32_t* cnt;      // [0] next frontier size
 };
 
 static hn_status bfs_buffers(hn_graph* g, BfsBufs* b) {
     const int64_t nv = g->nv > 0 ? g->nv : 1;
     const int64_t words = (nv + 31) / 32;
-    const int64_t tiles = g->ne / EXP_TILE + 2;
+    const int64_t tiles = (g->ne / EXP_G + nv) / EXP_TILE + 2;
     auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
     int64_t o_bitmap = 0;
     int64_t o_q = al(o_bitmap + words * 4);
     int64_t o_qdeg = al(o_q + nv * 4);
     int64_t o_qoff = al(o_qdeg + nv * 4);
     int64_t o_P = al(o_qoff + nv * 8);
-    int64_t o_base = al(o_P + (nv + 1) * 4);
-    int64_t o_tile = al(o_base + nv * 8);
-    int64_t o_next = al(o_tile + tiles * 4);
-    int64_t o_cnt = al(o_next + words * 32);
+    int64_t o_tile = al(o_P + (nv + 1) * 4);
+    int64_t o_state = al(o_tile + tiles * 4);
+    int64_t o_tot = al(o_state + words * 32);
+    int64_t o_cnt = al(o_tot + 8);
     int64_t total = al(o_cnt + 64);
     HN_TRY(g->s_bfs.reserve((size_t)total));
     char* base = g->s_bfs.as<char>();
@@ -219,9 +269,9 @@ static hn_status bfs_buffers(hn_graph* g, BfsBufs* b) {
     b->qdeg = reinterpret_cast<int32_t*>(base + o_qdeg);
     b->qoff = reinterpret_cast<int64_t*>(base + o_qoff);
     b->P = reinterpret_cast<uint32_t*>(base + o_P);
-    b->base = reinterpret_cast<int64_t*>(base + o_base);
     b->tile_lo = reinterpret_cast<uint32_t*>(base + o_tile);
-    b->next = reinterpret_cast<uint8_t*>(base + o_next);
+    b->state = reinterpret_cast<uint8_t*>(base + o_state);
+    b->tot = reinterpret_cast<uint64_t*>(base + o_tot);
     b->cnt = reinterpret_cast<uint32_t*>(base + o_cnt);
     HN_TRY(g->s_scan.reserve(scan_temp_bytes(nv)));
     return HN_OK;
@@ -241,10 +291,10 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
     HN_TRY(bfs_buffers(g, &b));
     const int64_t nv = g->nv;
     const int64_t words = (nv + 31) / 32;
-    bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(g->nv, dist, b.bitmap, source, b.q);
+    HN_HIP(hipMemsetAsync(b.state, 0, (size_t)words * 32, st));
+    bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(g->nv, dist, b.bitmap, b.state, source, b.q);
     HN_CHECK_LAUNCH();
-    HN_HIP(hipMemsetAsync(b.next, 0, (size_t)words * 32, st));
-    const int64_t grid_cap = (int64_t)num_cus() * 8;
+    const int64_t grid_cap = (int64_t)num_cus() * 64;
     const bool timed = stats && stats->time_kernels;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (timed) {
@@ -255,30 +305,30 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
     int32_t level = 1;
     int64_t visited = 0, scanned = 0, kns = 0;
     int32_t nlevels = 0;
-    uint32_t* host = reinterpret_cast<uint32_t*>(g->h_pinned);
+    uint64_t* host = g->h_pinned;
     hn_status rc = HN_OK;
     while (F > 0) {
         visited += F;
         nlevels++;
         bfs_gather_kernel<<<(unsigned)ceil_div(F, BLOCK), BLOCK, 0, st>>>(F, b.q, g->rec, b.qdeg, b.qoff);
-        rc = exclusive_scan_fn<uint32_t>(FrontierDegIn{b.qdeg}, FrontierOut{b.qoff, b.P, b.base, b.tile_lo},
-                                         b.P + F, F, g->s_scan.ptr, st);
+        rc = exclusive_scan_fn<uint64_t>(FrontierIn{b.qdeg}, FrontierOut{b.P, b.tile_lo}, b.tot, F,
+                                         g->s_scan.ptr, st);
         if (rc != HN_OK) break;
         hipError_t e = hipMemsetAsync(b.cnt, 0, 4, st);
         if (timed && e == hipSuccess) e = hipEventRecord(ev0, st);
         if (e == hipSuccess) {
-            bfs_expand_kernel<<<(unsigned)grid_cap, BLOCK, 0, st>>>(F, b.P, b.base, b.tile_lo, g->arena,
-                                                                      b.bitmap, b.next);
+            bfs_expand_kernel<EXP_G><<<(unsigned)grid_cap, BLOCK, 0, st>>>(b.tot, b.P, b.tile_lo, b.qdeg, b.qoff,
+                                                                          F, g->arena, b.state);
             e = hipGetLastError();
         }
         if (timed && e == hipSuccess) e = hipEventRecord(ev1, st);
         if (e == hipSuccess) {
             bfs_collect_kernel<<<(unsigned)ceil_div(words, BLOCK), BLOCK, 0, st>>>(
-                words, b.bitmap, reinterpret_cast<uint32_t*>(b.next), dist, level, b.q, b.cnt);
+                words, reinterpret_cast<const uint32_t*>(b.state), b.bitmap, dist, level, b.q, b.cnt);
             e = hipGetLastError();
         }
-        if (e == hipSuccess) e = hipMemcpyAsync(host, b.cnt, 4, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(host + 1, b.P + F, 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, b.tot, 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(host + 1, b.cnt, 4, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (timed && e == hipSuccess) {
             float ms = 0.f;
@@ -290,8 +340,8 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
             rc = HN_E_HIP;
             break;
         }
-        scanned += host[1];
-        F = host[0];
+        scanned += (int64_t)(host[0] >> 32);
+        F = (int64_t)(uint32_t)host[1];
         level++;
     }
     if (timed) {
