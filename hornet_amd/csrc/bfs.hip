@@ -22,10 +22,12 @@
 //      and each is one aligned 16-byte load (slots past deg in the last group
 //      are masked).  Each tile stages its frontier slice (prefix, offset,
 //      degree) in LDS and maps a group to its vertex with a fixed-step LDS
-//      search.  Every neighbour probes the byte-per-vertex state array
-//      (0 = never reached) and an unreached one is marked with an idempotent
-//      byte store of 1: probe and mark touch the same cache line, and marks
-//      made earlier in the level suppress later ones.  No atomics: with eight
+//      search.  Every neighbour first probes the visited bitmap as of the
+//      start of the level (n/8 bytes, read-only here, so L2 resident); the
+//      survivors probe the byte-per-vertex state array (0 = never reached)
+//      and an unreached one is marked with an idempotent byte store of 1:
+//      probe and mark touch the same cache line, and marks made earlier in
+//      the level suppress later ones.  No atomics: with eight
 //      XCDs, each holding its own L2, device-scope atomics are not served from
 //      cache (claiming vertices with atomicOr measured 7x slower, and even
 //      XCD-private mark bitmaps with workgroup-scope atomics measured 1.5x
@@ -33,8 +35,8 @@
 //   4. collect: a lane owns 32 vertices (32 state bytes, one word of the
 //      visited bitmap as of the start of the level); a set state byte whose
 //      bitmap bit is clear is a newly reached vertex: it gets its distance, its
-//      bitmap bit and one next-frontier slot (one atomicAdd per wave), so every
-//      vertex enters a frontier once;
+//      bitmap bit and one next-frontier slot (one atomicAdd per workgroup),
+//      so every vertex enters a frontier once;
 //   5. one 12-byte readback (level totals + next frontier size).
 // Distances: source 0, its neighbours 1, unreached INT32_MAX (TopDown.cu:42,81).
 #include "graph.hpp"
@@ -99,8 +101,9 @@ struct ExpShared {
     int32_t deg[LbShared<EXP_IPT, uint32_t>::CAP];
 };
 
-// Every neighbour of the frontier probes state[] and marks the unreached
-// ones; state[] is only ever set to 1 here, so concurrent stores agree.
+// Every neighbour of the frontier is tested against the visited bitmap and,
+// if not visited before this level, against state[]; unreached ones are
+// marked.  state[] is only ever set to 1 here, so concurrent stores agree.
 template <int G>
 __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __restrict__ tot,
                                                            const uint32_t* __restrict__ P,
@@ -108,7 +111,8 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
                                                            const int32_t* __restrict__ qdeg,
                                                            const int64_t* __restrict__ qoff, int64_t F,
                                                            const int32_t* __restrict__ arena,
-                                                           uint8_t* __restrict__ state) {
+                                                           uint8_t* __restrict__ state,
+                                                           const uint32_t* __restrict__ bitmap) {
     typedef int32_t Vec __attribute__((ext_vector_type(G)));
     constexpr int CAP = LbShared<EXP_IPT, uint32_t>::CAP;
     __shared__ ExpShared sh;
@@ -161,18 +165,32 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
 #pragma unroll
         for (int j = 0; j < EXP_IPT; j++)
             v[j] = __builtin_nontemporal_load(reinterpret_cast<const Vec*>(arena + (nval[j] > 0 ? addr[j] : 0)));
-        // phase 3: all state probes in flight at once, folded branch-free
-        // into one bit per slot that needs a mark
+        // phase 3a: all visited-bitmap probes in flight at once (n/8 bytes,
+        // L2 resident), folded branch-free into one bit per candidate slot
+        uint32_t wb[EXP_IPT][G];
+#pragma unroll
+        for (int j = 0; j < EXP_IPT; j++)
+#pragma unroll
+            for (int e = 0; e < G; e++) wb[j][e] = bitmap[(e < nval[j] ? v[j][e] : 0) >> 5];
+        uint32_t cand = 0;
+#pragma unroll
+        for (int j = 0; j < EXP_IPT; j++)
+#pragma unroll
+            for (int e = 0; e < G; e++)
+                cand |= (uint32_t)((e < nval[j]) & (((wb[j][e] >> (v[j][e] & 31)) & 1u) == 0)) << (j * G + e);
+        // phase 3b: candidates (not visited before this level) probe their
+        // state byte; the rest probe state[0], one shared line
         uint8_t s[EXP_IPT][G];
 #pragma unroll
         for (int j = 0; j < EXP_IPT; j++)
 #pragma unroll
-            for (int e = 0; e < G; e++) s[j][e] = state[e < nval[j] ? v[j][e] : 0];
+            for (int e = 0; e < G; e++) s[j][e] = state[((cand >> (j * G + e)) & 1u) ? v[j][e] : 0];
         uint32_t need = 0;
 #pragma unroll
         for (int j = 0; j < EXP_IPT; j++)
 #pragma unroll
-            for (int e = 0; e < G; e++) need |= (uint32_t)((e < nval[j]) & (s[j][e] == 0)) << (j * G + e);
+            for (int e = 0; e < G; e++)
+                need |= (uint32_t)(((cand >> (j * G + e)) & 1u) & (s[j][e] == 0)) << (j * G + e);
         // phase 4: idempotent marks
 #pragma unroll
         for (int j = 0; j < EXP_IPT; j++)
@@ -185,8 +203,7 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
 
 // One lane owns 32 vertices (32 state bytes and one visited-bitmap word):
 // set state bytes not yet in the bitmap are this level's discoveries; they
-// get their distance, their bitmap bit and a slot in the next frontier (one
-// atomicAdd per wave).
+// get their distance, their bitmap bit and a slot in the next frontier.
 __global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, const uint32_t* __restrict__ state4,
                                                             uint32_t* __restrict__ bitmap,
                                                             int32_t* __restrict__ dist, int32_t level,
@@ -214,19 +231,42 @@ __global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, const
             if (found) bitmap[wi] = old | found;
         }
     }
+    // next-frontier slots: one atomicAdd per workgroup (the counter is a
+    // single device-scope address, so fewer of them is what matters)
+    __shared__ uint32_t wsum[NWAVES + 1];
     const uint32_t cnt = (uint32_t)__popc(found);
     const uint32_t inc = wave_inclusive_scan(cnt);
-    const uint32_t tot = __shfl(inc, WAVE - 1, WAVE);
-    if (tot == 0) return;
-    uint32_t b = 0;
-    if (__lane_id() == WAVE - 1) b = atomicAdd(qn_count, tot);
-    b = __shfl(b, WAVE - 1, WAVE) + inc - cnt;
-    while (found) {
-        const int j = __ffs(found) - 1;
-        found &= found - 1;
-        const int32_t v = (int32_t)(wi * 32 + j);
-        dist[v] = level;
-        qn[b++] = v;
+    if (__lane_id() == WAVE - 1) wsum[threadIdx.x / WAVE] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int k = 0; k < NWAVES; k++) {
+            const uint32_t x = wsum[k];
+            wsum[k] = acc;
+            acc += x;
+        }
+        wsum[NWAVES] = acc ? atomicAdd(qn_count, acc) : 0u;
+    }
+    __syncthreads();
+    if (__shfl(inc, WAVE - 1, WAVE) == 0) return;   // nothing found by this wave
+    const uint32_t b = wsum[NWAVES] + wsum[threadIdx.x / WAVE] + inc - cnt;
+    // Writes, transposed so that consecutive lanes write consecutive vertices:
+    // in round r lane l handles bit (l & 31) of the wave's word 2r + (l >> 5),
+    // fetching that word's found bits and first queue slot from its owner.
+    const int lane = __lane_id();
+    const int64_t wbase = wi - lane;
+    const int bit = lane & 31;
+    const uint32_t below = (1u << bit) - 1u;
+#pragma unroll 4
+    for (int r = 0; r < WAVE / 2; r++) {
+        const int owner = 2 * r + (lane >> 5);
+        const uint32_t fw = __shfl(found, owner, WAVE);
+        const uint32_t bo = __shfl(b, owner, WAVE);
+        if ((fw >> bit) & 1u) {
+            const int32_t v = (int32_t)((wbase + owner) * 32 + bit);
+            dist[v] = level;
+            qn[bo + (uint32_t)__popc(fw & below)] = v;
+        }
     }
 }
 
@@ -318,7 +358,7 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
         if (timed && e == hipSuccess) e = hipEventRecord(ev0, st);
         if (e == hipSuccess) {
             bfs_expand_kernel<EXP_G><<<(unsigned)grid_cap, BLOCK, 0, st>>>(b.tot, b.P, b.tile_lo, b.qdeg, b.qoff,
-                                                                          F, g->arena, b.state);
+                                                                          F, g->arena, b.state, b.bitmap);
             e = hipGetLastError();
         }
         if (timed && e == hipSuccess) e = hipEventRecord(ev1, st);
