@@ -119,6 +119,7 @@ SIGNATURES = {
     "hn_server_drain": [P],
     "hn_server_results": [P, ctypes.POINTER(HnResult), I64, PI64],
     "hn_server_slots": [P, PI32],
+    "hn_server_slot_bytes": [P, I32, PI64],
     "hn_server_destroy": [P],
     "hn_gen_chung_lu_pairs": [I32, D, I64, I64, U64, P, P, P],
     "hn_gen_chung_lu_graph": [I32, I32, D, U64, D, P, PP],

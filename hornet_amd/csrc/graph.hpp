@@ -127,6 +127,6 @@ hn_status ms_engine_create(hn_graph* g, MsEngine** out);
 void ms_engine_destroy(MsEngine* e);
 hn_status ms_run_engine(MsEngine* e, const int32_t* sources, int32_t k, const int32_t* depth_limit, int32_t* dist,
                         hn_msbfs_stats* stats, hipStream_t caller);
-// device bytes one engine holds for a graph of nv vertices
-inline int64_t ms_engine_bytes(int64_t nv) { return nv * 24 + 4096; }
+// device bytes one engine holds for a graph of nv vertices and ne edges
+int64_t ms_engine_bytes(int64_t nv, int64_t ne);
 }  // namespace hn

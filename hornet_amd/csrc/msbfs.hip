@@ -4,20 +4,41 @@
 // full traversal of the graph per query.  Here up to 64 sources share one
 // traversal: every vertex holds three 64-bit masks (visited, frontier, next),
 // bit b standing for source b, so one adjacency read serves every source
-// whose frontier touches it.  Per level, a one-thread decide kernel picks the
-// direction from counters the previous level left on the device:
-//   push  (sparse frontier): one wave per frontier vertex ORs its frontier
-//         mask into each neighbour's `next` with a 64-bit atomicOr;
-//   pull  (dense frontier): each vertex still missing some source ORs the
-//         frontier masks of its neighbours, stopping once every missing bit is
-//         found -- no atomics.  Vertices with degree <= T take one lane,
-//         heavier ones one wave (T picked by the autotuner).
-// A collect kernel turns `next` into visited/frontier bits and per-source
-// distances (source-major, int32, INT32_MAX unreached) and leaves the
-// counters for the next decision.  The whole level step is captured once into
-// a hipGraph and replayed; the host reads a 4-byte "done" flag every few
-// levels.  Sources may carry a depth limit (the serving loop's 2-hop queries
-// use 2): their bits are dropped from the frontier once the limit is reached.
+// whose frontier touches it.
+//
+// One level step, captured once into a hipGraph and replayed until the
+// frontiers empty (the host reads a 4-byte "done" flag every few levels):
+//   decide   one workgroup reduces the previous collect's per-block counters
+//            and picks the direction: push while frontier degree * alpha <
+//            degree of the vertices still missing a source, else pull;
+//   list     a scan of per-word popcounts of the frontier (push) or
+//            light-open (pull) bitset and an emit kernel turn the bitset
+//            into a compact work list carrying (vertex, degree, adjacency
+//            offset, mask) -- writes transposed so lanes write consecutive
+//            entries;
+//   scan     a scan of the list's 16-byte adjacency groups whose element
+//            count is read on the device (so it stays inside the graph),
+//            writing the group prefix and the tile partition;
+//   expand   edge-granular, the single-source BFS's load balancing: a lane
+//            loads four adjacency slots at once (every block is a power of
+//            two of >= 32 slots, 128-byte aligned) and probes the four
+//            neighbours' masks together.
+//              push: add = frontier(u) & ~visited(w), ORed into next(w) with
+//                    a 64-bit atomicOr when non-zero;
+//              pull: an open vertex u ORs its neighbours' frontier masks
+//                    (restricted to the bits it misses); a tile reduces each
+//                    vertex's bits with LDS atomics and stores them, with a
+//                    global atomicOr only for the two vertices it shares with
+//                    neighbouring tiles;
+//   heavy    pull only: open vertices of degree > T take one wave each and
+//            stop as soon as every missing bit is found (T is tuned);
+//   collect  one lane per vertex: new bits -> visited, frontier (masked by
+//            the per-source depth limits) and per-source distances (source
+//            major, int32, INT32_MAX unreached; written bit by bit so lanes
+//            store consecutive words), the three bitsets of the next level
+//            (frontier, light-open, heavy-open) from ballots, and per-block
+//            counters.  No atomics.
+// Sources may carry a depth limit (the serving loop's 2-hop queries use 2).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -29,12 +50,21 @@ namespace hn {
 namespace {
 
 constexpr int32_t INF = 0x7fffffff;
+constexpr int MS_G = 4;                          // adjacency slots per work unit (16 bytes)
+constexpr int MS_IPT = 4;                        // work units per lane per tile
+constexpr int MS_TILE = BLOCK * MS_IPT;          // work units per tile
+constexpr int MS_CAP = LbShared<MS_IPT, uint32_t>::CAP;
+constexpr int MS_COLLECT_PER_CU = 8;             // collect workgroups per CU
+static_assert(UNIT_SLOTS % MS_G == 0, "groups must not straddle a block start");
 
 struct MsState {
     int32_t level;                  // level being built (sources are level 0)
     int32_t mode;                   // 0 push, 1 pull
     int32_t done;
     int32_t levels_run;
+    uint32_t wn;                    // entries in this level's work list
+    uint32_t pad;
+    uint64_t tot;                   // group scan total: edges << 32 | groups
     unsigned long long active;      // vertices with a non-empty frontier
     unsigned long long active_deg;  // sum of their degrees
     unsigned long long open_deg;    // sum of degrees of vertices still missing a bit
@@ -46,72 +76,115 @@ struct MsState {
 };
 
 struct MsParams {
-    int32_t T;          // pull: degree threshold lane / wave
+    int32_t T;          // pull: open vertices of degree > T take one wave each
     int32_t alpha;      // push while active_deg * alpha < open_deg
     int32_t depth_limit[64];
 };
 
-// counters for level 0 (the sources) and clearing
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) v |= __shfl_xor(v, d, WAVE);
+    return v;
+}
+
+// bits that can still be found while expanding the frontier of `level`
+__device__ __forceinline__ uint64_t live_bits(const MsState* S, int level) {
+    return S->all & S->depth_mask[level < 63 ? level : 63];
+}
+
 __global__ void ms_init_kernel(int64_t nv, uint64_t* __restrict__ visited, uint64_t* __restrict__ frontier,
-                               uint64_t* __restrict__ next) {
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x) {
+                               uint64_t* __restrict__ next, int64_t nbits, uint32_t* __restrict__ bits) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride) {
         visited[v] = 0;
         frontier[v] = 0;
         next[v] = 0;
     }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbits; i += stride) bits[i] = 0;
 }
 
 __global__ void ms_fill_dist_kernel(int64_t total, int32_t* __restrict__ dist) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
-        dist[i] = INF;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n4 = ((uintptr_t)dist & 15) == 0 ? total / 4 : 0;
+    int4* d4 = reinterpret_cast<int4*>(dist);
+    const int4 inf4 = make_int4(INF, INF, INF, INF);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) d4[i] = inf4;
+    for (int64_t i = n4 * 4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) dist[i] = INF;
 }
 
-__global__ void ms_seed_kernel(int32_t k, const int32_t* __restrict__ src, int64_t nv, uint64_t* __restrict__ visited,
-                               uint64_t* __restrict__ frontier, int32_t* __restrict__ dist, const hn_vrec* __restrict__ rec,
-                               MsState* S, MsParams P) {
-    // single thread: sources may repeat, so OR serially
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint64_t all = 0;
-    unsigned long long edges = 0;
-    for (int b = 0; b < k; b++) {
+// one wave: lane b seeds source b; lane L builds the depth mask of level L
+__global__ __launch_bounds__(WAVE) void ms_seed_kernel(int32_t k, const int32_t* __restrict__ src, int64_t nv,
+                                                       uint64_t* __restrict__ visited, uint64_t* __restrict__ frontier,
+                                                       uint32_t* __restrict__ fbits, int32_t* __restrict__ dist,
+                                                       const hn_vrec* __restrict__ rec, MsState* S, MsParams P) {
+    const int b = threadIdx.x;
+    unsigned long long deg = 0, first = 0;
+    if (b < k) {
         const int32_t s = src[b];
-        visited[s] |= 1ull << b;
-        frontier[s] |= 1ull << b;
+        atomicOr(reinterpret_cast<unsigned long long*>(visited + s), 1ull << b);
+        atomicOr(reinterpret_cast<unsigned long long*>(frontier + s), 1ull << b);
+        atomicOr(fbits + (s >> 5), 1u << (s & 31));
         dist[(int64_t)b * nv + s] = 0;
-        all |= 1ull << b;
-        edges += (unsigned long long)rec[s].deg;
-    }
-    S->all = all;
-    S->level = 0;
-    S->done = 0;
-    S->levels_run = 0;
-    S->edges = edges;
-    S->push_levels = S->pull_levels = 0;
-    // distinct frontier vertices and their degrees; open degree taken as large
-    unsigned long long act = 0, adeg = 0;
-    for (int b = 0; b < k; b++) {
-        bool first = true;
+        deg = (unsigned long long)rec[s].deg;
+        first = 1;
         for (int c = 0; c < b; c++)
-            if (src[c] == src[b]) first = false;
-        if (first) {
-            act++;
-            adeg += (unsigned long long)rec[src[b]].deg;
+            if (src[c] == s) first = 0;
+    }
+    uint64_t m = 0;
+    for (int c = 0; c < k; c++)
+        if (P.depth_limit[c] <= 0 || b < P.depth_limit[c]) m |= 1ull << c;
+    S->depth_mask[b] = m;
+    const unsigned long long edges = wave_reduce_sum(deg);
+    const unsigned long long act = wave_reduce_sum(first);
+    const unsigned long long adeg = wave_reduce_sum(first * deg);
+    const uint64_t all = (uint64_t)__ballot(b < k);
+    if (b == 0) {
+        S->all = all;
+        S->level = 0;
+        S->mode = 0;
+        S->done = 0;
+        S->levels_run = 0;
+        S->wn = 0;
+        S->tot = 0;
+        S->edges = edges;
+        S->push_levels = S->pull_levels = 0;
+        S->active = act;
+        S->active_deg = adeg;
+        S->open_deg = ~0ull >> 1;   // the first expansion is always a push
+    }
+}
+
+// one workgroup: counters of the previous collect -> direction of this level
+__global__ __launch_bounds__(BLOCK) void ms_decide_kernel(MsState* S, MsParams P,
+                                                          const unsigned long long* __restrict__ part,
+                                                          int32_t nparts) {
+    // every thread reads the state before thread 0 changes it
+    const int32_t done = S->done, level = S->level;
+    __syncthreads();
+    if (done) return;
+    __shared__ unsigned long long red[4][NWAVES];
+    if (level > 0) {
+        unsigned long long c[4] = {0, 0, 0, 0};
+        for (int i = threadIdx.x; i < nparts; i += BLOCK)
+#pragma unroll
+            for (int j = 0; j < 4; j++) c[j] += part[(int64_t)i * 4 + j];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            c[j] = wave_reduce_sum(c[j]);
+            if (__lane_id() == 0) red[j][threadIdx.x / WAVE] = c[j];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long t[4] = {0, 0, 0, 0};
+            for (int j = 0; j < 4; j++)
+                for (int w = 0; w < NWAVES; w++) t[j] += red[j][w];
+            S->active = t[0];
+            S->active_deg = t[1];
+            S->open_deg = t[2];
+            S->edges += t[3];
         }
     }
-    S->active = act;
-    S->active_deg = adeg;
-    S->open_deg = ~0ull >> 1;
-    for (int L = 0; L < 64; L++) {
-        uint64_t m = 0;
-        for (int b = 0; b < k; b++)
-            if (P.depth_limit[b] <= 0 || L < P.depth_limit[b]) m |= 1ull << b;
-        S->depth_mask[L] = m;
-    }
-}
-
-// one thread: pick the direction of the next level or stop
-__global__ void ms_decide_kernel(MsState* S, MsParams P) {
-    if (S->done) return;
+    if (threadIdx.x != 0) return;
     if (S->active == 0) {
         S->done = 1;
         return;
@@ -121,170 +194,352 @@ __global__ void ms_decide_kernel(MsState* S, MsParams P) {
     const bool push = (unsigned long long)S->active_deg * (unsigned long long)P.alpha < S->open_deg;
     S->mode = push ? 0 : 1;
     if (push) S->push_levels++; else S->pull_levels++;
-    S->active = 0;
-    S->active_deg = 0;
-    S->open_deg = 0;
 }
 
-__device__ __forceinline__ uint64_t frontier_of(const uint64_t* frontier, const MsState* S, int64_t v) {
-    const int L = S->level - 1;   // frontier holds level L vertices
-    return frontier[v] & S->depth_mask[L < 63 ? L : 63];
-}
+// popcount of the word of this level's list bitset
+struct ListWordIn {
+    const MsState* S;
+    const uint32_t* fbits;
+    const uint32_t* obits;
+    __device__ __forceinline__ uint32_t operator()(int64_t w) const {
+        if (S->done) return 0u;
+        return (uint32_t)__popc(S->mode == 0 ? fbits[w] : obits[w]);
+    }
+};
 
-// push: one wave per vertex with a non-empty (depth-limited) frontier
-__global__ __launch_bounds__(BLOCK) void ms_push_kernel(int64_t nv, const hn_vrec* __restrict__ rec,
-                                                        const int32_t* __restrict__ arena,
-                                                        const uint64_t* __restrict__ frontier,
+// bitset -> work list (vertex, degree, offset, mask); lane per bitset word,
+// writes transposed so that consecutive lanes write consecutive entries
+__global__ __launch_bounds__(BLOCK) void ms_emit_kernel(int64_t words, const MsState* S,
+                                                        const uint32_t* __restrict__ fbits,
+                                                        const uint32_t* __restrict__ obits,
+                                                        const uint32_t* __restrict__ wpre,
+                                                        const hn_vrec* __restrict__ rec,
                                                         const uint64_t* __restrict__ visited,
-                                                        unsigned long long* __restrict__ next, const MsState* S) {
-    if (S->done || S->mode != 0) return;
-    const int64_t wid = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
-    const int64_t nw = (int64_t)gridDim.x * BLOCK / WAVE;
-    const int l = __lane_id();
-    for (int64_t v = wid; v < nv; v += nw) {
-        const uint64_t f = frontier_of(frontier, S, v);
-        if (!f) continue;
-        const hn_vrec r = rec[v];
-        for (int64_t j = l; j < r.deg; j += WAVE) {
-            const int32_t u = arena[r.off + j];
-            const uint64_t add = f & ~visited[u];
-            if (add) atomicOr(next + u, (unsigned long long)add);
+                                                        const uint64_t* __restrict__ frontier,
+                                                        int32_t* __restrict__ list, int32_t* __restrict__ qdeg,
+                                                        int64_t* __restrict__ qoff, uint64_t* __restrict__ qmask) {
+    if (S->done) return;
+    const bool push = S->mode == 0;
+    const uint64_t live = live_bits(S, S->level - 1);
+    const int64_t wi = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t found = wi < words ? (push ? fbits[wi] : obits[wi]) : 0u;
+    if (__ballot(found != 0) == 0) return;
+    const uint32_t b = wi < words ? wpre[wi] : 0u;
+    const int lane = __lane_id();
+    const int64_t wbase = wi - lane;
+    const int bit = lane & 31;
+    const uint32_t below = (1u << bit) - 1u;
+    for (int r = 0; r < WAVE / 2; r++) {
+        const int owner = 2 * r + (lane >> 5);
+        const uint32_t fw = __shfl(found, owner, WAVE);
+        if (__ballot(fw != 0) == 0) continue;
+        const uint32_t bo = __shfl(b, owner, WAVE);
+        if ((fw >> bit) & 1u) {
+            const int32_t v = (int32_t)((wbase + owner) * 32 + bit);
+            const uint32_t pos = bo + (uint32_t)__popc(fw & below);
+            const hn_vrec rv = rec[v];
+            const uint64_t m = push ? frontier[v] : (live & ~visited[v]);
+            list[pos] = v;
+            qdeg[pos] = m ? rv.deg : 0;
+            qoff[pos] = rv.off;
+            qmask[pos] = m;
         }
     }
 }
 
-// pull, light vertices: one lane per vertex
-__global__ __launch_bounds__(BLOCK) void ms_pull_light_kernel(int64_t nv, const hn_vrec* __restrict__ rec,
-                                                              const int32_t* __restrict__ arena,
-                                                              const uint64_t* __restrict__ frontier,
-                                                              const uint64_t* __restrict__ visited,
-                                                              uint64_t* __restrict__ next, const MsState* S,
-                                                              MsParams P) {
-    if (S->done || S->mode != 1) return;
-    const uint64_t all = S->all;
-    const int L = S->level - 1;
-    const uint64_t dm = S->depth_mask[L < 63 ? L : 63];
-    for (int64_t u = (int64_t)blockIdx.x * BLOCK + threadIdx.x; u < nv; u += (int64_t)gridDim.x * BLOCK) {
-        const uint64_t miss = all & ~visited[u];
-        if (!miss) continue;
-        const hn_vrec r = rec[u];
-        if (r.deg > P.T) continue;
-        uint64_t acc = 0;
-        for (int32_t j = 0; j < r.deg; j++) {
-            acc |= frontier[arena[r.off + j]] & dm;
-            if ((acc & miss) == miss) break;
-        }
-        acc &= miss;
-        if (acc) next[u] = acc;
+struct GroupIn {
+    const int32_t* qdeg;
+    __device__ __forceinline__ uint64_t operator()(int64_t i) const {
+        const uint32_t d = (uint32_t)qdeg[i];
+        return ((uint64_t)d << 32) | ((d + MS_G - 1) / MS_G);
     }
-}
+};
 
-// pull, heavy vertices: one wave per vertex, 64 neighbours per step
-__global__ __launch_bounds__(BLOCK) void ms_pull_heavy_kernel(int64_t nv, const hn_vrec* __restrict__ rec,
-                                                              const int32_t* __restrict__ arena,
-                                                              const uint64_t* __restrict__ frontier,
-                                                              const uint64_t* __restrict__ visited,
-                                                              uint64_t* __restrict__ next, const MsState* S,
-                                                              MsParams P, const int32_t* __restrict__ heavy,
-                                                              int64_t nheavy) {
-    if (S->done || S->mode != 1) return;
-    const uint64_t all = S->all;
-    const int L = S->level - 1;
-    const uint64_t dm = S->depth_mask[L < 63 ? L : 63];
-    const int64_t wid = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
-    const int64_t nw = (int64_t)gridDim.x * BLOCK / WAVE;
-    const int l = __lane_id();
-    for (int64_t i = wid; i < nheavy; i += nw) {
-        const int32_t u = heavy[i];
-        const uint64_t miss = all & ~visited[u];
-        if (!miss) continue;
-        const hn_vrec r = rec[u];
-        uint64_t acc = 0;
-        for (int64_t j0 = 0; j0 < r.deg; j0 += WAVE) {
-            const int64_t j = j0 + l;
-            uint64_t f = j < r.deg ? (frontier[arena[r.off + j]] & dm) : 0ull;
+struct GroupOut {
+    uint32_t* P;
+    uint32_t* tile_lo;
+    __device__ __forceinline__ void operator()(int64_t i, uint64_t ex, uint64_t x) const {
+        const uint32_t g0 = (uint32_t)ex, ng = (uint32_t)x;
+        P[i] = g0;
+        if (ng) {
+            const uint32_t t0 = (uint32_t)(((uint64_t)g0 + MS_TILE - 1) / MS_TILE);
+            const uint32_t t1 = (uint32_t)(((uint64_t)g0 + ng + MS_TILE - 1) / MS_TILE);
+            for (uint32_t t = t0; t < t1; t++) tile_lo[t] = (uint32_t)i;
+        }
+    }
+};
+
+struct MsShared {
+    LbShared<MS_IPT, uint32_t> lb;   // pref = group prefix, base = adjacency offset
+    int32_t deg[MS_CAP];
+    uint64_t mask[MS_CAP];           // push: frontier mask; pull: missing bits
+    unsigned long long acc[MS_CAP];  // pull: bits found this tile
+};
+
+// Edge-granular expansion of this level's work list (see the file comment).
+template <bool PULL>
+__global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, const uint32_t* __restrict__ P,
+                                                          const uint32_t* __restrict__ tile_lo,
+                                                          const int32_t* __restrict__ qdeg,
+                                                          const int64_t* __restrict__ qoff,
+                                                          const uint64_t* __restrict__ qmask,
+                                                          const int32_t* __restrict__ list,
+                                                          const int32_t* __restrict__ arena,
+                                                          const uint64_t* __restrict__ probe,
+                                                          unsigned long long* __restrict__ next) {
+    if (S->done || S->mode != (PULL ? 1 : 0)) return;
+    typedef int32_t Vec __attribute__((ext_vector_type(MS_G)));
+    __shared__ MsShared sh;
+    const int64_t F = S->wn;
+    const int64_t W = (uint32_t)S->tot;
+    const int64_t ntiles = (W + MS_TILE - 1) / MS_TILE;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t lo = tile_lo[t];
+        const int64_t hi = (t + 1 < ntiles) ? (int64_t)tile_lo[t + 1] : F - 1;
+        const int64_t w0 = t * MS_TILE;
+        const int64_t span = hi - lo + 1;
+        const bool staged = span <= MS_CAP;
+        if (staged) {
+            for (int i = threadIdx.x; i < (int)span; i += BLOCK) {
+                sh.lb.pref[i] = P[lo + i];
+                sh.lb.base[i] = qoff[lo + i];
+                sh.deg[i] = qdeg[lo + i];
+                sh.mask[i] = qmask[lo + i];
+                if (PULL) sh.acc[i] = 0;
+            }
+        }
+        __syncthreads();
+        // phase 1: every unit's segment, slot address, valid count and mask
+        int64_t addr[MS_IPT];
+        int nval[MS_IPT];
+        int64_t seg[MS_IPT];
+        uint64_t msk[MS_IPT];
+        if (staged) {
 #pragma unroll
-            for (int d = WAVE / 2; d > 0; d >>= 1) f |= __shfl_xor(f, d, WAVE);
-            acc |= f;
-            if ((acc & miss) == miss) break;
+            for (int j = 0; j < MS_IPT; j++) {
+                const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+                const int s = lb_locate<MS_IPT>(sh.lb, (int)span, w);
+                const int64_t k = (w - (int64_t)sh.lb.pref[s]) * MS_G;
+                addr[j] = sh.lb.base[s] + k;
+                nval[j] = w < W ? (int)min((int64_t)MS_G, (int64_t)sh.deg[s] - k) : 0;
+                seg[j] = s;
+                msk[j] = sh.mask[s];
+            }
+        } else {   // slice wider than LDS (runs of work-free list entries)
+            for (int j = 0; j < MS_IPT; j++) {
+                const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+                addr[j] = 0;
+                nval[j] = 0;
+                seg[j] = lo;
+                msk[j] = 0;
+                if (w < W) {
+                    const int64_t s = upper_bound_m1<int64_t>(P, lo, hi + 1, w);
+                    const int64_t k = (w - (int64_t)P[s]) * MS_G;
+                    addr[j] = qoff[s] + k;
+                    nval[j] = (int)min((int64_t)MS_G, (int64_t)qdeg[s] - k);
+                    seg[j] = s;
+                    msk[j] = qmask[s];
+                }
+            }
         }
-        acc &= miss;
-        if (l == 0 && acc) next[u] = acc;
+        // phase 2: adjacency groups, all in flight (a unit without work reads arena[0..4))
+        Vec v[MS_IPT];
+#pragma unroll
+        for (int j = 0; j < MS_IPT; j++)
+            v[j] = __builtin_nontemporal_load(reinterpret_cast<const Vec*>(arena + (nval[j] > 0 ? addr[j] : 0)));
+        // phase 3: neighbour masks, all in flight (masked slots read probe[0])
+        uint64_t pr[MS_IPT][MS_G];
+#pragma unroll
+        for (int j = 0; j < MS_IPT; j++)
+#pragma unroll
+            for (int e = 0; e < MS_G; e++) pr[j][e] = probe[e < nval[j] ? v[j][e] : 0];
+        if (!PULL) {
+            // push: bits of the frontier vertex its neighbour has not seen
+#pragma unroll
+            for (int j = 0; j < MS_IPT; j++)
+#pragma unroll
+                for (int e = 0; e < MS_G; e++) {
+                    const uint64_t add = msk[j] & ~pr[j][e];
+                    if (e < nval[j] && add) atomicOr(next + v[j][e], (unsigned long long)add);
+                }
+        } else {
+            // pull: bits the open vertex misses, found in its neighbours' frontiers
+#pragma unroll
+            for (int j = 0; j < MS_IPT; j++) {
+                uint64_t a = 0;
+#pragma unroll
+                for (int e = 0; e < MS_G; e++) a |= e < nval[j] ? pr[j][e] : 0ull;
+                a &= msk[j];
+                if (a) {
+                    if (staged) atomicOr(&sh.acc[seg[j]], (unsigned long long)a);
+                    else atomicOr(next + list[seg[j]], (unsigned long long)a);
+                }
+            }
+            __syncthreads();
+            if (staged) {
+                for (int i = threadIdx.x; i < (int)span; i += BLOCK) {
+                    const unsigned long long a = sh.acc[i];
+                    if (!a) continue;
+                    const int32_t u = list[lo + i];
+                    // the first and last vertex may share their groups with
+                    // neighbouring tiles; the others belong to this tile only
+                    if (i == 0 || i == (int)span - 1) atomicOr(next + u, a);
+                    else next[u] = a;
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
-// collect: new bits -> visited, frontier, distances; counters for the next decision
-__global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, int32_t k, const hn_vrec* __restrict__ rec,
+// pull, heavy open vertices (degree > T): one wave each, 256 slots per step,
+// stopping once every missing bit is found
+__global__ __launch_bounds__(BLOCK) void ms_pull_heavy_kernel(int64_t words, const MsState* S,
+                                                              const uint32_t* __restrict__ hbits,
+                                                              const hn_vrec* __restrict__ rec,
+                                                              const int32_t* __restrict__ arena,
+                                                              const uint64_t* __restrict__ frontier,
+                                                              const uint64_t* __restrict__ visited,
+                                                              uint64_t* __restrict__ next) {
+    if (S->done || S->mode != 1) return;
+    typedef int32_t Vec __attribute__((ext_vector_type(MS_G)));
+    const uint64_t live = live_bits(S, S->level - 1);
+    const int64_t wid = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
+    const int64_t nw = (int64_t)gridDim.x * BLOCK / WAVE;
+    const int l = __lane_id();
+    for (int64_t wd = wid; wd < words; wd += nw) {
+        uint32_t hw = hbits[wd];
+        while (hw) {
+            const int j = __ffs(hw) - 1;
+            hw &= hw - 1;
+            const int32_t u = (int32_t)(wd * 32 + j);
+            const uint64_t miss = live & ~visited[u];
+            if (!miss) continue;
+            const hn_vrec r = rec[u];
+            uint64_t acc = 0;
+            for (int64_t j0 = 0; j0 < r.deg; j0 += (int64_t)WAVE * MS_G) {
+                const int64_t k = j0 + (int64_t)l * MS_G;
+                const int n = (int)min((int64_t)MS_G, (int64_t)r.deg - k);
+                const Vec nb = __builtin_nontemporal_load(reinterpret_cast<const Vec*>(arena + (n > 0 ? r.off + k : 0)));
+                uint64_t f = 0;
+#pragma unroll
+                for (int e = 0; e < MS_G; e++) {
+                    const uint64_t x = frontier[e < n ? nb[e] : 0];
+                    f |= e < n ? x : 0ull;
+                }
+                acc |= wave_or64(f & miss);
+                if ((acc & miss) == miss) break;
+            }
+            if (l == 0 && acc) next[u] = acc;
+        }
+    }
+}
+
+// collect: new bits -> visited, frontier, distances; next level's bitsets and
+// this block's counters (active, active degree, open degree, edges)
+__global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const hn_vrec* __restrict__ rec,
                                                            uint64_t* __restrict__ visited,
                                                            uint64_t* __restrict__ frontier,
                                                            uint64_t* __restrict__ next, int32_t* __restrict__ dist,
-                                                           MsState* S) {
+                                                           const MsState* S, int32_t T, uint32_t* __restrict__ fbits,
+                                                           uint32_t* __restrict__ obits, uint32_t* __restrict__ hbits,
+                                                           unsigned long long* __restrict__ part) {
     if (S->done) return;
-    const uint64_t all = S->all;
     const int32_t level = S->level;
+    const uint64_t dmn = S->depth_mask[level < 63 ? level : 63];
+    const uint64_t live = S->all & dmn;
+    const int lane = __lane_id();
     unsigned long long act = 0, adeg = 0, odeg = 0, edges = 0;
-    for (int64_t u = (int64_t)blockIdx.x * BLOCK + threadIdx.x; u < nv; u += (int64_t)gridDim.x * BLOCK) {
-        const uint64_t vis = visited[u];
-        const uint64_t nb = next[u] & ~vis;
-        uint64_t f = 0;
-        if (nb) {
-            next[u] = 0;
-            visited[u] = vis | nb;
-            const int32_t deg = rec[u].deg;
-            act += 1;
-            adeg += (unsigned long long)deg;
-            edges += (unsigned long long)deg * (unsigned long long)__popcll(nb);
-            uint64_t m = nb;
-            while (m) {
-                const int b = __ffsll((unsigned long long)m) - 1;
-                m &= m - 1;
-                dist[(int64_t)b * nv + u] = level;
-            }
-            f = nb;
-            if ((all & ~(vis | nb)) != 0) odeg += (unsigned long long)deg;
-        } else if (next[u]) {
-            next[u] = 0;
-            if ((all & ~vis) != 0) odeg += (unsigned long long)rec[u].deg;
-        } else if ((all & ~vis) != 0) {
-            odeg += (unsigned long long)rec[u].deg;
+    for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < nv; base += (int64_t)gridDim.x * BLOCK) {
+        const int64_t u = base + threadIdx.x;
+        const bool valid = u < nv;
+        uint64_t vis = 0, nx = 0, fo = 0;
+        if (valid) {
+            vis = visited[u];
+            nx = next[u];
+            fo = frontier[u];
         }
-        frontier[u] = f;
+        const uint64_t nb = nx & ~vis;
+        if (nx) next[u] = 0;
+        const uint64_t visn = vis | nb;
+        if (nb) visited[u] = visn;
+        const uint64_t f = nb & dmn;
+        if (f != fo) frontier[u] = f;
+        const uint64_t miss = valid ? (live & ~visn) : 0ull;
+        const int32_t deg = (f | miss | nb) ? rec[u].deg : 0;
+        const bool inF = f != 0;
+        const bool inO = miss != 0 && deg > 0 && deg <= T;
+        const bool inH = miss != 0 && deg > T;
+        act += inF ? 1ull : 0ull;
+        adeg += inF ? (unsigned long long)deg : 0ull;
+        odeg += miss ? (unsigned long long)deg : 0ull;
+        edges += (unsigned long long)deg * (unsigned long long)__popcll(nb);
+        // distances, one source bit at a time so that lanes store consecutive words
+        uint64_t M = wave_or64(nb);
+        while (M) {
+            const int b = __ffsll((unsigned long long)M) - 1;
+            M &= M - 1;
+            if ((nb >> b) & 1ull) dist[(int64_t)b * nv + u] = level;
+        }
+        // next level's bitsets: one 64-bit ballot = two bitset words
+        const uint64_t bf = (uint64_t)__ballot(inF), bo = (uint64_t)__ballot(inO), bh = (uint64_t)__ballot(inH);
+        const int64_t wv = base + (threadIdx.x & ~(WAVE - 1));   // first vertex of this wave
+        if (lane == 0 && wv < nv) {
+            const int64_t wq = wv / 64;                            // 64-vertex word pair
+            reinterpret_cast<uint64_t*>(fbits)[wq] = bf;
+            reinterpret_cast<uint64_t*>(obits)[wq] = bo;
+            reinterpret_cast<uint64_t*>(hbits)[wq] = bh;
+        }
     }
-    act = wave_reduce_sum(act);
-    adeg = wave_reduce_sum(adeg);
-    odeg = wave_reduce_sum(odeg);
-    edges = wave_reduce_sum(edges);
-    if (__lane_id() == 0) {
-        if (act) atomicAdd(&S->active, act);
-        if (adeg) atomicAdd(&S->active_deg, adeg);
-        if (odeg) atomicAdd(&S->open_deg, odeg);
-        if (edges) atomicAdd(&S->edges, edges);
+    __shared__ unsigned long long red[4][NWAVES];
+    unsigned long long c[4] = {act, adeg, odeg, edges};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        c[j] = wave_reduce_sum(c[j]);
+        if (lane == 0) red[j][threadIdx.x / WAVE] = c[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        unsigned long long t = 0;
+        for (int w = 0; w < NWAVES; w++) t += red[threadIdx.x][w];
+        part[(int64_t)blockIdx.x * 4 + threadIdx.x] = t;
     }
 }
 
-__global__ void heavy_flag_kernel(int64_t nv, const hn_vrec* __restrict__ rec, int32_t T, uint32_t* __restrict__ flag) {
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < nv) flag[v] = rec[v].deg > T ? 1u : 0u;
-}
-__global__ void heavy_write_kernel(int64_t nv, const uint32_t* __restrict__ pos, int32_t* __restrict__ heavy) {
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < nv && pos[v + 1] != pos[v]) heavy[pos[v]] = (int32_t)v;
+// engine buffer sizes
+int64_t ms_words(int64_t nv) { return (nv + 31) / 32; }
+int64_t ms_bit_words(int64_t nv) { return ((nv + 63) / 64) * 2; }   // whole 64-vertex pairs
+int64_t ms_tiles(int64_t nv, int64_t ne) { return (ne / MS_G + nv) / MS_TILE + 2; }
+int64_t ms_collect_grid(int64_t nv) {
+    return std::max<int64_t>(1, std::min<int64_t>(ceil_div(nv, BLOCK), (int64_t)num_cus() * MS_COLLECT_PER_CU));
 }
 
 }  // namespace
 
-// Per-graph multi-source engine state (lives in the graph handle's s_misc-free storage)
+// Per-engine device state: masks, bitsets, work list and load-balancing
+// buffers, own stream and pinned readback buffer, captured level graph.
 struct MsEngine {
     hn_graph* g = nullptr;
+    char* mem = nullptr;              // one allocation for everything below
+    size_t mem_bytes = 0;
     uint64_t* visited = nullptr;
     uint64_t* frontier = nullptr;
     uint64_t* next = nullptr;
-    int32_t* heavy = nullptr;
-    int64_t nheavy = 0;
-    int32_t heavy_T = -1;
-    uint64_t heavy_version = 0;       // graph version the heavy list was built for
+    uint32_t* fbits = nullptr;
+    uint32_t* obits = nullptr;
+    uint32_t* hbits = nullptr;
+    uint32_t* wpre = nullptr;
+    int32_t* list = nullptr;
+    int32_t* qdeg = nullptr;
+    int64_t* qoff = nullptr;
+    uint64_t* qmask = nullptr;
+    uint32_t* P = nullptr;
+    uint32_t* tile_lo = nullptr;
+    unsigned long long* part = nullptr;
+    void* scan_tmp = nullptr;
     MsState* S = nullptr;
     int32_t* d_src = nullptr;
+    int64_t nv = 0, tile_cap = 0, nparts = 0;
     hipGraphExec_t exec = nullptr;
     hipStream_t exec_stream = nullptr;
     int32_t* exec_dist = nullptr;
@@ -293,36 +548,112 @@ struct MsEngine {
     hipStream_t own = nullptr;        // non-blocking stream the level graph is captured on
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     int32_t* pinned = nullptr;        // engine-private readback buffer
-    void release() {
-        if (pinned) (void)hipHostFree(pinned);
-        pinned = nullptr;
+    void release_mem() {
         if (exec) (void)hipGraphExecDestroy(exec);
         exec = nullptr;
+        dfree(mem);
+        mem = nullptr;
+        mem_bytes = 0;
+    }
+    void release() {
+        release_mem();
+        if (pinned) (void)hipHostFree(pinned);
+        pinned = nullptr;
         if (own) (void)hipStreamDestroy(own);
         if (ev_in) (void)hipEventDestroy(ev_in);
         if (ev_out) (void)hipEventDestroy(ev_out);
         own = nullptr;
         ev_in = ev_out = nullptr;
-        dfree(visited); dfree(frontier); dfree(next); dfree(heavy); dfree(S); dfree(d_src);
-        visited = frontier = next = nullptr;
-        heavy = nullptr;
-        S = nullptr;
-        d_src = nullptr;
     }
 };
 
-// a standalone engine: masks, state, its own stream and pinned readback
+namespace {
+
+// Lay out (and with `base` set, assign) the engine buffers; returns bytes.
+size_t ms_layout(int64_t nv, int64_t tiles, int64_t nparts, MsEngine* e, char* base) {
+    const int64_t n = nv > 0 ? nv : 1;
+    const int64_t words = ms_words(n), bw = ms_bit_words(n);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = (off + bytes + 255) & ~size_t(255);
+        return base ? base + at : nullptr;
+    };
+    char* p_vis = take((size_t)n * 8);
+    char* p_fr = take((size_t)n * 8);
+    char* p_nx = take((size_t)n * 8);
+    char* p_fb = take((size_t)bw * 4 * 3);   // frontier, light-open, heavy-open: contiguous
+    char* p_wp = take((size_t)words * 4);
+    char* p_li = take((size_t)n * 4);
+    char* p_qd = take((size_t)n * 4);
+    char* p_qo = take((size_t)n * 8);
+    char* p_qm = take((size_t)n * 8);
+    char* p_P = take((size_t)(n + 1) * 4);
+    char* p_tl = take((size_t)tiles * 4);
+    char* p_pa = take((size_t)nparts * 4 * 8);
+    char* p_st = take(std::max(scan_temp_bytes(n), scan_temp_bytes(words)));
+    char* p_S = take(sizeof(MsState));
+    char* p_src = take(64 * 4);
+    if (base) {
+        e->visited = reinterpret_cast<uint64_t*>(p_vis);
+        e->frontier = reinterpret_cast<uint64_t*>(p_fr);
+        e->next = reinterpret_cast<uint64_t*>(p_nx);
+        e->fbits = reinterpret_cast<uint32_t*>(p_fb);
+        e->obits = e->fbits + bw;
+        e->hbits = e->fbits + 2 * bw;
+        e->wpre = reinterpret_cast<uint32_t*>(p_wp);
+        e->list = reinterpret_cast<int32_t*>(p_li);
+        e->qdeg = reinterpret_cast<int32_t*>(p_qd);
+        e->qoff = reinterpret_cast<int64_t*>(p_qo);
+        e->qmask = reinterpret_cast<uint64_t*>(p_qm);
+        e->P = reinterpret_cast<uint32_t*>(p_P);
+        e->tile_lo = reinterpret_cast<uint32_t*>(p_tl);
+        e->part = reinterpret_cast<unsigned long long*>(p_pa);
+        e->scan_tmp = p_st;
+        e->S = reinterpret_cast<MsState*>(p_S);
+        e->d_src = reinterpret_cast<int32_t*>(p_src);
+    }
+    return off;
+}
+
+// tile capacity with room for edge growth, so updates rarely force a resize
+int64_t ms_tile_cap(int64_t nv, int64_t ne) { return ms_tiles(nv, ne + ne / 4 + 4096); }
+
+// (re)allocate the buffers when the graph outgrew them
+hn_status ms_reserve(MsEngine* e) {
+    hn_graph* g = e->g;
+    const int64_t need = ms_tiles(g->nv, g->ne);
+    if (e->mem && e->nv == g->nv && e->tile_cap >= need) return HN_OK;
+    e->release_mem();
+    const int64_t cap = ms_tile_cap(g->nv, g->ne);
+    const int64_t nparts = ms_collect_grid(g->nv);
+    const size_t bytes = ms_layout(g->nv, cap, nparts, e, nullptr);
+    char* mem = nullptr;
+    HN_TRY(dmalloc(reinterpret_cast<void**>(&mem), bytes));
+    e->mem = mem;
+    e->mem_bytes = bytes;
+    e->nv = g->nv;
+    e->tile_cap = cap;
+    e->nparts = nparts;
+    ms_layout(g->nv, cap, nparts, e, mem);
+    return HN_OK;
+}
+
+}  // namespace
+
+int64_t ms_engine_bytes(int64_t nv, int64_t ne) {
+    return (int64_t)ms_layout(nv, ms_tile_cap(nv, ne), ms_collect_grid(nv), nullptr, nullptr) + 4096;
+}
+
+// a standalone engine: buffers, state, its own stream and pinned readback
 // buffer, so several engines may run on one graph from different threads
 hn_status ms_engine_create(hn_graph* g, MsEngine** out) {
     MsEngine* e = new MsEngine();
     e->g = g;
-    e->params.T = 32;
+    e->params.T = 256;
     e->params.alpha = 16;
-    const size_t nv = (size_t)(g->nv > 0 ? g->nv : 1);
-    hn_status s;
-    if ((s = dalloc(&e->visited, nv)) != HN_OK || (s = dalloc(&e->frontier, nv)) != HN_OK ||
-        (s = dalloc(&e->next, nv)) != HN_OK || (s = dalloc(&e->S, 1)) != HN_OK ||
-        (s = dalloc(&e->d_src, 64)) != HN_OK) {
+    hn_status s = ms_reserve(e);
+    if (s != HN_OK) {
         e->release();
         delete e;
         return s;
@@ -367,57 +698,34 @@ static hn_status ms_read(MsEngine* e, const void* dptr, size_t bytes, void* host
     return HN_OK;
 }
 
-static hn_status ms_heavy_list(MsEngine* e, hipStream_t st) {
-    hn_graph* g = e->g;
-    // degrees change with every update: rebuild when the graph moved on
-    if (e->heavy_T == e->params.T && e->heavy && e->heavy_version == g->version) return HN_OK;
-    const int64_t nv = g->nv;
-    uint32_t* flag = nullptr;
-    uint32_t* pos = nullptr;
-    void* tmp = nullptr;
-    hn_status s;
-    if ((s = dalloc(&flag, (size_t)nv + 1)) != HN_OK) return s;
-    if ((s = dalloc(&pos, (size_t)nv + 1)) != HN_OK) { dfree(flag); return s; }
-    if ((s = dmalloc(&tmp, scan_temp_bytes(nv))) != HN_OK) { dfree(flag); dfree(pos); return s; }
-    if (nv > 0) heavy_flag_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(nv, g->rec, e->params.T, flag);
-    s = exclusive_scan<uint32_t>(ArrayIn<uint32_t>{flag}, pos, nv, tmp, st);
-    uint32_t cnt = 0;
-    if (s == HN_OK) s = ms_read(e, pos + nv, 4, &cnt, st);
-    if (s == HN_OK) {
-        dfree(e->heavy);
-        e->heavy = nullptr;
-        s = dalloc(&e->heavy, (size_t)(cnt > 0 ? cnt : 1));
-    }
-    if (s == HN_OK && nv > 0) {
-        heavy_write_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(nv, pos, e->heavy);
-        if (hipStreamSynchronize(st) != hipSuccess) s = HN_E_HIP;
-    }
-    dfree(flag);
-    dfree(pos);
-    dfree(tmp);
-    if (s == HN_OK) {
-        e->nheavy = cnt;
-        e->heavy_T = e->params.T;
-        e->heavy_version = g->version;
-        if (e->exec) (void)hipGraphExecDestroy(e->exec);
-        e->exec = nullptr;
-    }
-    return s;
-}
-
-// record one level step on `st` (also used for capture)
-static void ms_level(MsEngine* e, int32_t k, int32_t* dist, const MsParams& P, hipStream_t st) {
+// record one level step on `st` (also used for capture); returns the first launch error
+static hipError_t ms_level(MsEngine* e, int32_t* dist, const MsParams& P, hipStream_t st) {
     hn_graph* g = e->g;
     const int64_t nv = g->nv;
-    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(nv, BLOCK), (int64_t)num_cus() * 8);
+    const int64_t words = ms_words(nv);
     const unsigned wgrid = (unsigned)((int64_t)num_cus() * 8);
-    ms_decide_kernel<<<1, 1, 0, st>>>(e->S, P);
-    ms_push_kernel<<<wgrid, BLOCK, 0, st>>>(nv, g->rec, g->arena, e->frontier, e->visited,
-                                            reinterpret_cast<unsigned long long*>(e->next), e->S);
-    ms_pull_light_kernel<<<grid, BLOCK, 0, st>>>(nv, g->rec, g->arena, e->frontier, e->visited, e->next, e->S, P);
-    ms_pull_heavy_kernel<<<wgrid, BLOCK, 0, st>>>(nv, g->rec, g->arena, e->frontier, e->visited, e->next, e->S, P,
-                                                  e->heavy, e->nheavy);
-    ms_collect_kernel<<<grid, BLOCK, 0, st>>>(nv, k, g->rec, e->visited, e->frontier, e->next, dist, e->S);
+    const unsigned xgrid = (unsigned)((int64_t)num_cus() * 16);
+    ms_decide_kernel<<<1, BLOCK, 0, st>>>(e->S, P, e->part, (int32_t)e->nparts);
+    hipError_t err = exclusive_scan_dev(ListWordIn{e->S, e->fbits, e->obits}, ArrayOut<uint32_t>{e->wpre}, &e->S->wn,
+                                        words, e->scan_tmp, st);
+    if (err != hipSuccess) return err;
+    ms_emit_kernel<<<(unsigned)ceil_div(words, BLOCK), BLOCK, 0, st>>>(words, e->S, e->fbits, e->obits, e->wpre,
+                                                                       g->rec, e->visited, e->frontier, e->list,
+                                                                       e->qdeg, e->qoff, e->qmask);
+    err = exclusive_scan_dev_count<uint64_t>(GroupIn{e->qdeg}, GroupOut{e->P, e->tile_lo}, &e->S->tot, &e->S->wn, nv,
+                                             e->scan_tmp, st);
+    if (err != hipSuccess) return err;
+    ms_expand_kernel<false><<<xgrid, BLOCK, 0, st>>>(e->S, e->P, e->tile_lo, e->qdeg, e->qoff, e->qmask, e->list,
+                                                     g->arena, e->visited,
+                                                     reinterpret_cast<unsigned long long*>(e->next));
+    ms_expand_kernel<true><<<xgrid, BLOCK, 0, st>>>(e->S, e->P, e->tile_lo, e->qdeg, e->qoff, e->qmask, e->list,
+                                                    g->arena, e->frontier,
+                                                    reinterpret_cast<unsigned long long*>(e->next));
+    ms_pull_heavy_kernel<<<wgrid, BLOCK, 0, st>>>(words, e->S, e->hbits, g->rec, g->arena, e->frontier, e->visited,
+                                                  e->next);
+    ms_collect_kernel<<<(unsigned)e->nparts, BLOCK, 0, st>>>(nv, g->rec, e->visited, e->frontier, e->next, dist, e->S,
+                                                             P.T, e->fbits, e->obits, e->hbits, e->part);
+    return hipGetLastError();
 }
 
 static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const int32_t* depth_limit,
@@ -445,18 +753,19 @@ hn_status ms_run_engine(MsEngine* e, const int32_t* sources, int32_t k, const in
 static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const int32_t* depth_limit,
                            int32_t* dist, hn_msbfs_stats* stats, hipStream_t st) {
     hn_graph* g = e->g;
-    HN_TRY(ms_heavy_list(e, st));
+    HN_ARG(g->ne < (int64_t)0xFFFFFFFF, "edge count exceeds the 32-bit work prefix");
+    HN_TRY(ms_reserve(e));
     MsParams P = e->params;
     for (int b = 0; b < 64; b++) P.depth_limit[b] = (depth_limit && b < k) ? depth_limit[b] : 0;
     const int64_t nv = g->nv;
     const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(nv, BLOCK), (int64_t)num_cus() * 8);
     HN_HIP(hipMemcpyAsync(e->d_src, sources, sizeof(int32_t) * (size_t)k, hipMemcpyHostToDevice, st));
-    ms_init_kernel<<<grid, BLOCK, 0, st>>>(nv, e->visited, e->frontier, e->next);
-    ms_fill_dist_kernel<<<(unsigned)std::min<int64_t>(ceil_div((int64_t)k * nv, BLOCK), (int64_t)num_cus() * 16),
+    ms_init_kernel<<<grid, BLOCK, 0, st>>>(nv, e->visited, e->frontier, e->next, 3 * ms_bit_words(nv), e->fbits);
+    ms_fill_dist_kernel<<<(unsigned)std::min<int64_t>(ceil_div((int64_t)k * nv, 4 * BLOCK), (int64_t)num_cus() * 16),
                           BLOCK, 0, st>>>((int64_t)k * nv, dist);
-    ms_seed_kernel<<<1, 1, 0, st>>>(k, e->d_src, nv, e->visited, e->frontier, dist, g->rec, e->S, P);
+    ms_seed_kernel<<<1, WAVE, 0, st>>>(k, e->d_src, nv, e->visited, e->frontier, e->fbits, dist, g->rec, e->S, P);
     HN_CHECK_LAUNCH();
-    // capture the level step once per (stream, dist buffer, parameters, k)
+    // capture the level step once per (stream, dist buffer, parameters, buffers)
     const bool same = e->exec && e->exec_stream == st && e->exec_dist == dist &&
                       std::memcmp(&e->exec_params, &P, sizeof(P)) == 0;
     if (!same) {
@@ -464,10 +773,11 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
         e->exec = nullptr;
         hipGraph_t graph = nullptr;
         HN_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        ms_level(e, k, dist, P, st);
+        const hipError_t le = ms_level(e, dist, P, st);
         hipError_t ce = hipStreamEndCapture(st, &graph);
-        if (ce != hipSuccess) {
-            set_error(std::string("level-step capture: ") + hipGetErrorString(ce));
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            set_error(std::string("level-step capture: ") + hipGetErrorString(le != hipSuccess ? le : ce));
             return HN_E_HIP;
         }
         hipError_t ie = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
@@ -545,7 +855,7 @@ hn_status hn_bfs_multi_tune(hn_graph* g, const int32_t* sources, int32_t k, int3
     hipStream_t st = as_stream(stream);
     MsEngine* e = nullptr;
     HN_TRY(ms_engine(g, &e));
-    const int32_t Ts[] = {8, 16, 32, 64, 128, 256};
+    const int32_t Ts[] = {64, 256, 1024, 4096};
     const int32_t As[] = {4, 16, 64};
     hipEvent_t a = nullptr, b = nullptr;
     HN_HIP(hipEventCreate(&a));

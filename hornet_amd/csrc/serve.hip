@@ -242,6 +242,13 @@ static void dispatcher_loop(hn_server* s) {
     }
 }
 
+// device bytes one launch slot holds: an engine plus k * nv distances and
+// the digests
+static int64_t slot_bytes(const hn_graph* g, int64_t batch) {
+    const int64_t nv = g->nv > 0 ? g->nv : 1;
+    return ms_engine_bytes(nv, g->ne) + batch * nv * 4 + 64 * 24;
+}
+
 static void destroy_server(hn_server* s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -268,6 +275,13 @@ using namespace hn;
 
 extern "C" {
 
+hn_status hn_server_slot_bytes(const hn_graph* g, int32_t max_batch, int64_t* bytes) {
+    HN_ARG(g && bytes, "null argument");
+    HN_ARG(max_batch >= 1 && max_batch <= 64, "max_batch must be in [1, 64]");
+    *bytes = slot_bytes(g, max_batch);
+    return HN_OK;
+}
+
 hn_status hn_server_create(hn_graph* g, const hn_serve_config* cfg, hn_server** out) {
     HN_ARG(g && cfg && out, "null argument");
     *out = nullptr;
@@ -278,7 +292,7 @@ hn_status hn_server_create(hn_graph* g, const hn_serve_config* cfg, hn_server** 
     s->cfg = *cfg;
     const int64_t nv = g->nv > 0 ? g->nv : 1;
     const int64_t batch = cfg->plain ? 1 : cfg->max_batch;
-    s->slot_bytes = ms_engine_bytes(nv) + batch * nv * 4 + 64 * 24;
+    s->slot_bytes = slot_bytes(g, batch);
     int64_t budget = cfg->mem_budget_bytes;
     if (budget <= 0) {
         size_t fr = 0, tot = 0;

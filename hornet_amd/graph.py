@@ -398,6 +398,13 @@ class Server:
         return v.value
 
     @staticmethod
+    def slot_bytes(hornet: "Hornet", max_batch: int = 64) -> int:
+        """Device bytes one launch slot takes from the memory budget."""
+        v = ctypes.c_int64()
+        call("hn_server_slot_bytes", hornet._h, int(max_batch), ctypes.byref(v))
+        return v.value
+
+    @staticmethod
     def _requests(trace):
         """trace: list of (kind, payload) with payload = source (queries) or
         (src, dst) int32 arrays (updates)."""

@@ -98,10 +98,25 @@ inline size_t scan_temp_bytes(int64_t n) {
     return (size_t)(tiles + 1) * 8 + 256;
 }
 
-template <typename Out, typename InFn>
-__global__ __launch_bounds__(BLOCK) void scan_reduce_kernel(InFn in, int64_t n, Out* __restrict__ partial) {
+// Element count of a scan: a value fixed at launch, or a 32-bit word in
+// device memory read by every kernel (launched for an upper bound; tiles past
+// the count exit at once), so a scan can sit inside a captured hipGraph whose
+// sizes change from one replay to the next.
+struct HostCount {
+    int64_t n;
+    __device__ __forceinline__ int64_t get() const { return n; }
+};
+struct DevCount {
+    const uint32_t* p;
+    __device__ __forceinline__ int64_t get() const { return (int64_t)*p; }
+};
+
+template <typename Out, typename InFn, typename Cnt>
+__global__ __launch_bounds__(BLOCK) void scan_reduce_kernel(InFn in, Cnt cnt, Out* __restrict__ partial) {
     __shared__ Out lds[NWAVES];
+    const int64_t n = cnt.get();
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    if (base >= n && blockIdx.x > 0) return;
     Out acc = 0;
 #pragma unroll
     for (int j = 0; j < SCAN_IPT; j++) {
@@ -119,10 +134,12 @@ __global__ __launch_bounds__(BLOCK) void scan_reduce_kernel(InFn in, int64_t n, 
 }
 
 // one workgroup: exclusive scan of `count` partials in place, total -> *total
-template <typename Out>
-__global__ __launch_bounds__(BLOCK) void scan_partials_kernel(Out* __restrict__ partial, int64_t count,
+template <typename Out, typename Cnt>
+__global__ __launch_bounds__(BLOCK) void scan_partials_kernel(Out* __restrict__ partial, Cnt cnt,
                                                               Out* __restrict__ total) {
     __shared__ Out lds[NWAVES + 1];
+    const int64_t n = cnt.get();
+    const int64_t count = n > 0 ? (n + SCAN_TILE - 1) / SCAN_TILE : 1;
     Out carry = 0;
     for (int64_t b = 0; b < count; b += BLOCK) {
         int64_t i = b + threadIdx.x;
@@ -136,10 +153,12 @@ __global__ __launch_bounds__(BLOCK) void scan_partials_kernel(Out* __restrict__ 
 }
 
 // down-sweep that hands (index, exclusive prefix, value) to a functor
-template <typename Out, typename InFn, typename OutFn>
-__global__ __launch_bounds__(BLOCK) void scan_down_fn_kernel(InFn in, int64_t n, const Out* __restrict__ partial,
+template <typename Out, typename InFn, typename OutFn, typename Cnt>
+__global__ __launch_bounds__(BLOCK) void scan_down_fn_kernel(InFn in, Cnt cnt, const Out* __restrict__ partial,
                                                              OutFn out) {
     __shared__ Out wtot[NWAVES];
+    const int64_t n = cnt.get();
+    if ((int64_t)blockIdx.x * SCAN_TILE >= n) return;
     const int w = threadIdx.x / WAVE;
     const int l = __lane_id();
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)w * SCAN_WAVE_CHUNK;
@@ -191,9 +210,24 @@ hipError_t exclusive_scan_dev(InFn in, OutFn out, Out* total, int64_t n, void* t
     }
     int64_t tiles = ceil_div(n, SCAN_TILE);
     Out* partial = reinterpret_cast<Out*>(tmp);
-    scan_reduce_kernel<Out, InFn><<<(unsigned)tiles, BLOCK, 0, st>>>(in, n, partial);
-    scan_partials_kernel<Out><<<1, BLOCK, 0, st>>>(partial, tiles, total);
-    scan_down_fn_kernel<Out, InFn, OutFn><<<(unsigned)tiles, BLOCK, 0, st>>>(in, n, partial, out);
+    scan_reduce_kernel<Out, InFn, HostCount><<<(unsigned)tiles, BLOCK, 0, st>>>(in, HostCount{n}, partial);
+    scan_partials_kernel<Out, HostCount><<<1, BLOCK, 0, st>>>(partial, HostCount{n}, total);
+    scan_down_fn_kernel<Out, InFn, OutFn, HostCount><<<(unsigned)tiles, BLOCK, 0, st>>>(in, HostCount{n}, partial,
+                                                                                        out);
+    return hipGetLastError();
+}
+
+// Same, with the count read on the device (*n_dev <= max_n); a zero count
+// leaves *total = 0.  No host round trip, so it may be captured.
+template <typename Out, typename InFn, typename OutFn>
+hipError_t exclusive_scan_dev_count(InFn in, OutFn out, Out* total, const uint32_t* n_dev, int64_t max_n, void* tmp,
+                                    hipStream_t st) {
+    const int64_t tiles = ceil_div(max_n > 0 ? max_n : 1, SCAN_TILE);
+    Out* partial = reinterpret_cast<Out*>(tmp);
+    scan_reduce_kernel<Out, InFn, DevCount><<<(unsigned)tiles, BLOCK, 0, st>>>(in, DevCount{n_dev}, partial);
+    scan_partials_kernel<Out, DevCount><<<1, BLOCK, 0, st>>>(partial, DevCount{n_dev}, total);
+    scan_down_fn_kernel<Out, InFn, OutFn, DevCount><<<(unsigned)tiles, BLOCK, 0, st>>>(in, DevCount{n_dev}, partial,
+                                                                                       out);
     return hipGetLastError();
 }
 
