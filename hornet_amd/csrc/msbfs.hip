@@ -55,6 +55,7 @@ constexpr int MS_IPT = 4;                        // work units per lane per tile
 constexpr int MS_TILE = BLOCK * MS_IPT;          // work units per tile
 constexpr int MS_CAP = LbShared<MS_IPT, uint32_t>::CAP;
 constexpr int MS_COLLECT_PER_CU = 8;             // collect workgroups per CU
+constexpr int32_t MS_NO_HEAVY = 1 << 30;         // T: every open vertex goes through the edge-granular pull
 static_assert(UNIT_SLOTS % MS_G == 0, "groups must not straddle a block start");
 
 struct MsState {
@@ -71,6 +72,7 @@ struct MsState {
     unsigned long long edges;       // sum over sources of degrees of reached vertices
     uint64_t depth_mask[64];        // per level: bits whose depth limit allows expanding
     uint64_t all;                   // bits in use
+    uint64_t live;                  // bits with a non-empty frontier (OR over the frontier)
     unsigned long long push_levels;
     unsigned long long pull_levels;
 };
@@ -87,10 +89,11 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
     return v;
 }
 
-// bits that can still be found while expanding the frontier of `level`
-__device__ __forceinline__ uint64_t live_bits(const MsState* S, int level) {
-    return S->all & S->depth_mask[level < 63 ? level : 63];
-}
+// Bits that can still be found: sources whose frontier is non-empty (the
+// frontier is already cut by the depth limits).  A source that stops (depth
+// limit, or its component is exhausted) no longer keeps every vertex it did
+// not reach on the pull list.
+constexpr int MS_NPART = 5;   // per collect block: active, active degree, open degree, edges, OR of frontier
 
 __global__ void ms_init_kernel(int64_t nv, uint64_t* __restrict__ visited, uint64_t* __restrict__ frontier,
                                uint64_t* __restrict__ next, int64_t nbits, uint32_t* __restrict__ bits) {
@@ -140,6 +143,7 @@ __global__ __launch_bounds__(WAVE) void ms_seed_kernel(int32_t k, const int32_t*
     const uint64_t all = (uint64_t)__ballot(b < k);
     if (b == 0) {
         S->all = all;
+        S->live = all;
         S->level = 0;
         S->mode = 0;
         S->done = 0;
@@ -162,26 +166,32 @@ __global__ __launch_bounds__(BLOCK) void ms_decide_kernel(MsState* S, MsParams P
     const int32_t done = S->done, level = S->level;
     __syncthreads();
     if (done) return;
-    __shared__ unsigned long long red[4][NWAVES];
+    __shared__ unsigned long long red[MS_NPART][NWAVES];
     if (level > 0) {
-        unsigned long long c[4] = {0, 0, 0, 0};
-        for (int i = threadIdx.x; i < nparts; i += BLOCK)
+        unsigned long long c[MS_NPART] = {0, 0, 0, 0, 0};
+        for (int i = threadIdx.x; i < nparts; i += BLOCK) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) c[j] += part[(int64_t)i * 4 + j];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            c[j] = wave_reduce_sum(c[j]);
-            if (__lane_id() == 0) red[j][threadIdx.x / WAVE] = c[j];
+            for (int j = 0; j < 4; j++) c[j] += part[(int64_t)i * MS_NPART + j];
+            c[4] |= part[(int64_t)i * MS_NPART + 4];
         }
+#pragma unroll
+        for (int j = 0; j < 4; j++) c[j] = wave_reduce_sum(c[j]);
+        c[4] = wave_or64(c[4]);
+        if (__lane_id() == 0)
+#pragma unroll
+            for (int j = 0; j < MS_NPART; j++) red[j][threadIdx.x / WAVE] = c[j];
         __syncthreads();
         if (threadIdx.x == 0) {
-            unsigned long long t[4] = {0, 0, 0, 0};
-            for (int j = 0; j < 4; j++)
-                for (int w = 0; w < NWAVES; w++) t[j] += red[j][w];
+            unsigned long long t[MS_NPART] = {0, 0, 0, 0, 0};
+            for (int w = 0; w < NWAVES; w++) {
+                for (int j = 0; j < 4; j++) t[j] += red[j][w];
+                t[4] |= red[4][w];
+            }
             S->active = t[0];
             S->active_deg = t[1];
             S->open_deg = t[2];
             S->edges += t[3];
+            S->live = t[4];
         }
     }
     if (threadIdx.x != 0) return;
@@ -220,7 +230,7 @@ __global__ __launch_bounds__(BLOCK) void ms_emit_kernel(int64_t words, const MsS
                                                         int64_t* __restrict__ qoff, uint64_t* __restrict__ qmask) {
     if (S->done) return;
     const bool push = S->mode == 0;
-    const uint64_t live = live_bits(S, S->level - 1);
+    const uint64_t live = S->live;
     const int64_t wi = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t found = wi < words ? (push ? fbits[wi] : obits[wi]) : 0u;
     if (__ballot(found != 0) == 0) return;
@@ -403,7 +413,7 @@ __global__ __launch_bounds__(BLOCK) void ms_pull_heavy_kernel(int64_t words, con
                                                               uint64_t* __restrict__ next) {
     if (S->done || S->mode != 1) return;
     typedef int32_t Vec __attribute__((ext_vector_type(MS_G)));
-    const uint64_t live = live_bits(S, S->level - 1);
+    const uint64_t live = S->live;
     const int64_t wid = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) / WAVE;
     const int64_t nw = (int64_t)gridDim.x * BLOCK / WAVE;
     const int l = __lane_id();
@@ -437,7 +447,7 @@ __global__ __launch_bounds__(BLOCK) void ms_pull_heavy_kernel(int64_t words, con
 
 // collect: new bits -> visited, frontier, distances; next level's bitsets and
 // this block's counters (active, active degree, open degree, edges)
-__global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const hn_vrec* __restrict__ rec,
+__global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const int32_t* __restrict__ degs,
                                                            uint64_t* __restrict__ visited,
                                                            uint64_t* __restrict__ frontier,
                                                            uint64_t* __restrict__ next, int32_t* __restrict__ dist,
@@ -447,9 +457,10 @@ __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const hn_
     if (S->done) return;
     const int32_t level = S->level;
     const uint64_t dmn = S->depth_mask[level < 63 ? level : 63];
-    const uint64_t live = S->all & dmn;
+    // bits still expandable after this level: a superset of the next live set
+    const uint64_t live = S->live & dmn;
     const int lane = __lane_id();
-    unsigned long long act = 0, adeg = 0, odeg = 0, edges = 0;
+    unsigned long long act = 0, adeg = 0, odeg = 0, edges = 0, fany = 0;
     for (int64_t base = (int64_t)blockIdx.x * BLOCK; base < nv; base += (int64_t)gridDim.x * BLOCK) {
         const int64_t u = base + threadIdx.x;
         const bool valid = u < nv;
@@ -466,7 +477,7 @@ __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const hn_
         const uint64_t f = nb & dmn;
         if (f != fo) frontier[u] = f;
         const uint64_t miss = valid ? (live & ~visn) : 0ull;
-        const int32_t deg = (f | miss | nb) ? rec[u].deg : 0;
+        const int32_t deg = (f | miss | nb) ? degs[u] : 0;
         const bool inF = f != 0;
         const bool inO = miss != 0 && deg > 0 && deg <= T;
         const bool inH = miss != 0 && deg > T;
@@ -474,6 +485,7 @@ __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const hn_
         adeg += inF ? (unsigned long long)deg : 0ull;
         odeg += miss ? (unsigned long long)deg : 0ull;
         edges += (unsigned long long)deg * (unsigned long long)__popcll(nb);
+        fany |= f;
         // distances, one source bit at a time so that lanes store consecutive words
         uint64_t M = wave_or64(nb);
         while (M) {
@@ -491,19 +503,26 @@ __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const hn_
             reinterpret_cast<uint64_t*>(hbits)[wq] = bh;
         }
     }
-    __shared__ unsigned long long red[4][NWAVES];
-    unsigned long long c[4] = {act, adeg, odeg, edges};
+    __shared__ unsigned long long red[MS_NPART][NWAVES];
+    unsigned long long c[MS_NPART] = {act, adeg, odeg, edges, fany};
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        c[j] = wave_reduce_sum(c[j]);
-        if (lane == 0) red[j][threadIdx.x / WAVE] = c[j];
-    }
+    for (int j = 0; j < 4; j++) c[j] = wave_reduce_sum(c[j]);
+    c[4] = wave_or64(c[4]);
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < MS_NPART; j++) red[j][threadIdx.x / WAVE] = c[j];
     __syncthreads();
-    if (threadIdx.x < 4) {
+    if (threadIdx.x < MS_NPART) {
         unsigned long long t = 0;
-        for (int w = 0; w < NWAVES; w++) t += red[threadIdx.x][w];
-        part[(int64_t)blockIdx.x * 4 + threadIdx.x] = t;
+        for (int w = 0; w < NWAVES; w++) t = threadIdx.x == 4 ? (t | red[4][w]) : t + red[threadIdx.x][w];
+        part[(int64_t)blockIdx.x * MS_NPART + threadIdx.x] = t;
     }
+}
+
+// the engine's compact degree copy (4 bytes per vertex instead of a 16-byte record)
+__global__ void ms_degrees_kernel(int64_t nv, const hn_vrec* __restrict__ rec, int32_t* __restrict__ degs) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nv) degs[v] = rec[v].deg;
 }
 
 // engine buffer sizes
@@ -536,6 +555,8 @@ struct MsEngine {
     uint32_t* P = nullptr;
     uint32_t* tile_lo = nullptr;
     unsigned long long* part = nullptr;
+    int32_t* degs = nullptr;          // degree copy, valid for graph version degs_version
+    uint64_t degs_version = ~0ull;
     void* scan_tmp = nullptr;
     MsState* S = nullptr;
     int32_t* d_src = nullptr;
@@ -590,7 +611,8 @@ size_t ms_layout(int64_t nv, int64_t tiles, int64_t nparts, MsEngine* e, char* b
     char* p_qm = take((size_t)n * 8);
     char* p_P = take((size_t)(n + 1) * 4);
     char* p_tl = take((size_t)tiles * 4);
-    char* p_pa = take((size_t)nparts * 4 * 8);
+    char* p_pa = take((size_t)nparts * MS_NPART * 8);
+    char* p_dg = take((size_t)n * 4);
     char* p_st = take(std::max(scan_temp_bytes(n), scan_temp_bytes(words)));
     char* p_S = take(sizeof(MsState));
     char* p_src = take(64 * 4);
@@ -609,6 +631,8 @@ size_t ms_layout(int64_t nv, int64_t tiles, int64_t nparts, MsEngine* e, char* b
         e->P = reinterpret_cast<uint32_t*>(p_P);
         e->tile_lo = reinterpret_cast<uint32_t*>(p_tl);
         e->part = reinterpret_cast<unsigned long long*>(p_pa);
+        e->degs = reinterpret_cast<int32_t*>(p_dg);
+        e->degs_version = ~0ull;
         e->scan_tmp = p_st;
         e->S = reinterpret_cast<MsState*>(p_S);
         e->d_src = reinterpret_cast<int32_t*>(p_src);
@@ -650,7 +674,7 @@ int64_t ms_engine_bytes(int64_t nv, int64_t ne) {
 hn_status ms_engine_create(hn_graph* g, MsEngine** out) {
     MsEngine* e = new MsEngine();
     e->g = g;
-    e->params.T = 256;
+    e->params.T = MS_NO_HEAVY;
     e->params.alpha = 16;
     hn_status s = ms_reserve(e);
     if (s != HN_OK) {
@@ -723,7 +747,7 @@ static hipError_t ms_level(MsEngine* e, int32_t* dist, const MsParams& P, hipStr
                                                     reinterpret_cast<unsigned long long*>(e->next));
     ms_pull_heavy_kernel<<<wgrid, BLOCK, 0, st>>>(words, e->S, e->hbits, g->rec, g->arena, e->frontier, e->visited,
                                                   e->next);
-    ms_collect_kernel<<<(unsigned)e->nparts, BLOCK, 0, st>>>(nv, g->rec, e->visited, e->frontier, e->next, dist, e->S,
+    ms_collect_kernel<<<(unsigned)e->nparts, BLOCK, 0, st>>>(nv, e->degs, e->visited, e->frontier, e->next, dist, e->S,
                                                              P.T, e->fbits, e->obits, e->hbits, e->part);
     return hipGetLastError();
 }
@@ -760,6 +784,10 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
     const int64_t nv = g->nv;
     const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(nv, BLOCK), (int64_t)num_cus() * 8);
     HN_HIP(hipMemcpyAsync(e->d_src, sources, sizeof(int32_t) * (size_t)k, hipMemcpyHostToDevice, st));
+    if (e->degs_version != g->version) {   // degrees change with every applied update
+        if (nv > 0) ms_degrees_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(nv, g->rec, e->degs);
+        e->degs_version = g->version;
+    }
     ms_init_kernel<<<grid, BLOCK, 0, st>>>(nv, e->visited, e->frontier, e->next, 3 * ms_bit_words(nv), e->fbits);
     ms_fill_dist_kernel<<<(unsigned)std::min<int64_t>(ceil_div((int64_t)k * nv, 4 * BLOCK), (int64_t)num_cus() * 16),
                           BLOCK, 0, st>>>((int64_t)k * nv, dist);
@@ -855,7 +883,7 @@ hn_status hn_bfs_multi_tune(hn_graph* g, const int32_t* sources, int32_t k, int3
     hipStream_t st = as_stream(stream);
     MsEngine* e = nullptr;
     HN_TRY(ms_engine(g, &e));
-    const int32_t Ts[] = {64, 256, 1024, 4096};
+    const int32_t Ts[] = {64, 256, 1024, MS_NO_HEAVY};
     const int32_t As[] = {4, 16, 64};
     hipEvent_t a = nullptr, b = nullptr;
     HN_HIP(hipEventCreate(&a));

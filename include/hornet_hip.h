@@ -180,8 +180,10 @@ hn_status hn_bfs_multi(hn_graph* g, const int32_t* sources, int32_t k, int32_t* 
 hn_status hn_bfs_multi_depth(hn_graph* g, const int32_t* sources, const int32_t* depth_limit,
                              int32_t k, int32_t* dist, hn_msbfs_stats* stats, void* stream);
 /* Load-balancing parameters of the batched BFS: pull-side degree threshold
- * between one lane and one wave per vertex, and the push/pull switch
- * (push while frontier degree * alpha < unvisited degree).  0 keeps a value. */
+ * (open vertices of degree > T take one wave each and stop early; the others
+ * go through the edge-granular pull; default 2^30 = none), and the push/pull
+ * switch (push while frontier degree * alpha < unvisited degree).  0 keeps a
+ * value. */
 hn_status hn_bfs_multi_params(hn_graph* g, int32_t degree_threshold, int32_t push_alpha);
 /* Autotuner: times the batched BFS of `sources` over a grid of thresholds and
  * switch points and keeps the fastest for this graph.  dist is scratch of
