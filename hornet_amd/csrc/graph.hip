@@ -108,6 +108,34 @@ __global__ __launch_bounds__(BLOCK) void flatten_kernel(int32_t nv, int64_t ne,
     }
 }
 
+// does w's sorted adjacency hold u?
+__device__ __forceinline__ bool adj_contains(const int32_t* __restrict__ a, int32_t n, int32_t u) {
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (a[mid] < u) lo = mid + 1; else hi = mid;
+    }
+    return lo < n && a[lo] == u;
+}
+
+// every stored (s, v) must have its reverse (v, s); flag the first that does not
+__global__ __launch_bounds__(BLOCK) void symmetry_kernel(int32_t nv, int64_t ne, const int64_t* __restrict__ eoff,
+                                                         const hn_vrec* __restrict__ rec,
+                                                         const int32_t* __restrict__ arena,
+                                                         uint32_t* __restrict__ flag) {
+    __shared__ LbShared<COPY_IPT, int64_t> sh;
+    for (int64_t w0 = (int64_t)blockIdx.x * COPY_TILE; w0 < ne; w0 += (int64_t)gridDim.x * COPY_TILE) {
+        lb_tile<COPY_IPT>(
+            eoff, (int64_t)nv, w0, ne, sh,
+            [&](int64_t s) { return rec[s].off - eoff[s]; },
+            [&](int64_t s, int64_t w, int64_t base) {
+                const int32_t v = arena[base + w];
+                const hn_vrec r = rec[v];
+                if (!adj_contains(arena + r.off, r.deg, (int32_t)s)) *flag = 1u;
+            });
+    }
+}
+
 struct DegIn {
     const hn_vrec* rec;
     __device__ __forceinline__ int64_t operator()(int64_t v) const { return rec[v].deg; }
@@ -400,6 +428,30 @@ hn_status degree_prefix(hn_graph* g, int64_t* d_eoff, hipStream_t st) {
     return exclusive_scan<int64_t>(DegIn{g->rec}, d_eoff, g->nv, g->s_scan.ptr, st);
 }
 
+hn_status graph_is_symmetric(hn_graph* g, hipStream_t st, bool* yes) {
+    if (g->sym == 1 || (g->sym == -1 && g->sym_version == g->version)) {
+        *yes = g->sym == 1;
+        return HN_OK;
+    }
+    uint32_t flag = 0;
+    if (g->ne > 0) {
+        const size_t off_bytes = ((size_t)(g->nv + 1) * 8 + 255) & ~size_t(255);
+        HN_TRY(g->s_misc.reserve(off_bytes + 256));
+        int64_t* eoff = g->s_misc.as<int64_t>();
+        uint32_t* d_flag = reinterpret_cast<uint32_t*>(g->s_misc.as<char>() + off_bytes);
+        HN_HIP(hipMemsetAsync(d_flag, 0, 4, st));
+        HN_TRY(degree_prefix(g, eoff, st));
+        const int64_t grid = std::min<int64_t>(ceil_div(g->ne, COPY_TILE), (int64_t)num_cus() * 16);
+        symmetry_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(g->nv, g->ne, eoff, g->rec, g->arena, d_flag);
+        HN_CHECK_LAUNCH();
+        HN_TRY(read_back(g, d_flag, 4, &flag, st));
+    }
+    g->sym = flag ? -1 : 1;
+    g->sym_version = g->version;
+    *yes = g->sym == 1;
+    return HN_OK;
+}
+
 }  // namespace hn
 
 using namespace hn;
@@ -502,6 +554,7 @@ hn_status hn_graph_create_coo(int32_t nv, int64_t m, const int32_t* src, const i
     int64_t ne = 0;
     HN_TRY(csr_from_device_coo(nv, m, src, dst, flags, st, &d_off, &d_adj, &ne));
     hn_status s = graph_from_device_csr(nv, ne, d_off, d_adj, slack, st, out);
+    if (s == HN_OK && (flags & HN_SYMMETRIZE)) (*out)->sym = 1;
     dfree(d_off);
     dfree(d_adj);
     return s;

@@ -74,6 +74,10 @@ struct hn_graph {
     int32_t nv = 0;
     int64_t ne = 0;
     uint64_t version = 0;             // bumped by every applied insert / erase
+    // adjacency symmetry ((u,w) stored iff (w,u) stored): 1 known symmetric,
+    // -1 known not (for sym_version), 0 unknown
+    int32_t sym = 0;
+    uint64_t sym_version = 0;
     hn_vrec* rec = nullptr;           // device, nv records
     int32_t* arena = nullptr;         // device, cap_units * 32 slots
     int64_t cap_units = 0;
@@ -127,6 +131,9 @@ hn_status ms_engine_create(hn_graph* g, MsEngine** out);
 void ms_engine_destroy(MsEngine* e);
 hn_status ms_run_engine(MsEngine* e, const int32_t* sources, int32_t k, const int32_t* depth_limit, int32_t* dist,
                         hn_msbfs_stats* stats, hipStream_t caller);
+// whether the adjacency is symmetric (checked once per graph version when
+// not known; the multi-source BFS pulls only over symmetric graphs)
+hn_status graph_is_symmetric(hn_graph* g, hipStream_t st, bool* yes);
 // device bytes one engine holds for a graph of nv vertices and ne edges
 int64_t ms_engine_bytes(int64_t nv, int64_t ne);
 }  // namespace hn

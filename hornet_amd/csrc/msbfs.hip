@@ -80,6 +80,7 @@ struct MsState {
 struct MsParams {
     int32_t T;          // pull: open vertices of degree > T take one wave each
     int32_t alpha;      // push while active_deg * alpha < open_deg
+    int32_t push_only;  // adjacency not symmetric: pulling over out-edges would be wrong
     int32_t depth_limit[64];
 };
 
@@ -201,7 +202,8 @@ __global__ __launch_bounds__(BLOCK) void ms_decide_kernel(MsState* S, MsParams P
     }
     S->level += 1;
     S->levels_run += 1;
-    const bool push = (unsigned long long)S->active_deg * (unsigned long long)P.alpha < S->open_deg;
+    const bool push =
+        P.push_only || (unsigned long long)S->active_deg * (unsigned long long)P.alpha < S->open_deg;
     S->mode = push ? 0 : 1;
     if (push) S->push_levels++; else S->pull_levels++;
 }
@@ -779,7 +781,10 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
     hn_graph* g = e->g;
     HN_ARG(g->ne < (int64_t)0xFFFFFFFF, "edge count exceeds the 32-bit work prefix");
     HN_TRY(ms_reserve(e));
+    bool sym = false;
+    HN_TRY(graph_is_symmetric(g, st, &sym));
     MsParams P = e->params;
+    P.push_only = sym ? 0 : 1;
     for (int b = 0; b < 64; b++) P.depth_limit[b] = (depth_limit && b < k) ? depth_limit[b] : 0;
     const int64_t nv = g->nv;
     const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(nv, BLOCK), (int64_t)num_cus() * 8);

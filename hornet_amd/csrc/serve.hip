@@ -212,6 +212,10 @@ static void dispatcher_loop(hn_server* s) {
                 st = hn_graph_erase(s->g, head->src.data(), head->dst.data(), n, HN_DEDUP_BATCH, &applied,
                                     s->upd_stream);
             }
+            // settle the adjacency-symmetry state here, while no query runs,
+            // so concurrent launches only ever read it
+            bool sym = false;
+            if (st == HN_OK) st = graph_is_symmetric(s->g, s->upd_stream, &sym);
             const int64_t t_done = now_ns();
             lk.lock();
             hn_result& res = s->results[(size_t)head->id];
@@ -315,6 +319,16 @@ hn_status hn_server_create(hn_graph* g, const hn_serve_config* cfg, hn_server** 
         delete s;
         set_error("server: stream creation failed");
         return HN_E_HIP;
+    }
+    {
+        // known before any query runs (the slots only read it)
+        bool sym = false;
+        const hn_status ss = graph_is_symmetric(g, s->upd_stream, &sym);
+        if (ss != HN_OK) {
+            (void)hipStreamDestroy(s->upd_stream);
+            delete s;
+            return ss;
+        }
     }
     for (int i = 0; i < nslots; i++) {
         auto sl = std::make_unique<ServeSlot>();

@@ -49,7 +49,7 @@ struct KeyFmt {
 
 struct UpdScal {
     uint32_t err;
-    uint32_t pad;
+    uint32_t asym;                // a batch pair left (u,w) and (w,u) with different membership
     unsigned long long changed;   // entries inserted / erased
 };
 
@@ -583,6 +583,29 @@ __global__ void store_changed_kernel(const uint32_t* __restrict__ total, UpdScal
     if (!sc->err) sc->changed = *total;
 }
 
+// After a batch applied to a symmetric graph: pairs outside the batch kept
+// their membership, so the adjacency stays symmetric iff every batch pair
+// (u,w) has (u,w) and (w,u) both stored or both absent.
+__global__ void pair_symmetry_kernel(const uint32_t* __restrict__ count, const uint64_t* __restrict__ keys, KeyFmt f,
+                                     const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena,
+                                     UpdScal* sc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)*count || sc->err) return;
+    const int32_t u = f.src(keys[i]), w = f.dst(keys[i]);
+    const hn_vrec ru = rec[u], rw = rec[w];
+    const int64_t pu = lower_bound_i32(arena + ru.off, ru.deg, w);
+    const int64_t pw = lower_bound_i32(arena + rw.off, rw.deg, u);
+    const bool a = pu < ru.deg && arena[ru.off + pu] == w;
+    const bool b = pw < rw.deg && arena[rw.off + pw] == u;
+    if (a != b) sc->asym = 1u;   // every writer stores the same value
+}
+
+// symmetry bookkeeping after a successful batch (was_sym: known symmetric before)
+static void note_symmetry(hn_graph* g, bool was_sym, const UpdScal& h) {
+    g->sym = was_sym ? (h.asym ? -1 : 1) : 0;
+    g->sym_version = g->version;
+}
+
 // size the staging buffer (one readback) and copy the moving suffixes
 static hn_status stage_suffixes(hn_graph* g, UpdBufs& b, int64_t n, const uint32_t* ucnt, const uint8_t* skip,
                                 hipStream_t st, int32_t** stage_out) {
@@ -657,10 +680,16 @@ hn_status hn_graph_insert(hn_graph* g, const int32_t* src, const int32_t* dst, i
     HN_CHECK_LAUNCH();
     store_changed_kernel<<<1, 1, 0, st>>>(nb, b.sc);
     HN_CHECK_LAUNCH();
+    const bool was_sym = g->sym == 1;
+    if (was_sym) {
+        pair_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, g->rec, g->arena, b.sc);
+        HN_CHECK_LAUNCH();
+    }
     UpdScal h;
     HN_TRY(check_scal(g, b, st, &h));
     g->ne += (int64_t)h.changed;
     g->version++;
+    note_symmetry(g, was_sym, h);
     if (inserted) *inserted = (int64_t)h.changed;
     return HN_OK;
 }
@@ -719,10 +748,16 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     HN_CHECK_LAUNCH();
     store_changed_kernel<<<1, 1, 0, st>>>(b.dlp + n, b.sc);
     HN_CHECK_LAUNCH();
+    const bool was_sym = g->sym == 1;
+    if (was_sym) {
+        pair_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, g->rec, g->arena, b.sc);
+        HN_CHECK_LAUNCH();
+    }
     UpdScal h;
     HN_TRY(check_scal(g, b, st, &h));
     g->ne -= (int64_t)h.changed;
     g->version++;
+    note_symmetry(g, was_sym, h);
     if (erased) *erased = (int64_t)h.changed;
     return HN_OK;
 }

@@ -120,3 +120,50 @@ def test_chung_lu_generator_and_bfs(hn):
     srcs = O.pick_sources(off, 64, seed=2)
     got = hn.BfsMulti(h).run(srcs).cpu().numpy()
     assert np.array_equal(got, expected(off, adj, srcs))
+
+
+def test_directed_graph_runs_push_only(hn):
+    # pulling reads out-adjacency as in-adjacency, which only a symmetric
+    # graph allows: a directed graph must be traversed push-only
+    print("case msbfs directed", flush=True)
+    n = 1 << 12
+    s, d = O.rmat_pairs(12, seed=31)
+    off, adj = O.coo_to_csr(n, s, d, symmetrize=False)
+    h = hn.Hornet.from_csr(off, adj)
+    srcs = O.pick_sources(off, 48, seed=4)
+    ms = hn.BfsMulti(h)
+    ms.set_params(0, 1 << 20)   # would pull at every level if it could
+    got = ms.run(srcs).cpu().numpy()
+    assert np.array_equal(got, expected(off, adj, srcs))
+    assert ms.stats.pull_levels == 0
+
+
+def test_symmetry_tracked_through_updates(hn):
+    print("case msbfs symmetry after updates", flush=True)
+    off, adj = O.rmat_graph(12, seed=41)
+    h = hn.Hornet.from_csr(off, adj)      # symmetric, found by the full check
+    srcs = O.pick_sources(off, 32, seed=6)
+    ms = hn.BfsMulti(h)
+    ms.set_params(0, 1 << 20)
+    ms.run(srcs)
+    assert ms.stats.pull_levels >= 1
+    # a symmetric batch keeps pulling allowed
+    n = off.size - 1
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, n, 300).astype(np.int32)
+    b = rng.integers(0, n, 300).astype(np.int32)
+    a, b = a[a != b], b[a != b]
+    h.insert(hn.BatchUpdate(np.concatenate([a, b]), np.concatenate([b, a])), True, True)
+    off, adj, _ = O.insert(off, adj, np.concatenate([a, b]), np.concatenate([b, a]), True, True)
+    got = ms.run(srcs).cpu().numpy()
+    assert np.array_equal(got, expected(off, adj, srcs))
+    assert ms.stats.pull_levels >= 1
+    # one direction only: the adjacency is no longer symmetric
+    c = rng.integers(0, n, 200).astype(np.int32)
+    e = rng.integers(0, n, 200).astype(np.int32)
+    c, e = c[c != e], e[c != e]
+    h.insert(hn.BatchUpdate(c, e), True, True)
+    off, adj, _ = O.insert(off, adj, c, e, True, True)
+    got = ms.run(srcs).cpu().numpy()
+    assert np.array_equal(got, expected(off, adj, srcs))
+    assert ms.stats.pull_levels == 0
