@@ -228,6 +228,7 @@ __global__ __launch_bounds__(BLOCK) void ms_emit_kernel(int64_t words, const MsS
                                                         const hn_vrec* __restrict__ rec,
                                                         const uint64_t* __restrict__ visited,
                                                         const uint64_t* __restrict__ frontier,
+                                                        const uint8_t* __restrict__ dirty,
                                                         int32_t* __restrict__ list, int32_t* __restrict__ qdeg,
                                                         int64_t* __restrict__ qoff, uint64_t* __restrict__ qmask) {
     if (S->done) return;
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(BLOCK) void ms_emit_kernel(int64_t words, const MsS
             const hn_vrec rv = rec[v];
             const uint64_t m = push ? frontier[v] : (live & ~visited[v]);
             list[pos] = v;
-            qdeg[pos] = m ? rv.deg : 0;
+            qdeg[pos] = (m && !dirty[v]) ? rv.deg : 0;   // a dirty vertex's edges are side records
             qoff[pos] = rv.off;
             qmask[pos] = m;
         }
@@ -412,6 +413,7 @@ __global__ __launch_bounds__(BLOCK) void ms_pull_heavy_kernel(int64_t words, con
                                                               const int32_t* __restrict__ arena,
                                                               const uint64_t* __restrict__ frontier,
                                                               const uint64_t* __restrict__ visited,
+                                                              const uint8_t* __restrict__ dirty,
                                                               uint64_t* __restrict__ next) {
     if (S->done || S->mode != 1) return;
     typedef int32_t Vec __attribute__((ext_vector_type(MS_G)));
@@ -425,6 +427,7 @@ __global__ __launch_bounds__(BLOCK) void ms_pull_heavy_kernel(int64_t words, con
             const int j = __ffs(hw) - 1;
             hw &= hw - 1;
             const int32_t u = (int32_t)(wd * 32 + j);
+            if (dirty[u]) continue;   // its edges are side records
             const uint64_t miss = live & ~visited[u];
             if (!miss) continue;
             const hn_vrec r = rec[u];
@@ -521,6 +524,205 @@ __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const int
     }
 }
 
+// ---- multi-version windows (the served stream) -----------------------------
+// One launch may serve queries of several graph versions: storage holds the
+// newest version of the window; a vertex touched by the window's updates is
+// dirty, the main kernels skip its adjacency, and side records (u, w, S)
+// relax each of its edges in both directions for the sources S whose
+// version holds the edge.
+
+struct SideSV {
+    uint64_t v[64];   // sources whose graph version is window version t
+};
+
+__global__ __launch_bounds__(BLOCK) void ms_side_kernel(const MsState* S, const uint32_t* __restrict__ nside,
+                                                        const int2* __restrict__ uw,
+                                                        const uint64_t* __restrict__ sm,
+                                                        const uint64_t* __restrict__ frontier,
+                                                        const uint64_t* __restrict__ visited,
+                                                        unsigned long long* __restrict__ next) {
+    if (S->done) return;
+    const int64_t n = *nside;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const uint64_t s = sm[i];
+        if (!s) continue;
+        const int2 e = uw[i];
+        const uint64_t fu = frontier[e.x] & s, fw = frontier[e.y] & s;
+        if (fu) {
+            const uint64_t add = fu & ~visited[e.y];
+            if (add) atomicOr(next + e.y, (unsigned long long)add);
+        }
+        if (fw) {
+            const uint64_t add = fw & ~visited[e.x];
+            if (add) atomicOr(next + e.x, (unsigned long long)add);
+        }
+    }
+}
+
+// window entry i: pair key << 7 | update index (1..63) << 1 | 1 for an insert
+__global__ void win_pack_kernel(int64_t n, const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
+                                const uint8_t* __restrict__ tt, int sbits, uint64_t* __restrict__ keys) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = ((((uint64_t)(uint32_t)src[i] << sbits) | (uint32_t)dst[i]) << 7) | tt[i];
+}
+
+// every entry (u, w) of update t needs (w, u) of the same kind in update t:
+// side records relax both directions, which needs every version symmetric
+__global__ void win_sym_kernel(int64_t n, const uint64_t* __restrict__ keys, int sbits, uint32_t* __restrict__ bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i], pk = k >> 7, msk = (1ull << sbits) - 1;
+    const uint64_t rk = ((((pk & msk) << sbits) | (pk >> sbits)) << 7) | (k & 127);
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < rk) lo = mid + 1; else hi = mid;
+    }
+    if (lo >= n || keys[lo] != rk) *bad = 1u;
+}
+
+struct PairHeadIn {
+    const uint64_t* keys;
+    __device__ __forceinline__ uint32_t operator()(int64_t i) const {
+        return (i == 0 || (keys[i] >> 7) != (keys[i - 1] >> 7)) ? 1u : 0u;
+    }
+};
+
+__global__ void win_seg_kernel(int64_t n, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ hpos,
+                               uint32_t* __restrict__ seg_start) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i == 0 || (keys[i] >> 7) != (keys[i - 1] >> 7)) seg_start[hpos[i]] = (uint32_t)i;
+    if (i == n - 1) seg_start[hpos[n]] = (uint32_t)n;
+}
+
+__device__ __forceinline__ bool sorted_has(const int32_t* __restrict__ a, int32_t n, int32_t x) {
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo < n && a[lo] == x;
+}
+
+// membership of every touched pair before the window's first update
+__global__ void win_e0_kernel(const uint32_t* __restrict__ nseg, const uint32_t* __restrict__ seg_start,
+                              const uint64_t* __restrict__ keys, int sbits, const hn_vrec* __restrict__ rec,
+                              const int32_t* __restrict__ arena, uint8_t* __restrict__ e0) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (int64_t)*nseg) return;
+    const uint64_t pk = keys[seg_start[j]] >> 7;
+    const int32_t u = (int32_t)(pk >> sbits), w = (int32_t)(pk & ((1ull << sbits) - 1));
+    const hn_vrec r = rec[u];
+    e0[j] = sorted_has(arena + r.off, r.deg, w) ? 1 : 0;
+}
+
+// existence of each touched pair in window versions 0..m (update t makes
+// version t; failed updates change nothing); a pair whose existence varies
+// is changed and makes its source dirty
+__global__ void win_exist_kernel(const uint32_t* __restrict__ nseg, const uint32_t* __restrict__ seg_start,
+                                 const uint64_t* __restrict__ keys, const uint8_t* __restrict__ e0, int32_t m,
+                                 uint64_t failed, int sbits, uint64_t* __restrict__ E, uint32_t* __restrict__ chg,
+                                 uint8_t* __restrict__ dirty) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (int64_t)*nseg) return;
+    uint32_t p = seg_start[j];
+    const uint32_t end = seg_start[j + 1];
+    uint32_t state = e0[j];
+    uint64_t e = 0;
+    for (int t = 0; t <= m; t++) {
+        while (p < end && (int)((keys[p] >> 1) & 63) == t) {
+            if (!((failed >> t) & 1ull)) state = (uint32_t)(keys[p] & 1);
+            p++;
+        }
+        e |= (uint64_t)state << t;
+    }
+    const uint64_t all = m >= 63 ? ~0ull : ((2ull << m) - 1);
+    const bool changed = e != 0 && e != all;
+    E[j] = e;
+    chg[j] = changed ? 1u : 0u;
+    if (changed) dirty[(keys[seg_start[j]] >> 7) >> sbits] = 1;
+}
+
+struct FlagIn {
+    const uint32_t* f;
+    const uint32_t* n;
+    __device__ __forceinline__ uint32_t operator()(int64_t i) const { return i < (int64_t)*n ? f[i] : 0u; }
+};
+
+__global__ void win_chg_write_kernel(const uint32_t* __restrict__ nseg, const uint32_t* __restrict__ chg,
+                                     const uint32_t* __restrict__ cpos, const uint32_t* __restrict__ seg_start,
+                                     const uint64_t* __restrict__ keys, const uint64_t* __restrict__ E,
+                                     uint64_t* __restrict__ ckey, uint64_t* __restrict__ cE) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (int64_t)*nseg || !chg[j]) return;
+    ckey[cpos[j]] = keys[seg_start[j]] >> 7;
+    cE[cpos[j]] = E[j];
+}
+
+struct DirtyIn {
+    const uint8_t* d;
+    __device__ __forceinline__ uint32_t operator()(int64_t i) const { return d[i] ? 1u : 0u; }
+};
+
+__global__ void win_dlist_kernel(int64_t nv, const uint8_t* __restrict__ dirty, const uint32_t* __restrict__ dpos,
+                                 int32_t* __restrict__ dlist) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nv && dirty[v]) dlist[dpos[v]] = (int32_t)v;
+}
+
+struct DirtyDegIn {
+    const int32_t* dlist;
+    const hn_vrec* rec;
+    __device__ __forceinline__ int64_t operator()(int64_t i) const { return rec[dlist[i]].deg; }
+};
+
+// side records for the stored edges of the dirty vertices: every source,
+// except the changed pairs (their records come with their own masks)
+constexpr int WIN_IPT = 8;
+__global__ __launch_bounds__(BLOCK) void win_side_fill_kernel(int64_t nd, int64_t total,
+                                                              const int64_t* __restrict__ P,
+                                                              const int32_t* __restrict__ dlist,
+                                                              const hn_vrec* __restrict__ rec,
+                                                              const int32_t* __restrict__ arena, int sbits,
+                                                              const uint64_t* __restrict__ ckey, int64_t nchg,
+                                                              int2* __restrict__ uw, uint64_t* __restrict__ sm) {
+    __shared__ LbShared<WIN_IPT, int64_t> sh;
+    constexpr int TILE = BLOCK * WIN_IPT;
+    for (int64_t w0 = (int64_t)blockIdx.x * TILE; w0 < total; w0 += (int64_t)gridDim.x * TILE) {
+        lb_tile<WIN_IPT>(
+            P, nd, w0, total, sh, [&](int64_t s) { return rec[dlist[s]].off - P[s]; },
+            [&](int64_t s, int64_t w, int64_t base) {
+                const int32_t u = dlist[s], v = arena[base + w];
+                const uint64_t pk = ((uint64_t)(uint32_t)u << sbits) | (uint32_t)v;
+                int64_t lo = 0, hi = nchg;
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (ckey[mid] < pk) lo = mid + 1; else hi = mid;
+                }
+                uw[w] = make_int2(u, v);
+                sm[w] = (lo < nchg && ckey[lo] == pk) ? 0ull : ~0ull;
+            });
+    }
+}
+
+// side records of the changed pairs: the sources whose version holds them
+__global__ void win_side_chg_kernel(int64_t nchg, int64_t at, const uint64_t* __restrict__ ckey,
+                                    const uint64_t* __restrict__ cE, int32_t m, SideSV sv, int sbits,
+                                    int2* __restrict__ uw, uint64_t* __restrict__ sm) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchg) return;
+    const uint64_t pk = ckey[c], e = cE[c];
+    uint64_t s = 0;
+    for (int t = 0; t <= m; t++)
+        if ((e >> t) & 1ull) s |= sv.v[t];
+    uw[at + c] = make_int2((int32_t)(pk >> sbits), (int32_t)(pk & ((1ull << sbits) - 1)));
+    sm[at + c] = s;
+}
+
+__global__ void set_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
+
 // the engine's compact degree copy (4 bytes per vertex instead of a 16-byte record)
 __global__ void ms_degrees_kernel(int64_t nv, const hn_vrec* __restrict__ rec, int32_t* __restrict__ degs) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -571,6 +773,20 @@ struct MsEngine {
     hipStream_t own = nullptr;        // non-blocking stream the level graph is captured on
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     int32_t* pinned = nullptr;        // engine-private readback buffer
+    // multi-version window state (served stream)
+    uint8_t* dirty = nullptr;         // nv flags, in `mem`
+    uint32_t* d_nside = nullptr;      // side record count, in `mem`
+    Scratch side_uw, side_sm;         // side records (u, w) and source masks
+    Scratch w_keys, w_alt, w_tmp, w_seg, w_e0, w_E, w_chg, w_ckey, w_cE, w_dpos, w_dlist, w_P;
+    int64_t w_n = 0;                  // entries of the open window
+    uint64_t* w_sorted = nullptr;     // its sorted keys (in w_keys or w_alt)
+    int64_t w_ndirty = 0;
+    bool has_window = false;
+    void release_window() {
+        for (Scratch* x : {&side_uw, &side_sm, &w_keys, &w_alt, &w_tmp, &w_seg, &w_e0, &w_E, &w_chg, &w_ckey,
+                           &w_cE, &w_dpos, &w_dlist, &w_P})
+            x->release();
+    }
     void release_mem() {
         if (exec) (void)hipGraphExecDestroy(exec);
         exec = nullptr;
@@ -580,6 +796,7 @@ struct MsEngine {
     }
     void release() {
         release_mem();
+        release_window();
         if (pinned) (void)hipHostFree(pinned);
         pinned = nullptr;
         if (own) (void)hipStreamDestroy(own);
@@ -615,6 +832,8 @@ size_t ms_layout(int64_t nv, int64_t tiles, int64_t nparts, MsEngine* e, char* b
     char* p_tl = take((size_t)tiles * 4);
     char* p_pa = take((size_t)nparts * MS_NPART * 8);
     char* p_dg = take((size_t)n * 4);
+    char* p_dy = take((size_t)n);
+    char* p_ns = take(4);
     char* p_st = take(std::max(scan_temp_bytes(n), scan_temp_bytes(words)));
     char* p_S = take(sizeof(MsState));
     char* p_src = take(64 * 4);
@@ -635,6 +854,8 @@ size_t ms_layout(int64_t nv, int64_t tiles, int64_t nparts, MsEngine* e, char* b
         e->part = reinterpret_cast<unsigned long long*>(p_pa);
         e->degs = reinterpret_cast<int32_t*>(p_dg);
         e->degs_version = ~0ull;
+        e->dirty = reinterpret_cast<uint8_t*>(p_dy);
+        e->d_nside = reinterpret_cast<uint32_t*>(p_ns);
         e->scan_tmp = p_st;
         e->S = reinterpret_cast<MsState*>(p_S);
         e->d_src = reinterpret_cast<int32_t*>(p_src);
@@ -662,6 +883,10 @@ hn_status ms_reserve(MsEngine* e) {
     e->tile_cap = cap;
     e->nparts = nparts;
     ms_layout(g->nv, cap, nparts, e, mem);
+    // no window: nothing dirty, no side records
+    HN_HIP(hipMemset(e->dirty, 0, (size_t)(g->nv > 0 ? g->nv : 1)));
+    HN_HIP(hipMemset(e->d_nside, 0, 4));
+    e->has_window = false;
     return HN_OK;
 }
 
@@ -736,8 +961,8 @@ static hipError_t ms_level(MsEngine* e, int32_t* dist, const MsParams& P, hipStr
                                         words, e->scan_tmp, st);
     if (err != hipSuccess) return err;
     ms_emit_kernel<<<(unsigned)ceil_div(words, BLOCK), BLOCK, 0, st>>>(words, e->S, e->fbits, e->obits, e->wpre,
-                                                                       g->rec, e->visited, e->frontier, e->list,
-                                                                       e->qdeg, e->qoff, e->qmask);
+                                                                       g->rec, e->visited, e->frontier, e->dirty,
+                                                                       e->list, e->qdeg, e->qoff, e->qmask);
     err = exclusive_scan_dev_count<uint64_t>(GroupIn{e->qdeg}, GroupOut{e->P, e->tile_lo}, &e->S->tot, &e->S->wn, nv,
                                              e->scan_tmp, st);
     if (err != hipSuccess) return err;
@@ -748,7 +973,9 @@ static hipError_t ms_level(MsEngine* e, int32_t* dist, const MsParams& P, hipStr
                                                     g->arena, e->frontier,
                                                     reinterpret_cast<unsigned long long*>(e->next));
     ms_pull_heavy_kernel<<<wgrid, BLOCK, 0, st>>>(words, e->S, e->hbits, g->rec, g->arena, e->frontier, e->visited,
-                                                  e->next);
+                                                  e->dirty, e->next);
+    ms_side_kernel<<<wgrid, BLOCK, 0, st>>>(e->S, e->d_nside, e->side_uw.as<int2>(), e->side_sm.as<uint64_t>(),
+                                            e->frontier, e->visited, reinterpret_cast<unsigned long long*>(e->next));
     ms_collect_kernel<<<(unsigned)e->nparts, BLOCK, 0, st>>>(nv, e->degs, e->visited, e->frontier, e->next, dist, e->S,
                                                              P.T, e->fbits, e->obits, e->hbits, e->part);
     return hipGetLastError();
@@ -843,6 +1070,155 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
         stats->pad_ = 0;
         stats->edges = (int64_t)s.edges;
     }
+    return HN_OK;
+}
+
+// ---- multi-version windows: called by the serving loop ----------------------
+
+hn_status ms_window_begin(MsEngine* e, const int32_t* h_src, const int32_t* h_dst, const uint8_t* h_tt, int64_t n,
+                          hipStream_t st, bool* symmetric) {
+    hn_graph* g = e->g;
+    const int sbits = std::max(1, ceil_log2_host((int64_t)g->nv + 1));
+    HN_ARG(2 * sbits + 7 <= 64, "window keys need 2 * ceil(log2(nv + 1)) + 7 <= 64 bits");
+    HN_ARG(n >= 0 && n < (int64_t)0x7FFFFFFF, "window too large");
+    *symmetric = true;
+    e->w_n = n;
+    e->has_window = true;
+    if (n == 0) return HN_OK;
+    HN_TRY(e->w_keys.reserve((size_t)n * 8));
+    HN_TRY(e->w_alt.reserve((size_t)n * 9 + 512));
+    HN_TRY(e->w_tmp.reserve(std::max(sort_temp_bytes(n), scan_temp_bytes(std::max<int64_t>(n, g->nv)))));
+    HN_TRY(e->w_seg.reserve((size_t)(n + 1) * 8 + 256));
+    HN_TRY(e->w_e0.reserve((size_t)n));
+    // entries staged in w_alt (the sort's second buffer once they are packed)
+    char* stg = e->w_alt.as<char>();
+    int32_t* d_src = reinterpret_cast<int32_t*>(stg);
+    int32_t* d_dst = reinterpret_cast<int32_t*>(stg + (size_t)n * 4);
+    uint8_t* d_tt = reinterpret_cast<uint8_t*>(stg + (size_t)n * 8);
+    HN_HIP(hipMemcpyAsync(d_src, h_src, (size_t)n * 4, hipMemcpyHostToDevice, st));
+    HN_HIP(hipMemcpyAsync(d_dst, h_dst, (size_t)n * 4, hipMemcpyHostToDevice, st));
+    HN_HIP(hipMemcpyAsync(d_tt, h_tt, (size_t)n, hipMemcpyHostToDevice, st));
+    uint64_t* keys = e->w_keys.as<uint64_t>();
+    uint64_t* alt = e->w_alt.as<uint64_t>();
+    win_pack_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(n, d_src, d_dst, d_tt, sbits, keys);
+    HN_CHECK_LAUNCH();
+    HN_TRY(radix_sort_u64(&keys, &alt, n, 2 * sbits + 7, e->w_tmp.ptr, st));
+    e->w_sorted = keys;
+    {
+        uint32_t* d_bad = reinterpret_cast<uint32_t*>(e->w_seg.as<char>() + (size_t)(n + 1) * 8);
+        HN_HIP(hipMemsetAsync(d_bad, 0, 4, st));
+        win_sym_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(n, keys, sbits, d_bad);
+        HN_CHECK_LAUNCH();
+        uint32_t bad = 0;
+        HN_TRY(ms_read(e, d_bad, 4, &bad, st));
+        if (bad) {   // not a symmetric window: the caller applies these updates one by one
+            *symmetric = false;
+            e->has_window = false;
+            e->w_n = 0;
+            return HN_OK;
+        }
+    }
+    uint32_t* hpos = e->w_seg.as<uint32_t>();
+    uint32_t* seg_start = hpos + (n + 1);
+    HN_TRY(exclusive_scan<uint32_t>(PairHeadIn{keys}, hpos, n, e->w_tmp.ptr, st));
+    win_seg_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(n, keys, hpos, seg_start);
+    HN_CHECK_LAUNCH();
+    // membership before the first update of the window
+    win_e0_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(hpos + n, seg_start, keys, sbits, g->rec,
+                                                                  g->arena, e->w_e0.as<uint8_t>());
+    HN_CHECK_LAUNCH();
+    HN_HIP(hipStreamSynchronize(st));
+    return HN_OK;
+}
+
+hn_status ms_window_end(MsEngine* e, int32_t m, uint64_t failed, const uint64_t* sv, hipStream_t st) {
+    hn_graph* g = e->g;
+    HN_ARG(e->has_window, "no open window");
+    HN_ARG(m >= 0 && m <= 63, "a window holds at most 63 updates");
+    HN_TRY(ms_reserve(e));   // may re-lay out the buffers after the updates: before any flag is set
+    e->has_window = true;
+    const int64_t nv = g->nv, n = e->w_n;
+    const int sbits = std::max(1, ceil_log2_host((int64_t)nv + 1));
+    HN_TRY(e->w_tmp.reserve(scan_temp_bytes(std::max<int64_t>(n, nv))));
+    HN_HIP(hipMemsetAsync(e->dirty, 0, (size_t)(nv > 0 ? nv : 1), st));
+    uint32_t nchg = 0;
+    if (n > 0) {
+        HN_TRY(e->w_E.reserve((size_t)n * 8));
+        HN_TRY(e->w_chg.reserve((size_t)(2 * n + 1) * 4 + 256));
+        HN_TRY(e->w_ckey.reserve((size_t)n * 8));
+        HN_TRY(e->w_cE.reserve((size_t)n * 8));
+        uint32_t* hpos = e->w_seg.as<uint32_t>();
+        uint32_t* seg_start = hpos + (n + 1);
+        const uint32_t* nseg = hpos + n;
+        uint32_t* chg = e->w_chg.as<uint32_t>();
+        uint32_t* cpos = chg + n;
+        win_exist_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(nseg, seg_start, e->w_sorted,
+                                                                         e->w_e0.as<uint8_t>(), m, failed, sbits,
+                                                                         e->w_E.as<uint64_t>(), chg, e->dirty);
+        HN_CHECK_LAUNCH();
+        HN_TRY(exclusive_scan<uint32_t>(FlagIn{chg, nseg}, cpos, n, e->w_tmp.ptr, st));
+        win_chg_write_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(nseg, chg, cpos, seg_start, e->w_sorted,
+                                                                             e->w_E.as<uint64_t>(),
+                                                                             e->w_ckey.as<uint64_t>(),
+                                                                             e->w_cE.as<uint64_t>());
+        HN_CHECK_LAUNCH();
+        HN_TRY(ms_read(e, cpos + n, 4, &nchg, st));
+    }
+    // dirty vertices and their stored degrees
+    HN_TRY(e->w_dpos.reserve((size_t)(nv + 1) * 4));
+    uint32_t* dpos = e->w_dpos.as<uint32_t>();
+    HN_TRY(exclusive_scan<uint32_t>(DirtyIn{e->dirty}, dpos, nv, e->w_tmp.ptr, st));
+    uint32_t nd = 0;
+    HN_TRY(ms_read(e, dpos + nv, 4, &nd, st));
+    HN_TRY(e->w_dlist.reserve((size_t)(nd > 0 ? nd : 1) * 4));
+    HN_TRY(e->w_P.reserve((size_t)(nd + 1) * 8));
+    int32_t* dlist = e->w_dlist.as<int32_t>();
+    int64_t* P = e->w_P.as<int64_t>();
+    if (nv > 0) win_dlist_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(nv, e->dirty, dpos, dlist);
+    HN_CHECK_LAUNCH();
+    HN_TRY(exclusive_scan<int64_t>(DirtyDegIn{dlist, g->rec}, P, nd, e->w_tmp.ptr, st));
+    int64_t total = 0;
+    HN_TRY(ms_read(e, P + nd, 8, &total, st));
+    // side records: stored edges of the dirty vertices, then the changed pairs
+    const int64_t cap = total + nchg;
+    void* old_uw = e->side_uw.ptr;
+    void* old_sm = e->side_sm.ptr;
+    HN_TRY(e->side_uw.reserve((size_t)(cap > 0 ? cap : 1) * 8));
+    HN_TRY(e->side_sm.reserve((size_t)(cap > 0 ? cap : 1) * 8));
+    if (e->side_uw.ptr != old_uw || e->side_sm.ptr != old_sm) {   // the captured level step holds the pointers
+        if (e->exec) (void)hipGraphExecDestroy(e->exec);
+        e->exec = nullptr;
+    }
+    int2* uw = e->side_uw.as<int2>();
+    uint64_t* sm = e->side_sm.as<uint64_t>();
+    if (total > 0) {
+        const int64_t tile = (int64_t)BLOCK * WIN_IPT;
+        const int64_t grid = std::min<int64_t>(ceil_div(total, tile), (int64_t)num_cus() * 16);
+        win_side_fill_kernel<<<(unsigned)grid, BLOCK, 0, st>>>((int64_t)nd, total, P, dlist, g->rec, g->arena, sbits,
+                                                               e->w_ckey.as<uint64_t>(), (int64_t)nchg, uw, sm);
+        HN_CHECK_LAUNCH();
+    }
+    if (nchg > 0) {
+        SideSV v;
+        for (int t = 0; t < 64; t++) v.v[t] = t <= m ? sv[t] : 0ull;
+        win_side_chg_kernel<<<(unsigned)ceil_div(nchg, BLOCK), BLOCK, 0, st>>>(
+            (int64_t)nchg, total, e->w_ckey.as<uint64_t>(), e->w_cE.as<uint64_t>(), m, v, sbits, uw, sm);
+        HN_CHECK_LAUNCH();
+    }
+    set_u32_kernel<<<1, 1, 0, st>>>(e->d_nside, (uint32_t)cap);
+    HN_CHECK_LAUNCH();
+    HN_HIP(hipStreamSynchronize(st));
+    e->w_ndirty = nd;
+    return HN_OK;
+}
+
+hn_status ms_window_clear(MsEngine* e, hipStream_t st) {
+    if (!e->has_window) return HN_OK;
+    HN_HIP(hipMemsetAsync(e->dirty, 0, (size_t)(e->nv > 0 ? e->nv : 1), st));
+    HN_HIP(hipMemsetAsync(e->d_nside, 0, 4, st));
+    e->has_window = false;
+    e->w_n = 0;
+    e->w_ndirty = 0;
     return HN_OK;
 }
 

@@ -87,6 +87,8 @@ struct ServeSlot {
     hipStream_t stream = nullptr;
     std::thread th;
     std::vector<ServeReq*> batch;
+    std::vector<uint64_t> qver;              // per-query graph versions of a window launch
+    bool window = false;                     // the engine holds a window's side records
     bool has_work = false;
     bool busy = false;
     std::condition_variable cv;
@@ -135,6 +137,10 @@ static void worker_loop(hn_server* s, ServeSlot* slot) {
         if (!slot->has_work) break;
         std::vector<ServeReq*> batch;
         batch.swap(slot->batch);
+        std::vector<uint64_t> qver;
+        qver.swap(slot->qver);
+        const bool window = slot->window;
+        slot->window = false;
         slot->has_work = false;
         const uint64_t ver = g->version;   // stable: updates wait for in-flight launches
         lk.unlock();
@@ -164,9 +170,14 @@ static void worker_loop(hn_server* s, ServeSlot* slot) {
             if (e == hipSuccess) e = hipStreamSynchronize(slot->stream);
             if (e != hipSuccess) st = HN_E_HIP;
         }
+        if (window) {
+            const hn_status cs = ms_window_clear(slot->eng, slot->stream);
+            if (cs == HN_OK) (void)hipStreamSynchronize(slot->stream);
+        }
         const int64_t t_done = now_ns();
         lk.lock();
-        for (int32_t i = 0; i < k; i++) finish_query(s, batch[i], slot->h_digest + 3 * i, k, ver, st, t_done);
+        for (int32_t i = 0; i < k; i++)
+            finish_query(s, batch[i], slot->h_digest + 3 * i, k, qver.empty() ? ver : qver[(size_t)i], st, t_done);
         s->completed += k;
         s->inflight--;
         slot->busy = false;
@@ -183,53 +194,213 @@ static ServeSlot* free_slot(hn_server* s) {
     return nullptr;
 }
 
+// apply one update (barrier and window paths); the lock is not held
+static hn_status apply_update(hn_server* s, const ServeReq* r, int64_t* applied) {
+    *applied = 0;
+    const int64_t n = (int64_t)r->src.size();
+    if (n * 160 > s->update_budget) {
+        hn::set_error("update batch exceeds the server's memory budget");
+        return HN_E_OOM;
+    }
+    if (r->kind == HN_REQ_INSERT)
+        return hn_graph_insert(s->g, r->src.data(), r->dst.data(), n, HN_DEDUP_BATCH | HN_DEDUP_GRAPH, applied,
+                               s->upd_stream);
+    return hn_graph_erase(s->g, r->src.data(), r->dst.data(), n, HN_DEDUP_BATCH, applied, s->upd_stream);
+}
+
+// lock held
+static void record_update(hn_server* s, const ServeReq* r, hn_status st, int64_t applied, uint64_t version,
+                          int64_t t_start, int64_t t_done) {
+    hn_result& res = s->results[(size_t)r->id];
+    res.status = st;
+    res.applied = applied;
+    res.t_start_ns = t_start;
+    res.t_done_ns = t_done;
+    res.version = version;
+    res.batch = 1;
+    s->completed++;
+}
+
+// A multi-version window: a queue prefix of up to max_batch queries with the
+// updates between them (at most 63, WINDOW_MAX_ENTRIES entries), ending at
+// its last query.  One launch serves all its queries, each at the graph
+// version it arrived at.
+constexpr int64_t WINDOW_MAX_ENTRIES = 1 << 21;
+struct Window {
+    int nq = 0, nu = 0;
+    size_t len = 0;
+};
+
+static Window scan_window(hn_server* s, int maxb) {
+    Window w;
+    int nq = 0, nu = 0;
+    int64_t ent = 0;
+    for (size_t i = 0; i < s->queue.size(); i++) {
+        const ServeReq* r = s->queue[i];
+        if (is_query(r)) {
+            if (nq == maxb) break;
+            nq++;
+            w.nq = nq;
+            w.nu = nu;
+            w.len = i + 1;
+        } else {
+            const int64_t m = (int64_t)r->src.size();
+            if (nu == 63 || ent + m > WINDOW_MAX_ENTRIES) break;
+            nu++;
+            ent += m;
+        }
+    }
+    return w;
+}
+
+// Runs window w (lock held on entry and exit; inflight == 0).  Returns false,
+// with the queue untouched, when its updates are not symmetric batches.
+static bool run_window(hn_server* s, const Window& w, std::unique_lock<std::mutex>& lk) {
+    hn_graph* g = s->g;
+    ServeSlot* slot = free_slot(s);
+    std::vector<ServeReq*> qs, us;
+    std::vector<int> qt;   // window version of each query: updates before it
+    for (size_t i = 0; i < w.len; i++) {
+        ServeReq* r = s->queue[i];
+        if (is_query(r)) {
+            qs.push_back(r);
+            qt.push_back((int)us.size());
+        } else {
+            us.push_back(r);
+        }
+    }
+    const int nu = (int)us.size();
+    const int64_t t0 = now_ns();
+    const uint64_t base_version = g->version;
+    slot->busy = true;   // no other launch while the graph changes
+    s->inflight++;
+    lk.unlock();
+
+    // entries of the updates that can apply (bad ids or an over-budget batch
+    // fail as a whole and change nothing)
+    uint64_t failed = 0;
+    std::vector<int32_t> es, ed;
+    std::vector<uint8_t> et;
+    for (int t = 1; t <= nu; t++) {
+        const ServeReq* r = us[(size_t)t - 1];
+        bool ok = (int64_t)r->src.size() * 160 <= s->update_budget;
+        for (size_t j = 0; ok && j < r->src.size(); j++)
+            ok = r->src[j] >= 0 && r->src[j] < g->nv && r->dst[j] >= 0 && r->dst[j] < g->nv;
+        if (!ok) {
+            failed |= 1ull << t;
+            continue;
+        }
+        const uint8_t tag = (uint8_t)((t << 1) | (r->kind == HN_REQ_INSERT ? 1 : 0));
+        es.insert(es.end(), r->src.begin(), r->src.end());
+        ed.insert(ed.end(), r->dst.begin(), r->dst.end());
+        et.insert(et.end(), r->src.size(), tag);
+    }
+    bool symmetric = false;
+    hn_status st = ms_window_begin(slot->eng, es.data(), ed.data(), et.data(), (int64_t)es.size(), s->upd_stream,
+                                   &symmetric);
+    if (st != HN_OK || !symmetric) {
+        (void)ms_window_clear(slot->eng, s->upd_stream);
+        lk.lock();
+        slot->busy = false;
+        s->inflight--;
+        return false;
+    }
+    // the updates, in arrival order
+    std::vector<hn_status> ust((size_t)nu);
+    std::vector<int64_t> uapp((size_t)nu), udone((size_t)nu);
+    std::vector<uint64_t> uver((size_t)nu);
+    for (int t = 1; t <= nu; t++) {
+        int64_t applied = 0;
+        hn_status us_st = apply_update(s, us[(size_t)t - 1], &applied);
+        if (us_st != HN_OK) failed |= 1ull << t;
+        ust[(size_t)t - 1] = us_st;
+        uapp[(size_t)t - 1] = applied;
+        uver[(size_t)t - 1] = g->version;
+        udone[(size_t)t - 1] = now_ns();
+    }
+    bool sym = false;
+    st = graph_is_symmetric(g, s->upd_stream, &sym);
+    uint64_t sv[64] = {0};
+    for (size_t i = 0; i < qs.size(); i++) sv[qt[i]] |= 1ull << i;
+    if (st == HN_OK) st = ms_window_end(slot->eng, nu, failed, sv, s->upd_stream);
+
+    lk.lock();
+    for (size_t i = 0; i < w.len; i++) s->queue.pop_front();
+    for (int t = 0; t < nu; t++) record_update(s, us[(size_t)t], ust[(size_t)t], uapp[(size_t)t], uver[(size_t)t], t0,
+                                               udone[(size_t)t]);
+    // version a query saw: the base plus the updates before it that applied
+    std::vector<uint64_t> vat((size_t)nu + 1, base_version);
+    for (int t = 1; t <= nu; t++) vat[(size_t)t] = vat[(size_t)t - 1] + (((failed >> t) & 1ull) ? 0 : 1);
+    for (size_t i = 0; i < qs.size(); i++) s->results[(size_t)qs[i]->id].t_start_ns = t0;
+    s->cv_done.notify_all();
+    if (st != HN_OK) {
+        const int64_t t_done = now_ns();
+        const unsigned long long zero[3] = {0, 0, 0};
+        for (size_t i = 0; i < qs.size(); i++)
+            finish_query(s, qs[i], zero, (int32_t)qs.size(), vat[(size_t)qt[i]], st, t_done);
+        s->completed += (int64_t)qs.size();
+        (void)ms_window_clear(slot->eng, s->upd_stream);
+        slot->busy = false;
+        s->inflight--;
+        s->cv_done.notify_all();
+        return true;
+    }
+    slot->batch = qs;
+    slot->qver.clear();
+    for (size_t i = 0; i < qs.size(); i++) slot->qver.push_back(vat[(size_t)qt[i]]);
+    slot->window = true;
+    slot->has_work = true;
+    slot->cv.notify_one();
+    return true;
+}
+
 static void dispatcher_loop(hn_server* s) {
     std::unique_lock<std::mutex> lk(s->mu);
+    const int maxb = s->cfg.plain ? 1 : s->cfg.max_batch;
+    const int sbits = std::max(1, ceil_log2_host((int64_t)s->g->nv + 1));
+    const bool windows = !s->cfg.plain && 2 * sbits + 7 <= 64;
+    int barrier_updates = 0;   // updates to apply one by one (a window was not symmetric)
     while (true) {
-        s->cv_disp.wait(lk, [&] {
-            if (s->stop) return true;
-            if (s->queue.empty()) return false;
-            return is_query(s->queue.front()) ? free_slot(s) != nullptr : s->inflight == 0;
-        });
+        s->cv_disp.wait(lk, [&] { return s->stop || !s->queue.empty(); });
         if (s->stop) break;
+        // queries across updates, in one launch (symmetric graphs)
+        if (windows && barrier_updates == 0 && s->g->sym == 1) {
+            Window w = scan_window(s, maxb);
+            if (w.nq > 0 && w.nu > 0) {
+                s->cv_disp.wait(lk, [&] { return s->stop || s->inflight == 0; });
+                if (s->stop) break;
+                w = scan_window(s, maxb);   // the queue may have grown
+                if (run_window(s, w, lk)) continue;
+                barrier_updates = w.nu;
+                continue;
+            }
+        }
         ServeReq* head = s->queue.front();
         if (!is_query(head)) {
+            // an update waits for every launch before it; the queries after it wait for it
+            s->cv_disp.wait(lk, [&] { return s->stop || s->inflight == 0; });
+            if (s->stop) break;
             s->queue.pop_front();
-            // index, never hold a reference: submissions may grow `results`
-            // while the lock is released below
-            s->results[(size_t)head->id].t_start_ns = now_ns();
+            const int64_t t_start = now_ns();
             lk.unlock();
             int64_t applied = 0;
-            hn_status st;
-            const int64_t n = (int64_t)head->src.size();
-            if (n * 160 > s->update_budget) {
-                hn::set_error("update batch exceeds the server's memory budget");
-                st = HN_E_OOM;
-            } else if (head->kind == HN_REQ_INSERT) {
-                st = hn_graph_insert(s->g, head->src.data(), head->dst.data(), n, HN_DEDUP_BATCH | HN_DEDUP_GRAPH,
-                                     &applied, s->upd_stream);
-            } else {
-                st = hn_graph_erase(s->g, head->src.data(), head->dst.data(), n, HN_DEDUP_BATCH, &applied,
-                                    s->upd_stream);
-            }
+            hn_status st = apply_update(s, head, &applied);
             // settle the adjacency-symmetry state here, while no query runs,
             // so concurrent launches only ever read it
             bool sym = false;
             if (st == HN_OK) st = graph_is_symmetric(s->g, s->upd_stream, &sym);
             const int64_t t_done = now_ns();
             lk.lock();
-            hn_result& res = s->results[(size_t)head->id];
-            res.status = st;
-            res.applied = applied;
-            res.t_done_ns = t_done;
-            res.version = s->g->version;
-            res.batch = 1;
-            s->completed++;
+            // index, never hold a reference: submissions may grow `results`
+            // while the lock is released
+            record_update(s, head, st, applied, s->g->version, t_start, t_done);
+            if (barrier_updates > 0) barrier_updates--;
             s->cv_done.notify_all();
             continue;
         }
+        s->cv_disp.wait(lk, [&] { return s->stop || free_slot(s) != nullptr; });
+        if (s->stop) break;
         ServeSlot* slot = free_slot(s);
-        const int maxb = s->cfg.plain ? 1 : s->cfg.max_batch;
         const int64_t t0 = now_ns();
         while (!s->queue.empty() && is_query(s->queue.front()) && (int)slot->batch.size() < maxb) {
             ServeReq* r = s->queue.front();

@@ -118,3 +118,41 @@ def test_budget_admission(hn):
     srv.close()
     s, d = h.getCOO()
     assert s.numel() == adj.size
+
+
+def test_one_directional_updates_fall_back(hn):
+    # windows batch queries across updates only when every update is a
+    # symmetric batch; one-directional ones are applied one by one and the
+    # graph is then traversed push-only
+    print("case serve one-directional updates", flush=True)
+    off, adj = O.rmat_graph(11, seed=19)
+    h = hn.Hornet.from_csr(off, adj)
+    trace, arrivals = make_trace(off, adj, 240, seed=8)
+    rng = np.random.default_rng(1)
+    n = off.size - 1
+    for i, (kind, payload) in enumerate(trace):
+        if kind == 2 and rng.random() < 0.5:
+            a = rng.integers(0, n, 50).astype(np.int32)
+            b = rng.integers(0, n, 50).astype(np.int32)
+            k = a != b
+            trace[i] = (2, (a[k], b[k]))
+    srv = hn.Server(h, max_batch=64, slots=2)
+    res = srv.replay(trace, arrivals)
+    srv.close()
+    check(res, expected_results(off, adj, trace))
+
+
+def test_window_spans_updates(hn):
+    # a burst of queries and updates queued together is served by launches
+    # that span updates, each query still at its own version
+    print("case serve window burst", flush=True)
+    off, adj = O.rmat_graph(12, seed=29)
+    h = hn.Hornet.from_csr(off, adj)
+    trace, _ = make_trace(off, adj, 300, seed=12)
+    arrivals = np.zeros(len(trace), np.int64)   # everything queued at once
+    srv = hn.Server(h, max_batch=64, slots=1)
+    res = srv.replay(trace, arrivals)
+    srv.close()
+    check(res, expected_results(off, adj, trace))
+    q = res["kind"] <= 1
+    assert res["batch"][q].max() >= 16
