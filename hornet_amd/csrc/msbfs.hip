@@ -253,7 +253,7 @@ __global__ __launch_bounds__(BLOCK) void ms_emit_kernel(int64_t words, const MsS
             const hn_vrec rv = rec[v];
             const uint64_t m = push ? frontier[v] : (live & ~visited[v]);
             list[pos] = v;
-            qdeg[pos] = (m && !dirty[v]) ? rv.deg : 0;   // a dirty vertex's edges are side records
+            qdeg[pos] = m ? rv.deg : 0;
             qoff[pos] = rv.off;
             qmask[pos] = m;
         }
@@ -287,7 +287,27 @@ struct MsShared {
     int32_t deg[MS_CAP];
     uint64_t mask[MS_CAP];           // push: frontier mask; pull: missing bits
     unsigned long long acc[MS_CAP];  // pull: bits found this tile
+    uint64_t crange[MS_CAP];         // window: range of the vertex's changed pairs (lo << 32 | hi), 0 if none
 };
+
+// Multi-version window (served stream): the changed pairs of the window,
+// sorted by (u << sbits | w), with cS = the sources whose graph version holds
+// the pair.  A vertex flagged dirty has some; its range in the sorted list:
+__device__ __forceinline__ uint64_t changed_range(const uint64_t* __restrict__ ckey, uint32_t n, int32_t u,
+                                                  int sbits) {
+    const uint64_t k0 = (uint64_t)(uint32_t)u << sbits, k1 = (uint64_t)((uint32_t)u + 1) << sbits;
+    uint32_t a = 0, b = n;
+    while (a < b) {
+        const uint32_t m = (a + b) >> 1;
+        if (ckey[m] < k0) a = m + 1; else b = m;
+    }
+    uint32_t c = a, d = n;
+    while (c < d) {
+        const uint32_t m = (c + d) >> 1;
+        if (ckey[m] < k1) c = m + 1; else d = m;
+    }
+    return c > a ? ((uint64_t)a << 32) | c : 0ull;
+}
 
 // Edge-granular expansion of this level's work list (see the file comment).
 template <bool PULL>
@@ -299,10 +319,15 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
                                                           const int32_t* __restrict__ list,
                                                           const int32_t* __restrict__ arena,
                                                           const uint64_t* __restrict__ probe,
-                                                          unsigned long long* __restrict__ next) {
+                                                          unsigned long long* __restrict__ next,
+                                                          const uint8_t* __restrict__ dirty,
+                                                          const uint64_t* __restrict__ ckey,
+                                                          const uint64_t* __restrict__ cS,
+                                                          const uint32_t* __restrict__ nchg, int sbits) {
     if (S->done || S->mode != (PULL ? 1 : 0)) return;
     typedef int32_t Vec __attribute__((ext_vector_type(MS_G)));
     __shared__ MsShared sh;
+    const uint32_t nc = *nchg;
     const int64_t F = S->wn;
     const int64_t W = (uint32_t)S->tot;
     const int64_t ntiles = (W + MS_TILE - 1) / MS_TILE;
@@ -319,6 +344,8 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
                 sh.deg[i] = qdeg[lo + i];
                 sh.mask[i] = qmask[lo + i];
                 if (PULL) sh.acc[i] = 0;
+                const int32_t u = list[lo + i];
+                sh.crange[i] = (nc && dirty[u]) ? changed_range(ckey, nc, u, sbits) : 0ull;
             }
         }
         __syncthreads();
@@ -327,6 +354,7 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
         int nval[MS_IPT];
         int64_t seg[MS_IPT];
         uint64_t msk[MS_IPT];
+        uint64_t crg[MS_IPT];
         if (staged) {
 #pragma unroll
             for (int j = 0; j < MS_IPT; j++) {
@@ -337,6 +365,7 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
                 nval[j] = w < W ? (int)min((int64_t)MS_G, (int64_t)sh.deg[s] - k) : 0;
                 seg[j] = s;
                 msk[j] = sh.mask[s];
+                crg[j] = sh.crange[s];
             }
         } else {   // slice wider than LDS (runs of work-free list entries)
             for (int j = 0; j < MS_IPT; j++) {
@@ -345,6 +374,7 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
                 nval[j] = 0;
                 seg[j] = lo;
                 msk[j] = 0;
+                crg[j] = 0;
                 if (w < W) {
                     const int64_t s = upper_bound_m1<int64_t>(P, lo, hi + 1, w);
                     const int64_t k = (w - (int64_t)P[s]) * MS_G;
@@ -352,6 +382,8 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
                     nval[j] = (int)min((int64_t)MS_G, (int64_t)qdeg[s] - k);
                     seg[j] = s;
                     msk[j] = qmask[s];
+                    const int32_t u = list[s];
+                    crg[j] = (nc && dirty[u]) ? changed_range(ckey, nc, u, sbits) : 0ull;
                 }
             }
         }
@@ -366,6 +398,25 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
         for (int j = 0; j < MS_IPT; j++)
 #pragma unroll
             for (int e = 0; e < MS_G; e++) pr[j][e] = probe[e < nval[j] ? v[j][e] : 0];
+        // window: a changed pair carries only the sources whose version holds
+        // it (push: the others count as visited; pull: their bits are dropped)
+#pragma unroll
+        for (int j = 0; j < MS_IPT; j++) {
+            if (!crg[j]) continue;
+            const uint32_t clo = (uint32_t)(crg[j] >> 32), chi = (uint32_t)crg[j];
+            const uint64_t ub = ckey[clo] & ~((1ull << sbits) - 1);   // u << sbits
+#pragma unroll
+            for (int e = 0; e < MS_G; e++) {
+                if (e >= nval[j]) continue;
+                const uint64_t key = ub | (uint32_t)v[j][e];
+                uint32_t a = clo, b = chi;
+                while (a < b) {
+                    const uint32_t m = (a + b) >> 1;
+                    if (ckey[m] < key) a = m + 1; else b = m;
+                }
+                if (a < chi && ckey[a] == key) pr[j][e] = PULL ? (pr[j][e] & cS[a]) : (pr[j][e] | ~cS[a]);
+            }
+        }
         if (!PULL) {
             // push: bits of the frontier vertex its neighbour has not seen
 #pragma unroll
@@ -456,7 +507,8 @@ __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const int
                                                            uint64_t* __restrict__ visited,
                                                            uint64_t* __restrict__ frontier,
                                                            uint64_t* __restrict__ next, int32_t* __restrict__ dist,
-                                                           const MsState* S, int32_t T, uint32_t* __restrict__ fbits,
+                                                           const MsState* S, int32_t T, const uint8_t* __restrict__ dirty,
+                                                           uint32_t* __restrict__ fbits,
                                                            uint32_t* __restrict__ obits, uint32_t* __restrict__ hbits,
                                                            unsigned long long* __restrict__ part) {
     if (S->done) return;
@@ -484,8 +536,10 @@ __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const int
         const uint64_t miss = valid ? (live & ~visn) : 0ull;
         const int32_t deg = (f | miss | nb) ? degs[u] : 0;
         const bool inF = f != 0;
-        const bool inO = miss != 0 && deg > 0 && deg <= T;
-        const bool inH = miss != 0 && deg > T;
+        // a vertex with changed pairs in a window takes the edge-granular pull
+        const bool heavy = miss != 0 && deg > T && !dirty[u];
+        const bool inO = miss != 0 && deg > 0 && !heavy;
+        const bool inH = heavy;
         act += inF ? 1ull : 0ull;
         adeg += inF ? (unsigned long long)deg : 0ull;
         odeg += miss ? (unsigned long long)deg : 0ull;
@@ -535,6 +589,9 @@ struct SideSV {
     uint64_t v[64];   // sources whose graph version is window version t
 };
 
+// side records: changed pairs absent from the newest version (so not in
+// storage), each relaxed u -> w for its sources (the reverse pair is its own
+// record, the window's batches being symmetric)
 __global__ __launch_bounds__(BLOCK) void ms_side_kernel(const MsState* S, const uint32_t* __restrict__ nside,
                                                         const int2* __restrict__ uw,
                                                         const uint64_t* __restrict__ sm,
@@ -544,18 +601,11 @@ __global__ __launch_bounds__(BLOCK) void ms_side_kernel(const MsState* S, const 
     if (S->done) return;
     const int64_t n = *nside;
     for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-        const uint64_t s = sm[i];
-        if (!s) continue;
         const int2 e = uw[i];
-        const uint64_t fu = frontier[e.x] & s, fw = frontier[e.y] & s;
-        if (fu) {
-            const uint64_t add = fu & ~visited[e.y];
-            if (add) atomicOr(next + e.y, (unsigned long long)add);
-        }
-        if (fw) {
-            const uint64_t add = fw & ~visited[e.x];
-            if (add) atomicOr(next + e.x, (unsigned long long)add);
-        }
+        const uint64_t fu = frontier[e.x] & sm[i];
+        if (!fu) continue;
+        const uint64_t add = fu & ~visited[e.y];
+        if (add) atomicOr(next + e.y, (unsigned long long)add);
     }
 }
 
@@ -661,64 +711,30 @@ __global__ void win_chg_write_kernel(const uint32_t* __restrict__ nseg, const ui
     cE[cpos[j]] = E[j];
 }
 
-struct DirtyIn {
-    const uint8_t* d;
-    __device__ __forceinline__ uint32_t operator()(int64_t i) const { return d[i] ? 1u : 0u; }
-};
-
-__global__ void win_dlist_kernel(int64_t nv, const uint8_t* __restrict__ dirty, const uint32_t* __restrict__ dpos,
-                                 int32_t* __restrict__ dlist) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < nv && dirty[v]) dlist[dpos[v]] = (int32_t)v;
-}
-
-struct DirtyDegIn {
-    const int32_t* dlist;
-    const hn_vrec* rec;
-    __device__ __forceinline__ int64_t operator()(int64_t i) const { return rec[dlist[i]].deg; }
-};
-
-// side records for the stored edges of the dirty vertices: every source,
-// except the changed pairs (their records come with their own masks)
-constexpr int WIN_IPT = 8;
-__global__ __launch_bounds__(BLOCK) void win_side_fill_kernel(int64_t nd, int64_t total,
-                                                              const int64_t* __restrict__ P,
-                                                              const int32_t* __restrict__ dlist,
-                                                              const hn_vrec* __restrict__ rec,
-                                                              const int32_t* __restrict__ arena, int sbits,
-                                                              const uint64_t* __restrict__ ckey, int64_t nchg,
-                                                              int2* __restrict__ uw, uint64_t* __restrict__ sm) {
-    __shared__ LbShared<WIN_IPT, int64_t> sh;
-    constexpr int TILE = BLOCK * WIN_IPT;
-    for (int64_t w0 = (int64_t)blockIdx.x * TILE; w0 < total; w0 += (int64_t)gridDim.x * TILE) {
-        lb_tile<WIN_IPT>(
-            P, nd, w0, total, sh, [&](int64_t s) { return rec[dlist[s]].off - P[s]; },
-            [&](int64_t s, int64_t w, int64_t base) {
-                const int32_t u = dlist[s], v = arena[base + w];
-                const uint64_t pk = ((uint64_t)(uint32_t)u << sbits) | (uint32_t)v;
-                int64_t lo = 0, hi = nchg;
-                while (lo < hi) {
-                    const int64_t mid = (lo + hi) >> 1;
-                    if (ckey[mid] < pk) lo = mid + 1; else hi = mid;
-                }
-                uw[w] = make_int2(u, v);
-                sm[w] = (lo < nchg && ckey[lo] == pk) ? 0ull : ~0ull;
-            });
-    }
-}
-
-// side records of the changed pairs: the sources whose version holds them
-__global__ void win_side_chg_kernel(int64_t nchg, int64_t at, const uint64_t* __restrict__ ckey,
-                                    const uint64_t* __restrict__ cE, int32_t m, SideSV sv, int sbits,
-                                    int2* __restrict__ uw, uint64_t* __restrict__ sm) {
+// per changed pair: its sources, and whether it is absent from storage (a side record)
+__global__ void win_cs_kernel(const uint32_t* __restrict__ nchg, const uint64_t* __restrict__ cE, int32_t m,
+                              SideSV sv, uint64_t* __restrict__ cS, uint32_t* __restrict__ sflag) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nchg) return;
-    const uint64_t pk = ckey[c], e = cE[c];
+    if (c >= (int64_t)*nchg) return;
+    const uint64_t e = cE[c];
     uint64_t s = 0;
     for (int t = 0; t <= m; t++)
         if ((e >> t) & 1ull) s |= sv.v[t];
-    uw[at + c] = make_int2((int32_t)(pk >> sbits), (int32_t)(pk & ((1ull << sbits) - 1)));
-    sm[at + c] = s;
+    cS[c] = s;
+    sflag[c] = ((e >> m) & 1ull) ? 0u : 1u;
+}
+
+__global__ void win_side_write_kernel(const uint32_t* __restrict__ nchg, const uint32_t* __restrict__ sflag,
+                                      const uint32_t* __restrict__ spos, const uint64_t* __restrict__ ckey,
+                                      const uint64_t* __restrict__ cS, int sbits, int2* __restrict__ uw,
+                                      uint64_t* __restrict__ sm, uint32_t* __restrict__ nside) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = *nchg;
+    if (c == 0) *nside = spos[n];
+    if (c >= n || !sflag[c]) return;
+    const uint64_t pk = ckey[c];
+    uw[spos[c]] = make_int2((int32_t)(pk >> sbits), (int32_t)(pk & ((1ull << sbits) - 1)));
+    sm[spos[c]] = cS[c];
 }
 
 __global__ void set_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
@@ -774,17 +790,19 @@ struct MsEngine {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     int32_t* pinned = nullptr;        // engine-private readback buffer
     // multi-version window state (served stream)
-    uint8_t* dirty = nullptr;         // nv flags, in `mem`
+    uint8_t* dirty = nullptr;         // nv flags (vertex has changed pairs), in `mem`
     uint32_t* d_nside = nullptr;      // side record count, in `mem`
+    uint32_t* d_nchg = nullptr;       // changed pair count, in `mem`
     Scratch side_uw, side_sm;         // side records (u, w) and source masks
-    Scratch w_keys, w_alt, w_tmp, w_seg, w_e0, w_E, w_chg, w_ckey, w_cE, w_dpos, w_dlist, w_P;
+    Scratch w_keys, w_alt, w_tmp, w_seg, w_e0, w_E, w_chg, w_ckey, w_cE, w_cS, w_sflag;
+    void* exec_cS = nullptr;          // window buffers the captured level step was built with
+    void* exec_ckey = nullptr;
     int64_t w_n = 0;                  // entries of the open window
     uint64_t* w_sorted = nullptr;     // its sorted keys (in w_keys or w_alt)
-    int64_t w_ndirty = 0;
     bool has_window = false;
     void release_window() {
         for (Scratch* x : {&side_uw, &side_sm, &w_keys, &w_alt, &w_tmp, &w_seg, &w_e0, &w_E, &w_chg, &w_ckey,
-                           &w_cE, &w_dpos, &w_dlist, &w_P})
+                           &w_cE, &w_cS, &w_sflag})
             x->release();
     }
     void release_mem() {
@@ -834,6 +852,7 @@ size_t ms_layout(int64_t nv, int64_t tiles, int64_t nparts, MsEngine* e, char* b
     char* p_dg = take((size_t)n * 4);
     char* p_dy = take((size_t)n);
     char* p_ns = take(4);
+    char* p_nc = take(4);
     char* p_st = take(std::max(scan_temp_bytes(n), scan_temp_bytes(words)));
     char* p_S = take(sizeof(MsState));
     char* p_src = take(64 * 4);
@@ -856,6 +875,7 @@ size_t ms_layout(int64_t nv, int64_t tiles, int64_t nparts, MsEngine* e, char* b
         e->degs_version = ~0ull;
         e->dirty = reinterpret_cast<uint8_t*>(p_dy);
         e->d_nside = reinterpret_cast<uint32_t*>(p_ns);
+        e->d_nchg = reinterpret_cast<uint32_t*>(p_nc);
         e->scan_tmp = p_st;
         e->S = reinterpret_cast<MsState*>(p_S);
         e->d_src = reinterpret_cast<int32_t*>(p_src);
@@ -886,6 +906,7 @@ hn_status ms_reserve(MsEngine* e) {
     // no window: nothing dirty, no side records
     HN_HIP(hipMemset(e->dirty, 0, (size_t)(g->nv > 0 ? g->nv : 1)));
     HN_HIP(hipMemset(e->d_nside, 0, 4));
+    HN_HIP(hipMemset(e->d_nchg, 0, 4));
     e->has_window = false;
     return HN_OK;
 }
@@ -966,18 +987,26 @@ static hipError_t ms_level(MsEngine* e, int32_t* dist, const MsParams& P, hipStr
     err = exclusive_scan_dev_count<uint64_t>(GroupIn{e->qdeg}, GroupOut{e->P, e->tile_lo}, &e->S->tot, &e->S->wn, nv,
                                              e->scan_tmp, st);
     if (err != hipSuccess) return err;
+    const int sbits = std::max(1, ceil_log2_host((int64_t)nv + 1));
     ms_expand_kernel<false><<<xgrid, BLOCK, 0, st>>>(e->S, e->P, e->tile_lo, e->qdeg, e->qoff, e->qmask, e->list,
                                                      g->arena, e->visited,
-                                                     reinterpret_cast<unsigned long long*>(e->next));
+                                                     reinterpret_cast<unsigned long long*>(e->next), e->dirty,
+                                                     e->w_ckey.as<uint64_t>(), e->w_cS.as<uint64_t>(), e->d_nchg,
+                                                     sbits);
     ms_expand_kernel<true><<<xgrid, BLOCK, 0, st>>>(e->S, e->P, e->tile_lo, e->qdeg, e->qoff, e->qmask, e->list,
                                                     g->arena, e->frontier,
-                                                    reinterpret_cast<unsigned long long*>(e->next));
+                                                    reinterpret_cast<unsigned long long*>(e->next), e->dirty,
+                                                    e->w_ckey.as<uint64_t>(), e->w_cS.as<uint64_t>(), e->d_nchg,
+                                                    sbits);
+    e->exec_cS = e->w_cS.ptr;
+    e->exec_ckey = e->w_ckey.ptr;
     ms_pull_heavy_kernel<<<wgrid, BLOCK, 0, st>>>(words, e->S, e->hbits, g->rec, g->arena, e->frontier, e->visited,
                                                   e->dirty, e->next);
     ms_side_kernel<<<wgrid, BLOCK, 0, st>>>(e->S, e->d_nside, e->side_uw.as<int2>(), e->side_sm.as<uint64_t>(),
                                             e->frontier, e->visited, reinterpret_cast<unsigned long long*>(e->next));
     ms_collect_kernel<<<(unsigned)e->nparts, BLOCK, 0, st>>>(nv, e->degs, e->visited, e->frontier, e->next, dist, e->S,
-                                                             P.T, e->fbits, e->obits, e->hbits, e->part);
+                                                             P.T, e->dirty, e->fbits, e->obits, e->hbits,
+                                                             e->part);
     return hipGetLastError();
 }
 
@@ -1164,51 +1193,38 @@ hn_status ms_window_end(MsEngine* e, int32_t m, uint64_t failed, const uint64_t*
         HN_CHECK_LAUNCH();
         HN_TRY(ms_read(e, cpos + n, 4, &nchg, st));
     }
-    // dirty vertices and their stored degrees
-    HN_TRY(e->w_dpos.reserve((size_t)(nv + 1) * 4));
-    uint32_t* dpos = e->w_dpos.as<uint32_t>();
-    HN_TRY(exclusive_scan<uint32_t>(DirtyIn{e->dirty}, dpos, nv, e->w_tmp.ptr, st));
-    uint32_t nd = 0;
-    HN_TRY(ms_read(e, dpos + nv, 4, &nd, st));
-    HN_TRY(e->w_dlist.reserve((size_t)(nd > 0 ? nd : 1) * 4));
-    HN_TRY(e->w_P.reserve((size_t)(nd + 1) * 8));
-    int32_t* dlist = e->w_dlist.as<int32_t>();
-    int64_t* P = e->w_P.as<int64_t>();
-    if (nv > 0) win_dlist_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(nv, e->dirty, dpos, dlist);
-    HN_CHECK_LAUNCH();
-    HN_TRY(exclusive_scan<int64_t>(DirtyDegIn{dlist, g->rec}, P, nd, e->w_tmp.ptr, st));
-    int64_t total = 0;
-    HN_TRY(ms_read(e, P + nd, 8, &total, st));
-    // side records: stored edges of the dirty vertices, then the changed pairs
-    const int64_t cap = total + nchg;
+    // sources per changed pair; the pairs absent from storage become side records
+    HN_TRY(e->w_cS.reserve((size_t)(nchg > 0 ? nchg : 1) * 8));
+    HN_TRY(e->w_sflag.reserve((size_t)(2 * (int64_t)nchg + 2) * 4 + 256));
     void* old_uw = e->side_uw.ptr;
     void* old_sm = e->side_sm.ptr;
-    HN_TRY(e->side_uw.reserve((size_t)(cap > 0 ? cap : 1) * 8));
-    HN_TRY(e->side_sm.reserve((size_t)(cap > 0 ? cap : 1) * 8));
-    if (e->side_uw.ptr != old_uw || e->side_sm.ptr != old_sm) {   // the captured level step holds the pointers
+    HN_TRY(e->side_uw.reserve((size_t)(nchg > 0 ? nchg : 1) * 8));
+    HN_TRY(e->side_sm.reserve((size_t)(nchg > 0 ? nchg : 1) * 8));
+    if (e->side_uw.ptr != old_uw || e->side_sm.ptr != old_sm || e->w_cS.ptr != e->exec_cS ||
+        e->w_ckey.ptr != e->exec_ckey) {   // the captured level step holds these pointers
         if (e->exec) (void)hipGraphExecDestroy(e->exec);
         e->exec = nullptr;
     }
-    int2* uw = e->side_uw.as<int2>();
-    uint64_t* sm = e->side_sm.as<uint64_t>();
-    if (total > 0) {
-        const int64_t tile = (int64_t)BLOCK * WIN_IPT;
-        const int64_t grid = std::min<int64_t>(ceil_div(total, tile), (int64_t)num_cus() * 16);
-        win_side_fill_kernel<<<(unsigned)grid, BLOCK, 0, st>>>((int64_t)nd, total, P, dlist, g->rec, g->arena, sbits,
-                                                               e->w_ckey.as<uint64_t>(), (int64_t)nchg, uw, sm);
-        HN_CHECK_LAUNCH();
-    }
+    set_u32_kernel<<<1, 1, 0, st>>>(e->d_nchg, nchg);
+    HN_CHECK_LAUNCH();
     if (nchg > 0) {
         SideSV v;
         for (int t = 0; t < 64; t++) v.v[t] = t <= m ? sv[t] : 0ull;
-        win_side_chg_kernel<<<(unsigned)ceil_div(nchg, BLOCK), BLOCK, 0, st>>>(
-            (int64_t)nchg, total, e->w_ckey.as<uint64_t>(), e->w_cE.as<uint64_t>(), m, v, sbits, uw, sm);
+        uint32_t* sflag = e->w_sflag.as<uint32_t>();
+        uint32_t* spos = sflag + (nchg + 1);
+        win_cs_kernel<<<(unsigned)ceil_div(nchg, BLOCK), BLOCK, 0, st>>>(e->d_nchg, e->w_cE.as<uint64_t>(), m, v,
+                                                                         e->w_cS.as<uint64_t>(), sflag);
+        HN_CHECK_LAUNCH();
+        HN_TRY(exclusive_scan<uint32_t>(ArrayIn<uint32_t>{sflag}, spos, nchg, e->w_tmp.ptr, st));
+        win_side_write_kernel<<<(unsigned)ceil_div(nchg, BLOCK), BLOCK, 0, st>>>(
+            e->d_nchg, sflag, spos, e->w_ckey.as<uint64_t>(), e->w_cS.as<uint64_t>(), sbits,
+            e->side_uw.as<int2>(), e->side_sm.as<uint64_t>(), e->d_nside);
+        HN_CHECK_LAUNCH();
+    } else {
+        set_u32_kernel<<<1, 1, 0, st>>>(e->d_nside, 0u);
         HN_CHECK_LAUNCH();
     }
-    set_u32_kernel<<<1, 1, 0, st>>>(e->d_nside, (uint32_t)cap);
-    HN_CHECK_LAUNCH();
     HN_HIP(hipStreamSynchronize(st));
-    e->w_ndirty = nd;
     return HN_OK;
 }
 
@@ -1216,9 +1232,9 @@ hn_status ms_window_clear(MsEngine* e, hipStream_t st) {
     if (!e->has_window) return HN_OK;
     HN_HIP(hipMemsetAsync(e->dirty, 0, (size_t)(e->nv > 0 ? e->nv : 1), st));
     HN_HIP(hipMemsetAsync(e->d_nside, 0, 4, st));
+    HN_HIP(hipMemsetAsync(e->d_nchg, 0, 4, st));
     e->has_window = false;
     e->w_n = 0;
-    e->w_ndirty = 0;
     return HN_OK;
 }
 
