@@ -146,6 +146,10 @@ hn_status ms_window_begin(MsEngine* e, const int32_t* h_src, const int32_t* h_ds
                           hipStream_t st, bool* symmetric);
 hn_status ms_window_end(MsEngine* e, int32_t m, uint64_t failed, const uint64_t* sv, hipStream_t st);
 hn_status ms_window_clear(MsEngine* e, hipStream_t st);
+// ms_run_engine with dist == nullptr runs in digest mode: no distance matrix;
+// the device array returned here then holds {reached, distance sum,
+// eccentricity} per source (k x 3 uint64), ordered on the caller's stream.
+const unsigned long long* ms_engine_digest(const MsEngine* e);
 // device bytes one engine holds for a graph of nv vertices and ne edges
 int64_t ms_engine_bytes(int64_t nv, int64_t ne);
 }  // namespace hn

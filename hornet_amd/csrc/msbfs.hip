@@ -81,6 +81,7 @@ struct MsParams {
     int32_t T;          // pull: open vertices of degree > T take one wave each
     int32_t alpha;      // push while active_deg * alpha < open_deg
     int32_t push_only;  // adjacency not symmetric: pulling over out-edges would be wrong
+    int32_t digest;     // no distance matrix: per-source digests only
     int32_t depth_limit[64];
 };
 
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(WAVE) void ms_seed_kernel(int32_t k, const int32_t*
         atomicOr(reinterpret_cast<unsigned long long*>(visited + s), 1ull << b);
         atomicOr(reinterpret_cast<unsigned long long*>(frontier + s), 1ull << b);
         atomicOr(fbits + (s >> 5), 1u << (s & 31));
-        dist[(int64_t)b * nv + s] = 0;
+        if (dist) dist[(int64_t)b * nv + s] = 0;
         deg = (unsigned long long)rec[s].deg;
         first = 1;
         for (int c = 0; c < b; c++)
@@ -501,8 +502,9 @@ __global__ __launch_bounds__(BLOCK) void ms_pull_heavy_kernel(int64_t words, con
     }
 }
 
-// collect: new bits -> visited, frontier, distances; next level's bitsets and
-// this block's counters (active, active degree, open degree, edges)
+// collect: new bits -> visited, frontier, distances (or, in digest mode,
+// this block's per-source reached counts); next level's bitsets and this
+// block's counters (active, active degree, open degree, edges)
 __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const int32_t* __restrict__ degs,
                                                            uint64_t* __restrict__ visited,
                                                            uint64_t* __restrict__ frontier,
@@ -510,8 +512,12 @@ __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const int
                                                            const MsState* S, int32_t T, const uint8_t* __restrict__ dirty,
                                                            uint32_t* __restrict__ fbits,
                                                            uint32_t* __restrict__ obits, uint32_t* __restrict__ hbits,
-                                                           unsigned long long* __restrict__ part) {
+                                                           unsigned long long* __restrict__ part, int32_t digest,
+                                                           unsigned long long* __restrict__ dgp) {
     if (S->done) return;
+    __shared__ unsigned int dcnt[64];
+    if (threadIdx.x < 64) dcnt[threadIdx.x] = 0;
+    __syncthreads();
     const int32_t level = S->level;
     const uint64_t dmn = S->depth_mask[level < 63 ? level : 63];
     // bits still expandable after this level: a superset of the next live set
@@ -545,12 +551,22 @@ __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const int
         odeg += miss ? (unsigned long long)deg : 0ull;
         edges += (unsigned long long)deg * (unsigned long long)__popcll(nb);
         fany |= f;
-        // distances, one source bit at a time so that lanes store consecutive words
+        // distances, one source bit at a time so that lanes store consecutive
+        // words; in digest mode one count per source bit and wave
         uint64_t M = wave_or64(nb);
-        while (M) {
-            const int b = __ffsll((unsigned long long)M) - 1;
-            M &= M - 1;
-            if ((nb >> b) & 1ull) dist[(int64_t)b * nv + u] = level;
+        if (digest) {
+            while (M) {
+                const int b = __ffsll((unsigned long long)M) - 1;
+                M &= M - 1;
+                const unsigned int c = (unsigned int)__popcll((unsigned long long)__ballot((nb >> b) & 1ull));
+                if (lane == 0) atomicAdd(&dcnt[b], c);
+            }
+        } else {
+            while (M) {
+                const int b = __ffsll((unsigned long long)M) - 1;
+                M &= M - 1;
+                if ((nb >> b) & 1ull) dist[(int64_t)b * nv + u] = level;
+            }
         }
         // next level's bitsets: one 64-bit ballot = two bitset words
         const uint64_t bf = (uint64_t)__ballot(inF), bo = (uint64_t)__ballot(inO), bh = (uint64_t)__ballot(inH);
@@ -576,14 +592,63 @@ __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const int
         for (int w = 0; w < NWAVES; w++) t = threadIdx.x == 4 ? (t | red[4][w]) : t + red[threadIdx.x][w];
         part[(int64_t)blockIdx.x * MS_NPART + threadIdx.x] = t;
     }
+    // digest mode: this block's slot accumulates over the levels (it always
+    // covers the same vertices), so no atomics
+    if (digest && threadIdx.x < 64) {
+        const unsigned int cnt = dcnt[threadIdx.x];
+        if (cnt) {
+            unsigned long long* p = dgp + ((int64_t)blockIdx.x * 64 + threadIdx.x) * 3;
+            p[0] += cnt;
+            p[1] += (unsigned long long)cnt * (unsigned long long)level;
+            p[2] = (unsigned long long)level;   // levels only grow
+        }
+    }
+}
+
+// digest[b] = {reached, distance sum, eccentricity} of source b: the source
+// itself plus every collect block's slot
+__global__ __launch_bounds__(BLOCK) void ms_digest_kernel(const unsigned long long* __restrict__ dgp, int64_t nparts,
+                                                          unsigned long long* __restrict__ digest) {
+    const int b = blockIdx.x;
+    unsigned long long r = 0, s = 0, m = 0;
+    for (int64_t i = threadIdx.x; i < nparts; i += BLOCK) {
+        const unsigned long long* p = dgp + (i * 64 + b) * 3;
+        r += p[0];
+        s += p[1];
+        m = p[2] > m ? p[2] : m;
+    }
+    r = wave_reduce_sum(r);
+    s = wave_reduce_sum(s);
+    for (int d = WAVE / 2; d > 0; d >>= 1) {
+        const unsigned long long o = __shfl_xor(m, d, WAVE);
+        m = o > m ? o : m;
+    }
+    __shared__ unsigned long long red[3][NWAVES];
+    if (__lane_id() == 0) {
+        red[0][threadIdx.x / WAVE] = r;
+        red[1][threadIdx.x / WAVE] = s;
+        red[2][threadIdx.x / WAVE] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tr = 1, ts = 0, tm = 0;   // the source, at distance 0
+        for (int w = 0; w < NWAVES; w++) {
+            tr += red[0][w];
+            ts += red[1][w];
+            tm = red[2][w] > tm ? red[2][w] : tm;
+        }
+        digest[b * 3] = tr;
+        digest[b * 3 + 1] = ts;
+        digest[b * 3 + 2] = tm;
+    }
 }
 
 // ---- multi-version windows (the served stream) -----------------------------
 // One launch may serve queries of several graph versions: storage holds the
-// newest version of the window; a vertex touched by the window's updates is
-// dirty, the main kernels skip its adjacency, and side records (u, w, S)
-// relax each of its edges in both directions for the sources S whose
-// version holds the edge.
+// newest version of the window.  A pair whose existence differs between the
+// window's versions is a changed pair: the main kernels mask it per edge
+// with the sources whose version holds it, and the changed pairs absent from
+// storage are side records relaxed by one more kernel per level.
 
 struct SideSV {
     uint64_t v[64];   // sources whose graph version is window version t
@@ -793,6 +858,8 @@ struct MsEngine {
     uint8_t* dirty = nullptr;         // nv flags (vertex has changed pairs), in `mem`
     uint32_t* d_nside = nullptr;      // side record count, in `mem`
     uint32_t* d_nchg = nullptr;       // changed pair count, in `mem`
+    unsigned long long* dg_part = nullptr;   // digest mode: per collect block x source {reached, sum, max}
+    unsigned long long* digest = nullptr;    // digest mode: per source {reached, sum, max}
     Scratch side_uw, side_sm;         // side records (u, w) and source masks
     Scratch w_keys, w_alt, w_tmp, w_seg, w_e0, w_E, w_chg, w_ckey, w_cE, w_cS, w_sflag;
     void* exec_cS = nullptr;          // window buffers the captured level step was built with
@@ -853,6 +920,8 @@ size_t ms_layout(int64_t nv, int64_t tiles, int64_t nparts, MsEngine* e, char* b
     char* p_dy = take((size_t)n);
     char* p_ns = take(4);
     char* p_nc = take(4);
+    char* p_dp = take((size_t)nparts * 64 * 3 * 8);
+    char* p_dd = take(64 * 3 * 8);
     char* p_st = take(std::max(scan_temp_bytes(n), scan_temp_bytes(words)));
     char* p_S = take(sizeof(MsState));
     char* p_src = take(64 * 4);
@@ -876,6 +945,8 @@ size_t ms_layout(int64_t nv, int64_t tiles, int64_t nparts, MsEngine* e, char* b
         e->dirty = reinterpret_cast<uint8_t*>(p_dy);
         e->d_nside = reinterpret_cast<uint32_t*>(p_ns);
         e->d_nchg = reinterpret_cast<uint32_t*>(p_nc);
+        e->dg_part = reinterpret_cast<unsigned long long*>(p_dp);
+        e->digest = reinterpret_cast<unsigned long long*>(p_dd);
         e->scan_tmp = p_st;
         e->S = reinterpret_cast<MsState*>(p_S);
         e->d_src = reinterpret_cast<int32_t*>(p_src);
@@ -1006,7 +1077,7 @@ static hipError_t ms_level(MsEngine* e, int32_t* dist, const MsParams& P, hipStr
                                             e->frontier, e->visited, reinterpret_cast<unsigned long long*>(e->next));
     ms_collect_kernel<<<(unsigned)e->nparts, BLOCK, 0, st>>>(nv, e->degs, e->visited, e->frontier, e->next, dist, e->S,
                                                              P.T, e->dirty, e->fbits, e->obits, e->hbits,
-                                                             e->part);
+                                                             e->part, P.digest, e->dg_part);
     return hipGetLastError();
 }
 
@@ -1041,6 +1112,7 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
     HN_TRY(graph_is_symmetric(g, st, &sym));
     MsParams P = e->params;
     P.push_only = sym ? 0 : 1;
+    P.digest = dist ? 0 : 1;
     for (int b = 0; b < 64; b++) P.depth_limit[b] = (depth_limit && b < k) ? depth_limit[b] : 0;
     const int64_t nv = g->nv;
     const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(nv, BLOCK), (int64_t)num_cus() * 8);
@@ -1050,8 +1122,12 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
         e->degs_version = g->version;
     }
     ms_init_kernel<<<grid, BLOCK, 0, st>>>(nv, e->visited, e->frontier, e->next, 3 * ms_bit_words(nv), e->fbits);
-    ms_fill_dist_kernel<<<(unsigned)std::min<int64_t>(ceil_div((int64_t)k * nv, 4 * BLOCK), (int64_t)num_cus() * 16),
-                          BLOCK, 0, st>>>((int64_t)k * nv, dist);
+    if (dist)
+        ms_fill_dist_kernel<<<(unsigned)std::min<int64_t>(ceil_div((int64_t)k * nv, 4 * BLOCK),
+                                                          (int64_t)num_cus() * 16),
+                              BLOCK, 0, st>>>((int64_t)k * nv, dist);
+    else
+        HN_HIP(hipMemsetAsync(e->dg_part, 0, (size_t)e->nparts * 64 * 3 * 8, st));
     ms_seed_kernel<<<1, WAVE, 0, st>>>(k, e->d_src, nv, e->visited, e->frontier, e->fbits, dist, g->rec, e->S, P);
     HN_CHECK_LAUNCH();
     // capture the level step once per (stream, dist buffer, parameters, buffers)
@@ -1088,6 +1164,10 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
         HN_HIP(hipMemcpyAsync(host, &e->S->done, 4, hipMemcpyDeviceToHost, st));
         HN_HIP(hipStreamSynchronize(st));
         if (host[0]) break;
+    }
+    if (!dist) {
+        ms_digest_kernel<<<(unsigned)k, BLOCK, 0, st>>>(e->dg_part, e->nparts, e->digest);
+        HN_CHECK_LAUNCH();
     }
     if (stats) {
         MsState s;
@@ -1227,6 +1307,8 @@ hn_status ms_window_end(MsEngine* e, int32_t m, uint64_t failed, const uint64_t*
     HN_HIP(hipStreamSynchronize(st));
     return HN_OK;
 }
+
+const unsigned long long* ms_engine_digest(const MsEngine* e) { return e->digest; }
 
 hn_status ms_window_clear(MsEngine* e, hipStream_t st) {
     if (!e->has_window) return HN_OK;

@@ -13,10 +13,11 @@
 // * An update is applied once every launch before it has finished, and the
 //   queries after it wait for it: each query sees exactly the updates that
 //   arrived before it.  Its graph version is reported with the result.
-// * Slots are admitted against a device-memory budget: one slot holds an
-//   engine (3 x 8 B per vertex) and a distance block (max_batch x 4 B per
-//   vertex); an update is refused (HN_E_OOM result) when its scratch estimate
-//   does not fit what the slots leave.
+// * Slots are admitted against a device-memory budget: one slot holds a
+//   multi-source engine (masks, work lists, load-balancing buffers); batched
+//   launches keep only per-source digests, never k x nv distances.  An update
+//   is refused (HN_E_OOM result) when its scratch estimate does not fit what
+//   the slots leave.
 // * Results are digests computed on the device (reached count, distance sum,
 //   eccentricity), so a batch returns 24 B per query instead of its vectors.
 #include <algorithm>
@@ -155,9 +156,16 @@ static void worker_loop(hn_server* s, ServeSlot* slot) {
         if (s->cfg.plain && k == 1 && batch[0]->kind == HN_REQ_BFS) {
             st = hn_bfs_topdown(g, srcs[0], slot->dist, nullptr, slot->stream);
         } else {
-            st = ms_run_engine(slot->eng, srcs, k, limits, slot->dist, nullptr, slot->stream);
+            // digest mode: the launch never materialises k x nv distances
+            st = ms_run_engine(slot->eng, srcs, k, limits, nullptr, nullptr, slot->stream);
+            hipError_t e = st == HN_OK ? hipMemcpyAsync(slot->h_digest, ms_engine_digest(slot->eng),
+                                                        sizeof(unsigned long long) * 3 * (size_t)k,
+                                                        hipMemcpyDeviceToHost, slot->stream)
+                                       : hipSuccess;
+            if (e == hipSuccess && st == HN_OK) e = hipStreamSynchronize(slot->stream);
+            if (e != hipSuccess) st = HN_E_HIP;
         }
-        if (st == HN_OK) {
+        if (st == HN_OK && s->cfg.plain && k == 1 && batch[0]->kind == HN_REQ_BFS) {
             hipError_t e = hipMemsetAsync(slot->digest, 0, sizeof(unsigned long long) * 3 * 64, slot->stream);
             if (e == hipSuccess && nv > 0) {
                 dim3 grid((unsigned)std::min<int64_t>(ceil_div(nv, BLOCK), 1024), (unsigned)k);
@@ -225,7 +233,7 @@ static void record_update(hn_server* s, const ServeReq* r, hn_status st, int64_t
 // updates between them (at most 63, WINDOW_MAX_ENTRIES entries), ending at
 // its last query.  One launch serves all its queries, each at the graph
 // version it arrived at.
-constexpr int64_t WINDOW_MAX_ENTRIES = 1 << 21;
+constexpr int64_t WINDOW_MAX_ENTRIES = 1 << 23;
 struct Window {
     int nq = 0, nu = 0;
     size_t len = 0;
@@ -417,11 +425,12 @@ static void dispatcher_loop(hn_server* s) {
     }
 }
 
-// device bytes one launch slot holds: an engine plus k * nv distances and
-// the digests
-static int64_t slot_bytes(const hn_graph* g, int64_t batch) {
+// device bytes one launch slot holds: an engine and the digests (batched
+// launches run in digest mode and never hold k x nv distances); the plain
+// form adds one distance row for its single-source BFS
+static int64_t slot_bytes(const hn_graph* g, bool plain) {
     const int64_t nv = g->nv > 0 ? g->nv : 1;
-    return ms_engine_bytes(nv, g->ne) + batch * nv * 4 + 64 * 24;
+    return ms_engine_bytes(nv, g->ne) + (plain ? nv * 4 : 0) + 64 * 24;
 }
 
 static void destroy_server(hn_server* s) {
@@ -453,7 +462,7 @@ extern "C" {
 hn_status hn_server_slot_bytes(const hn_graph* g, int32_t max_batch, int64_t* bytes) {
     HN_ARG(g && bytes, "null argument");
     HN_ARG(max_batch >= 1 && max_batch <= 64, "max_batch must be in [1, 64]");
-    *bytes = slot_bytes(g, max_batch);
+    *bytes = slot_bytes(g, false);
     return HN_OK;
 }
 
@@ -466,8 +475,7 @@ hn_status hn_server_create(hn_graph* g, const hn_serve_config* cfg, hn_server** 
     s->g = g;
     s->cfg = *cfg;
     const int64_t nv = g->nv > 0 ? g->nv : 1;
-    const int64_t batch = cfg->plain ? 1 : cfg->max_batch;
-    s->slot_bytes = slot_bytes(g, batch);
+    s->slot_bytes = slot_bytes(g, cfg->plain != 0);
     int64_t budget = cfg->mem_budget_bytes;
     if (budget <= 0) {
         size_t fr = 0, tot = 0;
@@ -504,7 +512,7 @@ hn_status hn_server_create(hn_graph* g, const hn_serve_config* cfg, hn_server** 
     for (int i = 0; i < nslots; i++) {
         auto sl = std::make_unique<ServeSlot>();
         hn_status st = ms_engine_create(g, &sl->eng);
-        if (st == HN_OK) st = dalloc(&sl->dist, (size_t)(batch * nv));
+        if (st == HN_OK) st = dalloc(&sl->dist, (size_t)(cfg->plain ? nv : 1));
         if (st == HN_OK) st = dalloc(&sl->digest, 64 * 3);
         if (st == HN_OK && (hipHostMalloc(reinterpret_cast<void**>(&sl->h_digest), 64 * 3 * 8, 0) != hipSuccess ||
                             hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess)) {

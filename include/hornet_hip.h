@@ -261,8 +261,9 @@ hn_status hn_server_drain(hn_server* s);
 hn_status hn_server_results(hn_server* s, hn_result* out, int64_t cap, int64_t* n);
 /* Slots admitted after the memory budget. */
 hn_status hn_server_slots(hn_server* s, int32_t* slots);
-/* Device bytes one launch slot of a server on g with this max_batch takes
- * from the memory budget (multi-source engine + max_batch * nv distances). */
+/* Device bytes one launch slot of a batching server on g takes from the
+ * memory budget (the multi-source engine; batched launches keep per-source
+ * digests only, so max_batch does not change it). */
 hn_status hn_server_slot_bytes(const hn_graph* g, int32_t max_batch, int64_t* bytes);
 hn_status hn_server_destroy(hn_server* s);
 

@@ -106,7 +106,7 @@ def test_budget_admission(hn):
     h = hn.Hornet.from_csr(off, adj)
     n = off.size - 1
     one_slot = hn.Server.slot_bytes(h, 64)
-    assert one_slot > 64 * n * 4
+    assert one_slot > 24 * n
     with pytest.raises(HornetError):
         hn.Server(h, max_batch=64, slots=4, mem_budget_bytes=one_slot // 2)
     srv = hn.Server(h, max_batch=64, slots=4, mem_budget_bytes=int(one_slot * 2.5))
