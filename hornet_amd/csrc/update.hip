@@ -208,9 +208,11 @@ __global__ void insert_alloc_kernel(const uint32_t* __restrict__ ucnt, const uin
         t.p = (int32_t)lower_bound_i32(arena + r.off, r.deg, f.dst(keys[seg_start[j]]));
     } else {
         t.new_cap = class_of(t.new_deg);
-        t.new_off = alloc_block(A, t.new_cap);
         t.p = 0;
     }
+    // every active lane takes part (active lanes are a prefix of the wave)
+    const int64_t blk = alloc_block_wave(A, t.inplace ? 0 : t.new_cap, !t.inplace);
+    if (!t.inplace) t.new_off = blk;
     ti[j] = t;
 }
 
@@ -336,15 +338,18 @@ __global__ void commit_kernel(const uint32_t* __restrict__ ucnt, const uint64_t*
                               const TouchInfo* __restrict__ ti, const uint8_t* __restrict__ skip,
                               hn_vrec* __restrict__ rec, AllocView A, const UpdScal* sc) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= (int64_t)*ucnt || sc->err) return;
-    if (skip && skip[j]) return;
+    if (j >= (int64_t)*ucnt || sc->err) return;   // a suffix of the wave
+    const bool live = !(skip && skip[j]);
     const TouchInfo t = ti[j];
-    hn_vrec r;
-    r.off = t.new_cap >= 0 ? t.new_off : 0;
-    r.deg = t.new_deg;
-    r.cap_log2 = t.new_cap;
-    rec[f.src(seg_key[j])] = r;
-    if (!t.inplace && t.old_cap >= 0) free_block(A, t.old_cap, t.old_off);
+    if (live) {
+        hn_vrec r;
+        r.off = t.new_cap >= 0 ? t.new_off : 0;
+        r.deg = t.new_deg;
+        r.cap_log2 = t.new_cap;
+        rec[f.src(seg_key[j])] = r;
+    }
+    const bool rel = live && !t.inplace && t.old_cap >= 0;
+    free_block_wave(A, rel ? t.old_cap : 0, t.old_off, rel);
 }
 
 // ---------------------------------------------------------------- erase
@@ -400,6 +405,7 @@ __global__ void erase_alloc_kernel(const uint32_t* __restrict__ ucnt, const uint
     t.new_off = 0;
     t.new_cap = class_of(t.new_deg);
     skip[u] = del == 0 ? 1 : 0;
+    bool need = false;
     if (del > 0 && t.new_cap >= 0) {
         if (t.new_cap == r.cap_log2) {
             t.inplace = 1;
@@ -412,9 +418,12 @@ __global__ void erase_alloc_kernel(const uint32_t* __restrict__ ucnt, const uint
                 }
             }
         } else {
-            t.new_off = alloc_block(A, t.new_cap);
+            need = true;
         }
     }
+    // every active lane takes part (active lanes are a prefix of the wave)
+    const int64_t blk = alloc_block_wave(A, need ? t.new_cap : 0, need);
+    if (need) t.new_off = blk;
     ti[u] = t;
 }
 
