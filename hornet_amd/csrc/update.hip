@@ -427,50 +427,68 @@ __global__ void erase_alloc_kernel(const uint32_t* __restrict__ ucnt, const uint
     ti[u] = t;
 }
 
-struct EraseWorkIn {
-    const TouchInfo* ti;
-    const uint8_t* skip;
+// Survivors move as runs: between two deleted ranges of a touched vertex u
+// (and after the last one) the old entries keep their order and shift down by
+// the number deleted before them.  Run jj (before distinct key jj; the tail
+// run of u is n + u) copies [x0, x0 + len) of u's old list -- from the staged
+// suffix when u is rewritten in place, from the old block otherwise -- to
+// x0 - shift in u's block.  Sources carry GAP_STAGE when they index `stage`.
+constexpr int64_t GAP_STAGE = (int64_t)1 << 62;
+
+__global__ void erase_gaps_kernel(const uint32_t* __restrict__ ucnt, int64_t n, const uint32_t* __restrict__ ustart,
+                                  const uint32_t* __restrict__ dpos, const uint32_t* __restrict__ dlen,
+                                  const uint32_t* __restrict__ dlp, const TouchInfo* __restrict__ ti,
+                                  const uint8_t* __restrict__ skip, int64_t* __restrict__ glen,
+                                  int64_t* __restrict__ gsrc, int64_t* __restrict__ gdst, const UpdScal* sc) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= (int64_t)*ucnt) return;
+    const uint32_t a = ustart[u], b = ustart[u + 1];
+    const bool live = !sc->err && !skip[u];
+    const TouchInfo t = ti[u];
+    const int64_t p = t.p;
+    int64_t prev = 0;   // end of the previous deleted range
+    for (uint32_t jj = a; jj <= b; jj++) {
+        const int64_t start = prev > p ? prev : p;
+        const int64_t end = jj < b ? (int64_t)dpos[jj] : (int64_t)t.old_deg;
+        const int64_t len = (live && end > start) ? end - start : 0;
+        const int64_t shift = (int64_t)(dlp[jj] - dlp[a]);
+        const int64_t g = jj < b ? (int64_t)jj : n + u;
+        glen[g] = len;
+        if (len) {
+            gsrc[g] = t.inplace ? (GAP_STAGE | (t.soff + (start - p))) : (t.old_off + start);
+            gdst[g] = t.new_off + start - shift;
+        }
+        if (jj < b) prev = (int64_t)dpos[jj] + dlen[jj];
+    }
+}
+
+// run lengths: runs of the distinct keys [0, dcnt) and tail runs [n, n + ucnt)
+struct GapLenIn {
+    const int64_t* glen;
+    const uint32_t* dcnt;
     const uint32_t* ucnt;
-    const UpdScal* sc;
-    __device__ __forceinline__ int64_t operator()(int64_t u) const {
-        if (u >= (int64_t)*ucnt || sc->err || skip[u]) return 0;
-        const TouchInfo& t = ti[u];
-        return t.new_deg > 0 ? (int64_t)(t.old_deg - t.p) : 0;
+    int64_t n;
+    __device__ __forceinline__ int64_t operator()(int64_t g) const {
+        const bool valid = g < n ? g < (int64_t)*dcnt : (g - n) < (int64_t)*ucnt;
+        return valid ? glen[g] : 0;
     }
 };
 
-// survivors of source u's old list from slot p on, in order, into the block
-__global__ __launch_bounds__(BLOCK) void erase_compact_kernel(int64_t nseg, const int64_t* __restrict__ P,
-                                                              const TouchInfo* __restrict__ ti,
-                                                              const uint32_t* __restrict__ ustart,
-                                                              const uint32_t* __restrict__ dpos,
-                                                              const uint32_t* __restrict__ dlen,
-                                                              const uint32_t* __restrict__ dlp,
-                                                              int32_t* __restrict__ arena,
-                                                              const int32_t* __restrict__ stage, const UpdScal* sc) {
+__global__ __launch_bounds__(BLOCK) void erase_move_kernel(int64_t nruns, const int64_t* __restrict__ P,
+                                                           const int64_t* __restrict__ gsrc,
+                                                           const int64_t* __restrict__ gdst,
+                                                           int32_t* __restrict__ arena,
+                                                           const int32_t* __restrict__ stage, const UpdScal* sc) {
     __shared__ LbShared<UPD_IPT, int64_t> sh;
     if (sc->err) return;
-    const int64_t W = P[nseg];
+    const int64_t W = P[nruns];
     for (int64_t w0 = (int64_t)blockIdx.x * UPD_TILE; w0 < W; w0 += (int64_t)gridDim.x * UPD_TILE) {
         lb_tile<UPD_IPT>(
-            P, nseg, w0, W, sh, [&](int64_t s) { return P[s]; },
-            [&](int64_t u, int64_t w, int64_t pu) {
-                const TouchInfo t = ti[u];
-                const AView A = a_view(t, arena, stage);
-                const int64_t x = t.p + (w - pu);
-                const int64_t a = ustart[u], b = ustart[u + 1];
-                // largest jj in [a, b) with dpos[jj] <= x
-                int64_t lo = a - 1, hi = b;
-                while (hi - lo > 1) {
-                    int64_t mid = (lo + hi) >> 1;
-                    if ((int64_t)dpos[mid] <= x) lo = mid; else hi = mid;
-                }
-                int64_t before = 0;
-                if (lo >= a) {
-                    if (x < (int64_t)dpos[lo] + (int64_t)dlen[lo]) return;   // deleted
-                    before = (int64_t)(dlp[lo] - dlp[a]) + dlen[lo];
-                }
-                arena[t.new_off + (x - before)] = A[x];
+            P, nruns, w0, W, sh, [&](int64_t s) { return gsrc[s] - P[s]; },
+            [&](int64_t s, int64_t w, int64_t base) {
+                const int64_t src = base + w;
+                const int32_t v = (src & GAP_STAGE) ? stage[src & ~GAP_STAGE] : arena[src];
+                arena[gdst[s] - P[s] + w] = v;
             });
     }
 }
@@ -495,6 +513,10 @@ struct UpdBufs {
     int64_t* stpos;      // n+1
     TouchInfo* ti;       // n
     uint8_t* skip;       // n
+    int64_t* glen;       // 2n: erase runs (distinct keys, then tail runs)
+    int64_t* gsrc;       // 2n
+    int64_t* gdst;       // 2n
+    int64_t* gP;         // 2n+1
     UpdScal* sc;
     void* tmp;
 };
@@ -519,8 +541,12 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     int64_t o_stpos = o; o = al(o + n1 * 8);
     int64_t o_ti = o; o = al(o + n1 * (int64_t)sizeof(TouchInfo));
     int64_t o_skip = o; o = al(o + n1);
+    int64_t o_gl = o; o = al(o + 2 * n1 * 8);
+    int64_t o_gs = o; o = al(o + 2 * n1 * 8);
+    int64_t o_gd = o; o = al(o + 2 * n1 * 8);
+    int64_t o_gp = o; o = al(o + (2 * n1 + 1) * 8);
     int64_t o_sc = o; o = al(o + 64);
-    int64_t o_tmp = o; o = al(o + (int64_t)std::max(sort_temp_bytes(n1), scan_temp_bytes(n1)));
+    int64_t o_tmp = o; o = al(o + (int64_t)std::max(sort_temp_bytes(n1), scan_temp_bytes(2 * n1)));
     HN_TRY(g->s_upd.reserve((size_t)o));
     char* base = g->s_upd.as<char>();
     b->ka = reinterpret_cast<uint64_t*>(base + o_ka);
@@ -539,6 +565,10 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     b->stpos = reinterpret_cast<int64_t*>(base + o_stpos);
     b->ti = reinterpret_cast<TouchInfo*>(base + o_ti);
     b->skip = reinterpret_cast<uint8_t*>(base + o_skip);
+    b->glen = reinterpret_cast<int64_t*>(base + o_gl);
+    b->gsrc = reinterpret_cast<int64_t*>(base + o_gs);
+    b->gdst = reinterpret_cast<int64_t*>(base + o_gd);
+    b->gP = reinterpret_cast<int64_t*>(base + o_gp);
     b->sc = reinterpret_cast<UpdScal*>(base + o_sc);
     b->tmp = base + o_tmp;
     return HN_OK;
@@ -746,11 +776,13 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     HN_CHECK_LAUNCH();
     int32_t* stage = nullptr;
     HN_TRY(stage_suffixes(g, b, n, ucnt, b.skip, st, &stage));
-    // 6. ordered compaction of the survivors
-    HN_TRY(exclusive_scan<int64_t>(EraseWorkIn{b.ti, b.skip, ucnt, b.sc}, b.wpos, n, b.tmp, st));
+    // 6. ordered compaction of the survivors, as runs between deleted ranges
+    erase_gaps_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, n, b.seg2_start, b.dpos, b.dlen, b.dlp, b.ti, b.skip, b.glen,
+                                               b.gsrc, b.gdst, b.sc);
+    HN_CHECK_LAUNCH();
+    HN_TRY(exclusive_scan<int64_t>(GapLenIn{b.glen, dcnt, ucnt, n}, b.gP, 2 * n, b.tmp, st));
     const int64_t grid = (int64_t)num_cus() * 8;
-    erase_compact_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.wpos, b.ti, b.seg2_start, b.dpos, b.dlen, b.dlp,
-                                                           g->arena, stage, b.sc);
+    erase_move_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.gsrc, b.gdst, g->arena, stage, b.sc);
     HN_CHECK_LAUNCH();
     // 7. publish
     commit_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg2_key, f, b.ti, b.skip, g->rec, g->alloc_view(), b.sc);
