@@ -251,28 +251,76 @@ struct StageLenIn {
     }
 };
 
+// Buffers of the segmented copies: an index into the arena, or with
+// GAP_STAGE set into the staging buffer.
+constexpr int64_t GAP_STAGE = (int64_t)1 << 62;
+
+// the staged suffix of touched vertex j starts at spre[j]; its copy run
+// reads the old list from slot p
 __global__ void set_soff_kernel(const uint32_t* __restrict__ ucnt, const int64_t* __restrict__ spre,
-                                TouchInfo* __restrict__ ti, const UpdScal* sc) {
+                                TouchInfo* __restrict__ ti, int64_t* __restrict__ rsrc, int64_t* __restrict__ rdst,
+                                const UpdScal* sc) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= (int64_t)*ucnt || sc->err) return;
     ti[j].soff = spre[j];
+    const TouchInfo t = ti[j];
+    rsrc[j] = t.old_off + t.p;
+    rdst[j] = GAP_STAGE | spre[j];
 }
 
 // stage[soff_j + i] = A_j[p_j + i]
-__global__ __launch_bounds__(BLOCK) void stage_kernel(int64_t nseg, const int64_t* __restrict__ P,
-                                                      const TouchInfo* __restrict__ ti,
-                                                      const int32_t* __restrict__ arena, int32_t* __restrict__ stage,
-                                                      const UpdScal* sc) {
-    __shared__ LbShared<UPD_IPT, int64_t> sh;
+// Segmented copy: run s moves P[s+1] - P[s] entries from index src[s] on to
+// index dst[s] on.  A lane's items are located first, then all loads issue
+// (unconditionally: a lane past the end reads arena[0]), then all stores.
+struct CopyShared {
+    LbShared<UPD_IPT, int64_t> lb;   // base = src[s] - P[s]
+    int64_t dst[LbShared<UPD_IPT, int64_t>::CAP];
+};
+
+__global__ __launch_bounds__(BLOCK) void copy_runs_kernel(int64_t nruns, const int64_t* __restrict__ P,
+                                                          const int64_t* __restrict__ src,
+                                                          const int64_t* __restrict__ dst,
+                                                          int32_t* __restrict__ arena, int32_t* __restrict__ stage,
+                                                          const UpdScal* sc) {
+    __shared__ CopyShared sh;
     if (sc->err) return;
-    const int64_t W = P[nseg];
+    const int64_t W = P[nruns];
     for (int64_t w0 = (int64_t)blockIdx.x * UPD_TILE; w0 < W; w0 += (int64_t)gridDim.x * UPD_TILE) {
-        lb_tile<UPD_IPT>(
-            P, nseg, w0, W, sh, [&](int64_t s) { return P[s]; },
-            [&](int64_t j, int64_t w, int64_t pj) {
-                const TouchInfo t = ti[j];
-                stage[w] = arena[t.old_off + t.p + (w - pj)];
-            });
+        const int64_t wend = (w0 + (int64_t)UPD_TILE) < W ? (w0 + (int64_t)UPD_TILE) : W;
+        const int64_t lo = block_upper_bound_m1(P, 0, nruns + 1, w0);
+        const int64_t hi = block_upper_bound_m1(P, lo, nruns + 1, wend - 1);
+        const int count = lb_stage<UPD_IPT>(P, lo, hi, sh.lb, [&](int64_t s) { return src[s] - P[s]; });
+        if (count > 0)
+            for (int i = threadIdx.x; i < count; i += BLOCK) sh.dst[i] = dst[lo + i] - P[lo + i];
+        __syncthreads();
+        int64_t si[UPD_IPT], di[UPD_IPT];
+#pragma unroll
+        for (int j = 0; j < UPD_IPT; j++) {
+            const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+            if (count > 0) {
+                const int s = lb_locate<UPD_IPT>(sh.lb, count, w);
+                si[j] = sh.lb.base[s] + w;
+                di[j] = sh.dst[s] + w;
+            } else {
+                const int64_t s = upper_bound_m1<int64_t>(P, lo, lo + (int64_t)(-count), w);
+                si[j] = src[s] - P[s] + w;
+                di[j] = dst[s] - P[s] + w;
+            }
+            if (w >= wend) si[j] = 0, di[j] = -1;
+        }
+        int32_t v[UPD_IPT];
+#pragma unroll
+        for (int j = 0; j < UPD_IPT; j++) {
+            const int32_t* b = (si[j] & GAP_STAGE) ? stage : arena;
+            v[j] = b[si[j] & ~GAP_STAGE];
+        }
+#pragma unroll
+        for (int j = 0; j < UPD_IPT; j++) {
+            if (di[j] < 0) continue;
+            int32_t* b = (di[j] & GAP_STAGE) ? stage : arena;
+            b[di[j] & ~GAP_STAGE] = v[j];
+        }
+        __syncthreads();
     }
 }
 
@@ -432,33 +480,41 @@ __global__ void erase_alloc_kernel(const uint32_t* __restrict__ ucnt, const uint
 // the number deleted before them.  Run jj (before distinct key jj; the tail
 // run of u is n + u) copies [x0, x0 + len) of u's old list -- from the staged
 // suffix when u is rewritten in place, from the old block otherwise -- to
-// x0 - shift in u's block.  Sources carry GAP_STAGE when they index `stage`.
-constexpr int64_t GAP_STAGE = (int64_t)1 << 62;
+// x0 - shift in u's block.  One thread per distinct key and per touched
+// vertex; key jj belongs to vertex tpos[jj + 1] - 1.
+__device__ __forceinline__ void erase_run(int64_t g, int64_t prev, int64_t end, int64_t shift, bool live,
+                                          const TouchInfo& t, int64_t* __restrict__ glen, int64_t* __restrict__ gsrc,
+                                          int64_t* __restrict__ gdst) {
+    const int64_t p = t.p;
+    const int64_t start = prev > p ? prev : p;
+    const int64_t len = (live && end > start) ? end - start : 0;
+    glen[g] = len;
+    if (len) {
+        gsrc[g] = t.inplace ? (GAP_STAGE | (t.soff + (start - p))) : (t.old_off + start);
+        gdst[g] = t.new_off + start - shift;
+    }
+}
 
-__global__ void erase_gaps_kernel(const uint32_t* __restrict__ ucnt, int64_t n, const uint32_t* __restrict__ ustart,
+__global__ void erase_gaps_kernel(const uint32_t* __restrict__ dcnt, const uint32_t* __restrict__ ucnt, int64_t n,
+                                  const uint32_t* __restrict__ ustart, const uint32_t* __restrict__ tpos,
                                   const uint32_t* __restrict__ dpos, const uint32_t* __restrict__ dlen,
                                   const uint32_t* __restrict__ dlp, const TouchInfo* __restrict__ ti,
                                   const uint8_t* __restrict__ skip, int64_t* __restrict__ glen,
                                   int64_t* __restrict__ gsrc, int64_t* __restrict__ gdst, const UpdScal* sc) {
-    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= (int64_t)*ucnt) return;
-    const uint32_t a = ustart[u], b = ustart[u + 1];
-    const bool live = !sc->err && !skip[u];
-    const TouchInfo t = ti[u];
-    const int64_t p = t.p;
-    int64_t prev = 0;   // end of the previous deleted range
-    for (uint32_t jj = a; jj <= b; jj++) {
-        const int64_t start = prev > p ? prev : p;
-        const int64_t end = jj < b ? (int64_t)dpos[jj] : (int64_t)t.old_deg;
-        const int64_t len = (live && end > start) ? end - start : 0;
-        const int64_t shift = (int64_t)(dlp[jj] - dlp[a]);
-        const int64_t g = jj < b ? (int64_t)jj : n + u;
-        glen[g] = len;
-        if (len) {
-            gsrc[g] = t.inplace ? (GAP_STAGE | (t.soff + (start - p))) : (t.old_off + start);
-            gdst[g] = t.new_off + start - shift;
-        }
-        if (jj < b) prev = (int64_t)dpos[jj] + dlen[jj];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (int64_t)*dcnt) {   // the run before distinct key i
+        const int64_t u = (int64_t)tpos[i + 1] - 1;
+        const uint32_t a = ustart[u];
+        const int64_t prev = i > (int64_t)a ? (int64_t)dpos[i - 1] + dlen[i - 1] : 0;
+        erase_run(i, prev, (int64_t)dpos[i], (int64_t)(dlp[i] - dlp[a]), !sc->err && !skip[u], ti[u], glen, gsrc,
+                  gdst);
+    }
+    if (i < (int64_t)*ucnt) {   // the tail run of touched vertex i
+        const uint32_t a = ustart[i], b = ustart[i + 1];
+        const TouchInfo t = ti[i];
+        const int64_t prev = b > a ? (int64_t)dpos[b - 1] + dlen[b - 1] : 0;
+        erase_run(n + i, prev, (int64_t)t.old_deg, (int64_t)(dlp[b] - dlp[a]), !sc->err && !skip[i], t, glen, gsrc,
+                  gdst);
     }
 }
 
@@ -473,25 +529,6 @@ struct GapLenIn {
         return valid ? glen[g] : 0;
     }
 };
-
-__global__ __launch_bounds__(BLOCK) void erase_move_kernel(int64_t nruns, const int64_t* __restrict__ P,
-                                                           const int64_t* __restrict__ gsrc,
-                                                           const int64_t* __restrict__ gdst,
-                                                           int32_t* __restrict__ arena,
-                                                           const int32_t* __restrict__ stage, const UpdScal* sc) {
-    __shared__ LbShared<UPD_IPT, int64_t> sh;
-    if (sc->err) return;
-    const int64_t W = P[nruns];
-    for (int64_t w0 = (int64_t)blockIdx.x * UPD_TILE; w0 < W; w0 += (int64_t)gridDim.x * UPD_TILE) {
-        lb_tile<UPD_IPT>(
-            P, nruns, w0, W, sh, [&](int64_t s) { return gsrc[s] - P[s]; },
-            [&](int64_t s, int64_t w, int64_t base) {
-                const int64_t src = base + w;
-                const int32_t v = (src & GAP_STAGE) ? stage[src & ~GAP_STAGE] : arena[src];
-                arena[gdst[s] - P[s] + w] = v;
-            });
-    }
-}
 
 }  // namespace
 
@@ -650,7 +687,7 @@ static hn_status stage_suffixes(hn_graph* g, UpdBufs& b, int64_t n, const uint32
                                 hipStream_t st, int32_t** stage_out) {
     const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
     HN_TRY(exclusive_scan<int64_t>(StageLenIn{b.ti, ucnt, b.sc, skip}, b.stpos, n, b.tmp, st));
-    set_soff_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.stpos, b.ti, b.sc);
+    set_soff_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.stpos, b.ti, b.gsrc, b.gdst, b.sc);
     HN_CHECK_LAUNCH();
     int64_t total = 0;
     UpdScal h;
@@ -660,7 +697,7 @@ static hn_status stage_suffixes(hn_graph* g, UpdBufs& b, int64_t n, const uint32
     int32_t* stage = g->s_misc.as<int32_t>();
     if (total > 0) {
         const int64_t grid = std::min<int64_t>(ceil_div(total, UPD_TILE), (int64_t)num_cus() * 8);
-        stage_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.stpos, b.ti, g->arena, stage, b.sc);
+        copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.stpos, b.gsrc, b.gdst, g->arena, stage, b.sc);
         HN_CHECK_LAUNCH();
     }
     *stage_out = stage;
@@ -777,12 +814,12 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     int32_t* stage = nullptr;
     HN_TRY(stage_suffixes(g, b, n, ucnt, b.skip, st, &stage));
     // 6. ordered compaction of the survivors, as runs between deleted ranges
-    erase_gaps_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, n, b.seg2_start, b.dpos, b.dlen, b.dlp, b.ti, b.skip, b.glen,
-                                               b.gsrc, b.gdst, b.sc);
+    erase_gaps_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, ucnt, n, b.seg2_start, b.tpos, b.dpos, b.dlen, b.dlp, b.ti,
+                                               b.skip, b.glen, b.gsrc, b.gdst, b.sc);
     HN_CHECK_LAUNCH();
     HN_TRY(exclusive_scan<int64_t>(GapLenIn{b.glen, dcnt, ucnt, n}, b.gP, 2 * n, b.tmp, st));
     const int64_t grid = (int64_t)num_cus() * 8;
-    erase_move_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.gsrc, b.gdst, g->arena, stage, b.sc);
+    copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.gsrc, b.gdst, g->arena, stage, b.sc);
     HN_CHECK_LAUNCH();
     // 7. publish
     commit_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg2_key, f, b.ti, b.skip, g->rec, g->alloc_view(), b.sc);
