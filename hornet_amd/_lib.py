@@ -26,6 +26,11 @@ HN_DEDUP_GRAPH = 0x8
 HN_DROP_SELF_LOOPS = 0x10
 HN_SYMMETRIZE = 0x20
 
+# StructureProp / ParsingProp bits of the graph-file reader
+HN_GS_DIRECTED, HN_GS_UNDIRECTED, HN_GS_ENABLE_INGOING = 0x1, 0x2, 0x4
+HN_PARSE_RANDOMIZE, HN_PARSE_SORT, HN_PARSE_PRINT_INFO = 0x1, 0x2, 0x4
+HN_PARSE_RM_SINGLETON, HN_PARSE_DIRECTED_BY_DEGREE = 0x8, 0x10
+
 
 class HornetError(RuntimeError):
     def __init__(self, status: int, where: str, msg: str):
@@ -123,6 +128,14 @@ SIGNATURES = {
     "hn_server_destroy": [P],
     "hn_gen_chung_lu_pairs": [I32, D, I64, I64, U64, P, P, P],
     "hn_gen_chung_lu_graph": [I32, I32, D, U64, D, P, PP],
+    "hn_gstd_read": [ctypes.c_char_p, U32, U32, U64, PP],
+    "hn_gstd_destroy": [P],
+    "hn_gstd_info": [P, PI32, PI64, ctypes.POINTER(ctypes.c_uint32)],
+    "hn_gstd_csr_out": [P, PP, PP],
+    "hn_gstd_csr_in": [P, PP, PP],
+    "hn_gstd_max_out_degree_id": [P, PI32],
+    "hn_gstd_write_binary": [P, ctypes.c_char_p, I32],
+    "hn_gstd_write_market": [P, ctypes.c_char_p],
 }
 _RESTYPES = {"hn_status_string": ctypes.c_char_p, "hn_last_error": ctypes.c_char_p,
              "hn_version": ctypes.c_char_p}

@@ -312,6 +312,86 @@ static hn_status new_graph(int32_t nv, hn_graph** out) {
     return HN_OK;
 }
 
+// Key (row << sbits) | dst of CSR entry j, its row found by binary search of
+// the offsets; flags a row whose entries are not ascending (one atomic per
+// wave that saw one).
+__global__ void row_keys_kernel(int32_t nv, int64_t ne, const int64_t* __restrict__ off,
+                                const int32_t* __restrict__ adj, int sbits, uint64_t* __restrict__ keys,
+                                unsigned* __restrict__ unsorted) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool down = false;
+    if (j < ne) {
+        int64_t lo = 0, hi = nv;   // off[lo] <= j < off[hi]
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (off[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        const int32_t d = adj[j];
+        keys[j] = ((uint64_t)lo << sbits) | (uint64_t)(uint32_t)d;
+        down = j > off[lo] && adj[j - 1] > d;
+    }
+    if (__ballot(down) != 0 && (threadIdx.x & (WAVE - 1)) == 0) atomicOr(unsorted, 1u);
+}
+
+__global__ void keys_to_adj_kernel(int64_t ne, const uint64_t* __restrict__ keys, uint64_t mask,
+                                   int32_t* __restrict__ adj) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < ne) adj[j] = (int32_t)(keys[j] & mask);
+}
+
+// HornetInit accepts rows in any order (HornetInit.cuh:70-73) while the
+// storage keeps every adjacency ascending: *out = nullptr when the rows of
+// (d_off, d_adj) are already ascending, else a new device array holding each
+// row sorted (a radix sort of (row, dst) keys, which keeps the rows in place).
+static hn_status sort_rows_if_needed(int32_t nv, int64_t ne, const int64_t* d_off, const int32_t* d_adj,
+                                     hipStream_t st, int32_t** out) {
+    *out = nullptr;
+    if (ne == 0 || nv == 0) return HN_OK;
+    int sbits = ceil_log2_host((int64_t)nv + 1);
+    if (sbits < 1) sbits = 1;
+    uint64_t* ka = nullptr;
+    uint64_t* kb = nullptr;
+    unsigned* flag = nullptr;
+    void* tmp = nullptr;
+    int32_t* res = nullptr;
+    hn_status s = HN_OK;
+    auto cleanup = [&]() {
+        dfree(ka); dfree(kb); dfree(flag); dfree(tmp);
+    };
+    if ((s = dalloc(&ka, (size_t)ne)) != HN_OK || (s = dalloc(&flag, 1)) != HN_OK) {
+        cleanup();
+        return s;
+    }
+    HN_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned), st));
+    row_keys_kernel<<<(unsigned)ceil_div(ne, BLOCK), BLOCK, 0, st>>>(nv, ne, d_off, d_adj, sbits, ka, flag);
+    HN_CHECK_LAUNCH();
+    unsigned h = 0;
+    HN_HIP(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, st));
+    HN_HIP(hipStreamSynchronize(st));
+    if (!h) {
+        cleanup();
+        return HN_OK;
+    }
+    if ((s = dalloc(&kb, (size_t)ne)) != HN_OK || (s = dmalloc(&tmp, sort_temp_bytes(ne))) != HN_OK ||
+        (s = radix_sort_u64(&ka, &kb, ne, 2 * sbits, tmp, st)) != HN_OK ||
+        (s = dalloc(&res, (size_t)ne)) != HN_OK) {
+        cleanup();
+        return s;
+    }
+    keys_to_adj_kernel<<<(unsigned)ceil_div(ne, BLOCK), BLOCK, 0, st>>>(ne, ka, (1ull << sbits) - 1, res);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    cleanup();
+    if (e != hipSuccess) {
+        dfree(res);
+        set_error(std::string("row sort: ") + hipGetErrorString(e));
+        return HN_E_HIP;
+    }
+    *out = res;
+    return HN_OK;
+}
+
 hn_status graph_from_device_csr(int32_t nv, int64_t ne, const int64_t* d_off, const int32_t* d_adj,
                                 double slack, hipStream_t st, hn_graph** out) {
     hn_graph* g = nullptr;
@@ -506,12 +586,7 @@ hn_status hn_graph_create_csr(int32_t nv, int64_t ne, const void* offsets, const
             hoff[v] = o32 ? (int64_t)((const int32_t*)offsets)[v] : ((const int64_t*)offsets)[v];
         HN_ARG(hoff[0] == 0 && hoff[nv] == ne, "offsets[0] must be 0 and offsets[nv] == ne");
         for (int64_t v = 0; v < nv; v++) HN_ARG(hoff[v] <= hoff[v + 1], "offsets not monotone");
-        for (int64_t v = 0; v < nv; v++) {
-            for (int64_t j = hoff[v]; j < hoff[v + 1]; j++) {
-                HN_ARG(edges[j] >= 0 && edges[j] < nv, "edge endpoint outside [0, nv)");
-                HN_ARG(j == hoff[v] || edges[j - 1] <= edges[j], "adjacency not sorted ascending");
-            }
-        }
+        for (int64_t j = 0; j < ne; j++) HN_ARG(edges[j] >= 0 && edges[j] < nv, "edge endpoint outside [0, nv)");
     }
     int64_t* d_off = nullptr;
     int32_t* d_adj = nullptr;
@@ -534,9 +609,12 @@ hn_status hn_graph_create_csr(int32_t nv, int64_t ne, const void* offsets, const
         if (ne > 0) HN_HIP(hipMemcpyAsync(d_adj, edges, sizeof(int32_t) * (size_t)ne, hipMemcpyHostToDevice, st));
         adj_in = d_adj;
     }
-    s = graph_from_device_csr(nv, ne, d_off, adj_in, slack, st, out);
+    int32_t* d_sorted = nullptr;
+    s = sort_rows_if_needed(nv, ne, d_off, adj_in, st, &d_sorted);
+    if (s == HN_OK) s = graph_from_device_csr(nv, ne, d_off, d_sorted ? d_sorted : adj_in, slack, st, out);
     dfree(d_off);
     dfree(d_adj);
+    dfree(d_sorted);
     return s;
 }
 

@@ -9,6 +9,8 @@ Names, argument meaning and error behaviour follow the reference:
 * ``BatchUpdatePtr(n, src, dst)`` / ``BatchUpdate`` -- hornet/include/Core/BatchUpdate/BatchUpdate.cuh:95-197
 * ``BfsTopDown``                          -- hornetsnest/include/Static/BreadthFirstSearch/TopDown.cuh:52-70
 * ``TriangleCounting2``                   -- hornetsnest/include/Static/TriangleCounting/triangle2.cuh:18-37
+* ``GraphStd(structure).read(path, parsing)`` -- xlib/include/Graph/GraphStd.hpp,
+  xlib/src/Graph/GraphBase.cpp:130-206 (host graph files; no device needed)
 
 Device memory is held in torch tensors (plumbing only); every computation runs
 in libhornet_hip.so.  Where the reference exits the process on an error, these
@@ -60,6 +62,122 @@ class HornetInit:
         self.edges = np.ascontiguousarray(edges, dtype=np.int32)
         if self.offsets.size != self.nV + 1 or self.edges.size != self.nE:
             raise HornetError(_lib.HN_E_INVALID_ARG, "HornetInit", "size mismatch")
+
+
+class structure_prop:
+    """StructureProp bits (xlib/include/Graph/GraphBase.hpp:92-93,115-123)."""
+    NONE = 0
+    DIRECTED = _lib.HN_GS_DIRECTED
+    UNDIRECTED = _lib.HN_GS_UNDIRECTED
+    ENABLE_INGOING = _lib.HN_GS_ENABLE_INGOING
+
+
+class parsing_prop:
+    """ParsingProp bits (GraphBase.hpp:46-48); PRINT_INFO prints nothing."""
+    NONE = 0
+    RANDOMIZE = _lib.HN_PARSE_RANDOMIZE
+    SORT = _lib.HN_PARSE_SORT
+    PRINT_INFO = _lib.HN_PARSE_PRINT_INFO
+    RM_SINGLETON = _lib.HN_PARSE_RM_SINGLETON
+    DIRECTED_BY_DEGREE = _lib.HN_PARSE_DIRECTED_BY_DEGREE
+
+
+class GraphStd:
+    """Host graph read from a MatrixMarket (.mtx), DIMACS9 (.gr) or binary
+    (.bin) file: ``graph::GraphStd`` (xlib/include/Graph/GraphStd.hpp).
+
+    ``GraphStd(structure)`` then ``read(filename, parsing)`` as in the
+    reference drivers (hornetsnest/test/TriangleTest2.cu:86-87); the CSR
+    accessors return numpy views valid while the object lives.  Parsing and
+    the COO -> CSR pass run in libhornet_hip.so on the host.  RANDOMIZE takes
+    an explicit ``seed`` (the reference seeds from the clock).
+    """
+
+    def __init__(self, structure: int = structure_prop.NONE):
+        self._structure = int(structure)
+        self._h = None
+
+    def read(self, filename, parsing: int = parsing_prop.NONE, seed: int = 0) -> "GraphStd":
+        self._free()
+        h = ctypes.c_void_p()
+        call("hn_gstd_read", str(filename).encode(), self._structure, int(parsing), int(seed),
+             ctypes.byref(h))
+        self._h = h
+        return self
+
+    def _info(self):
+        if self._h is None:
+            raise HornetError(_lib.HN_E_INVALID_ARG, "GraphStd", "no graph read")
+        nv, ne, st = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_uint32()
+        call("hn_gstd_info", self._h, ctypes.byref(nv), ctypes.byref(ne), ctypes.byref(st))
+        return nv.value, ne.value, st.value
+
+    def nV(self) -> int:
+        return self._info()[0]
+
+    def nE(self) -> int:
+        return self._info()[1]
+
+    def is_directed(self) -> bool:
+        return bool(self._info()[2] & structure_prop.DIRECTED)
+
+    def is_undirected(self) -> bool:
+        return bool(self._info()[2] & structure_prop.UNDIRECTED)
+
+    def _csr(self, name):
+        nv, ne, _ = self._info()
+        o, e = ctypes.c_void_p(), ctypes.c_void_p()
+        call(name, self._h, ctypes.byref(o), ctypes.byref(e))
+        off = np.ctypeslib.as_array(ctypes.cast(o, ctypes.POINTER(ctypes.c_int64)), (nv + 1,))
+        adj = (np.ctypeslib.as_array(ctypes.cast(e, ctypes.POINTER(ctypes.c_int32)), (ne,))
+               if ne > 0 else np.empty(0, np.int32))
+        return off, adj
+
+    def csr_out_offsets(self) -> np.ndarray:
+        return self._csr("hn_gstd_csr_out")[0]
+
+    def csr_out_edges(self) -> np.ndarray:
+        return self._csr("hn_gstd_csr_out")[1]
+
+    def csr_in_offsets(self) -> np.ndarray:
+        return self._csr("hn_gstd_csr_in")[0]
+
+    def csr_in_edges(self) -> np.ndarray:
+        return self._csr("hn_gstd_csr_in")[1]
+
+    def out_degrees(self) -> np.ndarray:
+        return np.diff(self.csr_out_offsets()).astype(np.int32)
+
+    def max_out_degree_id(self) -> int:
+        self._info()
+        v = ctypes.c_int32()
+        call("hn_gstd_max_out_degree_id", self._h, ctypes.byref(v))
+        return v.value
+
+    def writeBinary(self, filename, offset_bytes: int = 4):
+        self._info()
+        call("hn_gstd_write_binary", self._h, str(filename).encode(), int(offset_bytes))
+
+    def writeMarket(self, filename):
+        self._info()
+        call("hn_gstd_write_market", self._h, str(filename).encode())
+
+    def hornet_init(self) -> "HornetInit":
+        """HornetInit(graph.nV(), graph.nE(), graph.csr_out_offsets(),
+        graph.csr_out_edges()) -- hornetsnest/test/BFSTest.cu:22-23."""
+        off, adj = self._csr("hn_gstd_csr_out")
+        return HornetInit(off.size - 1, adj.size, off, adj)
+
+    def _free(self):
+        if self._h is not None:
+            _lib.load().hn_gstd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self._free()
+        except Exception:
+            pass
 
 
 class BatchUpdatePtr:

@@ -290,6 +290,55 @@ hn_status hn_gen_chung_lu_graph(int32_t n, int32_t avg_degree, double gamma, uin
 hn_status hn_gen_uniform_pairs(int32_t n, int64_t count, uint64_t seed, int32_t* src,
                                int32_t* dst, void* stream);
 
+/* ---------------------------------------------------------- graph files */
+/* graph::GraphStd<int, int64> -- xlib/include/Graph/GraphStd.hpp: a host CSR
+ * read from a file, handed to hn_graph_create_csr the way the reference
+ * drivers hand graph.csr_out_offsets() / csr_out_edges() to HornetInit
+ * (hornetsnest/test/BFSTest.cu:18-23).  Host memory only; no device call. */
+typedef struct hn_gstd hn_gstd;
+/* StructureProp -- xlib/include/Graph/GraphBase.hpp:92-93,115-123 */
+#define HN_GS_DIRECTED        0x1u
+#define HN_GS_UNDIRECTED      0x2u
+#define HN_GS_ENABLE_INGOING  0x4u
+/* ParsingProp -- GraphBase.hpp:46-48 (PRINT_INFO is accepted and prints nothing) */
+#define HN_PARSE_RANDOMIZE           0x1u
+#define HN_PARSE_SORT                0x2u
+#define HN_PARSE_PRINT_INFO          0x4u
+#define HN_PARSE_RM_SINGLETON        0x8u
+#define HN_PARSE_DIRECTED_BY_DEGREE  0x10u
+/* GraphStd(structure) + read(filename, parsing) -- xlib/src/Graph/GraphBase.cpp:130-206
+ * (dispatch on .mtx "%%MatrixMarket" / .gr "c"|"p" / .bin), readers
+ * GraphStdRead.cpp:45-105,206-236, COOtoCSR GraphStd.cpp:173-351.
+ * structure 0 takes the file's direction (MatrixMarket "symmetric" and DIMACS9:
+ * UNDIRECTED, else DIRECTED); a directed file read UNDIRECTED is mirrored and
+ * deduplicated.  RANDOMIZE relabels by a Fisher-Yates permutation drawn from
+ * SplitMix64 stream `seed` (the reference seeds from the clock).  Errors
+ * (HN_E_INVALID_ARG, message in hn_last_error): unreadable or unrecognised
+ * file, ids outside [1, n], entry count different from the header, malformed
+ * binary, an undirected file read into a DIRECTED structure. */
+hn_status hn_gstd_read(const char* path, uint32_t structure, uint32_t parse, uint64_t seed,
+                       hn_gstd** out);
+/* ~GraphStd() */
+hn_status hn_gstd_destroy(hn_gstd* g);
+/* nV(), nE() -- GraphBase.hpp:137-138,192-199 -- and the resolved StructureProp bits */
+hn_status hn_gstd_info(const hn_gstd* g, int32_t* nv, int64_t* ne, uint32_t* structure);
+/* csr_out_offsets() / csr_out_edges() -- GraphStd.i.hpp:239-251: nv+1 int64
+ * offsets, ne int32 edges, each row in file order (sorted with
+ * HN_PARSE_SORT); valid until hn_gstd_destroy. */
+hn_status hn_gstd_csr_out(const hn_gstd* g, const int64_t** offsets, const int32_t** edges);
+/* csr_in_offsets() / csr_in_edges() -- GraphStd.i.hpp:244-256: DIRECTED graphs
+ * read with ENABLE_INGOING; an UNDIRECTED graph returns its out-CSR. */
+hn_status hn_gstd_csr_in(const hn_gstd* g, const int64_t** offsets, const int32_t** edges);
+/* max_out_degree_id() -- GraphStd.i.hpp:296-299 (first vertex of largest degree) */
+hn_status hn_gstd_max_out_degree_id(const hn_gstd* g, int32_t* id);
+/* writeBinary(filename) -- GraphStd.cpp:387-421: class id "intint"
+ * (offset_bytes 4, GraphStd<int, int>) or "intlong" (8, GraphStd<int, int64_t>),
+ * nV, nE, the 16-byte StructureProp (zero vtable word, state DIRECTED or
+ * DIRECTED|ENABLE_INGOING), offsets, edges. */
+hn_status hn_gstd_write_binary(const hn_gstd* g, const char* path, int32_t offset_bytes);
+/* writeMarket(filename) -- GraphStd.cpp:425-437 */
+hn_status hn_gstd_write_market(const hn_gstd* g, const char* path);
+
 #ifdef __cplusplus
 }
 #endif

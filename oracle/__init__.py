@@ -26,6 +26,15 @@ Pinning status (what independently checks each function, see
   specified by this build (the reference has none): parity unpinned against the
   reference; pinned by distribution invariants and by the GPU generator.
 * ``two_hop``         -- networkx ego-graph (radius 2) size.
+* ``graph_std`` / ``read_market_std`` / ``read_dimacs9_std`` / ``read_binary``
+  -- ``example/G.bin``, the reference's own binary dump of ``example/G.mtx``
+  (default direction, file-order rows, binary layout); golden.json for the
+  UNDIRECTED read; ``directed_by_degree`` for DIRECTED_BY_DEGREE; transpose
+  and relabelling identities for ENABLE_INGOING, RM_SINGLETON and RANDOMIZE
+  (``tests/test_graphio.py``).
+* ``vertex_permutation`` -- specified by this build (the reference's RANDOMIZE
+  is clock-seeded): parity unpinned against the reference; pinned as a
+  permutation whose inverse restores the input graph.
 """
 from __future__ import annotations
 
@@ -279,3 +288,142 @@ def read_dimacs9(path: str):
                 src.append(int(p[1]) - 1)
                 dst.append(int(p[2]) - 1)
     return n, np.array(src, np.int32), np.array(dst, np.int32)
+
+
+# --------------------------------------------------------------------------
+# GraphStd: file direction rules, COO -> CSR and the binary graph format
+# --------------------------------------------------------------------------
+# StructureProp values (xlib/include/Graph/GraphBase.hpp:92-93)
+GS_DIRECTED, GS_UNDIRECTED, GS_ENABLE_INGOING = 1, 2, 4
+# ParsingProp values (GraphBase.hpp:46-48)
+P_RANDOMIZE, P_SORT, P_PRINT_INFO, P_RM_SINGLETON, P_DIRECTED_BY_DEGREE = 1, 2, 4, 8, 16
+
+
+def vertex_permutation(n: int, seed: int) -> np.ndarray:
+    """RANDOMIZE relabelling as the library specifies it (the reference shuffles
+    with a clock seed, GraphStd.cpp:217-229, so no fixed permutation of its can
+    be matched): Fisher-Yates from i = n-1 down to 1, j = uniform_below(draw(i),
+    i+1) of SplitMix64 stream `seed`; vertex v becomes perm[v]."""
+    key = stream_key(seed)
+    perm = np.arange(n, dtype=np.int64)
+    for i in range(n - 1, 0, -1):
+        r = int(lib().orc_draw(key, i))
+        j = ((r >> 32) * (i + 1)) >> 32
+        perm[i], perm[j] = perm[j], perm[i]
+    return perm
+
+
+def _stable_csr(n, a, b):
+    """Rows a, entries b, each row in input order (GraphStd.cpp:333-341)."""
+    order = np.argsort(a, kind="stable")
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, a + 1, 1)
+    return np.cumsum(off), b[order].astype(np.int32)
+
+
+def graph_std(n, src, dst, file_direction, stored_undirected, structure=0, parse=0, seed=0):
+    """GraphStd::read after the file's entries are in COO form: direction
+    resolution (GraphStd.cpp:85-95), mirror of stored-undirected /
+    directed-to-undirected input (:175-190, self loops not mirrored), sort +
+    unique for directed-to-undirected (:192-206), RANDOMIZE (:217-229, with
+    vertex_permutation), degree count (:233-245), DIRECTED_BY_DEGREE (:247-304,
+    deg(u) < deg(v) or equal degrees and u < v), SORT (:306-310),
+    RM_SINGLETON (:312-331), CSR placement in COO order (:333-351).
+
+    Where the reference's behaviour is undefined this restatement fixes it:
+    RM_SINGLETON drops the edges that touch a removed vertex (the reference
+    leaves their labels unset), and an UNDIRECTED graph's in-CSR is its out-CSR
+    (the reference zeroes its in-degrees after DIRECTED_BY_DEGREE).  A file
+    stored undirected read into a DIRECTED structure (random edge removal,
+    :96-101) is rejected.  Returns a dict n, structure, off, adj, in_off, in_adj
+    (in_* None unless DIRECTED with ENABLE_INGOING)."""
+    s = np.asarray(src, np.int64).copy()
+    d = np.asarray(dst, np.int64).copy()
+    if not structure & (GS_DIRECTED | GS_UNDIRECTED):
+        structure |= file_direction
+    if file_direction == GS_UNDIRECTED and structure & GS_DIRECTED:
+        raise ValueError("undirected file into a directed structure")
+    d2u = file_direction == GS_DIRECTED and bool(structure & GS_UNDIRECTED)
+    if d2u or stored_undirected:
+        keep = s != d
+        s, d = np.concatenate([s, d[keep]]), np.concatenate([d, s[keep]])
+    if d2u:
+        pairs = sorted(set(zip(s.tolist(), d.tolist())))
+        s = np.array([p[0] for p in pairs], np.int64)
+        d = np.array([p[1] for p in pairs], np.int64)
+    if parse & P_RANDOMIZE:
+        perm = vertex_permutation(n, seed)
+        s, d = perm[s], perm[d]
+    directed = bool(structure & GS_DIRECTED)
+    ingoing = directed and bool(structure & GS_ENABLE_INGOING)
+    reverse = bool(structure & (GS_ENABLE_INGOING | GS_UNDIRECTED))
+
+    def degrees():
+        o = np.bincount(s, minlength=n)
+        return o, (np.bincount(d, minlength=n) if ingoing else o)
+
+    outd, ind = degrees()
+    if parse & P_DIRECTED_BY_DEGREE:
+        keep = (outd[s] < outd[d]) | ((outd[s] == outd[d]) & (s < d))
+        s, d = s[keep], d[keep]
+        outd, ind = degrees()
+    if parse & P_SORT and (not d2u or parse & P_RANDOMIZE):
+        order = np.lexsort((d, s))
+        s, d = s[order], d[order]
+    if parse & P_RM_SINGLETON and reverse:
+        kept = (outd > 0) & (ind > 0)
+        label = np.cumsum(kept) - 1
+        both = kept[s] & kept[d]
+        s, d = label[s[both]], label[d[both]]
+        n = int(kept.sum())
+    off, adj = _stable_csr(n, s, d)
+    in_off = in_adj = None
+    if ingoing:
+        in_off, in_adj = _stable_csr(n, d, s)
+    return {"n": n, "structure": structure, "off": off, "adj": adj, "in_off": in_off,
+            "in_adj": in_adj}
+
+
+def read_market_std(path: str, structure=0, parse=0, seed=0):
+    """GraphStd::read of a MatrixMarket file: header 'symmetric' -> UNDIRECTED
+    stored undirected, else DIRECTED (GraphBase.cpp:214-238)."""
+    n, s, d, sym = read_market(path)
+    return graph_std(n, s, d, GS_UNDIRECTED if sym else GS_DIRECTED, sym, structure, parse, seed)
+
+
+def read_dimacs9_std(path: str, structure=0, parse=0, seed=0):
+    """GraphStd::read of a DIMACS9 file: UNDIRECTED, arcs as listed, nothing
+    mirrored (GraphBase.cpp:252-262)."""
+    n, s, d = read_dimacs9(path)
+    return graph_std(n, s, d, GS_UNDIRECTED, False, structure, parse, seed)
+
+
+def read_binary(path: str):
+    """Binary graph (GraphStd.cpp:360-421 writer, GraphStdRead.cpp:206-236
+    reader): the class id type_name<vid_t>() + type_name<eoff_t>() ("intint"
+    or "intlong"), nV, nE, the 16-byte StructureProp object (its uint64 state at
+    byte 8), then out offsets [nV+1], (in offsets), out edges [nE], (in edges)
+    -- the in arrays present when the state is DIRECTED | ENABLE_INGOING.
+    Degrees are offsets[i+1] - offsets[i] (the reference reader subtracts
+    offsets[i-1])."""
+    raw = open(path, "rb").read()
+    for cid, ot in ((b"intint", np.int32), (b"intlong", np.int64)):
+        if raw.startswith(cid):
+            break
+    else:
+        raise ValueError("unknown class identifier")
+    p = len(cid)
+    nv = int(np.frombuffer(raw, np.int32, 1, p)[0]); p += 4
+    ne = int(np.frombuffer(raw, ot, 1, p)[0]); p += np.dtype(ot).itemsize
+    state = int(np.frombuffer(raw, np.uint64, 1, p + 8)[0]); p += 16
+    twice = (state & GS_DIRECTED) and (state & GS_ENABLE_INGOING)
+    off = np.frombuffer(raw, ot, nv + 1, p).astype(np.int64); p += (nv + 1) * np.dtype(ot).itemsize
+    in_off = in_adj = None
+    if twice:
+        in_off = np.frombuffer(raw, ot, nv + 1, p).astype(np.int64); p += (nv + 1) * np.dtype(ot).itemsize
+    adj = np.frombuffer(raw, np.int32, ne, p).copy(); p += 4 * ne
+    if twice:
+        in_adj = np.frombuffer(raw, np.int32, ne, p).copy(); p += 4 * ne
+    if p != len(raw):
+        raise ValueError("file size does not match its header")
+    return {"n": nv, "structure": state, "off": off, "adj": adj, "in_off": in_off, "in_adj": in_adj}

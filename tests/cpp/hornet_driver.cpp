@@ -14,6 +14,11 @@
 //   update <graph> <batch> <out.bin>    insert (both dedups) then erase half;
 //                                       batch file: int64 n, int32 src[n], dst[n];
 //                                       writes the COO after each step
+//   bfsfile <file> <out.bin>            BFSTest.cu:18-44 shape: graph::GraphStd
+//                                       read of a .mtx/.gr/.bin file, BFS from
+//                                       max_out_degree_id; writes int32 dist[nv]
+//   tcfile  <file>                      TriangleTest2.cu:86-100 shape: read
+//                                       UNDIRECTED, DIRECTED_BY_DEGREE | SORT
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -21,6 +26,7 @@
 #include <string>
 #include <vector>
 
+#include "Graph/GraphStd.hpp"
 #include "hornet/hornet.hpp"
 #include "hornet/operators.hpp"
 #include "hornetsnest/bfs_topdown.hpp"
@@ -93,6 +99,35 @@ int main(int argc, char** argv) {
         return 2;
     }
     const std::string cmd = argv[1];
+    if (cmd == "bfsfile" && argc >= 4) {
+        graph::GraphStd<vid_t, eoff_t> graph;
+        graph.read(argv[2]);
+        HornetInit hornet_init(graph.nV(), graph.nE(), graph.csr_out_offsets(), graph.csr_out_edges());
+        HornetGraph hornet_graph(hornet_init);
+        BfsTopDown bfs_top_down(hornet_graph);
+        const vid_t root = graph.max_out_degree_id();
+        bfs_top_down.set_parameters(root);
+        bfs_top_down.run();
+        std::vector<int> d((size_t)graph.nV());
+        gpu::copyToHost(bfs_top_down.distances(), d.size(), d.data());
+        std::printf("bfsfile nv=%d ne=%d root=%d levels=%d validate=%d\n", graph.nV(), graph.nE(), root,
+                    bfs_top_down.levels(), bfs_top_down.validate() ? 1 : 0);
+        std::ofstream out(argv[3], std::ios::binary);
+        out.write(reinterpret_cast<const char*>(d.data()), (std::streamsize)(4 * d.size()));
+        return 0;
+    }
+    if (cmd == "tcfile") {
+        graph::GraphStd<vid_t, eoff_t> graph(graph::structure_prop::UNDIRECTED);
+        graph.read(argv[2], graph::parsing_prop::DIRECTED_BY_DEGREE | graph::parsing_prop::PRINT_INFO |
+                                graph::parsing_prop::SORT);
+        HornetInit hornet_init(graph.nV(), graph.nE(), graph.csr_out_offsets(), graph.csr_out_edges());
+        HornetGraph hornet_graph(hornet_init);
+        TriangleCounting2 tc(hornet_graph);
+        tc.init();
+        tc.run(1);
+        std::printf("tcfile nv=%d ne=%d total=%llu\n", graph.nV(), graph.nE(), tc.total_count());
+        return 0;
+    }
     Csr g = read_graph(argv[2]);
     HornetInit init((int)g.nv, (int)g.ne, g.off.data(), g.adj.data());
     HornetGraph hornet_graph(init);

@@ -147,8 +147,10 @@ def test_invalid_inputs_raise(hn):
     from hornet_amd import HornetError
     with pytest.raises(HornetError):
         hn.Hornet.from_csr(np.array([0, 2, 3]), np.array([1, 5, 0], np.int32))  # id out of range
-    with pytest.raises(HornetError):
-        hn.Hornet.from_csr(np.array([0, 2, 3]), np.array([1, 0, 0], np.int32))  # unsorted
+    # rows in any order are accepted (HornetInit) and stored ascending
+    hu = hn.Hornet.from_csr(np.array([0, 2, 3]), np.array([1, 0, 0], np.int32))
+    us, ud = hu.getCOO()
+    assert us.cpu().tolist() == [0, 0, 1] and ud.cpu().tolist() == [0, 1, 0]
     h = hn.Hornet.from_csr(np.array([0, 1, 2]), np.array([1, 0], np.int32))
     b = hn.BfsTopDown(h)
     b.set_parameters(7)
