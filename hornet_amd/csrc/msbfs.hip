@@ -479,7 +479,7 @@ __global__ __launch_bounds__(BLOCK) void ms_pull_heavy_kernel(int64_t words, con
             const int j = __ffs(hw) - 1;
             hw &= hw - 1;
             const int32_t u = (int32_t)(wd * 32 + j);
-            if (dirty[u]) continue;   // its edges are side records
+            if (dirty[u]) continue;   // dirty vertices take the masked grouped path
             const uint64_t miss = live & ~visited[u];
             if (!miss) continue;
             const hn_vrec r = rec[u];
