@@ -4,8 +4,10 @@
 // load-balancing kernels are instantiated in the caller's translation unit
 // (compiled with hipcc for gfx950), exactly as in the reference, and work on
 // the POD device view exported by libhornet_hip.  Reference counterparts:
-//   forAll / forAllnumV / forAllVertices / forAllEdges / forAllEdgeVertexPairs
+//   forAll / forAllnumV / forAllnumE / forAllVertices / forAllEdges /
+//   forAllEdgeVertexPairs / forAllVertexPairs
 //                              primitives/Operator++.cuh:68-215, Operator++.i.cuh:524-645
+//   forAllAdjUnions            Operator++.cuh:171-180, Operator++.i.cuh:298-476
 //   OPERATOR                   primitives/Operator++.cuh:57-59
 //   TwoLevelQueue<T>           primitives/Queue/TwoLevelQueue.cuh:62-204
 //   load_balancing::BinarySearch  primitives/LoadBalancing/BinarySearch.cuh:52-94
@@ -39,6 +41,7 @@ using degree_t = int;
 using eoff_t = int;
 using Vertex = hornet::Vertex;
 using Edge = hornet::Edge;
+using vid2_t = int2;   // xlib::Make2Str<vid_t>::type
 
 inline void hip_check(hipError_t e, const char* where) {
     if (e != hipSuccess) throw hornet::Error(HN_E_HIP, where, hipGetErrorString(e));
@@ -280,6 +283,129 @@ inline unsigned grid_for(int64_t n) {
     return (unsigned)std::min<int64_t>(g, 256 * 16);
 }
 
+template <typename T, typename Operator>
+__global__ void for_all_items_kernel(const T* __restrict__ items, int64_t n, Operator op) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        T value = items[i];
+        op(value);
+    }
+}
+
+template <typename T, typename Operator>
+__global__ void for_all_vertex_pairs_kernel(hornet::HornetDevice h, const T* __restrict__ pairs, int64_t n,
+                                            Operator op) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T p = pairs[i];
+        Vertex a(h, p.x), b(h, p.y);
+        op(a, b);
+    }
+}
+
+// ---- forAllAdjUnions
+// One (src, dst) pair becomes an intersection of u = the endpoint of smaller
+// degree (dst on ties) with v = the other, split over a power-of-two number of
+// lanes (at most a wave) from its work: FLAG 1 (binary search) when
+// WORK_FACTOR * (|u| + |v| + log|u|) >= |u| * log|v|, as the reference's bin
+// rule (Operator++.i.cuh:312-320), else FLAG 0 (merge).  Pairs with src == dst
+// or deg(src) < 2 are skipped (Operator++.i.cuh:212-214).
+constexpr int ADJ_UNION_WORK = 64;   // merge steps / probes per lane
+
+struct AdjUnionCase {
+    vid_t u, v;
+    degree_t ul, vl;
+    int flag;
+    int threads;   // 0: skipped
+};
+
+struct CooPairsAt {
+    const vid_t* s;
+    const vid_t* d;
+    __device__ __forceinline__ int2 operator()(int64_t i) const { return make_int2(s[i], d[i]); }
+};
+template <typename T>
+struct ArrayPairsAt {
+    const T* p;
+    __device__ __forceinline__ int2 operator()(int64_t i) const { return make_int2(p[i].x, p[i].y); }
+};
+
+template <typename Pairs>
+__device__ __forceinline__ AdjUnionCase adj_union_case(const hornet::HornetDevice& h, const Pairs& pairs, int64_t i,
+                                                       int work_factor) {
+    const int2 p = pairs(i);
+    const degree_t sl = h.degree(p.x), dl = h.degree(p.y);
+    AdjUnionCase c{0, 0, 0, 0, 0, 0};
+    if (p.x == p.y || sl < 2) return c;
+    const bool ss = sl < dl;
+    c.u = ss ? p.x : p.y;
+    c.v = ss ? p.y : p.x;
+    c.ul = ss ? sl : dl;
+    c.vl = ss ? dl : sl;
+    const int log_u = min(32 - __clz(c.ul), 31), log_v = min(32 - __clz(c.vl), 31);
+    const int64_t bin_work = (int64_t)c.ul * log_v;
+    c.flag = (int64_t)work_factor * ((int64_t)c.ul + c.vl + log_u) >= bin_work;
+    const int64_t work = c.flag ? bin_work : (int64_t)c.ul + c.vl;
+    int64_t t = (work + ADJ_UNION_WORK - 1) / ADJ_UNION_WORK;
+    t = min(t, (int64_t)c.ul);
+    int lanes = 1;
+    while (lanes < t && lanes < hn::WAVE) lanes <<= 1;
+    c.threads = t > 0 ? lanes : 0;
+    return c;
+}
+
+template <typename Pairs>
+struct AdjUnionThreadsIn {
+    hornet::HornetDevice h;
+    Pairs pairs;
+    int work_factor;
+    __device__ __forceinline__ int64_t operator()(int64_t i) const {
+        return adj_union_case(h, pairs, i, work_factor).threads;
+    }
+};
+
+// Merge-path split of sorted A (a) and B (b) after d merged elements, A first
+// on ties: i from A, j = d - i from B.  A match A[i-1] == B[j] is moved into
+// the earlier part (j + 1), so no equal pair is cut.
+__device__ __forceinline__ void merge_split(const vid_t* A, int a, const vid_t* B, int b, int d, int& i, int& j) {
+    int lo = d - b > 0 ? d - b : 0, hi = d < a ? d : a;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (A[mid] <= B[d - mid - 1]) lo = mid + 1;
+        else hi = mid;
+    }
+    i = lo;
+    j = d - lo;
+    if (i > 0 && j < b && A[i - 1] == B[j]) j++;
+}
+
+template <typename Pairs, typename Operator>
+__global__ __launch_bounds__(hn::BLOCK) void adj_union_kernel(hornet::HornetDevice h, Pairs pairs, int64_t n,
+                                                              const int64_t* P, int work_factor, Operator op) {
+    __shared__ hn::LbShared<LB_IPT, int64_t> sh;
+    const int64_t W = P[n];
+    for (int64_t w0 = (int64_t)blockIdx.x * LB_TILE; w0 < W; w0 += (int64_t)gridDim.x * LB_TILE) {
+        hn::lb_tile<LB_IPT>(
+            P, n, w0, W, sh, [&](int64_t s) { return P[s]; },
+            [&](int64_t s, int64_t w, int64_t base) {
+                const AdjUnionCase c = adj_union_case(h, pairs, s, work_factor);
+                const int k = (int)(w - base);
+                Vertex uv(h, c.u), vv(h, c.v);
+                vid_t* un = uv.neighbor_ptr();
+                vid_t* vn = vv.neighbor_ptr();
+                if (c.flag) {
+                    const int b = (int)((int64_t)c.ul * k / c.threads);
+                    const int e = (int)((int64_t)c.ul * (k + 1) / c.threads);
+                    if (b < e) op(uv, vv, un + b, un + e - 1, vn, vn + c.vl - 1, 1);
+                } else {
+                    const int64_t tot = (int64_t)c.ul + c.vl;
+                    int i0, j0, i1, j1;
+                    merge_split(un, c.ul, vn, c.vl, (int)(tot * k / c.threads), i0, j0);
+                    merge_split(un, c.ul, vn, c.vl, (int)(tot * (k + 1) / c.threads), i1, j1);
+                    if (i0 < i1 && j0 < j1) op(uv, vv, un + i0, un + i1 - 1, vn + j0, vn + j1 - 1, 0);
+                }
+            });
+    }
+}
+
 }  // namespace detail
 
 // --------------------------------------------------------- load balancing
@@ -438,6 +564,88 @@ void forAllEdgeVertexPairs(HornetClass& hornet, const Operator& op, LoadBalancin
 template <typename HornetClass, typename Operator, int VW>
 void forAllEdgeVertexPairs(HornetClass& hornet, const Operator& op, load_balancing::VertexBased<VW>& lb) {
     lb.applyVertexPairs(hornet, op);
+}
+
+// op(item) for every item of the queue's input side (Operator++.i.cuh:534-543)
+template <typename T, typename Operator>
+void forAll(const TwoLevelQueue<T>& queue, const Operator& op) {
+    const int64_t n = queue.size();
+    if (n == 0) return;
+    detail::for_all_items_kernel<T, Operator><<<detail::grid_for(n), hn::BLOCK, 0, queue.stream()>>>(
+        queue.device_input_ptr(), n, op);
+    hip_check(hipGetLastError(), "forAll");
+}
+
+// op(Vertex&, Vertex&) for every (x, y) pair of the queue (Operator++.i.cuh:545-556)
+template <typename HornetClass, typename T, typename Operator>
+void forAllVertexPairs(HornetClass& hornet, const TwoLevelQueue<T>& queue, const Operator& op) {
+    const int64_t n = queue.size();
+    if (n == 0) return;
+    detail::for_all_vertex_pairs_kernel<T, Operator><<<detail::grid_for(n), hn::BLOCK, 0, hornet.stream()>>>(
+        hornet.device(), queue.device_input_ptr(), n, op);
+    hip_check(hipGetLastError(), "forAllVertexPairs");
+}
+
+// op(i) for i in [0, nE) (Operator++.i.cuh:570-577)
+template <typename HornetClass, typename Operator>
+void forAllnumE(HornetClass& hornet, const Operator& op) {
+    const int64_t n = hornet.nE64();
+    if (n == 0) return;
+    detail::for_all_kernel<Operator><<<detail::grid_for(n), hn::BLOCK, 0, hornet.stream()>>>(n, op);
+    hip_check(hipGetLastError(), "forAllnumE");
+}
+
+// forAllEdges without a load-balancing object (declared at Operator++.cuh:129-130):
+// BinarySearch
+template <typename HornetClass, typename Operator>
+void forAllEdges(HornetClass& hornet, const Operator& op) {
+    load_balancing::BinarySearch lb(hornet, 1.0f);
+    lb.apply(hornet, op);
+    hip_check(hipStreamSynchronize(hornet.stream()), "forAllEdges");
+}
+
+namespace detail {
+template <typename HornetClass, typename Pairs, typename Operator>
+void adj_unions_run(HornetClass& hornet, Pairs pairs, int64_t n, const Operator& op, int work_factor) {
+    if (n <= 0) return;
+    hipStream_t st = hornet.stream();
+    hornet::HornetDevice h = hornet.device();
+    int64_t* P = nullptr;
+    void* tmp = nullptr;
+    gpu::allocate(P, (size_t)n + 1);
+    hip_check(hipMalloc(&tmp, hn::scan_temp_bytes(n)), "forAllAdjUnions");
+    hip_check(hn::exclusive_scan_dev<int64_t>(AdjUnionThreadsIn<Pairs>{h, pairs, work_factor},
+                                              hn::ArrayOut<int64_t>{P}, P + n, n, tmp, st),
+              "forAllAdjUnions (scan)");
+    adj_union_kernel<Pairs, Operator><<<256 * 8, hn::BLOCK, 0, st>>>(h, pairs, n, P, work_factor, op);
+    hip_check(hipGetLastError(), "forAllAdjUnions");
+    hip_check(hipStreamSynchronize(st), "forAllAdjUnions");
+    gpu::free(P);
+    (void)hipFree(tmp);
+}
+}  // namespace detail
+
+// op(Vertex& u, Vertex& v, vid_t* ui_begin, vid_t* ui_end, vid_t* vi_begin,
+// vid_t* vi_end, int FLAG) over every stored edge (Operator++.i.cuh:369-375)
+// or the queue's vertex pairs (:377-476; an empty queue means every edge).
+// Ends are inclusive.  FLAG 0: merge the two sub-ranges; the lanes' sub-ranges
+// partition the merge path so each common element is seen by exactly one
+// lane.  FLAG 1: binary-search each element of the u sub-range in all of v.
+template <typename HornetClass, typename Operator>
+void forAllAdjUnions(HornetClass& hornet, const Operator& op, const int WORK_FACTOR = 1) {
+    auto coo = hornet.getCOO();
+    detail::adj_unions_run(hornet, detail::CooPairsAt{coo.srcPtr(), coo.dstPtr()}, coo.size(), op, WORK_FACTOR);
+}
+
+template <typename HornetClass, typename Operator>
+void forAllAdjUnions(HornetClass& hornet, TwoLevelQueue<vid2_t> vertex_pairs, const Operator& op,
+                     const int WORK_FACTOR = 1) {
+    if (vertex_pairs.size() == 0) {
+        forAllAdjUnions(hornet, op, WORK_FACTOR);
+        return;
+    }
+    detail::adj_unions_run(hornet, detail::ArrayPairsAt<vid2_t>{vertex_pairs.device_input_ptr()},
+                           vertex_pairs.size(), op, WORK_FACTOR);
 }
 
 // --------------------------------------------------------- StaticAlgorithm

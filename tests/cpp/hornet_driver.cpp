@@ -14,6 +14,12 @@
 //   update <graph> <batch> <out.bin>    insert (both dedups) then erase half;
 //                                       batch file: int64 n, int32 src[n], dst[n];
 //                                       writes the COO after each step
+//   ops2   <graph> <source> <out.bin>   BfsTopDown2, forAll(queue), forAllVertexPairs,
+//                                       forAllnumE, forAllEdges without LB object,
+//                                       forAllAdjUnions (triangle credits per vertex
+//                                       with WORK_FACTOR 0, 1, 1000 and from a pair
+//                                       queue); writes int32 dist[nv] then 4 x
+//                                       uint64 credits[nv]
 //   bfsfile <file> <out.bin>            BFSTest.cu:18-44 shape: graph::GraphStd
 //                                       read of a .mtx/.gr/.bin file, BFS from
 //                                       max_out_degree_id; writes int32 dist[nv]
@@ -84,6 +90,43 @@ struct CountPairs {
 struct SumDegrees {
     unsigned long long* acc;
     OPERATOR(Vertex& v) { atomicAdd(acc, (unsigned long long)v.degree()); }
+};
+
+// |N(u) ∩ N(v)| credited to u, honouring forAllAdjUnions' FLAG contract
+struct AdjCount {
+    unsigned long long* per_vertex;
+    unsigned long long* calls;   // [0] FLAG 0 calls, [1] FLAG 1 calls
+    OPERATOR(Vertex& u, Vertex&, vid_t* ub, vid_t* ue, vid_t* vb, vid_t* ve, int FLAG) {
+        unsigned long long c = 0;
+        if (!FLAG) {
+            while (ub <= ue && vb <= ve) {
+                if (*ub < *vb) ub++;
+                else if (*vb < *ub) vb++;
+                else c++, ub++, vb++;
+            }
+        } else {
+            for (; ub <= ue; ub++) {
+                const vid_t* lo = vb;
+                const vid_t* hi = ve + 1;
+                while (lo < hi) {
+                    const vid_t* mid = lo + (hi - lo) / 2;
+                    if (*mid < *ub) lo = mid + 1;
+                    else hi = mid;
+                }
+                c += (lo <= ve && *lo == *ub);
+            }
+        }
+        if (c) atomicAdd(per_vertex + u.id(), c);
+        atomicAdd(calls + FLAG, 1ull);
+    }
+};
+struct SumItems {
+    unsigned long long* acc;
+    OPERATOR(vid_t& x) { atomicAdd(acc, (unsigned long long)x); }
+};
+struct SumPairDegrees {
+    unsigned long long* acc;
+    OPERATOR(Vertex& a, Vertex& b) { atomicAdd(acc, (unsigned long long)(a.degree() * 3 + b.degree())); }
 };
 
 static std::vector<unsigned long long> fetch(unsigned long long* d, int n) {
@@ -194,6 +237,72 @@ int main(int argc, char** argv) {
         forAllnumV(hornet_graph, [=] __device__(int i) { atomicAdd(cnt, (unsigned long long)i); });
         a = fetch(acc, 1);
         std::printf("ops numv id_sum=%llu\n", a[0]);
+        gpu::free(acc);
+        return 0;
+    }
+    if (cmd == "ops2" && argc >= 5) {
+        std::ofstream out(argv[4], std::ios::binary);
+        BfsTopDown2 bfs2(hornet_graph);
+        bfs2.set_parameters(std::atoi(argv[3]));
+        bfs2.run();
+        std::vector<int> dist((size_t)g.nv);
+        gpu::copyToHost(bfs2.distances(), (size_t)g.nv, dist.data());
+        std::printf("ops2 bfs2 levels=%d validate=%d enqueued=%d\n", bfs2.levels(), bfs2.validate() ? 1 : 0,
+                    bfs2.enqueue_items());
+        out.write(reinterpret_cast<const char*>(dist.data()), (std::streamsize)(4 * dist.size()));
+        unsigned long long* acc = nullptr;
+        gpu::allocate(acc, 4);
+        gpu::memsetZero(acc, 4);
+        unsigned long long* cnt = acc;
+        forAllnumE(hornet_graph, [=] __device__(int i) { atomicAdd(cnt, (unsigned long long)i); });
+        auto a = fetch(acc, 1);
+        std::printf("ops2 nume id_sum=%llu\n", a[0]);
+        TwoLevelQueue<vid_t> q(hornet_graph, 1.0f);
+        std::vector<int> third;
+        for (int v = 0; v < (int)g.nv; v += 3) third.push_back(v);
+        q.insert(third.data(), (int)third.size());
+        forAll(q, SumItems{acc});
+        a = fetch(acc, 1);
+        std::printf("ops2 queue_items sum=%llu\n", a[0]);
+        TwoLevelQueue<vid2_t> pq(hornet_graph, 1.0f);
+        std::vector<vid2_t> pairs;
+        for (int v = 0; v + 1 < (int)g.nv; v += 2) pairs.push_back(make_int2(v, v + 1));
+        pq.insert(pairs.data(), (int)pairs.size());
+        forAllVertexPairs(hornet_graph, pq, SumPairDegrees{acc});
+        a = fetch(acc, 1);
+        std::printf("ops2 vertex_pairs sum=%llu\n", a[0]);
+        forAllEdges(hornet_graph, CountEdges{acc});
+        a = fetch(acc, 3);
+        std::printf("ops2 edges_default edges=%llu dst_sum=%llu src_sum=%llu\n", a[0], a[1], a[2]);
+        unsigned long long* per = nullptr;
+        gpu::allocate(per, (size_t)g.nv);
+        std::vector<unsigned long long> hper((size_t)g.nv);
+        auto adj_run = [&](const char* tag, int wf, bool from_queue) {
+            gpu::memsetZero(per, (size_t)g.nv);
+            if (from_queue) {
+                auto coo = hornet_graph.getCOO();
+                std::vector<int> cs, cd;
+                coo.toHost(cs, cd);
+                std::vector<vid2_t> ep(cs.size());
+                for (size_t i = 0; i < cs.size(); i++) ep[i] = make_int2(cs[i], cd[i]);
+                TwoLevelQueue<vid2_t> eq((size_t)std::max<size_t>(ep.size(), 1));
+                eq.insert(ep.data(), (int)ep.size());
+                forAllAdjUnions(hornet_graph, eq, AdjCount{per, acc}, wf);
+            } else {
+                forAllAdjUnions(hornet_graph, AdjCount{per, acc}, wf);
+            }
+            auto c = fetch(acc, 2);
+            gpu::copyToHost(per, (size_t)g.nv, hper.data());
+            unsigned long long tot = 0;
+            for (auto x : hper) tot += x;
+            std::printf("ops2 adj_unions_%s total=%llu merge_calls=%llu search_calls=%llu\n", tag, tot, c[0], c[1]);
+            out.write(reinterpret_cast<const char*>(hper.data()), (std::streamsize)(8 * hper.size()));
+        };
+        adj_run("wf0", 0, false);
+        adj_run("wf1", 1, false);
+        adj_run("wf1000", 1000, false);
+        adj_run("queue", 1, true);
+        gpu::free(per);
         gpu::free(acc);
         return 0;
     }

@@ -115,3 +115,44 @@ def test_batch_update_through_headers(graph, tmp_path):
         pos += 4 * m
         es = np.repeat(np.arange(oo.size - 1), np.diff(oo))
         assert np.array_equal(src, es) and np.array_equal(dst, aa)
+
+
+def test_operator_surface_and_atomic_bfs(tmp_path):
+    # oriented (DIRECTED_BY_DEGREE) R-MAT: forAllAdjUnions credits |N(u) ∩ N(v)|
+    # to the smaller-degree endpoint, which is TriangleCounting2's per-vertex rule
+    off, adj = O.rmat_graph(12, seed=6)
+    do, da = O.directed_by_degree(off, adj)
+    p = str(tmp_path / "o.bin")
+    write_graph(p, do, da)
+    src = O.max_degree_id(do)
+    out = run("ops2", p, str(src), str(tmp_path / "r.bin"))
+    n = do.size - 1
+    raw = open(tmp_path / "r.bin", "rb").read()
+    dist = np.frombuffer(raw, np.int32, n)
+    exp, _ = O.bfs(do, da, src)
+    assert np.array_equal(dist, exp)
+    assert field(out, "validate") == 1
+    reached = int((exp != INF).sum())
+    assert field(out, "enqueued") == reached - 1           # each vertex enqueued once
+    assert field(out, "levels") == int(exp[exp != INF].max()) + 1
+    ne = da.size
+    assert field(out, "id_sum") == ne * (ne - 1) // 2
+    assert field(out, "sum") == sum(range(0, n, 3))
+    deg = np.diff(do)
+    pair_line = next(x for x in out.splitlines() if x.startswith("ops2 vertex_pairs"))
+    assert field(pair_line, "sum") == int(sum(deg[v] * 3 + deg[v + 1] for v in range(0, n - 1, 2)))
+    line = next(x for x in out.splitlines() if x.startswith("ops2 edges_default"))
+    s = np.repeat(np.arange(n), deg)
+    assert field(line, "edges") == ne and field(line, "dst_sum") == int(da.sum())
+    assert field(line, "src_sum") == int(s.sum())
+    per_exp = O.tc2_per_vertex(do, da)
+    tri = O.triangles(off, adj)
+    for k, tag in enumerate(("wf0", "wf1", "wf1000", "queue")):
+        per = np.frombuffer(raw, np.uint64, n, 4 * n + 8 * n * k)
+        assert np.array_equal(per, per_exp), tag
+        line = next(x for x in out.splitlines() if x.startswith(f"ops2 adj_unions_{tag} "))
+        assert field(line, "total") == tri
+        if tag == "wf0":
+            assert field(line, "search_calls") == 0 and field(line, "merge_calls") > 0
+        if tag == "wf1000":
+            assert field(line, "search_calls") > 0
