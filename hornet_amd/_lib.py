@@ -105,6 +105,7 @@ SIGNATURES = {
     "hn_graph_export_coo": [P, P, P, I64, PI64, P],
     "hn_graph_export_csr": [P, P, P, I64, P],
     "hn_graph_degrees": [P, P, P],
+    "hn_graph_snapshot": [P, D, P, PP],
     "hn_graph_max_degree": [P, PI32, PI32, P],
     "hn_graph_insert": [P, P, P, I64, U32, PI64, P],
     "hn_graph_erase": [P, P, P, I64, U32, PI64, P],
@@ -130,6 +131,7 @@ SIGNATURES = {
     "hn_gen_chung_lu_graph": [I32, I32, D, U64, D, P, PP],
     "hn_gstd_read": [ctypes.c_char_p, U32, U32, U64, PP],
     "hn_gstd_destroy": [P],
+    "hn_gstd_from_csr": [I32, I64, P, P, PP],
     "hn_gstd_info": [P, PI32, PI64, ctypes.POINTER(ctypes.c_uint32)],
     "hn_gstd_csr_out": [P, PP, PP],
     "hn_gstd_csr_in": [P, PP, PP],

@@ -710,6 +710,27 @@ hn_status hn_graph_export_csr(hn_graph* g, int64_t* offsets, int32_t* adj, int64
     return flatten(g, nullptr, adj, offsets, as_stream(stream));
 }
 
+hn_status hn_graph_snapshot(hn_graph* g, double slack, void* stream, hn_graph** out) {
+    HN_ARG(g && out, "null argument");
+    *out = nullptr;
+    hipStream_t st = as_stream(stream);
+    int64_t* d_off = nullptr;
+    int32_t* d_adj = nullptr;
+    hn_status s = dalloc(&d_off, (size_t)g->nv + 1);
+    if (s == HN_OK) s = dalloc(&d_adj, (size_t)(g->ne > 0 ? g->ne : 1));
+    if (s == HN_OK) s = flatten(g, nullptr, d_adj, d_off, st);
+    if (s == HN_OK) s = graph_from_device_csr(g->nv, g->ne, d_off, d_adj, slack < 1.0 ? 1.0 : slack, st, out);
+    dfree(d_off);
+    dfree(d_adj);
+    if (s == HN_OK) {
+        // the copy starts at version 0 with what is known of the source's symmetry
+        const bool known = g->sym == 1 || (g->sym == -1 && g->sym_version == g->version);
+        (*out)->sym = known ? g->sym : 0;
+        (*out)->sym_version = 0;
+    }
+    return s;
+}
+
 hn_status hn_graph_degrees(hn_graph* g, int32_t* deg, void* stream) {
     HN_ARG(g && deg, "null argument");
     if (g->nv > 0) {

@@ -583,6 +583,23 @@ hn_status hn_gstd_read(const char* path, uint32_t structure, uint32_t parse, uin
     return HN_OK;
 }
 
+hn_status hn_gstd_from_csr(int32_t nv, int64_t ne, const int64_t* offsets, const int32_t* edges, hn_gstd** out) {
+    HN_ARG(out != nullptr && offsets != nullptr && (ne == 0 || edges != nullptr), "null argument");
+    *out = nullptr;
+    HN_ARG(nv >= 0 && ne >= 0, "negative size");
+    HN_ARG(offsets[0] == 0 && offsets[nv] == ne, "offsets[0] must be 0 and offsets[nv] == ne");
+    for (int32_t v = 0; v < nv; v++) HN_ARG(offsets[v] <= offsets[v + 1], "offsets not monotone");
+    for (int64_t j = 0; j < ne; j++) HN_ARG(edges[j] >= 0 && edges[j] < nv, "edge endpoint outside [0, nv)");
+    hn_gstd* g = new hn_gstd();
+    g->nv = nv;
+    g->ne = ne;
+    g->structure = S_UNDIRECTED;
+    g->out_off.assign(offsets, offsets + nv + 1);
+    g->out_adj.assign(edges, edges + ne);
+    *out = g;
+    return HN_OK;
+}
+
 hn_status hn_gstd_destroy(hn_gstd* g) {
     delete g;
     return HN_OK;

@@ -105,6 +105,20 @@ class GraphStd:
         self._h = h
         return self
 
+    @classmethod
+    def from_csr(cls, offsets, edges) -> "GraphStd":
+        """GraphStd(csr_offsets, nV, csr_edges, nE) (xlib/src/Graph/GraphStd.cpp:50-59):
+        an UNDIRECTED host graph copied from a CSR (e.g. Hornet.getCSR()),
+        so it can be written with writeBinary and read back later."""
+        off = np.ascontiguousarray(offsets, np.int64)
+        adj = np.ascontiguousarray(edges, np.int32)
+        g = cls(structure_prop.UNDIRECTED)
+        h = ctypes.c_void_p()
+        call("hn_gstd_from_csr", off.size - 1, adj.size, off.ctypes.data_as(ctypes.c_void_p),
+             adj.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h))
+        g._h = h
+        return g
+
     def _info(self):
         if self._h is None:
             raise HornetError(_lib.HN_E_INVALID_ARG, "GraphStd", "no graph read")
@@ -166,7 +180,9 @@ class GraphStd:
         """HornetInit(graph.nV(), graph.nE(), graph.csr_out_offsets(),
         graph.csr_out_edges()) -- hornetsnest/test/BFSTest.cu:22-23."""
         off, adj = self._csr("hn_gstd_csr_out")
-        return HornetInit(off.size - 1, adj.size, off, adj)
+        init = HornetInit(off.size - 1, adj.size, off, adj)
+        init._owner = self   # the arrays view this graph's memory
+        return init
 
     def _free(self):
         if self._h is not None:
@@ -309,6 +325,13 @@ class Hornet:
         call("hn_graph_export_csr", self._h, _ptr(off), _ptr(adj), ne, _stream())
         return off, adj[:ne]
 
+    def snapshot(self, slack: float = 1.0) -> "HornetStatic":
+        """A frozen device copy of the graph as it is now (hn_graph_snapshot):
+        later updates of this graph do not reach it."""
+        h = ctypes.c_void_p()
+        call("hn_graph_snapshot", self._h, float(slack), _stream(), ctypes.byref(h))
+        return HornetStatic(_handle=h)
+
     # ------------------------------------------------------------------ updates
     def insert(self, batch: BatchUpdate, removeBatchDuplicates: bool = False,
                removeGraphDuplicates: bool = False) -> int:
@@ -340,6 +363,18 @@ class Hornet:
             self.release()
         except Exception:
             pass
+
+
+class HornetStatic(Hornet):
+    """hornet::gpu::HornetStatic<int> (hornet/include/Core/Static/HornetStatic.cuh:16-66):
+    an immutable graph with the same queries and device view; insert and erase
+    raise."""
+
+    def insert(self, batch, removeBatchDuplicates: bool = False, removeGraphDuplicates: bool = False):
+        raise HornetError(_lib.HN_E_INVALID_ARG, "HornetStatic.insert", "the graph is immutable")
+
+    def erase(self, batch, removeBatchDuplicates: bool = False):
+        raise HornetError(_lib.HN_E_INVALID_ARG, "HornetStatic.erase", "the graph is immutable")
 
 
 class StaticAlgorithm:

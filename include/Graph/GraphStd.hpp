@@ -83,6 +83,14 @@ public:
         : _structure(structure) {
         read(filename, prop);
     }
+    // GraphStd(csr_offsets, nV, csr_edges, nE) -- xlib/src/Graph/GraphStd.cpp:50-59
+    // (structure UNDIRECTED), e.g. to write a graph's CSR with writeBinary
+    GraphStd(const eoff_t* csr_offsets, vid_t nV, const vid_t* csr_edges, eoff_t nE)
+        : _structure(structure_prop::UNDIRECTED) {
+        std::vector<int64_t> off(csr_offsets, csr_offsets + nV + 1);
+        check(hn_gstd_from_csr(nV, (int64_t)nE, off.data(), csr_edges, &_g), "GraphStd(csr)");
+        load();
+    }
     GraphStd(const GraphStd&) = delete;
     GraphStd& operator=(const GraphStd&) = delete;
     ~GraphStd() { hn_gstd_destroy(_g); }
@@ -92,6 +100,11 @@ public:
         hn_gstd_destroy(_g);
         _g = nullptr;
         check(hn_gstd_read(filename, _structure.value(), prop.value(), seed, &_g), "GraphStd::read");
+        load();
+    }
+
+private:
+    void load() {
         int32_t nv;
         int64_t ne;
         uint32_t st;
@@ -114,6 +127,7 @@ public:
         }
     }
 
+public:
     vid_t nV() const noexcept { return _nV; }
     eoff_t nE() const noexcept { return _nE; }
     bool is_directed() const noexcept { return _st & HN_GS_DIRECTED; }

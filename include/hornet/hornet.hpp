@@ -4,6 +4,7 @@
 // names, arguments and defaults of the reference:
 //   hornet::HornetInit<vid_t>        hornet/include/Core/HornetInitialize/HornetInit.cuh:54-97
 //   hornet::gpu::Hornet<vid_t>       hornet/include/Core/Hornet.cuh:54-131
+//   hornet::gpu::HornetStatic<vid_t> hornet/include/Core/Static/HornetStatic.cuh:16-66
 //   hornet::BatchUpdatePtr / gpu::BatchUpdate
 //                                    hornet/include/Core/BatchUpdate/BatchUpdate.cuh:77-272
 //   hornet::COO                      hornet/include/Core/Static/Static.cuh:19-130
@@ -150,8 +151,10 @@ private:
     int64_t _applied = 0;
 };
 
+// HornetStatic (hornet/include/Core/Static/HornetStatic.cuh:16-66): an
+// immutable graph with Hornet's queries and device view; Hornet adds updates.
 template <typename VID = vid_t, typename VMeta = EMPTY, typename EMeta = EMPTY, typename DEG = degree_t>
-class Hornet {
+class HornetStatic {
     static_assert(std::is_same<VID, int>::value && std::is_same<DEG, int>::value, "int ids and degrees");
     static_assert(std::is_same<VMeta, EMPTY>::value && std::is_same<EMeta, EMPTY>::value, "no meta data");
 
@@ -161,18 +164,14 @@ public:
     using HornetDeviceT = HornetDevice;
     using HInitT = HornetInit<VID, VMeta, EMeta, DEG>;
 
-    // Hornet(HornetInit&) (HornetInitialize.i.cuh:86-146); `arena_slack` sizes
-    // the edge arena as a multiple of the blocks the input needs
-    explicit Hornet(HInitT& init, double arena_slack = 0.0, hipStream_t stream = nullptr) : _stream(stream) {
-        check(hn_graph_create_csr(init.nV(), init.nE(), init.csr_offsets(), init.csr_edges(), HN_OFFSETS_INT32,
-                                  arena_slack, stream, &_g),
-              "Hornet(HornetInit&)");
-    }
+    // HornetStatic(HInitT&): no room for growth is reserved
+    explicit HornetStatic(HInitT& init, hipStream_t stream = nullptr) : HornetStatic(init, 1.0, stream) {}
     // adopt a handle created through the C ABI
-    explicit Hornet(hn_graph* g, hipStream_t stream = nullptr) noexcept : _g(g), _stream(stream) {}
-    Hornet(const Hornet&) = delete;
-    Hornet& operator=(const Hornet&) = delete;
-    ~Hornet() { hn_graph_destroy(_g); }
+    explicit HornetStatic(hn_graph* g, hipStream_t stream = nullptr) noexcept : _g(g), _stream(stream) {}
+    HornetStatic(const HornetStatic&) = delete;
+    HornetStatic& operator=(const HornetStatic&) = delete;
+    HornetStatic(HornetStatic&& o) noexcept : _g(o._g), _stream(o._stream) { o._g = nullptr; }
+    ~HornetStatic() { hn_graph_destroy(_g); }
 
     DEG nV() const {
         int32_t v = 0;
@@ -193,22 +192,6 @@ public:
         hn_dview v;
         check(hn_graph_device_view(_g, &v), "Hornet::device");
         return HornetDevice(v);
-    }
-    // Hornet::insert (HornetInsert.i.cuh:12-27)
-    void insert(BatchUpdate<VID, EMeta, DEG>& batch, bool removeBatchDuplicates = false,
-                bool removeGraphDuplicates = false) {
-        uint32_t f = batch.flags() | (removeBatchDuplicates ? HN_DEDUP_BATCH : 0u) |
-                     (removeGraphDuplicates ? HN_DEDUP_GRAPH : 0u);
-        int64_t n = 0;
-        check(hn_graph_insert(_g, batch.src(), batch.dst(), batch.nE(), f, &n, _stream), "Hornet::insert");
-        batch.set_applied(n);
-    }
-    // Hornet::erase (HornetInsert.i.cuh:71-89)
-    void erase(BatchUpdate<VID, EMeta, DEG>& batch, bool removeBatchDuplicates = false) {
-        uint32_t f = batch.flags() | (removeBatchDuplicates ? HN_DEDUP_BATCH : 0u);
-        int64_t n = 0;
-        check(hn_graph_erase(_g, batch.src(), batch.dst(), batch.nE(), f, &n, _stream), "Hornet::erase");
-        batch.set_applied(n);
     }
     // HornetQuery.i.cuh:10-36 (lowest id on ties)
     VID max_degree_id() const {
@@ -234,10 +217,64 @@ public:
     hn_graph* handle() const noexcept { return _g; }
     hipStream_t stream() const noexcept { return _stream; }
 
-private:
+protected:
+    HornetStatic(HInitT& init, double arena_slack, hipStream_t stream) : _stream(stream) {
+        check(hn_graph_create_csr(init.nV(), init.nE(), init.csr_offsets(), init.csr_edges(), HN_OFFSETS_INT32,
+                                  arena_slack, stream, &_g),
+              "Hornet(HornetInit&)");
+    }
+
     hn_graph* _g = nullptr;
     hipStream_t _stream = nullptr;
 };
 
+template <typename VID = vid_t, typename VMeta = EMPTY, typename EMeta = EMPTY, typename DEG = degree_t>
+class Hornet : public HornetStatic<VID, VMeta, EMeta, DEG> {
+    using Base = HornetStatic<VID, VMeta, EMeta, DEG>;
+    using Base::_g;
+    using Base::_stream;
+
+public:
+    using typename Base::HInitT;
+
+    // Hornet(HornetInit&) (HornetInitialize.i.cuh:86-146); `arena_slack` sizes
+    // the edge arena as a multiple of the blocks the input needs
+    explicit Hornet(HInitT& init, double arena_slack = 0.0, hipStream_t stream = nullptr)
+        : Base(init, arena_slack, stream) {}
+    // adopt a handle created through the C ABI
+    explicit Hornet(hn_graph* g, hipStream_t stream = nullptr) noexcept : Base(g, stream) {}
+
+    // Hornet::insert (HornetInsert.i.cuh:12-27)
+    void insert(BatchUpdate<VID, EMeta, DEG>& batch, bool removeBatchDuplicates = false,
+                bool removeGraphDuplicates = false) {
+        uint32_t f = batch.flags() | (removeBatchDuplicates ? HN_DEDUP_BATCH : 0u) |
+                     (removeGraphDuplicates ? HN_DEDUP_GRAPH : 0u);
+        int64_t n = 0;
+        check(hn_graph_insert(_g, batch.src(), batch.dst(), batch.nE(), f, &n, _stream), "Hornet::insert");
+        batch.set_applied(n);
+    }
+    // Hornet::erase (HornetInsert.i.cuh:71-89)
+    void erase(BatchUpdate<VID, EMeta, DEG>& batch, bool removeBatchDuplicates = false) {
+        uint32_t f = batch.flags() | (removeBatchDuplicates ? HN_DEDUP_BATCH : 0u);
+        int64_t n = 0;
+        check(hn_graph_erase(_g, batch.src(), batch.dst(), batch.nE(), f, &n, _stream), "Hornet::erase");
+        batch.set_applied(n);
+    }
+    // a frozen copy of the graph as it is now (hn_graph_snapshot; no reference
+    // counterpart): later updates of this graph do not reach it
+    Base snapshot(double arena_slack = 1.0) const {
+        hn_graph* c = nullptr;
+        check(hn_graph_snapshot(_g, arena_slack, _stream, &c), "Hornet::snapshot");
+        return Base(c, _stream);
+    }
+};
+
 }  // namespace gpu
+
+// HornetStatic.cuh:76-80
+template <typename>
+struct IsHornetStatic : std::false_type {};
+template <typename V, typename VM, typename EM, typename D>
+struct IsHornetStatic<gpu::HornetStatic<V, VM, EM, D>> : std::true_type {};
+
 }  // namespace hornet

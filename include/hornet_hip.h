@@ -139,6 +139,13 @@ hn_status hn_graph_export_coo(hn_graph* g, int32_t* src, int32_t* dst, int64_t c
  * adj[cap] int32, device buffers.  Synchronises. */
 hn_status hn_graph_export_csr(hn_graph* g, int64_t* offsets, int32_t* adj, int64_t cap,
                               void* stream);
+/* An independent device copy of g (no reference counterpart: the reference's
+ * immutable graph, HornetStatic, hornet/include/Core/Static/HornetStatic.cuh:16-66,
+ * is built from a HornetInit).  Updates to either graph do not reach the other,
+ * so queries can read a fixed version while the original keeps changing.
+ * `arena_slack` as in hn_graph_create_csr (values < 1.0 mean 1.0: no room for
+ * growth).  Synchronises. */
+hn_status hn_graph_snapshot(hn_graph* g, double arena_slack, void* stream, hn_graph** out);
 /* Per-vertex degrees into a device int32[nv]. */
 hn_status hn_graph_degrees(hn_graph* g, int32_t* deg, void* stream);
 /* Hornet::max_degree_id / max_degree -- HornetQuery.i.cuh:10-36 (lowest id
@@ -318,6 +325,11 @@ typedef struct hn_gstd hn_gstd;
  * binary, an undirected file read into a DIRECTED structure. */
 hn_status hn_gstd_read(const char* path, uint32_t structure, uint32_t parse, uint64_t seed,
                        hn_gstd** out);
+/* GraphStd(csr_offsets, nV, csr_edges, nE) -- xlib/src/Graph/GraphStd.cpp:50-59:
+ * a host graph (structure UNDIRECTED) copied from a host CSR, e.g. a graph's
+ * hn_graph_export_csr, to be written with hn_gstd_write_binary. */
+hn_status hn_gstd_from_csr(int32_t nv, int64_t ne, const int64_t* offsets, const int32_t* edges,
+                           hn_gstd** out);
 /* ~GraphStd() */
 hn_status hn_gstd_destroy(hn_gstd* g);
 /* nV(), nE() -- GraphBase.hpp:137-138,192-199 -- and the resolved StructureProp bits */

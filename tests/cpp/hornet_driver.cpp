@@ -20,6 +20,12 @@
 //                                       with WORK_FACTOR 0, 1, 1000 and from a pair
 //                                       queue); writes int32 dist[nv] then 4 x
 //                                       uint64 credits[nv]
+//   snapshot <graph> <batch> <ckpt.bin> <out.bin>
+//                                       freeze a HornetStatic copy, insert the batch
+//                                       into the live graph, checkpoint the live
+//                                       graph through GraphStd::writeBinary, resume it
+//                                       as a HornetStatic; writes the COO of the
+//                                       frozen copy, then of the resumed graph
 //   bfsfile <file> <out.bin>            BFSTest.cu:18-44 shape: graph::GraphStd
 //                                       read of a .mtx/.gr/.bin file, BFS from
 //                                       max_out_degree_id; writes int32 dist[nv]
@@ -304,6 +310,55 @@ int main(int argc, char** argv) {
         adj_run("queue", 1, true);
         gpu::free(per);
         gpu::free(acc);
+        return 0;
+    }
+    if (cmd == "snapshot" && argc >= 6) {
+        static_assert(hornet::IsHornetStatic<hornet::gpu::HornetStatic<int>>::value, "trait");
+        static_assert(!hornet::IsHornetStatic<HornetGraph>::value, "trait");
+        hornet::gpu::HornetStatic<int> frozen = hornet_graph.snapshot();
+        std::ifstream f(argv[3], std::ios::binary);
+        int64_t n = 0;
+        f.read(reinterpret_cast<char*>(&n), 8);
+        std::vector<int> s((size_t)n), d((size_t)n);
+        f.read(reinterpret_cast<char*>(s.data()), (std::streamsize)(4 * n));
+        f.read(reinterpret_cast<char*>(d.data()), (std::streamsize)(4 * n));
+        hornet::BatchUpdatePtr<int, hornet::EMPTY, hornet::DeviceType::HOST> ptr((int)n, s.data(), d.data());
+        hornet::gpu::BatchUpdate<int> batch(ptr);
+        hornet_graph.insert(batch, true, true);
+        // checkpoint: CSR of the live graph -> GraphStd -> binary file
+        const int64_t nv = hornet_graph.nV(), ne = hornet_graph.nE64();
+        int64_t* d_off = nullptr;
+        int32_t* d_adj = nullptr;
+        gpu::allocate(d_off, (size_t)nv + 1);
+        gpu::allocate(d_adj, (size_t)std::max<int64_t>(ne, 1));
+        hornet::check(hn_graph_export_csr(hornet_graph.handle(), d_off, d_adj, ne, nullptr), "export_csr");
+        std::vector<int64_t> off((size_t)nv + 1);
+        std::vector<int> adj((size_t)ne);
+        gpu::copyToHost(d_off, off.size(), off.data());
+        if (ne > 0) gpu::copyToHost(d_adj, adj.size(), adj.data());
+        gpu::free(d_off, d_adj);
+        {
+            graph::GraphStd<int, int64_t> ckpt(off.data(), (int)nv, adj.data(), ne);
+            ckpt.writeBinary(argv[4]);
+        }
+        graph::GraphStd<int, int> back;
+        back.read(argv[4]);
+        HornetInit rinit(back.nV(), back.nE(), back.csr_out_offsets(), back.csr_out_edges());
+        hornet::gpu::HornetStatic<int> resumed(rinit);
+        std::printf("snapshot frozen_ne=%d live_ne=%d resumed_ne=%d inserted=%lld\n", frozen.nE(), hornet_graph.nE(),
+                    resumed.nE(), (long long)batch.applied());
+        std::ofstream out(argv[5], std::ios::binary);
+        auto put = [&](const hornet::gpu::HornetStatic<int>& h) {
+            auto coo = h.getCOO();
+            std::vector<int> cs, cd;
+            coo.toHost(cs, cd);
+            int64_t m = (int64_t)cs.size();
+            out.write(reinterpret_cast<const char*>(&m), 8);
+            out.write(reinterpret_cast<const char*>(cs.data()), (std::streamsize)(4 * cs.size()));
+            out.write(reinterpret_cast<const char*>(cd.data()), (std::streamsize)(4 * cd.size()));
+        };
+        put(frozen);
+        put(resumed);
         return 0;
     }
     if (cmd == "update" && argc >= 5) {

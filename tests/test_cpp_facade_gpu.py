@@ -156,3 +156,33 @@ def test_operator_surface_and_atomic_bfs(tmp_path):
             assert field(line, "search_calls") == 0 and field(line, "merge_calls") > 0
         if tag == "wf1000":
             assert field(line, "search_calls") > 0
+
+
+def test_snapshot_and_checkpoint_through_headers(graph, tmp_path):
+    off, adj, p = graph
+    n = off.size - 1
+    bs, bd = O.uniform_pairs(n, 3000, seed=41)
+    bp = tmp_path / "b.bin"
+    with open(bp, "wb") as f:
+        np.array([bs.size], np.int64).tofile(f)
+        bs.astype(np.int32).tofile(f)
+        bd.astype(np.int32).tofile(f)
+    out = run("snapshot", p, str(bp), str(tmp_path / "ckpt.bin"), str(tmp_path / "coo.bin"))
+    o1, a1, added = O.insert(off, adj, bs, bd, True, True)
+    assert field(out, "inserted") == added and added > 0
+    assert field(out, "frozen_ne") == adj.size
+    assert field(out, "live_ne") == field(out, "resumed_ne") == a1.size
+    raw = np.fromfile(tmp_path / "coo.bin", np.int8)
+    pos = 0
+    for (oo, aa) in ((off, adj), (o1, a1)):    # frozen copy = before; resumed = after
+        m = int(raw[pos:pos + 8].view(np.int64)[0])
+        pos += 8
+        src = raw[pos:pos + 4 * m].view(np.int32)
+        pos += 4 * m
+        dst = raw[pos:pos + 4 * m].view(np.int32)
+        pos += 4 * m
+        assert np.array_equal(src, np.repeat(np.arange(oo.size - 1), np.diff(oo)))
+        assert np.array_equal(dst, aa)
+    # the checkpoint is the reference's binary format ("intlong": int64 offsets)
+    ck = O.read_binary(str(tmp_path / "ckpt.bin"))
+    assert np.array_equal(ck["off"], o1) and np.array_equal(ck["adj"], a1)

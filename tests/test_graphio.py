@@ -261,3 +261,22 @@ def test_hornet_init_shape():
     init = g.hornet_init()
     assert init.nV == 9 and init.nE == 22
     assert init.offsets.dtype == np.int64 and init.edges.dtype == np.int32
+
+
+def test_from_csr_binary_round_trip(tmp_path):
+    exp = O.read_dimacs9_std(gold("rome99.gr"), 0, PP.SORT)
+    g = GraphStd.from_csr(exp["off"], exp["adj"])
+    assert g.is_undirected() and g.nV() == exp["n"]
+    p = str(tmp_path / "c.bin")
+    g.writeBinary(p)
+    r = GraphStd().read(p)
+    check_same(r, O.read_binary(p))
+    assert np.array_equal(r.csr_out_offsets(), exp["off"]) and np.array_equal(r.csr_out_edges(), exp["adj"])
+
+
+@pytest.mark.parametrize("off,adj", [([0, 2, 1], [0, 1]), ([0, 1, 3], [0, 1]), ([1, 1, 2], [0, 1]),
+                                     ([0, 1, 2], [0, 2])])
+def test_from_csr_rejects_bad_csr(off, adj):
+    with pytest.raises(HornetError) as e:
+        GraphStd.from_csr(np.array(off), np.array(adj, np.int32))
+    assert e.value.status == _lib.HN_E_INVALID_ARG
