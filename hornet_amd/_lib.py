@@ -42,9 +42,18 @@ class HnVrec(ctypes.Structure):
     _fields_ = [("off", ctypes.c_int64), ("deg", ctypes.c_int32), ("cap_log2", ctypes.c_int32)]
 
 
+HN_MAX_EDGE_META = 4
+
+
+class HnEdgeMeta(ctypes.Structure):
+    _fields_ = [("nfields", ctypes.c_int32), ("bytes", ctypes.c_int32 * HN_MAX_EDGE_META)]
+
+
 class HnDview(ctypes.Structure):
-    _fields_ = [("nv", ctypes.c_int32), ("pad_", ctypes.c_int32), ("ne", ctypes.c_int64),
-                ("rec", ctypes.c_void_p), ("arena", ctypes.c_void_p)]
+    _fields_ = [("nv", ctypes.c_int32), ("nmeta", ctypes.c_int32), ("ne", ctypes.c_int64),
+                ("rec", ctypes.c_void_p), ("arena", ctypes.c_void_p),
+                ("meta", ctypes.c_void_p * HN_MAX_EDGE_META),
+                ("meta_bytes", ctypes.c_int32 * HN_MAX_EDGE_META)]
 
 
 class HnBfsStats(ctypes.Structure):
@@ -97,6 +106,10 @@ SIGNATURES = {
     "hn_device_info": [PI32, PI64, PI64, PI32],
     "hn_graph_create_csr": [I32, I64, P, P, U32, D, P, PP],
     "hn_graph_create_coo": [I32, I64, P, P, U32, D, P, PP],
+    "hn_graph_create_csr_meta": [I32, I64, P, P, ctypes.POINTER(HnEdgeMeta), P, U32, D, P, PP],
+    "hn_graph_edge_meta": [P, ctypes.POINTER(HnEdgeMeta)],
+    "hn_graph_export_coo_meta": [P, P, P, P, I64, PI64, P],
+    "hn_graph_insert_meta": [P, P, P, P, I64, U32, PI64, P],
     "hn_graph_destroy": [P],
     "hn_graph_nv": [P, PI32],
     "hn_graph_ne": [P, PI64],

@@ -94,5 +94,9 @@ inline int ceil_log2_host(int64_t x) {
 size_t sort_temp_bytes(int64_t n);
 hn_status radix_sort_u64(uint64_t** keys, uint64_t** alt, int64_t n, int key_bits, void* tmp,
                          hipStream_t st);
+// The same sort carrying a 32-bit value per key (vals / valt swap with the
+// keys): the values of equal keys keep their input order.
+hn_status radix_sort_u64_kv(uint64_t** keys, uint64_t** alt, uint32_t** vals, uint32_t** valt, int64_t n,
+                            int key_bits, void* tmp, hipStream_t st);
 
 }  // namespace hn

@@ -79,13 +79,16 @@ __global__ __launch_bounds__(BLOCK) void copy_in_kernel(int32_t nv, int64_t ne,
                                                         const int64_t* __restrict__ off,
                                                         const int32_t* __restrict__ adj,
                                                         const hn_vrec* __restrict__ rec,
-                                                        int32_t* __restrict__ arena) {
+                                                        int32_t* __restrict__ arena, MetaSet am, MetaSet in) {
     __shared__ LbShared<COPY_IPT, int64_t> sh;
     for (int64_t w0 = (int64_t)blockIdx.x * COPY_TILE; w0 < ne; w0 += (int64_t)gridDim.x * COPY_TILE) {
         lb_tile<COPY_IPT>(
             off, (int64_t)nv, w0, ne, sh,
             [&](int64_t s) { return rec[s].off - off[s]; },
-            [&](int64_t, int64_t w, int64_t base) { arena[base + w] = adj[w]; });
+            [&](int64_t, int64_t w, int64_t base) {
+                arena[base + w] = adj[w];
+                meta_put(am, base + w, in, w);
+            });
     }
 }
 
@@ -95,7 +98,7 @@ __global__ __launch_bounds__(BLOCK) void flatten_kernel(int32_t nv, int64_t ne,
                                                         const hn_vrec* __restrict__ rec,
                                                         const int32_t* __restrict__ arena,
                                                         int32_t* __restrict__ src,
-                                                        int32_t* __restrict__ dst) {
+                                                        int32_t* __restrict__ dst, MetaSet out, MetaSet am) {
     __shared__ LbShared<COPY_IPT, int64_t> sh;
     for (int64_t w0 = (int64_t)blockIdx.x * COPY_TILE; w0 < ne; w0 += (int64_t)gridDim.x * COPY_TILE) {
         lb_tile<COPY_IPT>(
@@ -104,6 +107,7 @@ __global__ __launch_bounds__(BLOCK) void flatten_kernel(int32_t nv, int64_t ne,
             [&](int64_t s, int64_t w, int64_t base) {
                 dst[w] = arena[base + w];
                 if (src) src[w] = (int32_t)s;
+                meta_put(out, w, am, base + w);
             });
     }
 }
@@ -263,6 +267,7 @@ static void destroy_graph(hn_graph* g) {
     g->ms = nullptr;
     dfree(g->rec);
     dfree(g->arena);
+    for (int f = 0; f < g->meta.n; f++) dfree(g->meta.p[f]);
     dfree(g->alloc);
     dfree(g->stacks);
     dfree(g->stack_off);
@@ -270,6 +275,7 @@ static void destroy_graph(hn_graph* g) {
     g->s_scan.release();
     g->s_upd.release();
     g->s_tc.release();
+    g->s_mstage.release();
     g->s_misc.release();
     if (g->h_pinned) (void)hipHostFree(g->h_pinned);
     delete g;
@@ -282,6 +288,8 @@ static hn_status setup_arena(hn_graph* g, uint64_t used_units, double slack, hip
     HN_ARG(cap < (1ull << 32), "edge arena would exceed 2^32 units of 128 B");
     g->cap_units = (int64_t)cap;
     HN_TRY(dalloc(&g->arena, (size_t)cap * UNIT_SLOTS));
+    for (int f = 0; f < g->meta.n; f++)
+        HN_TRY(dmalloc(reinterpret_cast<void**>(&g->meta.p[f]), (size_t)cap * UNIT_SLOTS * (size_t)g->meta.bytes[f]));
     // class k stack holds at most cap >> (k - MIN_CLASS) blocks
     int64_t tot = 0;
     for (int k = 0; k < NCLASS; k++) {
@@ -334,35 +342,57 @@ __global__ void row_keys_kernel(int32_t nv, int64_t ne, const int64_t* __restric
     if (__ballot(down) != 0 && (threadIdx.x & (WAVE - 1)) == 0) atomicOr(unsorted, 1u);
 }
 
+// sorted (row, dst) keys back to entries; with `idx` (each key's input
+// position) the meta-data fields are gathered along
 __global__ void keys_to_adj_kernel(int64_t ne, const uint64_t* __restrict__ keys, uint64_t mask,
-                                   int32_t* __restrict__ adj) {
+                                   int32_t* __restrict__ adj, const uint32_t* __restrict__ idx, MetaSet out,
+                                   MetaSet in) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < ne) adj[j] = (int32_t)(keys[j] & mask);
+    if (j >= ne) return;
+    adj[j] = (int32_t)(keys[j] & mask);
+    if (idx) meta_put(out, j, in, idx[j]);
+}
+
+__global__ void iota_u32_kernel(int64_t n, uint32_t* __restrict__ a) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) a[j] = (uint32_t)j;
 }
 
 // HornetInit accepts rows in any order (HornetInit.cuh:70-73) while the
 // storage keeps every adjacency ascending: *out = nullptr when the rows of
 // (d_off, d_adj) are already ascending, else a new device array holding each
 // row sorted (a radix sort of (row, dst) keys, which keeps the rows in place).
+// *out = nullptr when the rows of (d_off, d_adj) are already ascending, else a
+// new device array holding each row sorted (a radix sort of (row, dst) keys,
+// which keeps the rows in place); with meta-data the sort carries each
+// entry's position and *out_meta receives the fields in the new order (new
+// device arrays, caller frees).
 static hn_status sort_rows_if_needed(int32_t nv, int64_t ne, const int64_t* d_off, const int32_t* d_adj,
-                                     hipStream_t st, int32_t** out) {
+                                     const MetaSet& in_meta, hipStream_t st, int32_t** out, MetaSet* out_meta) {
     *out = nullptr;
+    *out_meta = meta_like(in_meta, nullptr);
     if (ne == 0 || nv == 0) return HN_OK;
     int sbits = ceil_log2_host((int64_t)nv + 1);
     if (sbits < 1) sbits = 1;
     uint64_t* ka = nullptr;
     uint64_t* kb = nullptr;
+    uint32_t* ia = nullptr;
+    uint32_t* ib = nullptr;
     unsigned* flag = nullptr;
     void* tmp = nullptr;
     int32_t* res = nullptr;
+    MetaSet om = meta_like(in_meta, nullptr);
     hn_status s = HN_OK;
     auto cleanup = [&]() {
-        dfree(ka); dfree(kb); dfree(flag); dfree(tmp);
+        dfree(ka); dfree(kb); dfree(ia); dfree(ib); dfree(flag); dfree(tmp);
     };
-    if ((s = dalloc(&ka, (size_t)ne)) != HN_OK || (s = dalloc(&flag, 1)) != HN_OK) {
+    auto fail = [&](hn_status e) {
         cleanup();
-        return s;
-    }
+        dfree(res);
+        for (int f = 0; f < om.n; f++) dfree(om.p[f]);
+        return e;
+    };
+    if ((s = dalloc(&ka, (size_t)ne)) != HN_OK || (s = dalloc(&flag, 1)) != HN_OK) return fail(s);
     HN_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned), st));
     row_keys_kernel<<<(unsigned)ceil_div(ne, BLOCK), BLOCK, 0, st>>>(nv, ne, d_off, d_adj, sbits, ka, flag);
     HN_CHECK_LAUNCH();
@@ -374,28 +404,38 @@ static hn_status sort_rows_if_needed(int32_t nv, int64_t ne, const int64_t* d_of
         return HN_OK;
     }
     if ((s = dalloc(&kb, (size_t)ne)) != HN_OK || (s = dmalloc(&tmp, sort_temp_bytes(ne))) != HN_OK ||
-        (s = radix_sort_u64(&ka, &kb, ne, 2 * sbits, tmp, st)) != HN_OK ||
-        (s = dalloc(&res, (size_t)ne)) != HN_OK) {
-        cleanup();
-        return s;
+        (s = dalloc(&res, (size_t)ne)) != HN_OK)
+        return fail(s);
+    if (in_meta.any()) {
+        if ((s = dalloc(&ia, (size_t)ne)) != HN_OK || (s = dalloc(&ib, (size_t)ne)) != HN_OK) return fail(s);
+        for (int f = 0; f < om.n; f++)
+            if ((s = dmalloc(reinterpret_cast<void**>(&om.p[f]), (size_t)ne * (size_t)om.bytes[f])) != HN_OK)
+                return fail(s);
+        iota_u32_kernel<<<(unsigned)ceil_div(ne, BLOCK), BLOCK, 0, st>>>(ne, ia);
+        HN_CHECK_LAUNCH();
+        if ((s = radix_sort_u64_kv(&ka, &kb, &ia, &ib, ne, 2 * sbits, tmp, st)) != HN_OK) return fail(s);
+    } else if ((s = radix_sort_u64(&ka, &kb, ne, 2 * sbits, tmp, st)) != HN_OK) {
+        return fail(s);
     }
-    keys_to_adj_kernel<<<(unsigned)ceil_div(ne, BLOCK), BLOCK, 0, st>>>(ne, ka, (1ull << sbits) - 1, res);
+    keys_to_adj_kernel<<<(unsigned)ceil_div(ne, BLOCK), BLOCK, 0, st>>>(ne, ka, (1ull << sbits) - 1, res, ia, om,
+                                                                        in_meta);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    cleanup();
     if (e != hipSuccess) {
-        dfree(res);
         set_error(std::string("row sort: ") + hipGetErrorString(e));
-        return HN_E_HIP;
+        return fail(HN_E_HIP);
     }
+    cleanup();
     *out = res;
+    *out_meta = om;
     return HN_OK;
 }
 
 hn_status graph_from_device_csr(int32_t nv, int64_t ne, const int64_t* d_off, const int32_t* d_adj,
-                                double slack, hipStream_t st, hn_graph** out) {
+                                double slack, hipStream_t st, hn_graph** out, const MetaSet& d_meta) {
     hn_graph* g = nullptr;
     HN_TRY(new_graph(nv, &g));
+    g->meta = meta_like(d_meta, nullptr);   // layout only; setup_arena allocates
     auto fail = [&](hn_status s) { destroy_graph(g); return s; };
     g->ne = ne;
     hn_status s;
@@ -416,7 +456,7 @@ hn_status graph_from_device_csr(int32_t nv, int64_t ne, const int64_t* d_off, co
     if (ne > 0) {
         int64_t tiles = ceil_div(ne, COPY_TILE);
         int64_t grid = std::min<int64_t>(tiles, (int64_t)num_cus() * 16);
-        copy_in_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(nv, ne, d_off, d_adj, g->rec, g->arena);
+        copy_in_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(nv, ne, d_off, d_adj, g->rec, g->arena, g->meta, d_meta);
     }
     hipError_t e = hipStreamSynchronize(st);
     dfree(uoff);
@@ -569,12 +609,29 @@ hn_status hn_device_info(int32_t* cus, int64_t* lds, int64_t* hbm, int32_t* wave
 
 hn_status hn_graph_create_csr(int32_t nv, int64_t ne, const void* offsets, const int32_t* edges,
                               uint32_t flags, double slack, void* stream, hn_graph** out) {
+    return hn_graph_create_csr_meta(nv, ne, offsets, edges, nullptr, nullptr, flags, slack, stream, out);
+}
+
+hn_status hn_graph_create_csr_meta(int32_t nv, int64_t ne, const void* offsets, const int32_t* edges,
+                                   const hn_edge_meta* layout, const void* const* meta, uint32_t flags,
+                                   double slack, void* stream, hn_graph** out) {
     HN_ARG(out != nullptr, "out is null");
     *out = nullptr;
     HN_ARG(nv >= 0 && ne >= 0, "negative size");
     HN_ARG(offsets != nullptr, "offsets is null");
     HN_ARG(ne == 0 || edges != nullptr, "edges is null");
     HN_ARG(ne < (int64_t)0xFFFFFFFF, "more than 2^32-1 edges");
+    MetaSet in_meta;
+    if (layout && layout->nfields > 0) {
+        HN_ARG(layout->nfields <= HN_MAX_EDGE_META, "more than HN_MAX_EDGE_META meta-data fields");
+        HN_ARG(ne == 0 || meta != nullptr, "meta-data arrays are null");
+        in_meta.n = layout->nfields;
+        for (int f = 0; f < in_meta.n; f++) {
+            HN_ARG(layout->bytes[f] == 4 || layout->bytes[f] == 8, "meta-data fields are 4 or 8 bytes");
+            HN_ARG(ne == 0 || meta[f] != nullptr, "a meta-data array is null");
+            in_meta.bytes[f] = layout->bytes[f];
+        }
+    }
     hipStream_t st = as_stream(stream);
     const bool dev = flags & HN_DEVICE_PTRS;
     const bool o32 = flags & HN_OFFSETS_INT32;
@@ -590,6 +647,12 @@ hn_status hn_graph_create_csr(int32_t nv, int64_t ne, const void* offsets, const
     }
     int64_t* d_off = nullptr;
     int32_t* d_adj = nullptr;
+    MetaSet up = meta_like(in_meta, nullptr);   // host meta-data copied to the device
+    auto release = [&]() {
+        dfree(d_off);
+        dfree(d_adj);
+        for (int f = 0; f < up.n; f++) dfree(up.p[f]);
+    };
     hn_status s;
     if ((s = dalloc(&d_off, (size_t)nv + 1)) != HN_OK) return s;
     if (dev && !o32) {
@@ -604,18 +667,38 @@ hn_status hn_graph_create_csr(int32_t nv, int64_t ne, const void* offsets, const
         HN_HIP(hipMemcpyAsync(d_off, hoff.data(), sizeof(int64_t) * hoff.size(), hipMemcpyHostToDevice, st));
     }
     const int32_t* adj_in = edges;
+    MetaSet dmeta = in_meta;
+    for (int f = 0; f < in_meta.n; f++) dmeta.p[f] = const_cast<char*>(static_cast<const char*>(meta[f]));
     if (!dev) {
-        if ((s = dalloc(&d_adj, (size_t)(ne > 0 ? ne : 1))) != HN_OK) { dfree(d_off); return s; }
+        if ((s = dalloc(&d_adj, (size_t)(ne > 0 ? ne : 1))) != HN_OK) { release(); return s; }
         if (ne > 0) HN_HIP(hipMemcpyAsync(d_adj, edges, sizeof(int32_t) * (size_t)ne, hipMemcpyHostToDevice, st));
         adj_in = d_adj;
+        for (int f = 0; f < up.n; f++) {
+            const size_t bytes = (size_t)(ne > 0 ? ne : 1) * (size_t)up.bytes[f];
+            if ((s = dmalloc(reinterpret_cast<void**>(&up.p[f]), bytes)) != HN_OK) { release(); return s; }
+            if (ne > 0)
+                HN_HIP(hipMemcpyAsync(up.p[f], meta[f], (size_t)ne * (size_t)up.bytes[f], hipMemcpyHostToDevice, st));
+            dmeta.p[f] = up.p[f];
+        }
     }
     int32_t* d_sorted = nullptr;
-    s = sort_rows_if_needed(nv, ne, d_off, adj_in, st, &d_sorted);
-    if (s == HN_OK) s = graph_from_device_csr(nv, ne, d_off, d_sorted ? d_sorted : adj_in, slack, st, out);
-    dfree(d_off);
-    dfree(d_adj);
+    MetaSet sorted_meta;
+    s = sort_rows_if_needed(nv, ne, d_off, adj_in, dmeta, st, &d_sorted, &sorted_meta);
+    if (s == HN_OK)
+        s = graph_from_device_csr(nv, ne, d_off, d_sorted ? d_sorted : adj_in, slack, st, out,
+                                  d_sorted ? sorted_meta : dmeta);
+    release();
     dfree(d_sorted);
+    if (d_sorted)
+        for (int f = 0; f < sorted_meta.n; f++) dfree(sorted_meta.p[f]);
     return s;
+}
+
+hn_status hn_graph_edge_meta(const hn_graph* g, hn_edge_meta* layout) {
+    HN_ARG(g && layout, "null argument");
+    layout->nfields = g->meta.n;
+    for (int f = 0; f < HN_MAX_EDGE_META; f++) layout->bytes[f] = f < g->meta.n ? g->meta.bytes[f] : 0;
+    return HN_OK;
 }
 
 hn_status hn_graph_create_coo(int32_t nv, int64_t m, const int32_t* src, const int32_t* dst,
@@ -675,7 +758,8 @@ hn_status hn_graph_memory(hn_graph* g, int64_t* used_slots, int64_t* capacity_sl
     return HN_OK;
 }
 
-static hn_status flatten(hn_graph* g, int32_t* src, int32_t* dst, int64_t* eoff_out, hipStream_t st) {
+static hn_status flatten(hn_graph* g, int32_t* src, int32_t* dst, int64_t* eoff_out, hipStream_t st,
+                         const MetaSet& out = MetaSet()) {
     int64_t* eoff = eoff_out;
     if (!eoff) HN_TRY(g->s_misc.reserve(sizeof(int64_t) * ((size_t)g->nv + 1)));
     if (!eoff) eoff = g->s_misc.as<int64_t>();
@@ -683,7 +767,8 @@ static hn_status flatten(hn_graph* g, int32_t* src, int32_t* dst, int64_t* eoff_
     if (g->ne > 0) {
         int64_t tiles = ceil_div(g->ne, COPY_TILE);
         int64_t grid = std::min<int64_t>(tiles, (int64_t)num_cus() * 16);
-        flatten_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(g->nv, g->ne, eoff, g->rec, g->arena, src, dst);
+        flatten_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(g->nv, g->ne, eoff, g->rec, g->arena, src, dst, out,
+                                                         g->meta);
         HN_CHECK_LAUNCH();
     }
     HN_HIP(hipStreamSynchronize(st));
@@ -701,6 +786,18 @@ hn_status hn_graph_export_coo(hn_graph* g, int32_t* src, int32_t* dst, int64_t c
     return flatten(g, src, dst, nullptr, as_stream(stream));
 }
 
+hn_status hn_graph_export_coo_meta(hn_graph* g, int32_t* src, int32_t* dst, void* const* meta, int64_t cap,
+                                   int64_t* n_out, void* stream) {
+    HN_ARG(g && src && dst, "null argument");
+    if (n_out) *n_out = g->ne;
+    if (cap < g->ne) {
+        set_error("export_coo: capacity " + std::to_string(cap) + " < ne " + std::to_string(g->ne));
+        return HN_E_CAPACITY;
+    }
+    MetaSet out = meta_like(g->meta, meta ? reinterpret_cast<char* const*>(meta) : nullptr);
+    return flatten(g, src, dst, nullptr, as_stream(stream), meta ? out : MetaSet());
+}
+
 hn_status hn_graph_export_csr(hn_graph* g, int64_t* offsets, int32_t* adj, int64_t cap, void* stream) {
     HN_ARG(g && offsets && adj, "null argument");
     if (cap < g->ne) {
@@ -716,12 +813,17 @@ hn_status hn_graph_snapshot(hn_graph* g, double slack, void* stream, hn_graph** 
     hipStream_t st = as_stream(stream);
     int64_t* d_off = nullptr;
     int32_t* d_adj = nullptr;
+    MetaSet tm = meta_like(g->meta, nullptr);
     hn_status s = dalloc(&d_off, (size_t)g->nv + 1);
     if (s == HN_OK) s = dalloc(&d_adj, (size_t)(g->ne > 0 ? g->ne : 1));
-    if (s == HN_OK) s = flatten(g, nullptr, d_adj, d_off, st);
-    if (s == HN_OK) s = graph_from_device_csr(g->nv, g->ne, d_off, d_adj, slack < 1.0 ? 1.0 : slack, st, out);
+    for (int f = 0; f < tm.n && s == HN_OK; f++)
+        s = dmalloc(reinterpret_cast<void**>(&tm.p[f]), (size_t)(g->ne > 0 ? g->ne : 1) * (size_t)tm.bytes[f]);
+    if (s == HN_OK) s = flatten(g, nullptr, d_adj, d_off, st, tm);
+    if (s == HN_OK)
+        s = graph_from_device_csr(g->nv, g->ne, d_off, d_adj, slack < 1.0 ? 1.0 : slack, st, out, tm);
     dfree(d_off);
     dfree(d_adj);
+    for (int f = 0; f < tm.n; f++) dfree(tm.p[f]);
     if (s == HN_OK) {
         // the copy starts at version 0 with what is known of the source's symmetry
         const bool known = g->sym == 1 || (g->sym == -1 && g->sym_version == g->version);

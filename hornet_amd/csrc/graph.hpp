@@ -142,6 +142,37 @@ __device__ __forceinline__ void free_block_wave(const AllocView& A, int k, int64
 
 }  // namespace hn
 
+namespace hn {
+// Edge meta-data arrays (parallel to the arena, a staging area, a batch or an
+// export): field f of entry i is p[f][i], bytes[f] wide.  p[f] == nullptr
+// reads as zero.
+struct MetaSet {
+    int32_t n = 0;
+    int32_t bytes[HN_MAX_EDGE_META] = {};
+    char* p[HN_MAX_EDGE_META] = {};
+    __host__ __device__ bool any() const { return n > 0; }
+};
+// the same layout over other arrays
+inline MetaSet meta_like(const MetaSet& m, char* const* ptrs) {
+    MetaSet r = m;
+    for (int f = 0; f < HN_MAX_EDGE_META; f++) r.p[f] = f < m.n && ptrs ? ptrs[f] : nullptr;
+    return r;
+}
+// d[di] = s[si] for every field
+__device__ __forceinline__ void meta_put(const MetaSet& d, int64_t di, const MetaSet& s, int64_t si) {
+    for (int f = 0; f < d.n; f++) {
+        if (!d.p[f]) continue;
+        if (d.bytes[f] == 8) {
+            const uint64_t v = s.p[f] ? reinterpret_cast<const uint64_t*>(s.p[f])[si] : 0ull;
+            reinterpret_cast<uint64_t*>(d.p[f])[di] = v;
+        } else {
+            const uint32_t v = s.p[f] ? reinterpret_cast<const uint32_t*>(s.p[f])[si] : 0u;
+            reinterpret_cast<uint32_t*>(d.p[f])[di] = v;
+        }
+    }
+}
+}  // namespace hn
+
 struct hn_graph {
     int32_t nv = 0;
     int64_t ne = 0;
@@ -152,6 +183,7 @@ struct hn_graph {
     uint64_t sym_version = 0;
     hn_vrec* rec = nullptr;           // device, nv records
     int32_t* arena = nullptr;         // device, cap_units * 32 slots
+    hn::MetaSet meta;                 // device, edge meta-data parallel to the arena
     int64_t cap_units = 0;
     hn::AllocState* alloc = nullptr;  // device
     uint32_t* stacks = nullptr;       // device
@@ -163,6 +195,7 @@ struct hn_graph {
     hn::Scratch s_upd;                // batch update buffers
     hn::Scratch s_tc;                 // triangle counting
     hn::Scratch s_misc;
+    hn::Scratch s_mstage;             // staging of edge meta-data during updates
     uint64_t* h_pinned = nullptr;     // small pinned readback area
     void* ms = nullptr;               // multi-source BFS engine (msbfs.hip)
     void (*ms_release)(void*) = nullptr;
@@ -171,10 +204,14 @@ struct hn_graph {
     hn_dview view() const {
         hn_dview v;
         v.nv = nv;
-        v.pad_ = 0;
+        v.nmeta = meta.n;
         v.ne = ne;
         v.rec = rec;
         v.arena = arena;
+        for (int f = 0; f < HN_MAX_EDGE_META; f++) {
+            v.meta[f] = f < meta.n ? meta.p[f] : nullptr;
+            v.meta_bytes[f] = f < meta.n ? meta.bytes[f] : 0;
+        }
         return v;
     }
 };
@@ -183,7 +220,8 @@ namespace hn {
 // build a graph from device CSR (offsets int64[nv+1], adj int32[ne]); adjacency
 // must already be sorted per vertex.  Does not take ownership of the inputs.
 hn_status graph_from_device_csr(int32_t nv, int64_t ne, const int64_t* d_off, const int32_t* d_adj,
-                                double slack, hipStream_t st, hn_graph** out);
+                                double slack, hipStream_t st, hn_graph** out,
+                                const MetaSet& d_meta = MetaSet());
 // device CSR from device COO: sort + dedup (+ symmetrize / drop self loops);
 // returns buffers allocated with dalloc (caller frees).
 hn_status csr_from_device_coo(int32_t nv, int64_t m, const int32_t* src, const int32_t* dst,

@@ -1,4 +1,5 @@
-// prims.hip -- wave64 LSD radix sort of 64-bit keys.
+// prims.hip -- wave64 LSD radix sort of 64-bit keys (optionally with a 32-bit
+// value per key).
 //
 // Replaces the lexicographic (src,dst) batch sort of the reference, two stable
 // thrust::sort_by_key calls (hornet/include/Core/SoA/impl/SoAPtr.i.cuh:560-571),
@@ -8,6 +9,8 @@
 // ballots) and stages the tile in LDS in digit order so that global writes
 // come out as per-bucket runs.
 #include "prims.hpp"
+
+#include <utility>
 
 namespace hn {
 
@@ -36,14 +39,19 @@ __global__ __launch_bounds__(BLOCK) void rs_hist_kernel(const uint64_t* __restri
     hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// KV: a 32-bit value rides with each key (staged beside it in LDS)
+template <bool KV>
 __global__ __launch_bounds__(BLOCK) void rs_scatter_kernel(const uint64_t* __restrict__ keys, int64_t n,
                                                            int shift, int64_t ntiles,
                                                            const uint32_t* __restrict__ hoff,
-                                                           uint64_t* __restrict__ out) {
+                                                           uint64_t* __restrict__ out,
+                                                           const uint32_t* __restrict__ vin,
+                                                           uint32_t* __restrict__ vout) {
     __shared__ uint32_t wcnt[NWAVES][RS_BUCKETS];   // per-wave digit counts
     __shared__ uint32_t dstart[RS_BUCKETS];         // tile-local start of each digit
     __shared__ uint32_t gbase[RS_BUCKETS];          // global start of this tile's digit run
     __shared__ uint64_t stage[RS_TILE];
+    __shared__ uint32_t vstage[KV ? RS_TILE : 1];
     __shared__ uint32_t sc[NWAVES + 1];
 
     const int w = threadIdx.x / WAVE;
@@ -97,6 +105,7 @@ __global__ __launch_bounds__(BLOCK) void rs_scatter_kernel(const uint64_t* __res
         if (i < n) {
             uint32_t d = (uint32_t)(k[j] >> shift) & (RS_BUCKETS - 1);
             stage[dstart[d] + wcnt[w][d] + rank[j]] = k[j];
+            if constexpr (KV) vstage[dstart[d] + wcnt[w][d] + rank[j]] = vin[i];
         }
     }
     __syncthreads();
@@ -105,6 +114,7 @@ __global__ __launch_bounds__(BLOCK) void rs_scatter_kernel(const uint64_t* __res
         uint64_t key = stage[p];
         uint32_t d = (uint32_t)(key >> shift) & (RS_BUCKETS - 1);
         out[(int64_t)gbase[d] + (p - dstart[d])] = key;
+        if constexpr (KV) vout[(int64_t)gbase[d] + (p - dstart[d])] = vstage[p];
     }
 }
 
@@ -116,8 +126,8 @@ size_t sort_temp_bytes(int64_t n) {
     return (size_t)(hn_ + 1) * 4 * 2 + scan_temp_bytes(hn_) + 512;
 }
 
-hn_status radix_sort_u64(uint64_t** keys, uint64_t** alt, int64_t n, int key_bits, void* tmp,
-                         hipStream_t st) {
+static hn_status radix_sort_impl(uint64_t** keys, uint64_t** alt, uint32_t** vals, uint32_t** valt, int64_t n,
+                                 int key_bits, void* tmp, hipStream_t st) {
     if (n <= 1) return HN_OK;
     int64_t tiles = ceil_div(n, RS_TILE);
     int64_t hn_ = tiles * RS_BUCKETS;
@@ -130,13 +140,26 @@ hn_status radix_sort_u64(uint64_t** keys, uint64_t** alt, int64_t n, int key_bit
         rs_hist_kernel<<<(unsigned)tiles, BLOCK, 0, st>>>(*keys, n, shift, tiles, hist);
         HN_CHECK_LAUNCH();
         HN_TRY((exclusive_scan<uint32_t>(ArrayIn<uint32_t>{hist}, hoff, hn_, stmp, st)));
-        rs_scatter_kernel<<<(unsigned)tiles, BLOCK, 0, st>>>(*keys, n, shift, tiles, hoff, *alt);
+        if (vals)
+            rs_scatter_kernel<true><<<(unsigned)tiles, BLOCK, 0, st>>>(*keys, n, shift, tiles, hoff, *alt, *vals,
+                                                                       *valt);
+        else
+            rs_scatter_kernel<false><<<(unsigned)tiles, BLOCK, 0, st>>>(*keys, n, shift, tiles, hoff, *alt,
+                                                                        nullptr, nullptr);
         HN_CHECK_LAUNCH();
-        uint64_t* t = *keys;
-        *keys = *alt;
-        *alt = t;
+        std::swap(*keys, *alt);
+        if (vals) std::swap(*vals, *valt);
     }
     return HN_OK;
+}
+
+hn_status radix_sort_u64(uint64_t** keys, uint64_t** alt, int64_t n, int key_bits, void* tmp, hipStream_t st) {
+    return radix_sort_impl(keys, alt, nullptr, nullptr, n, key_bits, tmp, st);
+}
+
+hn_status radix_sort_u64_kv(uint64_t** keys, uint64_t** alt, uint32_t** vals, uint32_t** valt, int64_t n,
+                            int key_bits, void* tmp, hipStream_t st) {
+    return radix_sort_impl(keys, alt, vals, valt, n, key_bits, tmp, st);
 }
 
 }  // namespace hn

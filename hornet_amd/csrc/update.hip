@@ -53,10 +53,13 @@ struct UpdScal {
     unsigned long long changed;   // entries inserted / erased
 };
 
+// idx (non-null with edge meta-data) records each entry's batch position
 __global__ void pack_batch_kernel(int64_t n, const int32_t* __restrict__ src, const int32_t* __restrict__ dst,
-                                  int32_t nv, int sbits, uint64_t* __restrict__ keys, UpdScal* sc) {
+                                  int32_t nv, int sbits, uint64_t* __restrict__ keys, uint32_t* __restrict__ idx,
+                                  UpdScal* sc) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (idx) idx[i] = (uint32_t)i;
     int32_t s = src[i], d = dst[i];
     if (s < 0 || s >= nv || d < 0 || d >= nv) {
         atomicOr(&sc->err, ERR_BAD_ID);
@@ -115,9 +118,13 @@ struct FirstKeepIn {
 };
 
 __global__ void compact_keys_kernel(int64_t n, const uint64_t* __restrict__ in, const uint32_t* __restrict__ pos,
-                                    uint64_t* __restrict__ out) {
+                                    uint64_t* __restrict__ out, const uint32_t* __restrict__ iin,
+                                    uint32_t* __restrict__ iout) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && pos[i + 1] != pos[i]) out[pos[i]] = in[i];
+    if (i < n && pos[i + 1] != pos[i]) {
+        out[pos[i]] = in[i];
+        if (iin) iout[pos[i]] = iin[i];
+    }
 }
 
 // segment starts over the first `*cnt` keys: key i starts a segment when its
@@ -277,11 +284,13 @@ struct CopyShared {
     int64_t dst[LbShared<UPD_IPT, int64_t>::CAP];
 };
 
+// Edge meta-data (am: parallel to the arena, sm: to the staging buffer) move
+// with their entries.
 __global__ __launch_bounds__(BLOCK) void copy_runs_kernel(int64_t nruns, const int64_t* __restrict__ P,
                                                           const int64_t* __restrict__ src,
                                                           const int64_t* __restrict__ dst,
                                                           int32_t* __restrict__ arena, int32_t* __restrict__ stage,
-                                                          const UpdScal* sc) {
+                                                          const UpdScal* sc, MetaSet am, MetaSet sm) {
     __shared__ CopyShared sh;
     if (sc->err) return;
     const int64_t W = P[nruns];
@@ -320,6 +329,14 @@ __global__ __launch_bounds__(BLOCK) void copy_runs_kernel(int64_t nruns, const i
             int32_t* b = (di[j] & GAP_STAGE) ? stage : arena;
             b[di[j] & ~GAP_STAGE] = v[j];
         }
+        if (am.any()) {
+#pragma unroll
+            for (int j = 0; j < UPD_IPT; j++) {
+                if (di[j] < 0) continue;
+                meta_put((di[j] & GAP_STAGE) ? sm : am, di[j] & ~GAP_STAGE, (si[j] & GAP_STAGE) ? sm : am,
+                         si[j] & ~GAP_STAGE);
+            }
+        }
         __syncthreads();
     }
 }
@@ -342,7 +359,9 @@ __global__ __launch_bounds__(BLOCK) void insert_merge_kernel(int64_t nseg, const
                                                              const uint32_t* __restrict__ seg_start,
                                                              const uint64_t* __restrict__ keys, KeyFmt f,
                                                              int32_t* __restrict__ arena,
-                                                             const int32_t* __restrict__ stage, const UpdScal* sc) {
+                                                             const int32_t* __restrict__ stage, const UpdScal* sc,
+                                                             MetaSet am, MetaSet sm, MetaSet bm,
+                                                             const uint32_t* __restrict__ bidx) {
     __shared__ LbShared<UPD_IPT, int64_t> sh;
     if (sc->err) return;
     const int64_t W = P[nseg];
@@ -377,6 +396,16 @@ __global__ __launch_bounds__(BLOCK) void insert_merge_kernel(int64_t nseg, const
                     dest = q + lo;
                 }
                 arena[t.new_off + dest] = val;
+                if (am.any()) {
+                    // the entry's meta-data: an old entry from where A read it, a
+                    // batch entry from its batch position
+                    if (r < na) {
+                        if (r >= t.p && t.inplace) meta_put(am, t.new_off + dest, sm, t.soff + (r - t.p));
+                        else meta_put(am, t.new_off + dest, am, t.old_off + r);
+                    } else {
+                        meta_put(am, t.new_off + dest, bm, bidx[seg_start[j] + (r - na)]);
+                    }
+                }
             });
     }
 }
@@ -556,6 +585,9 @@ struct UpdBufs {
     int64_t* gP;         // 2n+1
     UpdScal* sc;
     void* tmp;
+    uint32_t* ia;        // n+1: batch positions (edge meta-data only)
+    uint32_t* ib;        // n+1
+    MetaSet sm;          // staging area of the edge meta-data
 };
 
 static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
@@ -583,6 +615,9 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     int64_t o_gd = o; o = al(o + 2 * n1 * 8);
     int64_t o_gp = o; o = al(o + (2 * n1 + 1) * 8);
     int64_t o_sc = o; o = al(o + 64);
+    const bool meta = g->meta.any();
+    int64_t o_ia = o; o = al(o + (meta ? n1 * 4 : 0));
+    int64_t o_ib = o; o = al(o + (meta ? n1 * 4 : 0));
     int64_t o_tmp = o; o = al(o + (int64_t)std::max(sort_temp_bytes(n1), scan_temp_bytes(2 * n1)));
     HN_TRY(g->s_upd.reserve((size_t)o));
     char* base = g->s_upd.as<char>();
@@ -608,6 +643,9 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     b->gP = reinterpret_cast<int64_t*>(base + o_gp);
     b->sc = reinterpret_cast<UpdScal*>(base + o_sc);
     b->tmp = base + o_tmp;
+    b->ia = meta ? reinterpret_cast<uint32_t*>(base + o_ia) : nullptr;
+    b->ib = meta ? reinterpret_cast<uint32_t*>(base + o_ib) : nullptr;
+    b->sm = meta_like(g->meta, nullptr);
     return HN_OK;
 }
 
@@ -625,7 +663,7 @@ static hn_status load_and_sort(hn_graph* g, const int32_t* src, const int32_t* d
         dd = staged + n;
     }
     HN_HIP(hipMemsetAsync(b.sc, 0, sizeof(UpdScal), st));
-    pack_batch_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(n, ds, dd, g->nv, f.sbits, b.ka, b.sc);
+    pack_batch_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(n, ds, dd, g->nv, f.sbits, b.ka, b.ia, b.sc);
     HN_CHECK_LAUNCH();
     if (staged) {
         HN_HIP(hipStreamSynchronize(st));
@@ -633,10 +671,14 @@ static hn_status load_and_sort(hn_graph* g, const int32_t* src, const int32_t* d
     }
     uint64_t* keys = b.ka;
     uint64_t* alt = b.kb;
-    HN_TRY(radix_sort_u64(&keys, &alt, n, 2 * f.sbits, b.tmp, st));
+    uint32_t* vals = b.ia;
+    uint32_t* valt = b.ib;
+    if (vals) HN_TRY(radix_sort_u64_kv(&keys, &alt, &vals, &valt, n, 2 * f.sbits, b.tmp, st));
+    else HN_TRY(radix_sort_u64(&keys, &alt, n, 2 * f.sbits, b.tmp, st));
     if (keys != b.ka) {
-        // keep the sorted keys in ka so kb is free for the compaction
+        // keep the sorted keys (and positions) in ka / ia so kb / ib are free for the compaction
         HN_HIP(hipMemcpyAsync(b.ka, keys, sizeof(uint64_t) * (size_t)n, hipMemcpyDeviceToDevice, st));
+        if (vals) HN_HIP(hipMemcpyAsync(b.ia, vals, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, st));
     }
     *sorted = b.ka;
     return HN_OK;
@@ -695,9 +737,21 @@ static hn_status stage_suffixes(hn_graph* g, UpdBufs& b, int64_t n, const uint32
     HN_TRY(check_scal(g, b, st, &h));
     HN_TRY(g->s_misc.reserve(sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
     int32_t* stage = g->s_misc.as<int32_t>();
+    if (g->meta.any()) {
+        // one staging array per meta-data field, 256-byte aligned, in s_mstage
+        size_t bytes = 0;
+        for (int f = 0; f < g->meta.n; f++) bytes += ((size_t)(total > 0 ? total : 1) * g->meta.bytes[f] + 255) & ~size_t(255);
+        HN_TRY(g->s_mstage.reserve(bytes));
+        size_t at = 0;
+        for (int f = 0; f < g->meta.n; f++) {
+            b.sm.p[f] = g->s_mstage.as<char>() + at;
+            at += ((size_t)(total > 0 ? total : 1) * g->meta.bytes[f] + 255) & ~size_t(255);
+        }
+    }
     if (total > 0) {
         const int64_t grid = std::min<int64_t>(ceil_div(total, UPD_TILE), (int64_t)num_cus() * 8);
-        copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.stpos, b.gsrc, b.gdst, g->arena, stage, b.sc);
+        copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.stpos, b.gsrc, b.gdst, g->arena, stage, b.sc,
+                                                           g->meta, b.sm);
         HN_CHECK_LAUNCH();
     }
     *stage_out = stage;
@@ -712,14 +766,45 @@ extern "C" {
 
 hn_status hn_graph_insert(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n, uint32_t flags,
                           int64_t* inserted, void* stream) {
+    return hn_graph_insert_meta(g, src, dst, nullptr, n, flags, inserted, stream);
+}
+
+// device copies of a host batch's meta-data fields, released with the object
+struct BatchMeta {
+    MetaSet m;
+    bool owned = false;
+    ~BatchMeta() {
+        if (owned)
+            for (int f = 0; f < m.n; f++) dfree(m.p[f]);
+    }
+};
+
+hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* dst, const void* const* meta,
+                               int64_t n, uint32_t flags, int64_t* inserted, void* stream) {
     HN_ARG(g, "null graph");
     HN_ARG(n >= 0, "negative batch size");
     if (inserted) *inserted = 0;
+    HN_ARG(meta == nullptr || g->meta.any(), "the graph has no edge meta-data");
     if (n == 0) return HN_OK;
     HN_ARG(src && dst, "null batch");
     HN_ARG(n < (int64_t)0x7FFFFFFF, "batch larger than 2^31-1 entries");
     HN_ARG(g->ne + n < (int64_t)0xFFFFFFFF, "edge count would exceed 2^32-1");
     hipStream_t st = as_stream(stream);
+    BatchMeta bm;
+    bm.m = meta_like(g->meta, nullptr);   // null fields insert zeros
+    if (meta) {
+        for (int f = 0; f < g->meta.n; f++) HN_ARG(meta[f] != nullptr, "a meta-data array is null");
+        if (flags & HN_DEVICE_PTRS) {
+            for (int f = 0; f < g->meta.n; f++) bm.m.p[f] = const_cast<char*>(static_cast<const char*>(meta[f]));
+        } else {
+            bm.owned = true;
+            for (int f = 0; f < g->meta.n; f++) {
+                const size_t bytes = (size_t)n * (size_t)g->meta.bytes[f];
+                HN_TRY(dmalloc(reinterpret_cast<void**>(&bm.m.p[f]), bytes));
+                HN_HIP(hipMemcpyAsync(bm.m.p[f], meta[f], bytes, hipMemcpyHostToDevice, st));
+            }
+        }
+    }
     UpdBufs b;
     HN_TRY(upd_buffers(g, n, &b));
     KeyFmt f{std::max(1, ceil_log2_host((int64_t)g->nv + 1))};
@@ -729,7 +814,7 @@ hn_status hn_graph_insert(hn_graph* g, const int32_t* src, const int32_t* dst, i
     // 1. keep flags (batch / graph dedup) -> compact into kb
     InsertKeepIn kin{keys, n, f, (flags & HN_DEDUP_BATCH) ? 1 : 0, (flags & HN_DEDUP_GRAPH) ? 1 : 0, g->rec, g->arena};
     HN_TRY(exclusive_scan<uint32_t>(kin, b.pos, n, b.tmp, st));
-    compact_keys_kernel<<<gridn, BLOCK, 0, st>>>(n, keys, b.pos, b.kb);
+    compact_keys_kernel<<<gridn, BLOCK, 0, st>>>(n, keys, b.pos, b.kb, b.ia, b.ib);
     HN_CHECK_LAUNCH();
     uint32_t* nb = b.pos + n;   // entries kept
     // 2. segments by source
@@ -749,7 +834,7 @@ hn_status hn_graph_insert(hn_graph* g, const int32_t* src, const int32_t* dst, i
     HN_TRY(exclusive_scan<int64_t>(InsertWorkIn{b.ti, ucnt, b.sc}, b.wpos, n, b.tmp, st));
     const int64_t grid = (int64_t)num_cus() * 8;
     insert_merge_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.wpos, b.ti, b.seg_start, b.kb, f, g->arena, stage,
-                                                          b.sc);
+                                                          b.sc, g->meta, b.sm, bm.m, b.ib);
     HN_CHECK_LAUNCH();
     // 5. publish
     commit_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg_key, f, b.ti, nullptr, g->rec, g->alloc_view(), b.sc);
@@ -787,7 +872,7 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
     // 1. optional in-batch dedup -> kb
     HN_TRY(exclusive_scan<uint32_t>(FirstKeepIn{keys, (flags & HN_DEDUP_BATCH) ? 1 : 0}, b.pos, n, b.tmp, st));
-    compact_keys_kernel<<<gridn, BLOCK, 0, st>>>(n, keys, b.pos, b.kb);
+    compact_keys_kernel<<<gridn, BLOCK, 0, st>>>(n, keys, b.pos, b.kb, nullptr, nullptr);
     HN_CHECK_LAUNCH();
     uint32_t* nb = b.pos + n;
     // 2. distinct keys with multiplicities
@@ -819,7 +904,8 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     HN_CHECK_LAUNCH();
     HN_TRY(exclusive_scan<int64_t>(GapLenIn{b.glen, dcnt, ucnt, n}, b.gP, 2 * n, b.tmp, st));
     const int64_t grid = (int64_t)num_cus() * 8;
-    copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.gsrc, b.gdst, g->arena, stage, b.sc);
+    copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.gsrc, b.gdst, g->arena, stage, b.sc, g->meta,
+                                                       b.sm);
     HN_CHECK_LAUNCH();
     // 7. publish
     commit_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg2_key, f, b.ti, b.skip, g->rec, g->alloc_view(), b.sc);

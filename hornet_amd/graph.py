@@ -52,16 +52,44 @@ def _as_device_i32(x) -> torch.Tensor:
     return t.to(device=_dev(), dtype=torch.int32).contiguous()
 
 
+def _meta_arrays(meta, n, where):
+    """Edge meta-data fields as contiguous 4- or 8-byte numpy arrays of n elements."""
+    out = []
+    for m in meta or ():
+        a = np.ascontiguousarray(m)
+        if a.dtype.itemsize not in (4, 8) or a.ndim != 1 or a.size != n:
+            raise HornetError(_lib.HN_E_INVALID_ARG, where,
+                              "meta-data fields are 1-d arrays of 4- or 8-byte elements, one per edge")
+        out.append(a)
+    if len(out) > _lib.HN_MAX_EDGE_META:
+        raise HornetError(_lib.HN_E_INVALID_ARG, where, "more than HN_MAX_EDGE_META meta-data fields")
+    return out
+
+
+def _ptr_array(ptrs):
+    arr = (ctypes.c_void_p * _lib.HN_MAX_EDGE_META)()
+    for i, p in enumerate(ptrs):
+        arr[i] = p
+    return arr
+
+
 class HornetInit:
-    """Host CSR (offsets[nV+1], edges[nE]) used to build a Hornet."""
+    """Host CSR (offsets[nV+1], edges[nE]) used to build a Hornet, with
+    optional edge meta-data (HornetInit::insertEdgeData, HornetInit.cuh:75-97)."""
 
     def __init__(self, nV: int, nE: int, offsets, edges):
         self.nV = int(nV)
         self.nE = int(nE)
         self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         self.edges = np.ascontiguousarray(edges, dtype=np.int32)
+        self.meta = []
         if self.offsets.size != self.nV + 1 or self.edges.size != self.nE:
             raise HornetError(_lib.HN_E_INVALID_ARG, "HornetInit", "size mismatch")
+
+    def insertEdgeData(self, *fields):
+        """One array per meta-data field, nE elements each (4 or 8 bytes)."""
+        self.meta = _meta_arrays(fields, self.nE, "HornetInit.insertEdgeData")
+        return self
 
 
 class structure_prop:
@@ -209,7 +237,7 @@ class BatchUpdate:
     """A batch copied to the device (BatchUpdate::reset copies it,
     hornet/include/Core/BatchUpdate/BatchUpdate.i.cuh:154-184)."""
 
-    def __init__(self, ptr_or_src, dst=None):
+    def __init__(self, ptr_or_src, dst=None, meta=None):
         if isinstance(ptr_or_src, BatchUpdatePtr):
             src, dst, n = ptr_or_src.src, ptr_or_src.dst, ptr_or_src.n
         else:
@@ -218,6 +246,9 @@ class BatchUpdate:
         self.dst = _as_device_i32(dst)[:n]
         if self.src.numel() != self.dst.numel():
             raise HornetError(_lib.HN_E_INVALID_ARG, "BatchUpdate", "src/dst length mismatch")
+        # edge meta-data fields (one per graph field), copied to the device
+        self.meta = [torch.from_numpy(a.copy()).to(_dev())
+                     for a in _meta_arrays(meta, self.nE(), "BatchUpdate")] if meta else []
 
     def nE(self) -> int:
         return int(self.src.numel())
@@ -233,9 +264,14 @@ class Hornet:
         if _handle is not None:
             self._h = _handle
         elif init is not None:
-            call("hn_graph_create_csr", init.nV, init.nE, init.offsets.ctypes.data_as(ctypes.c_void_p),
-                 init.edges.ctypes.data_as(ctypes.c_void_p), 0, float(slack), _stream(),
-                 ctypes.byref(self._h))
+            layout = _lib.HnEdgeMeta()
+            layout.nfields = len(init.meta)
+            for i, a in enumerate(init.meta):
+                layout.bytes[i] = a.dtype.itemsize
+            ptrs = _ptr_array([a.ctypes.data for a in init.meta])
+            call("hn_graph_create_csr_meta", init.nV, init.nE, init.offsets.ctypes.data_as(ctypes.c_void_p),
+                 init.edges.ctypes.data_as(ctypes.c_void_p), ctypes.byref(layout), ptrs, 0, float(slack),
+                 _stream(), ctypes.byref(self._h))
         else:
             raise HornetError(_lib.HN_E_INVALID_ARG, "Hornet", "need a HornetInit")
 
@@ -309,14 +345,28 @@ class Hornet:
         call("hn_graph_degrees", self._h, _ptr(out), _stream())
         return out[: self.nV()]
 
-    def getCOO(self, sort: bool = True):
-        """(src, dst) device tensors sorted by (src, dst) (adjacency is kept sorted)."""
+    def edge_meta_bytes(self) -> list:
+        """Element size of each edge meta-data field ([] without meta-data)."""
+        layout = _lib.HnEdgeMeta()
+        call("hn_graph_edge_meta", self._h, ctypes.byref(layout))
+        return [layout.bytes[i] for i in range(layout.nfields)]
+
+    def getCOO(self, sort: bool = True, meta: bool = False):
+        """(src, dst) device tensors sorted by (src, dst) (adjacency is kept sorted);
+        with meta=True also a list of the meta-data fields (int32 / int64 tensors
+        holding the fields' bits)."""
         ne = self.nE()
         src = torch.empty(max(ne, 1), dtype=torch.int32, device=_dev())
         dst = torch.empty(max(ne, 1), dtype=torch.int32, device=_dev())
         n = ctypes.c_int64()
-        call("hn_graph_export_coo", self._h, _ptr(src), _ptr(dst), ne, ctypes.byref(n), _stream())
-        return src[:ne], dst[:ne]
+        if not meta:
+            call("hn_graph_export_coo", self._h, _ptr(src), _ptr(dst), ne, ctypes.byref(n), _stream())
+            return src[:ne], dst[:ne]
+        fields = [torch.empty(max(ne, 1), dtype=torch.int64 if b == 8 else torch.int32, device=_dev())
+                  for b in self.edge_meta_bytes()]
+        ptrs = _ptr_array([t.data_ptr() for t in fields])
+        call("hn_graph_export_coo_meta", self._h, _ptr(src), _ptr(dst), ptrs, ne, ctypes.byref(n), _stream())
+        return src[:ne], dst[:ne], [t[:ne] for t in fields]
 
     def getCSR(self):
         nv, ne = self.nV(), self.nE()
@@ -341,8 +391,13 @@ class Hornet:
         if removeGraphDuplicates:
             flags |= _lib.HN_DEDUP_GRAPH
         n = ctypes.c_int64()
-        call("hn_graph_insert", self._h, _ptr(batch.src), _ptr(batch.dst), batch.nE(), flags,
-             ctypes.byref(n), _stream())
+        if batch.meta:
+            ptrs = _ptr_array([t.data_ptr() for t in batch.meta])
+            call("hn_graph_insert_meta", self._h, _ptr(batch.src), _ptr(batch.dst), ptrs, batch.nE(), flags,
+                 ctypes.byref(n), _stream())
+        else:
+            call("hn_graph_insert", self._h, _ptr(batch.src), _ptr(batch.dst), batch.nE(), flags,
+                 ctypes.byref(n), _stream())
         return n.value
 
     def erase(self, batch: BatchUpdate, removeBatchDuplicates: bool = False) -> int:

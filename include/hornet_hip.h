@@ -64,15 +64,26 @@ typedef struct hn_vrec {
     int32_t cap_log2;
 } hn_vrec;
 
+/* Edge meta-data (TypeList<EdgeMetaTypes...>, hornet/include/Core/HornetInitialize/HornetInit.cuh:54-97):
+ * up to HN_MAX_EDGE_META fields per edge, each 4 or 8 bytes, kept as arrays
+ * parallel to the edge arena (field f of the edge in slot i is meta[f][i]). */
+#define HN_MAX_EDGE_META 4
+typedef struct hn_edge_meta {
+    int32_t nfields;
+    int32_t bytes[HN_MAX_EDGE_META];   /* 4 or 8 */
+} hn_edge_meta;
+
 /* POD device view passed by value into kernels (HornetDevice,
  * hornet/include/Core/HornetDevice/HornetDevice.cuh:52-90).  Valid until the
  * next mutating call on the graph. */
 typedef struct hn_dview {
     int32_t nv;
-    int32_t pad_;
+    int32_t nmeta;                      /* edge meta-data fields (0 = none) */
     int64_t ne;
     hn_vrec* rec;
     int32_t* arena;
+    void* meta[HN_MAX_EDGE_META];       /* field f of arena slot i: meta[f][i] */
+    int32_t meta_bytes[HN_MAX_EDGE_META];
 } hn_dview;
 
 /* BFS statistics for one traversal. */
@@ -114,6 +125,16 @@ hn_status hn_device_info(int32_t* num_cus, int64_t* lds_per_cu, int64_t* hbm_byt
 hn_status hn_graph_create_csr(int32_t nv, int64_t ne, const void* offsets,
                               const int32_t* edges, uint32_t flags, double arena_slack,
                               void* stream, hn_graph** out);
+/* Hornet(HornetInit&) after HornetInit::insertEdgeData(meta...)
+ * (HornetInit.cuh:75-97): as hn_graph_create_csr, with `layout->nfields`
+ * meta-data arrays meta[f] of ne elements of layout->bytes[f] bytes, in the
+ * memory space the flags give the edges; they follow their edges through
+ * every update.  layout NULL or nfields 0 = hn_graph_create_csr. */
+hn_status hn_graph_create_csr_meta(int32_t nv, int64_t ne, const void* offsets, const int32_t* edges,
+                                   const hn_edge_meta* layout, const void* const* meta, uint32_t flags,
+                                   double arena_slack, void* stream, hn_graph** out);
+/* The graph's edge meta-data layout (nfields 0 for a graph without). */
+hn_status hn_graph_edge_meta(const hn_graph* g, hn_edge_meta* layout);
 /* Graph from a device COO (src,dst) of m entries: optional symmetrize /
  * self-loop drop (GraphStd::COOtoCSR, xlib/src/Graph/GraphStd.cpp:173-341),
  * always sorted and deduplicated.  Device pointers only. */
@@ -134,6 +155,11 @@ hn_status hn_graph_memory(hn_graph* g, int64_t* used_slots, int64_t* capacity_sl
  * of capacity `cap`; *n_out = ne.  Synchronises. */
 hn_status hn_graph_export_coo(hn_graph* g, int32_t* src, int32_t* dst, int64_t cap,
                               int64_t* n_out, void* stream);
+/* getCOO with the edge meta-data (COO<DEVICE, vid_t, TypeList<EdgeMetaTypes...>>,
+ * HornetQuery.i.cuh:111-151): as hn_graph_export_coo, plus meta[f] (device,
+ * cap elements of the field's size) for each field; meta may be NULL. */
+hn_status hn_graph_export_coo_meta(hn_graph* g, int32_t* src, int32_t* dst, void* const* meta, int64_t cap,
+                                   int64_t* n_out, void* stream);
 /* CSR export (the working replacement for the reference's broken getCSR,
  * hornet/include/Core/Static/Static.i.cuh:100-103): offsets[nv+1] int64,
  * adj[cap] int32, device buffers.  Synchronises. */
@@ -160,9 +186,21 @@ hn_status hn_graph_max_degree(hn_graph* g, int32_t* id, int32_t* deg, void* stre
  * be NULL) receives the number of entries added.  Synchronises. */
 hn_status hn_graph_insert(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n,
                           uint32_t flags, int64_t* inserted, void* stream);
+/* Hornet::insert of a BatchUpdate carrying edge meta-data (BatchUpdate.cuh:95-99,
+ * sorted with the batch as in SoAPtr.i.cuh:581-613): meta[f] holds n elements
+ * of the graph's field f, in the memory space of src / dst.  Equal (src, dst)
+ * entries keep their batch order, so HN_DEDUP_BATCH keeps the first one's
+ * meta-data (BatchUpdate.i.cuh:250-335); HN_DEDUP_GRAPH drops the first batch
+ * copy of a stored pair, whose stored meta-data stay.  Within a row, copies
+ * of one destination are kept in insertion order.  meta NULL inserts zeros.  hn_graph_insert on a graph with
+ * meta-data inserts zeros too. */
+hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* dst, const void* const* meta,
+                               int64_t n, uint32_t flags, int64_t* inserted, void* stream);
 /* Hornet::erase(batch, removeBatchDuplicates) -- HornetInsert.i.cuh:71-89
  * with BatchUpdate::preprocess_erase, BatchUpdate.i.cuh:504-524: multiset
- * difference, absent edges ignored.  Adjacency order is preserved.
+ * difference, absent edges ignored.  Adjacency order is preserved; of c stored
+ * copies of one (u, v) an erase of k removes the min(k, c) oldest (this
+ * decides which meta-data go).
  * *erased (may be NULL) receives the number of entries removed.
  * Synchronises. */
 hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n,

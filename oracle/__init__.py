@@ -35,9 +35,16 @@ Pinning status (what independently checks each function, see
 * ``vertex_permutation`` -- specified by this build (the reference's RANDOMIZE
   is clock-seeded): parity unpinned against the reference; pinned as a
   permutation whose inverse restores the input graph.
+* ``csr_meta`` / ``insert_meta`` / ``erase_meta`` -- a pure-Python list model
+  of the same rules on random small multigraphs, the edge multisets of
+  ``insert`` / ``erase`` (fields dropped), and hand cases for first-kept
+  batch duplicates (``tests/test_oracle.py``).  Which stored copy of a
+  multi-edge an erase removes is the library's choice (oldest first): parity
+  unpinned against the reference, whose choice is unspecified.
 """
 from __future__ import annotations
 
+import collections
 import ctypes
 import os
 import subprocess
@@ -427,3 +434,78 @@ def read_binary(path: str):
     if p != len(raw):
         raise ValueError("file size does not match its header")
     return {"n": nv, "structure": state, "off": off, "adj": adj, "in_off": in_off, "in_adj": in_adj}
+
+
+# --------------------------------------------------------------------------
+# edge meta-data: rows sorted by destination, equal destinations in
+# insertion order, fields carried with their edges
+# --------------------------------------------------------------------------
+def _meta_rows(n, s, d, order, fields):
+    """CSR of entries (s, d) ordered by (s, d, order), fields permuted along."""
+    p = np.lexsort((order, d, s))
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, np.asarray(s, np.int64) + 1, 1)
+    return np.cumsum(off), np.asarray(d, np.int32)[p], [np.asarray(f)[p] for f in fields]
+
+
+def csr_meta(off, adj, fields):
+    """Hornet(HornetInit) with insertEdgeData (HornetInit.cuh:75-97): each row
+    sorted by destination, equal destinations in input order."""
+    n = off.size - 1
+    s = np.repeat(np.arange(n), np.diff(off))
+    return _meta_rows(n, s, adj, np.arange(adj.size), fields)
+
+
+def insert_meta(off, adj, fields, bs, bd, bfields, dedup_batch=False, dedup_graph=False):
+    """Hornet::insert of a batch with meta-data: the batch is stably sorted by
+    (src, dst) with its fields (SoAPtr.i.cuh:581-613); removeBatchDuplicates
+    keeps the first of equal pairs (unique_copy, BatchUpdate.i.cuh:250-335);
+    removeGraphDuplicates drops the first batch copy of each pair already
+    stored (as orc_insert, after BatchUpdateKernels.cuh:27-58; the stored
+    fields stay).  New copies of a destination follow the stored ones.
+    Returns (off, adj, fields, added)."""
+    n = off.size - 1
+    bs = np.asarray(bs, np.int64)
+    bd = np.asarray(bd, np.int64)
+    keep = np.ones(bs.size, bool)
+    p = np.lexsort((np.arange(bs.size), bd, bs))
+    first = np.ones(bs.size, bool)
+    first[1:] = (bs[p][1:] != bs[p][:-1]) | (bd[p][1:] != bd[p][:-1])
+    if dedup_batch:
+        keep[:] = False
+        keep[p[first]] = True
+    if dedup_graph:
+        s0 = np.repeat(np.arange(n), np.diff(off)).astype(np.int64)
+        stored = set(zip(s0.tolist(), np.asarray(adj).tolist()))
+        for i in p[first]:
+            if (int(bs[i]), int(bd[i])) in stored:
+                keep[i] = False
+    s0 = np.repeat(np.arange(n), np.diff(off)).astype(np.int64)
+    s = np.concatenate([s0, bs[keep]])
+    d = np.concatenate([np.asarray(adj, np.int64), bd[keep]])
+    order = np.concatenate([np.arange(adj.size), adj.size + np.flatnonzero(keep)])
+    fl = [np.concatenate([np.asarray(f), np.asarray(bf)[keep]]) for f, bf in zip(fields, bfields)]
+    o, a, f = _meta_rows(n, s, d, order, fl)
+    return o, a, f, int(keep.sum())
+
+
+def erase_meta(off, adj, fields, bs, bd, dedup_batch=False):
+    """Hornet::erase on a graph with meta-data: a distinct (u, v) of batch
+    multiplicity c removes the min(c, stored) oldest stored copies (the
+    library's choice; the reference's copy choice is unspecified); fields of
+    the survivors stay with them.  Returns (off, adj, fields, removed)."""
+    n = off.size - 1
+    cnt = collections.Counter(zip(np.asarray(bs).tolist(), np.asarray(bd).tolist()))
+    if dedup_batch:
+        cnt = collections.Counter({k: 1 for k in cnt})
+    s0 = np.repeat(np.arange(n), np.diff(off))
+    keep = np.ones(adj.size, bool)
+    for j in range(adj.size):          # rows hold equal destinations oldest first
+        k = (int(s0[j]), int(adj[j]))
+        if cnt.get(k, 0) > 0:
+            cnt[k] -= 1
+            keep[j] = False
+    o = np.zeros(n + 1, np.int64)
+    np.add.at(o, s0[keep] + 1, 1)
+    return (np.cumsum(o), np.asarray(adj)[keep], [np.asarray(f)[keep] for f in fields],
+            int((~keep).sum()))

@@ -287,3 +287,84 @@ def test_two_hop_vs_networkx(rmat10):
     G = to_nx(off, adj)
     for v in [0, 5, O.max_degree_id(off)]:
         assert O.two_hop(off, adj, v) == len(nx.ego_graph(G, v, radius=2))
+
+
+# ------------------------------------------------------------ edge meta-data
+def _list_model_insert(rows, bs, bd, bw, dedup_batch, dedup_graph):
+    """rows: dict u -> list of (v, w) in row order; pure-Python restatement."""
+    rows = {u: list(r) for u, r in rows.items()}
+    seen = set()
+    stored = {(u, v) for u, r in rows.items() for v, _ in r}
+    batch = sorted(range(len(bs)), key=lambda i: (bs[i], bd[i], i))
+    added = 0
+    for i in batch:
+        k = (int(bs[i]), int(bd[i]))
+        first = k not in seen
+        seen.add(k)
+        if dedup_batch and not first:
+            continue
+        if dedup_graph and first and k in stored:   # the first batch copy of a stored pair
+            continue
+        r = rows.setdefault(k[0], [])
+        pos = len(r)
+        while pos > 0 and r[pos - 1][0] > k[1]:
+            pos -= 1
+        r.insert(pos, (k[1], int(bw[i])))
+        added += 1
+    return rows, added
+
+
+def _rows(off, adj, w):
+    return {u: [(int(adj[j]), int(w[j])) for j in range(off[u], off[u + 1])] for u in range(off.size - 1)}
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_meta_oracle_matches_list_model(seed):
+    rng = np.random.default_rng(seed)
+    n = 12
+    s = rng.integers(0, n, 60)
+    d = rng.integers(0, n, 60)
+    w = rng.integers(0, 1000, 60)
+    o0, a0, p0 = _coo_csr(n, s, d)
+    off, adj, (mw,) = O.csr_meta(o0, a0, [w[p0]])
+    rows = {u: [] for u in range(n)}
+    for i in sorted(range(60), key=lambda i: (s[i], d[i], i)):
+        rows[int(s[i])].append((int(d[i]), int(w[i])))
+    assert _rows(off, adj, mw) == rows
+    bs, bd, bw = rng.integers(0, n, 40), rng.integers(0, n, 40), rng.integers(0, 1000, 40)
+    for db in (False, True):
+        for dg in (False, True):
+            o1, a1, (w1,), added = O.insert_meta(off, adj, [mw], bs, bd, [bw], db, dg)
+            exp, eadded = _list_model_insert(rows, bs, bd, bw, db, dg)
+            assert added == eadded and _rows(o1, a1, w1) == exp
+            # the edge multiset agrees with the pinned insert model (fields dropped)
+            oo, aa, _ = O.insert(off, adj, bs.astype(np.int32), bd.astype(np.int32), db, dg)
+            assert np.array_equal(oo, o1) and np.array_equal(aa, a1)
+    es, ed = bs[:25], bd[:25]
+    o2, a2, (w2,), removed = O.erase_meta(off, adj, [mw], es, ed)
+    oo, aa, rem = O.erase(off, adj, es.astype(np.int32), ed.astype(np.int32))
+    assert removed == rem and np.array_equal(oo, o2) and np.array_equal(aa, a2)
+    # oldest copies go first: the survivors of each (u, v) are its newest copies
+    for u in range(n):
+        for v in set(adj[off[u]:off[u + 1]].tolist()):
+            before = [x for (y, x) in rows[u] if y == v]
+            after = [int(w2[j]) for j in range(o2[u], o2[u + 1]) if a2[j] == v]
+            assert after == before[len(before) - len(after):]
+
+
+def _coo_csr(n, s, d):
+    p = np.lexsort((np.arange(s.size), d, s))
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, s + 1, 1)
+    return np.cumsum(off), d[p].astype(np.int32), p
+
+
+def test_meta_batch_duplicates_keep_first():
+    off = np.array([0, 0, 0, 0], np.int64)
+    adj = np.array([], np.int32)
+    o, a, (w,), added = O.insert_meta(off, adj, [np.array([], np.int64)], [1, 1, 0, 1], [2, 2, 1, 2],
+                                      [np.array([7, 8, 9, 10])], dedup_batch=True)
+    assert added == 2 and a.tolist() == [1, 2] and w.tolist() == [9, 7]
+    o, a, (w,), added = O.insert_meta(off, adj, [np.array([], np.int64)], [1, 1], [2, 2],
+                                      [np.array([7, 8])])
+    assert a.tolist() == [2, 2] and w.tolist() == [7, 8]
