@@ -28,7 +28,9 @@ public:
     using DegreeType = degree_t;
 
     HornetDevice() = default;
-    __host__ __device__ explicit HornetDevice(const hn_dview& v) : _nv(v.nv), _ne(v.ne), _rec(v.rec), _arena(v.arena) {}
+    __host__ __device__ explicit HornetDevice(const hn_dview& v) : _nv(v.nv), _ne(v.ne), _rec(v.rec), _arena(v.arena) {
+        for (int f = 0; f < HN_MAX_EDGE_META; f++) _meta[f] = v.meta[f];
+    }
 
     __host__ __device__ __forceinline__ vid_t nV() const { return _nv; }
     __host__ __device__ __forceinline__ int64_t nE() const { return _ne; }
@@ -36,12 +38,15 @@ public:
     __device__ __forceinline__ degree_t degree(vid_t id) const { return _rec[id].deg; }
     __host__ __device__ __forceinline__ const hn_vrec* records() const { return _rec; }
     __host__ __device__ __forceinline__ vid_t* arena() const { return _arena; }
+    // edge meta-data field f, parallel to the arena
+    __host__ __device__ __forceinline__ void* edge_meta(int f) const { return _meta[f]; }
 
 private:
     vid_t _nv = 0;
     int64_t _ne = 0;
     const hn_vrec* _rec = nullptr;
     vid_t* _arena = nullptr;
+    void* _meta[HN_MAX_EDGE_META] = {};
 };
 
 class Vertex {
@@ -70,6 +75,12 @@ public:
     __device__ __forceinline__ vid_t dst_id() const { return *_ptr; }
     __device__ __forceinline__ Vertex src() const { return Vertex(_h, _src); }
     __device__ __forceinline__ Vertex dst() const { return Vertex(_h, *_ptr); }
+    // edge meta-data field N (Edge::field<N>(), hornet/include/Core/HornetDevice/Edge.cuh:105-110);
+    // T is the field's type (int by default)
+    template <unsigned N, typename T = int>
+    __device__ __forceinline__ T& field() const {
+        return reinterpret_cast<T*>(_h.edge_meta((int)N))[_ptr - _h.arena()];
+    }
 
 private:
     HornetDevice _h;

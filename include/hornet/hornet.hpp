@@ -8,8 +8,10 @@
 //   hornet::BatchUpdatePtr / gpu::BatchUpdate
 //                                    hornet/include/Core/BatchUpdate/BatchUpdate.cuh:77-272
 //   hornet::COO                      hornet/include/Core/Static/Static.cuh:19-130
-// Vertex / edge meta types are not supported (vid_t = degree_t = int, EMPTY
-// meta); the extra template parameters are accepted and must be EMPTY.
+// vid_t = degree_t = int; vertex meta types are not supported (EMPTY); edge
+// meta types are a TypeList of up to four 4- or 8-byte fields
+// (hornet/include/Core/Conf/HornetConf.cuh), read on the device with
+// edge.template field<N, T>() (T defaults to int).
 // Errors: the reference prints and exits (xlib/src/Device/Util/SafeCudaAPI.cpp:71-94);
 // here a failed call throws hornet::Error carrying the hn_status.
 #pragma once
@@ -44,62 +46,140 @@ inline void check(hn_status s, const char* where) {
     if (s != HN_OK) throw Error(s, where);
 }
 
-// Host CSR handed to the Hornet constructor (pointers are borrowed)
+// Edge meta-data types (hornet/include/Core/Conf/HornetConf.cuh TypeList):
+// up to HN_MAX_EDGE_META fields of 4 or 8 bytes, stored beside the edges
+template <typename... Ts>
+struct TypeList {};
+
+namespace detail {
+template <typename L>
+struct MetaTraits;
+template <>
+struct MetaTraits<EMPTY> {
+    static constexpr int n = 0;
+    static void layout(hn_edge_meta& m) { m = hn_edge_meta{}; }
+};
+template <typename... Ts>
+struct MetaTraits<TypeList<Ts...>> {
+    static constexpr int n = (int)sizeof...(Ts);
+    static_assert(n <= HN_MAX_EDGE_META, "at most HN_MAX_EDGE_META edge meta-data fields");
+    static_assert(((sizeof(Ts) == 4 || sizeof(Ts) == 8) && ...), "edge meta-data fields are 4 or 8 bytes");
+    static void layout(hn_edge_meta& m) {
+        m = hn_edge_meta{};
+        m.nfields = n;
+        int i = 0;
+        ((m.bytes[i++] = (int32_t)sizeof(Ts)), ...);
+    }
+};
+template <unsigned N, typename L>
+struct FieldAt;
+template <unsigned N, typename T, typename... Ts>
+struct FieldAt<N, TypeList<T, Ts...>> : FieldAt<N - 1, TypeList<Ts...>> {};
+template <typename T, typename... Ts>
+struct FieldAt<0, TypeList<T, Ts...>> {
+    using type = T;
+};
+}  // namespace detail
+
+// Host CSR handed to the Hornet constructor (pointers are borrowed), with the
+// edge meta-data of HornetInit::insertEdgeData (HornetInit.cuh:75-97)
 template <typename VID = vid_t, typename VMeta = EMPTY, typename EMeta = EMPTY, typename DEG = degree_t>
 class HornetInit {
     static_assert(std::is_same<VID, int>::value && std::is_same<DEG, int>::value, "int ids and degrees");
-    static_assert(std::is_same<VMeta, EMPTY>::value && std::is_same<EMeta, EMPTY>::value, "no meta data");
+    static_assert(std::is_same<VMeta, EMPTY>::value, "no vertex meta data");
 
 public:
+    static constexpr int num_edge_meta = detail::MetaTraits<EMeta>::n;
+
     HornetInit(VID nV, DEG nE, const DEG* csr_offsets, const VID* csr_edges) noexcept
         : _nV(nV), _nE(nE), _off(csr_offsets), _edges(csr_edges) {}
+    template <typename... Ps>
+    void insertEdgeData(const Ps*... edge_meta_data) noexcept {
+        static_assert(sizeof...(Ps) == (size_t)num_edge_meta, "one array per edge meta-data field");
+        int i = 0;
+        ((_meta[i++] = static_cast<const void*>(edge_meta_data)), ...);
+    }
     VID nV() const noexcept { return _nV; }
     DEG nE() const noexcept { return _nE; }
     const DEG* csr_offsets() const noexcept { return _off; }
     const VID* csr_edges() const noexcept { return _edges; }
+    const void* const* edge_meta() const noexcept { return _meta; }
 
 private:
     VID _nV;
     DEG _nE;
     const DEG* _off;
     const VID* _edges;
+    const void* _meta[HN_MAX_EDGE_META] = {};
 };
 
-// (n, src, dst) batch on the host or the device (BatchUpdate.cuh:95-99)
+// (n, src, dst, meta...) batch on the host or the device (BatchUpdate.cuh:95-99)
 template <typename VID = vid_t, typename EMeta = EMPTY, DeviceType device = DeviceType::DEVICE,
           typename DEG = degree_t>
 class BatchUpdatePtr {
 public:
-    BatchUpdatePtr(DEG n, VID* src, VID* dst) noexcept : _n(n), _src(src), _dst(dst) {}
+    template <typename... Ps>
+    BatchUpdatePtr(DEG n, VID* src, VID* dst, Ps*... edge_data) noexcept : _n(n), _src(src), _dst(dst) {
+        static_assert(sizeof...(Ps) == 0 || sizeof...(Ps) == (size_t)detail::MetaTraits<EMeta>::n,
+                      "one array per edge meta-data field");
+        int i = 0;
+        ((_meta[i++] = static_cast<const void*>(edge_data)), ...);
+    }
     DEG nE() const noexcept { return _n; }
     VID* src() const noexcept { return _src; }
     VID* dst() const noexcept { return _dst; }
+    const void* const* edge_meta() const noexcept { return _meta[0] ? _meta : nullptr; }
     static constexpr bool on_device = device == DeviceType::DEVICE;
 
 private:
     DEG _n;
     VID* _src;
     VID* _dst;
+    const void* _meta[HN_MAX_EDGE_META] = {};
 };
 
-// Flat (src, dst) edge list; device storage owned by the object
+// Flat (src, dst, meta...) edge list; device storage owned by the object
 template <DeviceType device = DeviceType::DEVICE, typename VID = vid_t, typename EMeta = EMPTY,
           typename DEG = degree_t>
 class COO {
+    static constexpr int NM = detail::MetaTraits<EMeta>::n;
+
 public:
     COO() = default;
     explicit COO(int64_t n) : _n(n) {
         if (n > 0) {
             check(hipMalloc(&_src, sizeof(VID) * (size_t)n) == hipSuccess ? HN_OK : HN_E_OOM, "COO");
             check(hipMalloc(&_dst, sizeof(VID) * (size_t)n) == hipSuccess ? HN_OK : HN_E_OOM, "COO");
+            hn_edge_meta m;
+            detail::MetaTraits<EMeta>::layout(m);
+            for (int f = 0; f < NM; f++)
+                check(hipMalloc(&_meta[f], (size_t)m.bytes[f] * (size_t)n) == hipSuccess ? HN_OK : HN_E_OOM, "COO");
         }
     }
     COO(const COO&) = delete;
     COO& operator=(const COO&) = delete;
-    COO(COO&& o) noexcept : _n(o._n), _src(o._src), _dst(o._dst) { o._src = o._dst = nullptr; o._n = 0; }
+    COO(COO&& o) noexcept : _n(o._n), _src(o._src), _dst(o._dst) {
+        for (int f = 0; f < HN_MAX_EDGE_META; f++) _meta[f] = o._meta[f], o._meta[f] = nullptr;
+        o._src = o._dst = nullptr;
+        o._n = 0;
+    }
     ~COO() {
         if (_src) (void)hipFree(_src);
         if (_dst) (void)hipFree(_dst);
+        for (int f = 0; f < HN_MAX_EDGE_META; f++)
+            if (_meta[f]) (void)hipFree(_meta[f]);
+    }
+    // device array of edge meta-data field N
+    template <unsigned N>
+    typename detail::FieldAt<N, EMeta>::type* edgeMetaPtr() const noexcept {
+        return static_cast<typename detail::FieldAt<N, EMeta>::type*>(_meta[N]);
+    }
+    void* const* edge_meta() const noexcept { return NM ? _meta : nullptr; }
+    template <unsigned N>
+    void edgeMetaToHost(std::vector<typename detail::FieldAt<N, EMeta>::type>& out) const {
+        out.resize((size_t)_n);
+        if (_n > 0)
+            (void)hipMemcpy(out.data(), _meta[N], sizeof(out[0]) * (size_t)_n, hipMemcpyDeviceToHost);
     }
     int64_t size() const noexcept { return _n; }
     VID* srcPtr() const noexcept { return _src; }
@@ -118,6 +198,7 @@ private:
     int64_t _n = 0;
     VID* _src = nullptr;
     VID* _dst = nullptr;
+    void* _meta[HN_MAX_EDGE_META] = {};
 };
 
 namespace gpu {
@@ -129,15 +210,18 @@ class BatchUpdate {
 public:
     template <DeviceType D>
     explicit BatchUpdate(const BatchUpdatePtr<VID, EMeta, D, DEG>& p) noexcept
-        : _n(p.nE()), _src(p.src()), _dst(p.dst()), _device(D == DeviceType::DEVICE) {}
+        : _n(p.nE()), _src(p.src()), _dst(p.dst()), _meta(p.edge_meta()), _device(D == DeviceType::DEVICE) {}
     template <DeviceType D>
     explicit BatchUpdate(COO<D, VID, EMeta, DEG>& coo) noexcept
-        : _n((DEG)coo.size()), _src(coo.srcPtr()), _dst(coo.dstPtr()), _device(D == DeviceType::DEVICE) {}
+        : _n((DEG)coo.size()), _src(coo.srcPtr()), _dst(coo.dstPtr()),
+          _meta(const_cast<const void* const*>(coo.edge_meta())), _device(D == DeviceType::DEVICE) {}
 
     DEG nE() const noexcept { return _n; }
     DEG size() const noexcept { return _n; }
     const VID* src() const noexcept { return _src; }
     const VID* dst() const noexcept { return _dst; }
+    // edge meta-data arrays, one per field (nullptr: none given)
+    const void* const* edge_meta() const noexcept { return _meta; }
     uint32_t flags() const noexcept { return _device ? HN_DEVICE_PTRS : 0u; }
     // entries that took effect in the last insert / erase
     int64_t applied() const noexcept { return _applied; }
@@ -147,6 +231,7 @@ private:
     DEG _n;
     const VID* _src;
     const VID* _dst;
+    const void* const* _meta;
     bool _device;
     int64_t _applied = 0;
 };
@@ -156,7 +241,7 @@ private:
 template <typename VID = vid_t, typename VMeta = EMPTY, typename EMeta = EMPTY, typename DEG = degree_t>
 class HornetStatic {
     static_assert(std::is_same<VID, int>::value && std::is_same<DEG, int>::value, "int ids and degrees");
-    static_assert(std::is_same<VMeta, EMPTY>::value && std::is_same<EMeta, EMPTY>::value, "no meta data");
+    static_assert(std::is_same<VMeta, EMPTY>::value, "no vertex meta data");
 
 public:
     using VertexType = VID;
@@ -211,7 +296,9 @@ public:
         int64_t ne = nE64();
         COO<DeviceType::DEVICE, VID, EMeta, DEG> coo(ne);
         int64_t n = 0;
-        if (ne > 0) check(hn_graph_export_coo(_g, coo.srcPtr(), coo.dstPtr(), ne, &n, _stream), "Hornet::getCOO");
+        if (ne > 0)
+            check(hn_graph_export_coo_meta(_g, coo.srcPtr(), coo.dstPtr(), coo.edge_meta(), ne, &n, _stream),
+                  "Hornet::getCOO");
         return coo;
     }
     hn_graph* handle() const noexcept { return _g; }
@@ -219,8 +306,10 @@ public:
 
 protected:
     HornetStatic(HInitT& init, double arena_slack, hipStream_t stream) : _stream(stream) {
-        check(hn_graph_create_csr(init.nV(), init.nE(), init.csr_offsets(), init.csr_edges(), HN_OFFSETS_INT32,
-                                  arena_slack, stream, &_g),
+        hn_edge_meta layout;
+        detail::MetaTraits<EMeta>::layout(layout);
+        check(hn_graph_create_csr_meta(init.nV(), init.nE(), init.csr_offsets(), init.csr_edges(), &layout,
+                                       init.edge_meta(), HN_OFFSETS_INT32, arena_slack, stream, &_g),
               "Hornet(HornetInit&)");
     }
 
@@ -250,7 +339,8 @@ public:
         uint32_t f = batch.flags() | (removeBatchDuplicates ? HN_DEDUP_BATCH : 0u) |
                      (removeGraphDuplicates ? HN_DEDUP_GRAPH : 0u);
         int64_t n = 0;
-        check(hn_graph_insert(_g, batch.src(), batch.dst(), batch.nE(), f, &n, _stream), "Hornet::insert");
+        check(hn_graph_insert_meta(_g, batch.src(), batch.dst(), batch.edge_meta(), batch.nE(), f, &n, _stream),
+              "Hornet::insert");
         batch.set_applied(n);
     }
     // Hornet::erase (HornetInsert.i.cuh:71-89)

@@ -26,6 +26,10 @@
 //                                       graph through GraphStd::writeBinary, resume it
 //                                       as a HornetStatic; writes the COO of the
 //                                       frozen copy, then of the resumed graph
+//   meta   <graph> <batch> <out.bin>    HornetInsertTestWeighted.cu shape: a graph with
+//                                       TypeList<int, double> edge meta-data, a weighted
+//                                       insert, a forAllEdges operator reading
+//                                       edge.field<N>(); writes the COO with both fields
 //   bfsfile <file> <out.bin>            BFSTest.cu:18-44 shape: graph::GraphStd
 //                                       read of a .mtx/.gr/.bin file, BFS from
 //                                       max_out_degree_id; writes int32 dist[nv]
@@ -133,6 +137,15 @@ struct SumItems {
 struct SumPairDegrees {
     unsigned long long* acc;
     OPERATOR(Vertex& a, Vertex& b) { atomicAdd(acc, (unsigned long long)(a.degree() * 3 + b.degree())); }
+};
+
+// sums of the two meta-data fields over all edges (the double field doubled)
+struct SumFields {
+    unsigned long long* acc;
+    OPERATOR(Vertex&, Edge& e) {
+        atomicAdd(acc, (unsigned long long)(long long)e.template field<0>());
+        atomicAdd(acc + 1, (unsigned long long)(long long)(2.0 * e.template field<1, double>()));
+    }
 };
 
 static std::vector<unsigned long long> fetch(unsigned long long* d, int n) {
@@ -310,6 +323,50 @@ int main(int argc, char** argv) {
         adj_run("queue", 1, true);
         gpu::free(per);
         gpu::free(acc);
+        return 0;
+    }
+    if (cmd == "meta" && argc >= 5) {
+        using W = hornet::TypeList<int, double>;
+        std::vector<int> w0((size_t)g.ne);
+        std::vector<double> w1((size_t)g.ne);
+        for (size_t j = 0; j < w0.size(); j++) w0[j] = (int)(j * 7 + 1), w1[j] = 0.5 * (double)j;
+        hornet::HornetInit<int, hornet::EMPTY, W> winit((int)g.nv, (int)g.ne, g.off.data(), g.adj.data());
+        winit.insertEdgeData(w0.data(), w1.data());
+        hornet::gpu::Hornet<int, hornet::EMPTY, W> wh(winit);
+        std::ifstream f(argv[3], std::ios::binary);
+        int64_t n = 0;
+        f.read(reinterpret_cast<char*>(&n), 8);
+        std::vector<int> s((size_t)n), d((size_t)n), b0((size_t)n);
+        std::vector<double> b1((size_t)n);
+        f.read(reinterpret_cast<char*>(s.data()), (std::streamsize)(4 * n));
+        f.read(reinterpret_cast<char*>(d.data()), (std::streamsize)(4 * n));
+        for (int64_t i = 0; i < n; i++) b0[(size_t)i] = (int)(100000 + i), b1[(size_t)i] = -1.0 * (double)i;
+        hornet::BatchUpdatePtr<int, W, hornet::DeviceType::HOST> ptr((int)n, s.data(), d.data(), b0.data(), b1.data());
+        hornet::gpu::BatchUpdate<int, W> batch(ptr);
+        wh.insert(batch);
+        unsigned long long* acc = nullptr;
+        gpu::allocate(acc, 2);
+        gpu::memsetZero(acc, 2);
+        load_balancing::BinarySearch lb(wh);
+        forAllEdges(wh, SumFields{acc}, lb);
+        auto a = fetch(acc, 2);
+        gpu::free(acc);
+        std::printf("meta ne=%d inserted=%lld sum0=%lld sum1=%lld\n", wh.nE(), (long long)batch.applied(),
+                    (long long)a[0], (long long)a[1]);
+        auto coo = wh.getCOO();
+        std::vector<int> cs, cd;
+        std::vector<int> m0;
+        std::vector<double> m1;
+        coo.toHost(cs, cd);
+        coo.edgeMetaToHost<0>(m0);
+        coo.edgeMetaToHost<1>(m1);
+        std::ofstream out(argv[4], std::ios::binary);
+        int64_t m = (int64_t)cs.size();
+        out.write(reinterpret_cast<const char*>(&m), 8);
+        out.write(reinterpret_cast<const char*>(cs.data()), (std::streamsize)(4 * m));
+        out.write(reinterpret_cast<const char*>(cd.data()), (std::streamsize)(4 * m));
+        out.write(reinterpret_cast<const char*>(m0.data()), (std::streamsize)(4 * m));
+        out.write(reinterpret_cast<const char*>(m1.data()), (std::streamsize)(8 * m));
         return 0;
     }
     if (cmd == "snapshot" && argc >= 6) {

@@ -186,3 +186,36 @@ def test_snapshot_and_checkpoint_through_headers(graph, tmp_path):
     # the checkpoint is the reference's binary format ("intlong": int64 offsets)
     ck = O.read_binary(str(tmp_path / "ckpt.bin"))
     assert np.array_equal(ck["off"], o1) and np.array_equal(ck["adj"], a1)
+
+
+def test_edge_meta_through_headers(graph, tmp_path):
+    # TypeList<int, double> fields through HornetInit::insertEdgeData, a weighted
+    # BatchUpdatePtr, a device operator reading edge.field<N>() and getCOO
+    off, adj, p = graph
+    n = off.size - 1
+    bs, bd = O.uniform_pairs(n, 3000, seed=51)
+    bp = tmp_path / "b.bin"
+    with open(bp, "wb") as f:
+        np.array([bs.size], np.int64).tofile(f)
+        bs.astype(np.int32).tofile(f)
+        bd.astype(np.int32).tofile(f)
+    out = run("meta", p, str(bp), str(tmp_path / "coo.bin"))
+    j = np.arange(adj.size)
+    w0 = (j * 7 + 1).astype(np.int32)
+    w1 = 0.5 * j
+    i = np.arange(bs.size)
+    o1, a1, (f0, f1), added = O.insert_meta(off, adj, [w0, w1], bs, bd,
+                                           [(100000 + i).astype(np.int32), -1.0 * i])
+    line = next(x for x in out.splitlines() if x.startswith("meta "))
+    assert field(line, "inserted") == added and field(line, "ne") == a1.size
+    assert field(line, "sum0") == int(f0.astype(np.int64).sum())
+    assert field(line, "sum1") == int((2.0 * f1).astype(np.int64).sum())
+    raw = open(tmp_path / "coo.bin", "rb").read()
+    m = int(np.frombuffer(raw, np.int64, 1)[0])
+    pos = 8
+    src = np.frombuffer(raw, np.int32, m, pos); pos += 4 * m
+    dst = np.frombuffer(raw, np.int32, m, pos); pos += 4 * m
+    g0 = np.frombuffer(raw, np.int32, m, pos); pos += 4 * m
+    g1 = np.frombuffer(raw, np.float64, m, pos)
+    assert np.array_equal(src, np.repeat(np.arange(n), np.diff(o1))) and np.array_equal(dst, a1)
+    assert np.array_equal(g0, f0) and np.array_equal(g1, f1)
