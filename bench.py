@@ -110,6 +110,72 @@ def update_rounds(h, args, torch):
     return updates, tc_info
 
 
+def snapshot_timing(h, torch, reps=3):
+    """hn_graph_snapshot of the resident graph: CSR export + rebuild into a
+    tight arena (reads and writes every entry twice, plus the records)."""
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        s = h.snapshot()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+        s.release()
+    ne, nv = h.nE(), h.nV()
+    best = min(ts)
+    moved = 16 * ne + 40 * nv    # 4 B read + 4 B written per entry, twice; records, offsets
+    return {"ms": round(best * 1e3, 3), "ms_runs": [round(x * 1e3, 3) for x in ts],
+            "effective_GBps": round(moved / best / 1e9, 1), "ne": ne}
+
+
+def meta_rounds(h, args, torch):
+    """The same insert / erase rounds on a copy of the graph that carries one
+    4-byte field per edge (its position in the initial CSR); inserted entries
+    carry their batch position.  Reports the rates beside the plain ones."""
+    from hornet_amd.graph import BatchUpdate, Hornet, uniform_pairs_device
+
+    off, adj = h.getCSR()
+    w = torch.arange(adj.numel(), dtype=torch.int32, device=adj.device)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    hw = Hornet.from_device_csr(off, adj, meta=[w])
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t
+    del off, adj, w
+    nv = hw.nV()
+    ins_t = era_t = 0.0
+    ins_n = era_n = 0
+    for r in range(args.meta_rounds):
+        us, ud = uniform_pairs_device(nv, args.batch, seed=40 + r)
+        keep = us != ud
+        bs, bd = torch.cat([us[keep], ud[keep]]), torch.cat([ud[keep], us[keep]])
+        bw = torch.arange(bs.numel(), dtype=torch.int32, device=bs.device).cpu().numpy()
+        bi = BatchUpdate(bs, bd, meta=[bw])
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        hw.insert(bi, True, True)
+        torch.cuda.synchronize()
+        ins_t += time.perf_counter() - t
+        ins_n += bi.nE()
+        src, dst = hw.getCOO()
+        pos, _ = uniform_pairs_device(src.numel(), args.batch, seed=400 + r)
+        ps, pd = src[pos.long()], dst[pos.long()]
+        be = BatchUpdate(torch.cat([ps, pd]), torch.cat([pd, ps]))
+        del src, dst
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        hw.erase(be, True)
+        torch.cuda.synchronize()
+        era_t += time.perf_counter() - t
+        era_n += be.nE()
+    out = {"rounds": args.meta_rounds, "fields": "1 x int32", "build_s": round(t_build, 3),
+           "insert_updates_per_s": round(ins_n / ins_t, 1), "erase_updates_per_s": round(era_n / era_t, 1),
+           "insert_ms_avg": round(ins_t / args.meta_rounds * 1e3, 3),
+           "erase_ms_avg": round(era_t / args.meta_rounds * 1e3, 3), "final_ne": hw.nE()}
+    hw.release()
+    return out
+
+
 def batched_bfs(args, torch):
     """C4: 4,096 seeded sources in launches of 64 (bit-packed masks, captured
     level loop, autotuned load balancing) on R-MAT s23 and a Chung-Lu graph
@@ -272,6 +338,9 @@ def main() -> int:
                     help="budget for the CPU baseline sample (0 disables)")
     ap.add_argument("--rounds", type=int, default=10, help="dynamic-update rounds after the BFS steps")
     ap.add_argument("--batch", type=int, default=1 << 20, help="pairs inserted / deleted per round")
+    ap.add_argument("--meta-rounds", type=int, default=2,
+                    help="update rounds on a copy carrying one int32 field per edge (0 disables)")
+    ap.add_argument("--snapshot", type=int, default=1, help="time hn_graph_snapshot of the graph (0 disables)")
     ap.add_argument("--ms-sources", type=int, default=4096, help="batched-BFS sources (0 disables)")
     ap.add_argument("--ms-plain", type=int, default=64, help="sources timed one per launch for the ratio")
     ap.add_argument("--serve-requests", type=int, default=2000, help="served-stream requests (0 disables)")
@@ -361,7 +430,10 @@ def main() -> int:
         "launches": agg_l,
     }
 
-    updates = tc_info = ms_info = None
+    updates = tc_info = ms_info = snap_info = meta_info = None
+    if args.snapshot:
+        snap_info = snapshot_timing(h, torch)
+        print(f"# snapshot {snap_info['ms']} ms", file=sys.stderr, flush=True)
     if args.rounds > 0:
         updates, tc_info = update_rounds(h, args, torch)
 
@@ -402,6 +474,10 @@ def main() -> int:
                          f"from the first {used} of the {len(sources)} sources on the same graph",
                "cpu_model": cpu_model(), "nproc": os.cpu_count()}
 
+    if args.meta_rounds > 0:
+        meta_info = meta_rounds(h, args, torch)
+        print(f"# weighted updates: {meta_info}", file=sys.stderr, flush=True)
+
     if args.ms_sources > 0:
         del bfs
         h.release()
@@ -433,6 +509,8 @@ def main() -> int:
         "cpu_baseline": cpu,
         "updates": updates,
         "triangle_counting": tc_info,
+        "snapshot": snap_info,
+        "updates_with_edge_meta": meta_info,
         "batched_bfs": ms_info,
         "served": serve_info,
     }

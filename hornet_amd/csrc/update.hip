@@ -329,12 +329,24 @@ __global__ __launch_bounds__(BLOCK) void copy_runs_kernel(int64_t nruns, const i
             int32_t* b = (di[j] & GAP_STAGE) ? stage : arena;
             b[di[j] & ~GAP_STAGE] = v[j];
         }
-        if (am.any()) {
+        // meta-data: per field, all loads first (unconditional: a lane past the
+        // end reads slot 0), then the stores
+        for (int f = 0; f < am.n; f++) {
+            const bool wide = am.bytes[f] == 8;
+            uint64_t mv[UPD_IPT];
+#pragma unroll
+            for (int j = 0; j < UPD_IPT; j++) {
+                const char* b = (si[j] & GAP_STAGE) ? sm.p[f] : am.p[f];
+                const int64_t x = si[j] & ~GAP_STAGE;
+                mv[j] = wide ? reinterpret_cast<const uint64_t*>(b)[x] : reinterpret_cast<const uint32_t*>(b)[x];
+            }
 #pragma unroll
             for (int j = 0; j < UPD_IPT; j++) {
                 if (di[j] < 0) continue;
-                meta_put((di[j] & GAP_STAGE) ? sm : am, di[j] & ~GAP_STAGE, (si[j] & GAP_STAGE) ? sm : am,
-                         si[j] & ~GAP_STAGE);
+                char* b = (di[j] & GAP_STAGE) ? sm.p[f] : am.p[f];
+                const int64_t x = di[j] & ~GAP_STAGE;
+                if (wide) reinterpret_cast<uint64_t*>(b)[x] = mv[j];
+                else reinterpret_cast<uint32_t*>(b)[x] = (uint32_t)mv[j];
             }
         }
         __syncthreads();

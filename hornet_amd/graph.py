@@ -282,6 +282,29 @@ class Hornet:
         return cls(HornetInit(offsets.size - 1, int(offsets[-1]), offsets, edges), slack=slack)
 
     @classmethod
+    def from_device_csr(cls, offsets: torch.Tensor, edges: torch.Tensor, meta=None,
+                        slack: float = 0.0) -> "Hornet":
+        """Hornet(HornetInit) from device tensors (int64 offsets, int32 edges)
+        and optional device meta-data fields (4- or 8-byte dtypes, one
+        element per edge)."""
+        meta = list(meta or [])
+        layout = _lib.HnEdgeMeta()
+        layout.nfields = len(meta)
+        for i, t in enumerate(meta):
+            if t.element_size() not in (4, 8) or t.numel() != edges.numel() or not t.is_cuda:
+                raise HornetError(_lib.HN_E_INVALID_ARG, "Hornet.from_device_csr",
+                                  "meta-data fields are device tensors of 4- or 8-byte elements, one per edge")
+            layout.bytes[i] = t.element_size()
+        meta = [t.contiguous() for t in meta]
+        off = offsets.to(torch.int64).contiguous()
+        adj = edges.to(torch.int32).contiguous()
+        h = ctypes.c_void_p()
+        call("hn_graph_create_csr_meta", off.numel() - 1, adj.numel(), _ptr(off), _ptr(adj),
+             ctypes.byref(layout), _ptr_array([t.data_ptr() for t in meta]), _lib.HN_DEVICE_PTRS,
+             float(slack), _stream(), ctypes.byref(h))
+        return cls(_handle=h)
+
+    @classmethod
     def from_coo_device(cls, nv: int, src: torch.Tensor, dst: torch.Tensor, symmetrize=True,
                         drop_self_loops=True, slack: float = 0.0) -> "Hornet":
         h = ctypes.c_void_p()
