@@ -49,81 +49,62 @@ __device__ __forceinline__ int class_of(int32_t deg) {
     return k < MIN_CLASS ? MIN_CLASS : k;
 }
 
-// Pop a block of class k, or carve a fresh one; -1 when the arena is full.
-__device__ __forceinline__ int64_t alloc_block(const AllocView& A, int k) {
-    unsigned t = atomicAdd(&A.st->take[k], 1u);
-    unsigned fc = A.st->free_count[k];
-    if (t < fc) return (int64_t)A.stacks[A.stack_off[k] + (fc - 1 - t)] * UNIT_SLOTS;
-    unsigned long long units = 1ull << (k - MIN_CLASS);
-    unsigned long long u = atomicAdd(&A.st->bump_units, units);
-    if (u + units > A.st->cap_units) {
-        atomicOr(&A.st->error, 1u);
-        return -1;
-    }
-    return (int64_t)u * UNIT_SLOTS;
-}
+// Block allocation for kernels where many lanes allocate at once: tickets
+// and fresh units are first counted with LDS atomics, then each workgroup
+// takes one global atomic per size class it uses
+// and one on the bump pointer.  Every thread of the workgroup must call it
+// (it synchronises the workgroup); lanes with want == false pass through.
+// Class k's tickets below its stack depth pop blocks, the rest bump fresh
+// units; exhaustion returns -1 and sets the error bit.
+struct AllocShared {
+    unsigned int cnt[NCLASS];
+    unsigned int base[NCLASS];
+    unsigned long long units;
+    unsigned long long b0;
+};
 
-__device__ __forceinline__ void free_block(const AllocView& A, int k, int64_t slot_off) {
-    unsigned idx = atomicAdd(&A.st->free_count[k], 1u);
-    A.stacks[A.stack_off[k] + idx] = (uint32_t)(slot_off / UNIT_SLOTS);
-}
-
-// Wave-aggregated forms.  Every active lane of the wave calls them (lanes
-// that want nothing pass want = false), and the active lanes must be a prefix
-// of the wave (kernels that exit early do so only for indices past a count).
-// A class's tickets are one atomicAdd per (wave, class) instead of one per
-// lane; fresh blocks take one atomicAdd on the bump pointer per wave.  Same
-// results as alloc_block / free_block lane by lane: tickets below the stack
-// depth pop, the rest bump, exhaustion returns -1.
-__device__ __forceinline__ int64_t alloc_block_wave(const AllocView& A, int k, bool want) {
-    const int lane = __lane_id();
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    uint64_t pending = (uint64_t)__ballot(want);
-    unsigned ticket = 0;
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const int kl = __shfl(k, leader, 64);
-        const uint64_t grp = (uint64_t)__ballot(want && k == kl);
-        unsigned base = 0;
-        if (lane == leader) base = atomicAdd(&A.st->take[kl], (unsigned)__popcll(grp));
-        base = __shfl(base, leader, 64);
-        if (want && k == kl) ticket = base + (unsigned)__popcll(grp & lt);
-        pending &= ~grp;
-    }
+__device__ __forceinline__ int64_t alloc_block_group(const AllocView& A, int k, bool want, AllocShared& sh) {
+    for (int i = threadIdx.x; i < NCLASS; i += blockDim.x) sh.cnt[i] = 0;
+    if (threadIdx.x == 0) sh.units = 0;
+    __syncthreads();
+    unsigned int lt = 0;
+    if (want) lt = atomicAdd(&sh.cnt[k], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < NCLASS; i += blockDim.x)
+        sh.base[i] = sh.cnt[i] ? atomicAdd(&A.st->take[i], sh.cnt[i]) : 0u;
+    __syncthreads();
     int64_t res = -2;
     bool bump = false;
+    unsigned long long units = 0, uoff = 0;
     if (want) {
+        const unsigned ticket = sh.base[k] + lt;
         const unsigned fc = A.st->free_count[k];
-        if (ticket < fc) res = (int64_t)A.stacks[A.stack_off[k] + (fc - 1 - ticket)] * UNIT_SLOTS;
-        else bump = true;
+        if (ticket < fc) {
+            res = (int64_t)A.stacks[A.stack_off[k] + (fc - 1 - ticket)] * UNIT_SLOTS;
+        } else {
+            bump = true;
+            units = 1ull << (k - MIN_CLASS);
+            uoff = atomicAdd(&sh.units, units);
+        }
     }
-    // fresh blocks: an inclusive scan of the units over the (prefix of) active lanes
-    const unsigned long long units = bump ? (1ull << (k - MIN_CLASS)) : 0ull;
-    unsigned long long inc = units;
-    for (int d = 1; d < 64; d <<= 1) {
-        const unsigned long long o = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += o;
-    }
-    const uint64_t active = (uint64_t)__ballot(true);
-    const int last = 63 - __clzll((long long)active);
-    const unsigned long long total = __shfl(inc, last, 64);
-    if (total) {
-        unsigned long long b0 = 0;
-        if (lane == last) b0 = atomicAdd(&A.st->bump_units, total);
-        b0 = __shfl(b0, last, 64);
-        if (bump) {
-            const unsigned long long u = b0 + inc - units;
-            if (u + units > A.st->cap_units) {
-                atomicOr(&A.st->error, 1u);
-                res = -1;
-            } else {
-                res = (int64_t)u * UNIT_SLOTS;
-            }
+    __syncthreads();
+    if (threadIdx.x == 0) sh.b0 = sh.units ? atomicAdd(&A.st->bump_units, sh.units) : 0ull;
+    __syncthreads();
+    if (bump) {
+        const unsigned long long u = sh.b0 + uoff;
+        if (u + units > A.st->cap_units) {
+            atomicOr(&A.st->error, 1u);
+            res = -1;
+        } else {
+            res = (int64_t)u * UNIT_SLOTS;
         }
     }
     return res;
 }
 
+// Push freed blocks onto their class stacks: one atomicAdd per (wave, class)
+// on the stack depth.  Every active lane of the wave calls it (lanes that free
+// nothing pass want = false).
 __device__ __forceinline__ void free_block_wave(const AllocView& A, int k, int64_t slot_off, bool want) {
     const int lane = __lane_id();
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));

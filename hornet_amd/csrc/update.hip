@@ -192,16 +192,15 @@ __device__ __forceinline__ AView a_view(const TouchInfo& t, const int32_t* arena
 }
 
 // insert: per touched source j -- new degree, in place or a new block
-__global__ void insert_alloc_kernel(const uint32_t* __restrict__ ucnt, const uint32_t* __restrict__ seg_start,
-                                    const uint64_t* __restrict__ seg_key, const uint64_t* __restrict__ keys, KeyFmt f,
-                                    const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena, AllocView A,
-                                    TouchInfo* __restrict__ ti, const UpdScal* sc) {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= (int64_t)*ucnt || sc->err) return;
+// per touched source j: new degree, in place or a new block (no allocation)
+__device__ __forceinline__ void insert_touch(int64_t j, const uint32_t* __restrict__ seg_start,
+                                             const uint64_t* __restrict__ seg_key,
+                                             const uint64_t* __restrict__ keys, KeyFmt f,
+                                             const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena,
+                                             TouchInfo& t) {
     const int32_t v = f.src(seg_key[j]);
     const hn_vrec r = rec[v];
     const int32_t kj = (int32_t)(seg_start[j + 1] - seg_start[j]);
-    TouchInfo t;
     t.old_off = r.off;
     t.old_deg = r.deg;
     t.old_cap = r.cap_log2;
@@ -215,13 +214,31 @@ __global__ void insert_alloc_kernel(const uint32_t* __restrict__ ucnt, const uin
         t.p = (int32_t)lower_bound_i32(arena + r.off, r.deg, f.dst(keys[seg_start[j]]));
     } else {
         t.new_cap = class_of(t.new_deg);
+        t.new_off = 0;
         t.p = 0;
     }
-    // every active lane takes part (active lanes are a prefix of the wave)
-    const int64_t blk = alloc_block_wave(A, t.inplace ? 0 : t.new_cap, !t.inplace);
-    if (!t.inplace) t.new_off = blk;
+}
+
+// allocation kernels run 1024-lane workgroups: one global atomic per size class
+// and workgroup (alloc_block_group) instead of per wave
+constexpr int ALLOC_BLOCK = 1024;
+
+__global__ __launch_bounds__(ALLOC_BLOCK) void insert_alloc_kernel(
+    const uint32_t* __restrict__ ucnt, const uint32_t* __restrict__ seg_start, const uint64_t* __restrict__ seg_key,
+    const uint64_t* __restrict__ keys, KeyFmt f, const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena,
+    AllocView A, TouchInfo* __restrict__ ti, const UpdScal* sc) {
+    __shared__ AllocShared ash;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = j < (int64_t)*ucnt && !sc->err;   // every lane reaches the allocation below
+    TouchInfo t{};
+    if (active) insert_touch(j, seg_start, seg_key, keys, f, rec, arena, t);
+    const bool need = active && !t.inplace;
+    const int64_t blk = alloc_block_group(A, need ? t.new_cap : 0, need, ash);
+    if (!active) return;
+    if (need) t.new_off = blk;
     ti[j] = t;
 }
+
 
 // commit or roll back the tickets of this batch
 __global__ void alloc_fix_kernel(AllocState* st, UpdScal* sc) {
@@ -473,17 +490,16 @@ struct DLenIn {
 // per touched source u (segments of the distinct keys): deletions; in place
 // unless the list drops to a smaller size class (the reference shrinks when
 // new_degree <= limit/2, BatchUpdateKernels.cuh:180-186) or empties
-__global__ void erase_alloc_kernel(const uint32_t* __restrict__ ucnt, const uint32_t* __restrict__ ustart,
-                                   const uint64_t* __restrict__ ukey, KeyFmt f, const uint32_t* __restrict__ dpos,
-                                   const uint32_t* __restrict__ dlen, const uint32_t* __restrict__ dlp,
-                                   const hn_vrec* __restrict__ rec, AllocView A, TouchInfo* __restrict__ ti,
-                                   uint8_t* __restrict__ skip, const UpdScal* sc) {
-    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= (int64_t)*ucnt || sc->err) return;
+// per touched source u: deletions, in place / smaller block / no block;
+// returns whether a new block is needed
+__device__ __forceinline__ bool erase_touch(int64_t u, const uint32_t* __restrict__ ustart,
+                                            const uint64_t* __restrict__ ukey, KeyFmt f,
+                                            const uint32_t* __restrict__ dpos, const uint32_t* __restrict__ dlen,
+                                            const uint32_t* __restrict__ dlp, const hn_vrec* __restrict__ rec,
+                                            TouchInfo& t, uint8_t* __restrict__ skip) {
     const int32_t v = f.src(ukey[u]);
     const hn_vrec r = rec[v];
     const uint32_t del = dlp[ustart[u + 1]] - dlp[ustart[u]];
-    TouchInfo t;
     t.old_off = r.off;
     t.old_deg = r.deg;
     t.old_cap = r.cap_log2;
@@ -510,11 +526,26 @@ __global__ void erase_alloc_kernel(const uint32_t* __restrict__ ucnt, const uint
             need = true;
         }
     }
-    // every active lane takes part (active lanes are a prefix of the wave)
-    const int64_t blk = alloc_block_wave(A, need ? t.new_cap : 0, need);
+    return need;
+}
+
+__global__ __launch_bounds__(ALLOC_BLOCK) void erase_alloc_kernel(
+    const uint32_t* __restrict__ ucnt, const uint32_t* __restrict__ ustart, const uint64_t* __restrict__ ukey,
+    KeyFmt f, const uint32_t* __restrict__ dpos, const uint32_t* __restrict__ dlen, const uint32_t* __restrict__ dlp,
+    const hn_vrec* __restrict__ rec, AllocView A, TouchInfo* __restrict__ ti, uint8_t* __restrict__ skip,
+    const UpdScal* sc) {
+    __shared__ AllocShared ash;
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = u < (int64_t)*ucnt && !sc->err;   // every lane reaches the allocation below
+    TouchInfo t{};
+    bool need = false;
+    if (active) need = erase_touch(u, ustart, ukey, f, dpos, dlen, dlp, rec, t, skip);
+    const int64_t blk = alloc_block_group(A, need ? t.new_cap : 0, need, ash);
+    if (!active) return;
     if (need) t.new_off = blk;
     ti[u] = t;
 }
+
 
 // Survivors move as runs: between two deleted ranges of a touched vertex u
 // (and after the last one) the old entries keep their order and shift down by
@@ -835,7 +866,7 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     HN_CHECK_LAUNCH();
     uint32_t* ucnt = b.spos + n;
     // 3. in place or new blocks
-    insert_alloc_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg_start, b.seg_key, b.kb, f, g->rec, g->arena,
+    insert_alloc_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(ucnt, b.seg_start, b.seg_key, b.kb, f, g->rec, g->arena,
                                                  g->alloc_view(), b.ti, b.sc);
     HN_CHECK_LAUNCH();
     alloc_fix_kernel<<<1, 1, 0, st>>>(g->alloc, b.sc);
@@ -903,7 +934,7 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     HN_CHECK_LAUNCH();
     uint32_t* ucnt = b.tpos + n;
     // 5. in place, smaller block, or no block
-    erase_alloc_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg2_start, b.seg2_key, f, b.dpos, b.dlen, b.dlp, g->rec,
+    erase_alloc_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(ucnt, b.seg2_start, b.seg2_key, f, b.dpos, b.dlen, b.dlp, g->rec,
                                                 g->alloc_view(), b.ti, b.skip, b.sc);
     HN_CHECK_LAUNCH();
     alloc_fix_kernel<<<1, 1, 0, st>>>(g->alloc, b.sc);
