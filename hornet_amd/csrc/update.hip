@@ -21,9 +21,9 @@
 //   insert  each element of old list A and batch segment B finds its place in
 //           merge(A, B) with one binary search in the other list (stored
 //           copies first on ties), so adjacency stays sorted;
-//   erase   survivors are compacted in order; a survivor's shift is the number
-//           of deleted slots before it, found by a binary search over the
-//           vertex's deleted ranges.
+//   erase   survivors are compacted in order, moved as runs between deleted
+//           ranges (one run per distinct deleted key plus a tail run per
+//           vertex) by one segmented copy.
 // A device error word (bad vertex id, arena exhausted) turns every later
 // kernel into a no-op and rolls the allocator back, so a failed batch leaves
 // the graph unchanged.  Two host synchronisations per batch (staging size,
@@ -107,6 +107,13 @@ struct InsertKeepIn {
         return 1u;
     }
 };
+
+// the keep flags written once (the graph lookups of removeGraphDuplicates would
+// otherwise run in both passes of the scan)
+__global__ void insert_keep_kernel(int64_t n, InsertKeepIn kin, uint32_t* __restrict__ flags) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = kin(i);
+}
 
 // keep flag for the sorted batch (erase): optional in-batch dedup
 struct FirstKeepIn {
@@ -746,17 +753,22 @@ __global__ void store_changed_kernel(const uint32_t* __restrict__ total, UpdScal
 
 // After a batch applied to a symmetric graph: pairs outside the batch kept
 // their membership, so the adjacency stays symmetric iff every batch pair
-// (u,w) has (u,w) and (w,u) both stored or both absent.
+// (u,w) has (u,w) and (w,u) both stored or both absent.  After an insert the
+// kept pairs are stored, so only (w,u) is looked up.
 __global__ void pair_symmetry_kernel(const uint32_t* __restrict__ count, const uint64_t* __restrict__ keys, KeyFmt f,
                                      const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena,
-                                     UpdScal* sc) {
+                                     int inserted, UpdScal* sc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)*count || sc->err) return;
     const int32_t u = f.src(keys[i]), w = f.dst(keys[i]);
-    const hn_vrec ru = rec[u], rw = rec[w];
-    const int64_t pu = lower_bound_i32(arena + ru.off, ru.deg, w);
+    const hn_vrec rw = rec[w];
     const int64_t pw = lower_bound_i32(arena + rw.off, rw.deg, u);
-    const bool a = pu < ru.deg && arena[ru.off + pu] == w;
+    bool a = true;
+    if (!inserted) {
+        const hn_vrec ru = rec[u];
+        const int64_t pu = lower_bound_i32(arena + ru.off, ru.deg, w);
+        a = pu < ru.deg && arena[ru.off + pu] == w;
+    }
     const bool b = pw < rw.deg && arena[rw.off + pw] == u;
     if (a != b) sc->asym = 1u;   // every writer stores the same value
 }
@@ -856,7 +868,13 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
     // 1. keep flags (batch / graph dedup) -> compact into kb
     InsertKeepIn kin{keys, n, f, (flags & HN_DEDUP_BATCH) ? 1 : 0, (flags & HN_DEDUP_GRAPH) ? 1 : 0, g->rec, g->arena};
-    HN_TRY(exclusive_scan<uint32_t>(kin, b.pos, n, b.tmp, st));
+    if (flags & HN_DEDUP_GRAPH) {
+        insert_keep_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(n, kin, b.dpos);   // dpos: free in insert
+        HN_CHECK_LAUNCH();
+        HN_TRY(exclusive_scan<uint32_t>(ArrayIn<uint32_t>{b.dpos}, b.pos, n, b.tmp, st));
+    } else {
+        HN_TRY(exclusive_scan<uint32_t>(kin, b.pos, n, b.tmp, st));
+    }
     compact_keys_kernel<<<gridn, BLOCK, 0, st>>>(n, keys, b.pos, b.kb, b.ia, b.ib);
     HN_CHECK_LAUNCH();
     uint32_t* nb = b.pos + n;   // entries kept
@@ -886,7 +904,7 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     HN_CHECK_LAUNCH();
     const bool was_sym = g->sym == 1;
     if (was_sym) {
-        pair_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, g->rec, g->arena, b.sc);
+        pair_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, g->rec, g->arena, 1, b.sc);
         HN_CHECK_LAUNCH();
     }
     UpdScal h;
@@ -957,7 +975,7 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     HN_CHECK_LAUNCH();
     const bool was_sym = g->sym == 1;
     if (was_sym) {
-        pair_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, g->rec, g->arena, b.sc);
+        pair_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, g->rec, g->arena, 0, b.sc);
         HN_CHECK_LAUNCH();
     }
     UpdScal h;
