@@ -282,6 +282,21 @@ struct StageLenIn {
     }
 };
 
+// Scan output of the segmented copies' run lengths: P[i] = first item of run
+// i, and tile_lo[t] = the run holding item t * UPD_TILE, so a copy tile finds
+// its runs without a search (a run spanning several tiles writes them all).
+struct RunOut {
+    int64_t* P;
+    uint32_t* tile_lo;
+    __device__ __forceinline__ void operator()(int64_t i, int64_t ex, int64_t x) const {
+        P[i] = ex;
+        if (x) {
+            const int64_t t0 = (ex + UPD_TILE - 1) / UPD_TILE, t1 = (ex + x + UPD_TILE - 1) / UPD_TILE;
+            for (int64_t t = t0; t < t1; t++) tile_lo[t] = (uint32_t)i;
+        }
+    }
+};
+
 // Buffers of the segmented copies: an index into the arena, or with
 // GAP_STAGE set into the staging buffer.
 constexpr int64_t GAP_STAGE = (int64_t)1 << 62;
@@ -311,6 +326,7 @@ struct CopyShared {
 // Edge meta-data (am: parallel to the arena, sm: to the staging buffer) move
 // with their entries.
 __global__ __launch_bounds__(BLOCK) void copy_runs_kernel(int64_t nruns, const int64_t* __restrict__ P,
+                                                          const uint32_t* __restrict__ tile_lo,
                                                           const int64_t* __restrict__ src,
                                                           const int64_t* __restrict__ dst,
                                                           int32_t* __restrict__ arena, int32_t* __restrict__ stage,
@@ -318,10 +334,13 @@ __global__ __launch_bounds__(BLOCK) void copy_runs_kernel(int64_t nruns, const i
     __shared__ CopyShared sh;
     if (sc->err) return;
     const int64_t W = P[nruns];
-    for (int64_t w0 = (int64_t)blockIdx.x * UPD_TILE; w0 < W; w0 += (int64_t)gridDim.x * UPD_TILE) {
+    const int64_t ntiles = (W + UPD_TILE - 1) / UPD_TILE;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t w0 = tile * UPD_TILE;
         const int64_t wend = (w0 + (int64_t)UPD_TILE) < W ? (w0 + (int64_t)UPD_TILE) : W;
-        const int64_t lo = block_upper_bound_m1(P, 0, nruns + 1, w0);
-        const int64_t hi = block_upper_bound_m1(P, lo, nruns + 1, wend - 1);
+        const int64_t lo = tile_lo[tile];
+        // the run holding the next tile's first item bounds this tile's runs
+        const int64_t hi = tile + 1 < ntiles ? (int64_t)tile_lo[tile + 1] : block_upper_bound_m1(P, lo, nruns + 1, wend - 1);
         const int count = lb_stage<UPD_IPT>(P, lo, hi, sh.lb, [&](int64_t s) { return src[s] - P[s]; });
         if (count > 0)
             for (int i = threadIdx.x; i < count; i += BLOCK) sh.dst[i] = dst[lo + i] - P[lo + i];
@@ -633,6 +652,7 @@ struct UpdBufs {
     int64_t* gsrc;       // 2n
     int64_t* gdst;       // 2n
     int64_t* gP;         // 2n+1
+    uint32_t* tile_lo;   // first run of each copy tile
     UpdScal* sc;
     void* tmp;
     uint32_t* ia;        // n+1: batch positions (edge meta-data only)
@@ -664,6 +684,9 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     int64_t o_gs = o; o = al(o + 2 * n1 * 8);
     int64_t o_gd = o; o = al(o + 2 * n1 * 8);
     int64_t o_gp = o; o = al(o + (2 * n1 + 1) * 8);
+    // staged / moved entries never exceed the stored ones
+    const int64_t ntile_max = (g->ne + n) / UPD_TILE + 2;
+    int64_t o_tl = o; o = al(o + ntile_max * 4);
     int64_t o_sc = o; o = al(o + 64);
     const bool meta = g->meta.any();
     int64_t o_ia = o; o = al(o + (meta ? n1 * 4 : 0));
@@ -691,6 +714,7 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     b->gsrc = reinterpret_cast<int64_t*>(base + o_gs);
     b->gdst = reinterpret_cast<int64_t*>(base + o_gd);
     b->gP = reinterpret_cast<int64_t*>(base + o_gp);
+    b->tile_lo = reinterpret_cast<uint32_t*>(base + o_tl);
     b->sc = reinterpret_cast<UpdScal*>(base + o_sc);
     b->tmp = base + o_tmp;
     b->ia = meta ? reinterpret_cast<uint32_t*>(base + o_ia) : nullptr;
@@ -783,7 +807,8 @@ static void note_symmetry(hn_graph* g, bool was_sym, const UpdScal& h) {
 static hn_status stage_suffixes(hn_graph* g, UpdBufs& b, int64_t n, const uint32_t* ucnt, const uint8_t* skip,
                                 hipStream_t st, int32_t** stage_out) {
     const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
-    HN_TRY(exclusive_scan<int64_t>(StageLenIn{b.ti, ucnt, b.sc, skip}, b.stpos, n, b.tmp, st));
+    HN_TRY(exclusive_scan_fn<int64_t>(StageLenIn{b.ti, ucnt, b.sc, skip}, RunOut{b.stpos, b.tile_lo}, b.stpos + n, n,
+                                      b.tmp, st));
     set_soff_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.stpos, b.ti, b.gsrc, b.gdst, b.sc);
     HN_CHECK_LAUNCH();
     int64_t total = 0;
@@ -805,7 +830,7 @@ static hn_status stage_suffixes(hn_graph* g, UpdBufs& b, int64_t n, const uint32
     }
     if (total > 0) {
         const int64_t grid = std::min<int64_t>(ceil_div(total, UPD_TILE), (int64_t)num_cus() * 8);
-        copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.stpos, b.gsrc, b.gdst, g->arena, stage, b.sc,
+        copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.stpos, b.tile_lo, b.gsrc, b.gdst, g->arena, stage, b.sc,
                                                            g->meta, b.sm);
         HN_CHECK_LAUNCH();
     }
@@ -963,9 +988,11 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     erase_gaps_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, ucnt, n, b.seg2_start, b.tpos, b.dpos, b.dlen, b.dlp, b.ti,
                                                b.skip, b.glen, b.gsrc, b.gdst, b.sc);
     HN_CHECK_LAUNCH();
-    HN_TRY(exclusive_scan<int64_t>(GapLenIn{b.glen, dcnt, ucnt, n}, b.gP, 2 * n, b.tmp, st));
+    HN_TRY(exclusive_scan_fn<int64_t>(GapLenIn{b.glen, dcnt, ucnt, n}, RunOut{b.gP, b.tile_lo}, b.gP + 2 * n, 2 * n,
+                                      b.tmp, st));
     const int64_t grid = (int64_t)num_cus() * 8;
-    copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.gsrc, b.gdst, g->arena, stage, b.sc, g->meta,
+    copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst, g->arena, stage, b.sc,
+                                                       g->meta,
                                                        b.sm);
     HN_CHECK_LAUNCH();
     // 7. publish
