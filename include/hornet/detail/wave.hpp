@@ -259,15 +259,17 @@ __device__ __forceinline__ int64_t block_upper_bound_m1(const PrefT* __restrict_
 
 // Items w in [w0, min(w0+BLOCK*IPT, total)) with the tile's segment range
 // [lo, hi] (inclusive) known: lo holds item w0 and hi is at or past the
-// segment holding the tile's last item.  item(s, w, segbase(s)).
-template <int IPT, typename PrefT, typename SegFn, typename ItemFn>
+// segment holding the tile's last item.  item(s, w, segbase(s)).  The LDS
+// stage holds LbShared<CIPT>::CAP segments (a tile with more searches them
+// in global memory); a CIPT below IPT keeps the stage small.
+template <int IPT, int CIPT, typename PrefT, typename SegFn, typename ItemFn>
 __device__ __forceinline__ void lb_tile_range(const PrefT* __restrict__ P, int64_t lo, int64_t hi, int64_t w0,
-                                              int64_t total, LbShared<IPT, PrefT>& sh, SegFn segbase,
+                                              int64_t total, LbShared<CIPT, PrefT>& sh, SegFn segbase,
                                               ItemFn item) {
     constexpr int TILE = BLOCK * IPT;
     const int64_t wend = (w0 + (int64_t)TILE) < total ? (w0 + (int64_t)TILE) : total;
     const int64_t span = hi - lo + 1;
-    const int count = span <= LbShared<IPT, PrefT>::CAP ? (int)span : -(int)(span < 0x7fffffff ? span : 0x7fffffff);
+    const int count = span <= LbShared<CIPT, PrefT>::CAP ? (int)span : -(int)(span < 0x7fffffff ? span : 0x7fffffff);
     if (count > 0) {
         for (int i = threadIdx.x; i < count; i += BLOCK) {
             sh.pref[i] = P[lo + i];
@@ -334,9 +336,9 @@ __device__ __forceinline__ int lb_locate(const LbShared<IPT, PrefT>& sh, int cou
 // Items w in [w0, min(w0+BLOCK*IPT, total)): item(s, w, segbase(s)) where s is
 // the segment containing w (P exclusive prefix over nseg segments, P[nseg] =
 // total).  Items are striped: item w0 + j*BLOCK + tid.
-template <int IPT, typename PrefT, typename SegFn, typename ItemFn>
+template <int IPT, int CIPT, typename PrefT, typename SegFn, typename ItemFn>
 __device__ __forceinline__ void lb_tile(const PrefT* __restrict__ P, int64_t nseg, int64_t w0, int64_t total,
-                                        LbShared<IPT, PrefT>& sh, SegFn segbase, ItemFn item) {
+                                        LbShared<CIPT, PrefT>& sh, SegFn segbase, ItemFn item) {
     constexpr int TILE = BLOCK * IPT;
     const int64_t wend = (w0 + (int64_t)TILE) < total ? (w0 + (int64_t)TILE) : total;
     const int64_t lo = block_upper_bound_m1(P, 0, nseg + 1, w0);
