@@ -86,13 +86,15 @@ __device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t n, 
     }
     return lo;
 }
-__device__ __forceinline__ int64_t upper_bound_i32(const int32_t* a, int64_t n, int32_t x) {
+// whether the sorted keys a[0, n) hold x
+__device__ __forceinline__ bool has_key_u64(const uint64_t* a, int64_t n, uint64_t x, int64_t* at) {
     int64_t lo = 0, hi = n;
     while (lo < hi) {
         int64_t mid = (lo + hi) >> 1;
-        if (a[mid] <= x) lo = mid + 1; else hi = mid;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
     }
-    return lo;
+    *at = lo;
+    return lo < n && a[lo] == x;
 }
 
 // keep flag for the sorted batch (insert): in-batch dedup and graph dedup
@@ -494,26 +496,33 @@ __global__ void commit_kernel(const uint32_t* __restrict__ ucnt, const uint64_t*
 }
 
 // ---------------------------------------------------------------- erase
-// per distinct (u,v,c): stored range [lb, lb+e), deleted count min(c, e)
+// per distinct (u,v,c): stored copies from lb = lower_bound(v), deleted count
+// min(c, stored) and whether a copy survives (more than c stored).  The copies
+// are counted forward from lb, at most c + 1 of them: c is bounded by the batch.
 __global__ void erase_locate_kernel(const uint32_t* __restrict__ dcnt, const uint32_t* __restrict__ dstart,
                                     const uint64_t* __restrict__ dkey, KeyFmt f, const hn_vrec* __restrict__ rec,
                                     const int32_t* __restrict__ arena, uint32_t* __restrict__ dpos,
-                                    uint32_t* __restrict__ dlen, const UpdScal* sc) {
+                                    uint32_t* __restrict__ dlen, uint8_t* __restrict__ live, const UpdScal* sc) {
     int64_t jj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (jj >= (int64_t)*dcnt) return;
     if (sc->err) {
         dlen[jj] = 0;
         dpos[jj] = 0;
+        live[jj] = 0;
         return;
     }
     const uint64_t x = dkey[jj];
     const hn_vrec r = rec[f.src(x)];
     const int32_t d = f.dst(x);
-    const int64_t lb = lower_bound_i32(arena + r.off, r.deg, d);
-    const int64_t ub = upper_bound_i32(arena + r.off, r.deg, d);
+    const int32_t* row = arena + r.off;
+    const int64_t lb = lower_bound_i32(row, r.deg, d);
     const int64_t c = dstart[jj + 1] - dstart[jj];
+    const int64_t lim = c + 1 < r.deg - lb ? c + 1 : r.deg - lb;
+    int64_t e = 0;
+    while (e < lim && row[lb + e] == d) e++;
     dpos[jj] = (uint32_t)lb;
-    dlen[jj] = (uint32_t)(c < ub - lb ? c : ub - lb);
+    dlen[jj] = (uint32_t)(c < e ? c : e);
+    live[jj] = e > c ? 1 : 0;
 }
 
 struct DLenIn {
@@ -657,6 +666,7 @@ struct UpdBufs {
     int64_t* stpos;      // n+1
     TouchInfo* ti;       // n
     uint8_t* skip;       // n
+    uint8_t* live;       // n: erase, a copy of the distinct key survives
     int64_t* glen;       // 2n: erase runs (distinct keys, then tail runs)
     int64_t* gsrc;       // 2n
     int64_t* gdst;       // 2n
@@ -689,6 +699,7 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     int64_t o_stpos = o; o = al(o + n1 * 8);
     int64_t o_ti = o; o = al(o + n1 * (int64_t)sizeof(TouchInfo));
     int64_t o_skip = o; o = al(o + n1);
+    int64_t o_live = o; o = al(o + n1);
     int64_t o_gl = o; o = al(o + 2 * n1 * 8);
     int64_t o_gs = o; o = al(o + 2 * n1 * 8);
     int64_t o_gd = o; o = al(o + 2 * n1 * 8);
@@ -719,6 +730,7 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     b->stpos = reinterpret_cast<int64_t*>(base + o_stpos);
     b->ti = reinterpret_cast<TouchInfo*>(base + o_ti);
     b->skip = reinterpret_cast<uint8_t*>(base + o_skip);
+    b->live = reinterpret_cast<uint8_t*>(base + o_live);
     b->glen = reinterpret_cast<int64_t*>(base + o_gl);
     b->gsrc = reinterpret_cast<int64_t*>(base + o_gs);
     b->gdst = reinterpret_cast<int64_t*>(base + o_gd);
@@ -786,24 +798,39 @@ __global__ void store_changed_kernel(const uint32_t* __restrict__ total, UpdScal
 
 // After a batch applied to a symmetric graph: pairs outside the batch kept
 // their membership, so the adjacency stays symmetric iff every batch pair
-// (u,w) has (u,w) and (w,u) both stored or both absent.  After an insert the
-// kept pairs are stored, so only (w,u) is looked up.
-__global__ void pair_symmetry_kernel(const uint32_t* __restrict__ count, const uint64_t* __restrict__ keys, KeyFmt f,
-                                     const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena,
-                                     int inserted, UpdScal* sc) {
+// (u,w) has (u,w) and (w,u) both stored or both absent.
+//
+// Insert: every kept pair (u,w) is stored.  (w,u) is stored too when the kept
+// batch holds it (the sorted keys, searched in cache); only otherwise is it
+// looked up in w's list.
+__global__ void insert_symmetry_kernel(const uint32_t* __restrict__ count, const uint64_t* __restrict__ keys,
+                                       KeyFmt f, const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena,
+                                       UpdScal* sc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)*count || sc->err) return;
+    const int64_t n = (int64_t)*count;
+    if (i >= n || sc->err) return;
     const int32_t u = f.src(keys[i]), w = f.dst(keys[i]);
+    int64_t at;
+    if (has_key_u64(keys, n, ((uint64_t)w << f.sbits) | (uint64_t)u, &at)) return;
     const hn_vrec rw = rec[w];
     const int64_t pw = lower_bound_i32(arena + rw.off, rw.deg, u);
-    bool a = true;
-    if (!inserted) {
-        const hn_vrec ru = rec[u];
-        const int64_t pu = lower_bound_i32(arena + ru.off, ru.deg, w);
-        a = pu < ru.deg && arena[ru.off + pu] == w;
-    }
-    const bool b = pw < rw.deg && arena[rw.off + pw] == u;
-    if (a != b) sc->asym = 1u;   // every writer stores the same value
+    if (!(pw < rw.deg && arena[rw.off + pw] == u)) sc->asym = 1u;   // every writer stores the same value
+}
+
+// Erase: (u,w) is stored afterwards iff a copy survived its deletions.  (w,u)
+// is stored afterwards iff a copy of it survived, when the batch holds it, and
+// otherwise iff it was stored before, which (the graph being symmetric) is iff
+// (u,w) was stored before, i.e. iff (u,w) deleted something.  No list is read.
+__global__ void erase_symmetry_kernel(const uint32_t* __restrict__ dcnt, const uint64_t* __restrict__ dkey, KeyFmt f,
+                                      const uint32_t* __restrict__ dlen, const uint8_t* __restrict__ live,
+                                      UpdScal* sc) {
+    const int64_t jj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)*dcnt;
+    if (jj >= n || sc->err) return;
+    const int32_t u = f.src(dkey[jj]), w = f.dst(dkey[jj]);
+    int64_t at;
+    const bool b = has_key_u64(dkey, n, ((uint64_t)w << f.sbits) | (uint64_t)u, &at) ? live[at] != 0 : dlen[jj] > 0;
+    if ((live[jj] != 0) != b) sc->asym = 1u;
 }
 
 // symmetry bookkeeping after a successful batch (was_sym: known symmetric before)
@@ -938,7 +965,7 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     HN_CHECK_LAUNCH();
     const bool was_sym = g->sym == 1;
     if (was_sym) {
-        pair_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, g->rec, g->arena, 1, b.sc);
+        insert_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, g->rec, g->arena, b.sc);
         HN_CHECK_LAUNCH();
     }
     UpdScal h;
@@ -977,7 +1004,7 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     uint32_t* dcnt = b.spos + n;
     // 3. stored ranges and deletion counts per distinct key
     erase_locate_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, b.seg_start, b.seg_key, f, g->rec, g->arena, b.dpos, b.dlen,
-                                                 b.sc);
+                                                 b.live, b.sc);
     HN_CHECK_LAUNCH();
     HN_TRY(exclusive_scan<uint32_t>(DLenIn{b.dlen, dcnt}, b.dlp, n, b.tmp, st));
     // 4. group the distinct keys by source
@@ -1011,7 +1038,7 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     HN_CHECK_LAUNCH();
     const bool was_sym = g->sym == 1;
     if (was_sym) {
-        pair_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, g->rec, g->arena, 0, b.sc);
+        erase_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, b.seg_key, f, b.dlen, b.live, b.sc);
         HN_CHECK_LAUNCH();
     }
     UpdScal h;

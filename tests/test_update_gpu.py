@@ -204,3 +204,94 @@ def test_empty_batches(hn, g12):
     assert h.insert(hn.BatchUpdate(np.zeros(0, np.int32), np.zeros(0, np.int32))) == 0
     assert h.erase(hn.BatchUpdate(np.zeros(0, np.int32), np.zeros(0, np.int32))) == 0
     check_equal(h, off, adj)
+
+
+def presence_symmetric(off, adj):
+    """(u,w) stored (any number of copies) iff (w,u) stored."""
+    s, d = coo(off, adj)
+    n = off.size - 1
+    k = np.unique(s.astype(np.int64) * n + d)
+    rk = np.unique(d.astype(np.int64) * n + s)
+    return k.size == rk.size and np.array_equal(k, rk)
+
+
+def multigraph_base():
+    """R-MAT 2^10, symmetric, with 2 copies of every pair (u,w), u < w, u % 5 == 0."""
+    n = 1 << 10
+    s, d = O.rmat_pairs(10, seed=12)
+    keep = s != d
+    s, d = s[keep], d[keep]
+    lo, hi = np.minimum(s, d), np.maximum(s, d)
+    k = np.unique(lo.astype(np.int64) * n + hi)
+    lo, hi = (k // n).astype(np.int32), (k % n).astype(np.int32)
+    twice = lo % 5 == 0
+    s = np.concatenate([lo, lo[twice]])
+    d = np.concatenate([hi, hi[twice]])
+    return O.coo_to_csr(n, s, d, symmetrize=True, dedup=False)
+
+
+def sym_cases(off, adj):
+    """(name, op, src, dst, flags, symmetric after) over a symmetric multigraph."""
+    n = off.size - 1
+    s, d = coo(off, adj)
+    fwd = s < d
+    key = s.astype(np.int64) * n + d
+    u, c = np.unique(key[fwd], return_counts=True)
+    one = u[c == 1]
+    two = u[c == 2]
+    a1, b1 = (one // n).astype(np.int32), (one % n).astype(np.int32)
+    a2, b2 = (two // n).astype(np.int32), (two % n).astype(np.int32)
+    assert a1.size > 200 and a2.size > 50
+    stored = set(key.tolist())
+    rng = np.random.default_rng(9)
+    x = rng.integers(0, n, 400).astype(np.int32)
+    y = rng.integers(0, n, 400).astype(np.int32)
+    new = (x != y) & np.array([int(p) * n + int(q) not in stored and int(q) * n + int(p) not in stored
+                              for p, q in zip(x, y)])
+    x, y = x[new], y[new]
+    cat = np.concatenate
+    return [
+        ("erase both directions", "erase", cat([a1[:100], b1[:100]]), cat([b1[:100], a1[:100]]), (False,), True),
+        ("erase one direction", "erase", a1[:50], b1[:50], (False,), False),
+        ("erase absent pairs, one direction", "erase", cat([x[:50], a1[:20], b1[:20]]),
+         cat([y[:50], b1[:20], a1[:20]]), (False,), True),
+        ("erase one of two copies, both directions", "erase", cat([a2[:30], b2[:30]]), cat([b2[:30], a2[:30]]),
+         (False,), True),
+        ("erase one of two copies, one direction", "erase", a2[:30], b2[:30], (False,), True),
+        ("erase both copies one way, one the other way", "erase", cat([a2[:5], a2[:5], b2[:5]]),
+         cat([b2[:5], b2[:5], a2[:5]]), (False,), False),
+        ("erase both copies one way, deduplicated batch", "erase", cat([a2[:5], a2[:5], b2[:5]]),
+         cat([b2[:5], b2[:5], a2[:5]]), (True,), True),
+        ("insert new pairs, both directions", "insert", cat([x[:60], y[:60]]), cat([y[:60], x[:60]]),
+         (True, True), True),
+        ("insert new pairs, one direction", "insert", x[:60], y[:60], (True, True), False),
+        ("insert stored pairs, one direction", "insert", a1[:60], b1[:60], (False, False), True),
+        ("insert stored pairs, graph-deduplicated", "insert", a1[:60], b1[:60], (False, True), True),
+    ]
+
+
+def test_symmetry_tracked_by_each_batch(hn):
+    """After each batch on a graph known to be symmetric, the symmetry flag
+    (seen through BfsMulti, which pulls only over a symmetric adjacency) equals
+    the symmetry of the oracle's adjacency."""
+    off0, adj0 = multigraph_base()
+    assert presence_symmetric(off0, adj0)
+    srcs = O.pick_sources(off0, 8, seed=2)
+    for name, op, bs, bd, flags, sym in sym_cases(off0, adj0):
+        print(f"case symmetry: {name}", flush=True)
+        h = hn.Hornet.from_csr(off0, adj0)
+        ms = hn.BfsMulti(h)
+        ms.set_params(0, 1 << 20)   # pull at every level the adjacency allows
+        ms.run(srcs)
+        assert ms.stats.pull_levels >= 1
+        if op == "erase":
+            h.erase(hn.BatchUpdate(bs, bd), *flags)
+            off, adj, _ = O.erase(off0, adj0, bs, bd, *flags)
+        else:
+            h.insert(hn.BatchUpdate(bs, bd), *flags)
+            off, adj, _ = O.insert(off0, adj0, bs, bd, *flags)
+        assert presence_symmetric(off, adj) == sym
+        check_equal(h, off, adj)
+        got = ms.run(srcs).cpu().numpy()
+        assert np.array_equal(got, np.stack([O.bfs(off, adj, int(v))[0] for v in srcs]))
+        assert (ms.stats.pull_levels >= 1) == sym, name
