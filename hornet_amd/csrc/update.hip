@@ -23,11 +23,12 @@
 //           copies first on ties), so adjacency stays sorted;
 //   erase   survivors are compacted in order, moved as runs between deleted
 //           ranges (one run per distinct deleted key plus a tail run per
-//           vertex) by one segmented copy.
+//           vertex), in place in one pass; the few entries whose slot an
+//           earlier copy tile may not have read yet go through a side buffer.
 // A device error word (bad vertex id, arena exhausted) turns every later
 // kernel into a no-op and rolls the allocator back, so a failed batch leaves
-// the graph unchanged.  Two host synchronisations per batch (staging size,
-// final status).
+// the graph unchanged.  Three host synchronisations per batch (staging or
+// side-buffer size, error word, final status).
 #include "graph.hpp"
 #include "prims.hpp"
 
@@ -334,6 +335,23 @@ struct CopyShared {
     int64_t dst[LbShared<COPY_CAP_IPT, int64_t>::CAP];
 };
 
+// a tile's run range [lo, hi] (hi at or past the run holding its last item)
+// and its staged run count (> 0), or -(span) when the span overflows the stage
+struct CopyRange {
+    int64_t lo;
+    int count;
+};
+
+__device__ __forceinline__ CopyRange copy_range(const int64_t* __restrict__ P, int64_t nruns, int64_t W,
+                                                int64_t ntiles, int64_t tile, int64_t lo, int64_t next_lo) {
+    // the run holding the next tile's first item bounds this tile's runs; the
+    // last tile searches (the whole workgroup takes this branch together)
+    const int64_t hi = tile + 1 < ntiles ? next_lo : block_upper_bound_m1(P, lo, nruns + 1, W - 1);
+    const int64_t span = hi - lo + 1;
+    constexpr int cap = LbShared<COPY_CAP_IPT, int64_t>::CAP;
+    return {lo, span <= cap ? (int)span : -(int)(span < 0x7fffffff ? span : 0x7fffffff)};
+}
+
 // Edge meta-data (am: parallel to the arena, sm: to the staging buffer) move
 // with their entries.
 __global__ __launch_bounds__(BLOCK) void copy_runs_kernel(int64_t nruns, const int64_t* __restrict__ P,
@@ -593,11 +611,15 @@ __global__ __launch_bounds__(ALLOC_BLOCK) void erase_alloc_kernel(
 
 // Survivors move as runs: between two deleted ranges of a touched vertex u
 // (and after the last one) the old entries keep their order and shift down by
-// the number deleted before them.  Run jj (before distinct key jj; the tail
-// run of u is n + u) copies [x0, x0 + len) of u's old list -- from the staged
-// suffix when u is rewritten in place, from the old block otherwise -- to
-// x0 - shift in u's block.  One thread per distinct key and per touched
-// vertex; key jj belongs to vertex tpos[jj + 1] - 1.
+// the number deleted before them.  The runs of u are consecutive: run jj + u
+// (before distinct key jj), then u's tail run at ustart[u + 1] + u, so the
+// items of every vertex are contiguous and in slot order.  A run copies
+// [x0, x0 + len) of u's old block to x0 - shift in u's (new or same) block;
+// runs of a vertex rewritten in place carry RUN_INPLACE.  rtail[run] names
+// the tail run of the run's vertex.  One thread per distinct key and per
+// touched vertex; key jj belongs to vertex tpos[jj + 1] - 1.
+constexpr int64_t RUN_INPLACE = (int64_t)1 << 61;
+
 __device__ __forceinline__ void erase_run(int64_t g, int64_t prev, int64_t end, int64_t shift, bool live,
                                           const TouchInfo& t, int64_t* __restrict__ glen, int64_t* __restrict__ gsrc,
                                           int64_t* __restrict__ gdst) {
@@ -606,45 +628,191 @@ __device__ __forceinline__ void erase_run(int64_t g, int64_t prev, int64_t end, 
     const int64_t len = (live && end > start) ? end - start : 0;
     glen[g] = len;
     if (len) {
-        gsrc[g] = t.inplace ? (GAP_STAGE | (t.soff + (start - p))) : (t.old_off + start);
+        gsrc[g] = (t.old_off + start) | (t.inplace ? RUN_INPLACE : 0);
         gdst[g] = t.new_off + start - shift;
     }
 }
 
-__global__ void erase_gaps_kernel(const uint32_t* __restrict__ dcnt, const uint32_t* __restrict__ ucnt, int64_t n,
+__global__ void erase_gaps_kernel(const uint32_t* __restrict__ dcnt, const uint32_t* __restrict__ ucnt,
                                   const uint32_t* __restrict__ ustart, const uint32_t* __restrict__ tpos,
                                   const uint32_t* __restrict__ dpos, const uint32_t* __restrict__ dlen,
                                   const uint32_t* __restrict__ dlp, const TouchInfo* __restrict__ ti,
                                   const uint8_t* __restrict__ skip, int64_t* __restrict__ glen,
-                                  int64_t* __restrict__ gsrc, int64_t* __restrict__ gdst, const UpdScal* sc) {
+                                  int64_t* __restrict__ gsrc, int64_t* __restrict__ gdst,
+                                  uint32_t* __restrict__ rtail, const UpdScal* sc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < (int64_t)*dcnt) {   // the run before distinct key i
         const int64_t u = (int64_t)tpos[i + 1] - 1;
         const uint32_t a = ustart[u];
         const int64_t prev = i > (int64_t)a ? (int64_t)dpos[i - 1] + dlen[i - 1] : 0;
-        erase_run(i, prev, (int64_t)dpos[i], (int64_t)(dlp[i] - dlp[a]), !sc->err && !skip[u], ti[u], glen, gsrc,
+        erase_run(i + u, prev, (int64_t)dpos[i], (int64_t)(dlp[i] - dlp[a]), !sc->err && !skip[u], ti[u], glen, gsrc,
                   gdst);
+        rtail[i + u] = (uint32_t)(ustart[u + 1] + u);
     }
     if (i < (int64_t)*ucnt) {   // the tail run of touched vertex i
         const uint32_t a = ustart[i], b = ustart[i + 1];
         const TouchInfo t = ti[i];
         const int64_t prev = b > a ? (int64_t)dpos[b - 1] + dlen[b - 1] : 0;
-        erase_run(n + i, prev, (int64_t)t.old_deg, (int64_t)(dlp[b] - dlp[a]), !sc->err && !skip[i], t, glen, gsrc,
+        erase_run(b + i, prev, (int64_t)t.old_deg, (int64_t)(dlp[b] - dlp[a]), !sc->err && !skip[i], t, glen, gsrc,
                   gdst);
+        rtail[b + i] = (uint32_t)(b + i);
     }
 }
 
-// run lengths: runs of the distinct keys [0, dcnt) and tail runs [n, n + ucnt)
+// run lengths: dcnt + ucnt runs
 struct GapLenIn {
     const int64_t* glen;
     const uint32_t* dcnt;
     const uint32_t* ucnt;
-    int64_t n;
     __device__ __forceinline__ int64_t operator()(int64_t g) const {
-        const bool valid = g < n ? g < (int64_t)*dcnt : (g - n) < (int64_t)*ucnt;
-        return valid ? glen[g] : 0;
+        return g < (int64_t)*dcnt + (int64_t)*ucnt ? glen[g] : 0;
     }
 };
+
+// The erase compaction runs in place, in one pass over the survivors, except
+// for a few deferred entries.  Within a vertex the items are in slot order and
+// every entry moves down (destination <= source), so an entry's destination
+// can only be the source of an earlier item of its vertex.  A copy tile reads
+// all its sources before it writes (a barrier), which settles its own items;
+// the earlier items of its first vertex that lie in earlier tiles -- read in
+// any order by other workgroups -- have sources up to s = source(i0 - 1).  The
+// tile's first items whose destination is <= s are deferred: written to a
+// side buffer and placed by a second copy once every tile has read.  All other
+// destinations lie above every source of an earlier tile and below every
+// source of a later one.  Deferred entries are a prefix of the tile, within
+// its first vertex (their destinations are consecutive), at most the number
+// of entries deleted before them in that vertex; measured at 1-2 % of the
+// moved entries on R-MAT graphs, where the staged alternative copies every
+// moved entry twice.
+__global__ void erase_defer_kernel(int64_t ntile_max, int64_t nruns, const int64_t* __restrict__ P,
+                                   const uint32_t* __restrict__ tile_lo, const int64_t* __restrict__ src,
+                                   const int64_t* __restrict__ dst, const uint32_t* __restrict__ rtail,
+                                   uint32_t* __restrict__ tcnt, int64_t* __restrict__ tdst, const UpdScal* sc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntile_max) return;
+    const int64_t W = P[nruns];
+    const int64_t ntiles = (W + COPY_TILE - 1) / COPY_TILE;
+    int64_t c = 0, d0 = 0;
+    if (!sc->err && t > 0 && t < ntiles) {
+        const int64_t i0 = t * COPY_TILE;
+        const int64_t r = tile_lo[t];                                   // holds i0
+        const int64_t rp = upper_bound_m1<int64_t>(P, 0, r + 1, i0 - 1);  // holds i0 - 1
+        if ((src[r] & RUN_INPLACE) && rtail[r] == rtail[rp]) {
+            d0 = dst[r] - P[r] + i0;
+            const int64_t sprev = (src[rp] & ~RUN_INPLACE) - P[rp] + (i0 - 1);
+            if (sprev >= d0) {
+                c = sprev - d0 + 1;
+                const int64_t vend = P[rtail[r] + 1];   // end of the vertex's items
+                c = c < vend - i0 ? c : vend - i0;
+                c = c < (int64_t)COPY_TILE ? c : (int64_t)COPY_TILE;
+            }
+        }
+    }
+    tcnt[t] = (uint32_t)c;
+    tdst[t] = d0;
+}
+
+struct TileCountIn {
+    const uint32_t* c;
+    __device__ __forceinline__ int64_t operator()(int64_t t) const { return (int64_t)c[t]; }
+};
+
+// scan output of the deferred counts: P (the second copy's run starts), its
+// tile_lo, and each run's source in the side buffer
+struct DeferOut {
+    RunOut runs;
+    int64_t* src;
+    __device__ __forceinline__ void operator()(int64_t i, int64_t ex, int64_t x) const {
+        runs(i, ex, x);
+        src[i] = GAP_STAGE | ex;
+    }
+};
+
+// s_waitcnt vmcnt(0): every vector memory load of the wave has returned
+__device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// One pass of the in-place compaction: tile t's first tcnt[t] items go to the
+// side buffer at doff[t], every other item to its destination.  Sources are
+// read (values, then each meta-data field) by the whole tile before any of
+// its writes.
+__global__ __launch_bounds__(BLOCK) void erase_copy_kernel(int64_t nruns, const int64_t* __restrict__ P,
+                                                           const uint32_t* __restrict__ tile_lo,
+                                                           const int64_t* __restrict__ src,
+                                                           const int64_t* __restrict__ dst,
+                                                           const uint32_t* __restrict__ tcnt,
+                                                           const int64_t* __restrict__ doff,
+                                                           int32_t* __restrict__ arena, int32_t* __restrict__ side,
+                                                           const UpdScal* sc, MetaSet am, MetaSet sm) {
+    __shared__ CopyShared sh;
+    if (sc->err) return;
+    const int64_t W = P[nruns];
+    const int64_t ntiles = (W + COPY_TILE - 1) / COPY_TILE;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t w0 = tile * COPY_TILE;
+        const int64_t wend = (w0 + (int64_t)COPY_TILE) < W ? (w0 + (int64_t)COPY_TILE) : W;
+        const CopyRange rg = copy_range(P, nruns, W, ntiles, tile, tile_lo[tile],
+                                        tile + 1 < ntiles ? (int64_t)tile_lo[tile + 1] : 0);
+        if (rg.count > 0)
+            for (int i = threadIdx.x; i < rg.count; i += BLOCK) {
+                const int64_t s = rg.lo + i;
+                sh.lb.pref[i] = P[s];
+                sh.lb.base[i] = (src[s] & ~RUN_INPLACE) - P[s];
+                sh.dst[i] = dst[s] - P[s];
+            }
+        const int64_t ndef = tcnt[tile];
+        const int64_t o = doff[tile];
+        __syncthreads();
+        int64_t si[COPY_IPT], di[COPY_IPT];
+#pragma unroll
+        for (int j = 0; j < COPY_IPT; j++) {
+            const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+            if (rg.count > 0) {
+                const int s = lb_locate<COPY_CAP_IPT>(sh.lb, rg.count, w);
+                si[j] = sh.lb.base[s] + w;
+                di[j] = sh.dst[s] + w;
+            } else {
+                const int64_t s = upper_bound_m1<int64_t>(P, rg.lo, rg.lo + (int64_t)(-rg.count), w);
+                si[j] = (src[s] & ~RUN_INPLACE) - P[s] + w;
+                di[j] = dst[s] - P[s] + w;
+            }
+            if (w >= wend) si[j] = 0, di[j] = -1;
+            // deferred: the destination becomes a slot of the side buffer
+            if (di[j] >= 0 && w - w0 < ndef) di[j] = GAP_STAGE | (o + (w - w0));
+        }
+        int32_t v[COPY_IPT];
+#pragma unroll
+        for (int j = 0; j < COPY_IPT; j++) v[j] = arena[si[j]];
+        // the tile's sources are all read before any slot is rewritten: the
+        // loads have returned (the barrier alone does not wait for them)
+        wait_loads();
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < COPY_IPT; j++) {
+            if (di[j] < 0) continue;
+            if (di[j] & GAP_STAGE) side[di[j] & ~GAP_STAGE] = v[j];
+            else arena[di[j]] = v[j];
+        }
+        for (int f = 0; f < am.n; f++) {
+            const bool wide = am.bytes[f] == 8;
+            uint64_t mv[COPY_IPT];
+#pragma unroll
+            for (int j = 0; j < COPY_IPT; j++)
+                mv[j] = wide ? reinterpret_cast<const uint64_t*>(am.p[f])[si[j]]
+                             : reinterpret_cast<const uint32_t*>(am.p[f])[si[j]];
+            wait_loads();
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < COPY_IPT; j++) {
+                if (di[j] < 0) continue;
+                char* b = (di[j] & GAP_STAGE) ? sm.p[f] : am.p[f];
+                const int64_t x = di[j] & ~GAP_STAGE;
+                if (wide) reinterpret_cast<uint64_t*>(b)[x] = mv[j];
+                else reinterpret_cast<uint32_t*>(b)[x] = (uint32_t)mv[j];
+            }
+        }
+        __syncthreads();
+    }
+}
 
 }  // namespace
 
@@ -672,11 +840,18 @@ struct UpdBufs {
     int64_t* gdst;       // 2n
     int64_t* gP;         // 2n+1
     uint32_t* tile_lo;   // first run of each copy tile
+    uint32_t* rtail;     // 2n: erase, the tail run of each run's vertex
+    uint32_t* tcnt;      // per copy tile: erase, entries deferred
+    int64_t* tdst;       // per copy tile: erase, destination of the first deferred entry
+    int64_t* doff;       // per copy tile + 1: erase, side-buffer offset of the deferred entries
+    int64_t* dsrc;       // per copy tile: erase, GAP_STAGE | doff
+    uint32_t* tile_lo2;  // first deferred run of each copy tile
     UpdScal* sc;
     void* tmp;
     uint32_t* ia;        // n+1: batch positions (edge meta-data only)
     uint32_t* ib;        // n+1
     MetaSet sm;          // staging area of the edge meta-data
+    int64_t ntile_max;   // copy tiles the buffers above are sized for
 };
 
 static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
@@ -707,11 +882,18 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     // staged / moved entries never exceed the stored ones
     const int64_t ntile_max = (g->ne + n) / COPY_TILE + 2;
     int64_t o_tl = o; o = al(o + ntile_max * 4);
+    int64_t o_rt = o; o = al(o + 2 * n1 * 4);
+    int64_t o_tc = o; o = al(o + ntile_max * 4);
+    int64_t o_td = o; o = al(o + ntile_max * 8);
+    int64_t o_do = o; o = al(o + (ntile_max + 1) * 8);
+    int64_t o_ds = o; o = al(o + ntile_max * 8);
+    int64_t o_tl2 = o; o = al(o + ntile_max * 4);
     int64_t o_sc = o; o = al(o + 64);
     const bool meta = g->meta.any();
     int64_t o_ia = o; o = al(o + (meta ? n1 * 4 : 0));
     int64_t o_ib = o; o = al(o + (meta ? n1 * 4 : 0));
-    int64_t o_tmp = o; o = al(o + (int64_t)std::max(sort_temp_bytes(n1), scan_temp_bytes(2 * n1)));
+    int64_t o_tmp = o;
+    o = al(o + (int64_t)std::max(sort_temp_bytes(n1), scan_temp_bytes(std::max(2 * n1, ntile_max))));
     HN_TRY(g->s_upd.reserve((size_t)o));
     char* base = g->s_upd.as<char>();
     b->ka = reinterpret_cast<uint64_t*>(base + o_ka);
@@ -736,11 +918,18 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     b->gdst = reinterpret_cast<int64_t*>(base + o_gd);
     b->gP = reinterpret_cast<int64_t*>(base + o_gp);
     b->tile_lo = reinterpret_cast<uint32_t*>(base + o_tl);
+    b->rtail = reinterpret_cast<uint32_t*>(base + o_rt);
+    b->tcnt = reinterpret_cast<uint32_t*>(base + o_tc);
+    b->tdst = reinterpret_cast<int64_t*>(base + o_td);
+    b->doff = reinterpret_cast<int64_t*>(base + o_do);
+    b->dsrc = reinterpret_cast<int64_t*>(base + o_ds);
+    b->tile_lo2 = reinterpret_cast<uint32_t*>(base + o_tl2);
     b->sc = reinterpret_cast<UpdScal*>(base + o_sc);
     b->tmp = base + o_tmp;
     b->ia = meta ? reinterpret_cast<uint32_t*>(base + o_ia) : nullptr;
     b->ib = meta ? reinterpret_cast<uint32_t*>(base + o_ib) : nullptr;
     b->sm = meta_like(g->meta, nullptr);
+    b->ntile_max = ntile_max;
     return HN_OK;
 }
 
@@ -839,6 +1028,28 @@ static void note_symmetry(hn_graph* g, bool was_sym, const UpdScal& h) {
     g->sym_version = g->version;
 }
 
+// a staging buffer of `total` entries (and one per meta-data field, in b.sm),
+// after the batch's error word is checked
+static hn_status reserve_stage(hn_graph* g, UpdBufs& b, int64_t total, hipStream_t st, int32_t** stage) {
+    UpdScal h;
+    HN_TRY(check_scal(g, b, st, &h));
+    const size_t cnt = (size_t)(total > 0 ? total : 1);
+    HN_TRY(g->s_misc.reserve(sizeof(int32_t) * cnt));
+    *stage = g->s_misc.as<int32_t>();
+    if (g->meta.any()) {
+        // one staging array per meta-data field, 256-byte aligned, in s_mstage
+        size_t bytes = 0;
+        for (int f = 0; f < g->meta.n; f++) bytes += (cnt * g->meta.bytes[f] + 255) & ~size_t(255);
+        HN_TRY(g->s_mstage.reserve(bytes));
+        size_t at = 0;
+        for (int f = 0; f < g->meta.n; f++) {
+            b.sm.p[f] = g->s_mstage.as<char>() + at;
+            at += (cnt * g->meta.bytes[f] + 255) & ~size_t(255);
+        }
+    }
+    return HN_OK;
+}
+
 // size the staging buffer (one readback) and copy the moving suffixes
 static hn_status stage_suffixes(hn_graph* g, UpdBufs& b, int64_t n, const uint32_t* ucnt, const uint8_t* skip,
                                 hipStream_t st, int32_t** stage_out) {
@@ -848,22 +1059,9 @@ static hn_status stage_suffixes(hn_graph* g, UpdBufs& b, int64_t n, const uint32
     set_soff_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.stpos, b.ti, b.gsrc, b.gdst, b.sc);
     HN_CHECK_LAUNCH();
     int64_t total = 0;
-    UpdScal h;
     HN_TRY(read_back(g, b.stpos + n, 8, &total, st));
-    HN_TRY(check_scal(g, b, st, &h));
-    HN_TRY(g->s_misc.reserve(sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
-    int32_t* stage = g->s_misc.as<int32_t>();
-    if (g->meta.any()) {
-        // one staging array per meta-data field, 256-byte aligned, in s_mstage
-        size_t bytes = 0;
-        for (int f = 0; f < g->meta.n; f++) bytes += ((size_t)(total > 0 ? total : 1) * g->meta.bytes[f] + 255) & ~size_t(255);
-        HN_TRY(g->s_mstage.reserve(bytes));
-        size_t at = 0;
-        for (int f = 0; f < g->meta.n; f++) {
-            b.sm.p[f] = g->s_mstage.as<char>() + at;
-            at += ((size_t)(total > 0 ? total : 1) * g->meta.bytes[f] + 255) & ~size_t(255);
-        }
-    }
+    int32_t* stage = nullptr;
+    HN_TRY(reserve_stage(g, b, total, st, &stage));
     if (total > 0) {
         const int64_t grid = std::min<int64_t>(ceil_div(total, COPY_TILE), (int64_t)num_cus() * COPY_GRID_PER_CU);
         copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.stpos, b.tile_lo, b.gsrc, b.gdst, g->arena, stage, b.sc,
@@ -1018,20 +1216,35 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     HN_CHECK_LAUNCH();
     alloc_fix_kernel<<<1, 1, 0, st>>>(g->alloc, b.sc);
     HN_CHECK_LAUNCH();
-    int32_t* stage = nullptr;
-    HN_TRY(stage_suffixes(g, b, n, ucnt, b.skip, st, &stage));
-    // 6. ordered compaction of the survivors, as runs between deleted ranges
-    erase_gaps_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, ucnt, n, b.seg2_start, b.tpos, b.dpos, b.dlen, b.dlp, b.ti,
-                                               b.skip, b.glen, b.gsrc, b.gdst, b.sc);
+    // 6. the survivors as runs between deleted ranges, grouped by vertex
+    erase_gaps_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, ucnt, b.seg2_start, b.tpos, b.dpos, b.dlen, b.dlp, b.ti, b.skip,
+                                               b.glen, b.gsrc, b.gdst, b.rtail, b.sc);
     HN_CHECK_LAUNCH();
-    HN_TRY(exclusive_scan_fn<int64_t>(GapLenIn{b.glen, dcnt, ucnt, n}, RunOut{b.gP, b.tile_lo}, b.gP + 2 * n, 2 * n,
+    HN_TRY(exclusive_scan_fn<int64_t>(GapLenIn{b.glen, dcnt, ucnt}, RunOut{b.gP, b.tile_lo}, b.gP + 2 * n, 2 * n,
                                       b.tmp, st));
-    const int64_t grid = (int64_t)num_cus() * COPY_GRID_PER_CU;
-    copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst, g->arena, stage, b.sc,
-                                                       g->meta,
-                                                       b.sm);
+    // 7. in-place compaction in one pass; the few entries that would overwrite
+    // a source of an earlier tile go through a side buffer
+    const int64_t nt = b.ntile_max;
+    erase_defer_kernel<<<(unsigned)ceil_div(nt, BLOCK), BLOCK, 0, st>>>(nt, 2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst,
+                                                                       b.rtail, b.tcnt, b.tdst, b.sc);
     HN_CHECK_LAUNCH();
-    // 7. publish
+    HN_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, DeferOut{RunOut{b.doff, b.tile_lo2}, b.dsrc}, b.doff + nt,
+                                      nt, b.tmp, st));
+    int64_t ndefer = 0;
+    HN_TRY(read_back(g, b.doff + nt, 8, &ndefer, st));
+    int32_t* side = nullptr;
+    HN_TRY(reserve_stage(g, b, ndefer, st, &side));
+    const int64_t grid = (int64_t)num_cus() * COPY_GRID_PER_CU;
+    erase_copy_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst, b.tcnt, b.doff,
+                                                        g->arena, side, b.sc, g->meta, b.sm);
+    HN_CHECK_LAUNCH();
+    if (ndefer > 0) {
+        const int64_t g2 = std::min<int64_t>(ceil_div(ndefer, COPY_TILE), grid);
+        copy_runs_kernel<<<(unsigned)g2, BLOCK, 0, st>>>(nt, b.doff, b.tile_lo2, b.dsrc, b.tdst, g->arena, side,
+                                                         b.sc, g->meta, b.sm);
+        HN_CHECK_LAUNCH();
+    }
+    // 8. publish
     commit_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg2_key, f, b.ti, b.skip, g->rec, g->alloc_view(), b.sc);
     HN_CHECK_LAUNCH();
     store_changed_kernel<<<1, 1, 0, st>>>(b.dlp + n, b.sc);

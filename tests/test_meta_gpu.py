@@ -123,3 +123,19 @@ def test_meta_errors(base):
     with pytest.raises(HornetError) as e:
         plain.insert(BatchUpdate([0], [1], meta=[np.zeros(1, np.int32)]))
     assert e.value.status == _lib.HN_E_INVALID_ARG
+
+
+def test_erase_hub_rows_in_place_with_meta():
+    """Fields follow their entries through in-place compaction of long rows
+    (shifts of thousands of slots across many copy tiles)."""
+    print("case meta erase hub rows", flush=True)
+    from test_update_gpu import hub_erase_batch, hub_graph
+    off, adj, hubs = hub_graph(4)
+    f = fields_for(adj.size, 6)
+    h = build(off, adj, f)
+    for r in range(2):
+        es, ed = hub_erase_batch(off, adj, hubs, 20 + r, (0.4, 0.08)[r])
+        removed = h.erase(BatchUpdate(es, ed))
+        off, adj, f, erem = O.erase_meta(off, adj, f, es, ed)
+        assert removed == erem
+        check(h, off, adj, f)

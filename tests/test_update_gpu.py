@@ -295,3 +295,50 @@ def test_symmetry_tracked_by_each_batch(hn):
         got = ms.run(srcs).cpu().numpy()
         assert np.array_equal(got, np.stack([O.bfs(off, adj, int(v))[0] for v in srcs]))
         assert (ms.stats.pull_levels >= 1) == sym, name
+
+
+def hub_graph(seed):
+    """2^16 vertices: four hubs of ~30000 entries (capacity 32768) over uniform
+    background pairs, directed, with repeated pairs (a multigraph)."""
+    rng = np.random.default_rng(seed)
+    n = 1 << 16
+    hubs = [5, 77, 1000, 40000]
+    s = [np.full(30000, v, np.int32) for v in hubs]
+    d = [rng.choice(n, 30000, replace=False).astype(np.int32) for _ in hubs]
+    bs, bd = O.uniform_pairs(n, 60000, seed=seed)
+    s = np.concatenate(s + [bs, bs[:5000]])
+    d = np.concatenate(d + [bd, bd[:5000]])
+    off, adj = O.coo_to_csr(n, s, d, symmetrize=False, drop_self_loops=False, dedup=False)
+    return off, adj, hubs
+
+
+def hub_erase_batch(off, adj, hubs, seed, frac):
+    """A fraction of every hub row (40 % of 30000, then 8 % of 18000: the rows
+    stay in their 32768-entry blocks and are compacted in place, with shifts of
+    thousands of slots) and a seventh of the other entries."""
+    rng = np.random.default_rng(seed)
+    n = off.size - 1
+    s0, _ = coo(off, adj)
+    pick = np.zeros(adj.size, bool)
+    pick[::7] = True
+    for v in hubs:
+        deg = int(off[v + 1] - off[v])
+        pick[off[v]:off[v + 1]] = False
+        pick[off[v] + rng.choice(deg, int(deg * frac), replace=False)] = True
+    idx = rng.permutation(np.nonzero(pick)[0])
+    return s0[idx].astype(np.int32), adj[idx].astype(np.int32)
+
+
+def test_erase_hub_rows_in_place(hn):
+    """Long rows compacted in place over many copy tiles, where most tiles start
+    behind a deleted range of their row; two rounds, the second on the
+    compacted rows."""
+    print("case erase hub rows in place", flush=True)
+    off, adj, hubs = hub_graph(3)
+    h = hn.Hornet.from_csr(off, adj)
+    for r in range(2):
+        es, ed = hub_erase_batch(off, adj, hubs, 10 + r, (0.4, 0.08)[r])
+        got = h.erase(hn.BatchUpdate(es, ed))
+        off, adj, removed = O.erase(off, adj, es, ed)
+        assert got == removed
+        check_equal(h, off, adj)
