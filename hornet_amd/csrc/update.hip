@@ -98,33 +98,35 @@ __device__ __forceinline__ bool has_key_u64(const uint64_t* a, int64_t n, uint64
     return lo < n && a[lo] == x;
 }
 
-// keep flag for the sorted batch (insert): in-batch dedup and graph dedup
-struct InsertKeepIn {
-    const uint64_t* k;
-    int64_t n;
-    KeyFmt f;
-    int dedup_batch, dedup_graph;
-    const hn_vrec* rec;
-    const int32_t* arena;
-    __device__ __forceinline__ uint32_t operator()(int64_t i) const {
-        const uint64_t x = k[i];
-        const bool first = (i == 0) || k[i - 1] != x;
-        if (dedup_batch && !first) return 0u;
-        if (dedup_graph && first) {
-            const hn_vrec r = rec[f.src(x)];
-            const int32_t d = f.dst(x);
-            const int64_t p = lower_bound_i32(arena + r.off, r.deg, d);
-            if (p < r.deg && arena[r.off + p] == d) return 0u;
-        }
-        return 1u;
-    }
-};
-
-// the keep flags written once (the graph lookups of removeGraphDuplicates would
-// otherwise run in both passes of the scan)
-__global__ void insert_keep_kernel(int64_t n, InsertKeepIn kin, uint32_t* __restrict__ flags) {
+// Per sorted batch entry (u, d) of an insert: ub = #(A_u <= d) on u's stored
+// list A_u, found once and used three times -- graph dedup (stored iff
+// A_u[ub - 1] == d), the first slot of A_u that moves (ub of u's smallest
+// kept entry) and the entry's slot in the merged list (q + ub for the q-th
+// kept entry of u, stored copies first on ties).  Keep flag: in-batch dedup
+// keeps the first copy of a key; graph dedup drops the first batch copy of a
+// stored pair (lower_bound_left, BatchUpdateKernels.cuh:27-58).
+__global__ void insert_locate_kernel(int64_t n, const uint64_t* __restrict__ keys, KeyFmt f, int dedup_batch,
+                                     int dedup_graph, const hn_vrec* __restrict__ rec,
+                                     const int32_t* __restrict__ arena, uint32_t* __restrict__ keep,
+                                     uint32_t* __restrict__ ub, const UpdScal* sc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flags[i] = kin(i);
+    if (i >= n) return;
+    if (sc->err) {   // ids were out of range: nothing is inserted
+        keep[i] = 0;
+        ub[i] = 0;
+        return;
+    }
+    const uint64_t x = keys[i];
+    const bool first = (i == 0) || keys[i - 1] != x;
+    const hn_vrec r = rec[f.src(x)];
+    const int32_t d = f.dst(x);
+    const int32_t* row = arena + r.off;
+    const int64_t lb = lower_bound_i32(row, r.deg, d);
+    const bool stored = lb < r.deg && row[lb] == d;
+    // d < nv <= 2^31 - 1, so d + 1 does not overflow
+    const int64_t u = stored ? lb + lower_bound_i32(row + lb, r.deg - lb, d + 1) : lb;
+    keep[i] = ((dedup_batch && !first) || (dedup_graph && first && stored)) ? 0u : 1u;
+    ub[i] = (uint32_t)u;
 }
 
 // keep flag for the sorted batch (erase): optional in-batch dedup
@@ -136,13 +138,16 @@ struct FirstKeepIn {
     }
 };
 
+// kept keys (and, when given, the values of one or two parallel arrays)
 __global__ void compact_keys_kernel(int64_t n, const uint64_t* __restrict__ in, const uint32_t* __restrict__ pos,
                                     uint64_t* __restrict__ out, const uint32_t* __restrict__ iin,
-                                    uint32_t* __restrict__ iout) {
+                                    uint32_t* __restrict__ iout, const uint32_t* __restrict__ jin,
+                                    uint32_t* __restrict__ jout) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && pos[i + 1] != pos[i]) {
         out[pos[i]] = in[i];
         if (iin) iout[pos[i]] = iin[i];
+        if (jin) jout[pos[i]] = jin[i];
     }
 }
 
@@ -214,9 +219,8 @@ __device__ __forceinline__ AView a_view(const TouchInfo& t, const int32_t* arena
 // per touched source j: new degree, in place or a new block (no allocation)
 __device__ __forceinline__ void insert_touch(int64_t j, const uint32_t* __restrict__ seg_start,
                                              const uint64_t* __restrict__ seg_key,
-                                             const uint64_t* __restrict__ keys, KeyFmt f,
-                                             const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena,
-                                             TouchInfo& t) {
+                                             const uint32_t* __restrict__ ubk, KeyFmt f,
+                                             const hn_vrec* __restrict__ rec, TouchInfo& t) {
     const int32_t v = f.src(seg_key[j]);
     const hn_vrec r = rec[v];
     const int32_t kj = (int32_t)(seg_start[j + 1] - seg_start[j]);
@@ -229,8 +233,8 @@ __device__ __forceinline__ void insert_touch(int64_t j, const uint32_t* __restri
     if (t.inplace) {
         t.new_cap = r.cap_log2;
         t.new_off = r.off;
-        // old entries below the smallest batch value keep their slots
-        t.p = (int32_t)lower_bound_i32(arena + r.off, r.deg, f.dst(keys[seg_start[j]]));
+        // old entries up to the smallest batch value keep their slots
+        t.p = (int32_t)ubk[seg_start[j]];
     } else {
         t.new_cap = class_of(t.new_deg);
         t.new_off = 0;
@@ -244,13 +248,13 @@ constexpr int ALLOC_BLOCK = 1024;
 
 __global__ __launch_bounds__(ALLOC_BLOCK) void insert_alloc_kernel(
     const uint32_t* __restrict__ ucnt, const uint32_t* __restrict__ seg_start, const uint64_t* __restrict__ seg_key,
-    const uint64_t* __restrict__ keys, KeyFmt f, const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena,
-    AllocView A, TouchInfo* __restrict__ ti, const UpdScal* sc) {
+    const uint32_t* __restrict__ ubk, KeyFmt f, const hn_vrec* __restrict__ rec, AllocView A,
+    TouchInfo* __restrict__ ti, const UpdScal* sc) {
     __shared__ AllocShared ash;
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = j < (int64_t)*ucnt && !sc->err;   // every lane reaches the allocation below
     TouchInfo t{};
-    if (active) insert_touch(j, seg_start, seg_key, keys, f, rec, arena, t);
+    if (active) insert_touch(j, seg_start, seg_key, ubk, f, rec, t);
     const bool need = active && !t.inplace;
     const int64_t blk = alloc_block_group(A, need ? t.new_cap : 0, need, ash);
     if (!active) return;
@@ -441,7 +445,8 @@ struct InsertWorkIn {
 __global__ __launch_bounds__(BLOCK) void insert_merge_kernel(int64_t nseg, const int64_t* __restrict__ P,
                                                              const TouchInfo* __restrict__ ti,
                                                              const uint32_t* __restrict__ seg_start,
-                                                             const uint64_t* __restrict__ keys, KeyFmt f,
+                                                             const uint64_t* __restrict__ keys,
+                                                             const uint32_t* __restrict__ ubk, KeyFmt f,
                                                              int32_t* __restrict__ arena,
                                                              const int32_t* __restrict__ stage, const UpdScal* sc,
                                                              MetaSet am, MetaSet sm, MetaSet bm,
@@ -472,12 +477,7 @@ __global__ __launch_bounds__(BLOCK) void insert_merge_kernel(int64_t nseg, const
                 } else {
                     const int64_t q = r - na;
                     val = f.dst(B[q]);
-                    int64_t lo = 0, hi = na;   // #(A <= val)
-                    while (lo < hi) {
-                        int64_t mid = (lo + hi) >> 1;
-                        if (A[mid] <= val) lo = mid + 1; else hi = mid;
-                    }
-                    dest = q + lo;
+                    dest = q + ubk[seg_start[j] + q];   // q + #(A <= val)
                 }
                 arena[t.new_off + dest] = val;
                 if (am.any()) {
@@ -1125,16 +1125,16 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     uint64_t* keys = nullptr;
     HN_TRY(load_and_sort(g, src, dst, n, flags, b, f, st, &keys));
     const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
-    // 1. keep flags (batch / graph dedup) -> compact into kb
-    InsertKeepIn kin{keys, n, f, (flags & HN_DEDUP_BATCH) ? 1 : 0, (flags & HN_DEDUP_GRAPH) ? 1 : 0, g->rec, g->arena};
-    if (flags & HN_DEDUP_GRAPH) {
-        insert_keep_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(n, kin, b.dpos);   // dpos: free in insert
-        HN_CHECK_LAUNCH();
-        HN_TRY(exclusive_scan<uint32_t>(ArrayIn<uint32_t>{b.dpos}, b.pos, n, b.tmp, st));
-    } else {
-        HN_TRY(exclusive_scan<uint32_t>(kin, b.pos, n, b.tmp, st));
-    }
-    compact_keys_kernel<<<gridn, BLOCK, 0, st>>>(n, keys, b.pos, b.kb, b.ia, b.ib);
+    // 1. place in the stored list and keep flag (batch / graph dedup) of every
+    // entry -> compact into kb (keep flags in dpos, places in dlen, kept places
+    // in dlp: the erase arrays are free in insert)
+    insert_locate_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(
+        n, keys, f, (flags & HN_DEDUP_BATCH) ? 1 : 0, (flags & HN_DEDUP_GRAPH) ? 1 : 0, g->rec, g->arena, b.dpos,
+        b.dlen, b.sc);
+    HN_CHECK_LAUNCH();
+    HN_TRY(exclusive_scan<uint32_t>(ArrayIn<uint32_t>{b.dpos}, b.pos, n, b.tmp, st));
+    uint32_t* ubk = b.dlp;
+    compact_keys_kernel<<<gridn, BLOCK, 0, st>>>(n, keys, b.pos, b.kb, b.ia, b.ib, b.dlen, ubk);
     HN_CHECK_LAUNCH();
     uint32_t* nb = b.pos + n;   // entries kept
     // 2. segments by source
@@ -1143,7 +1143,7 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     HN_CHECK_LAUNCH();
     uint32_t* ucnt = b.spos + n;
     // 3. in place or new blocks
-    insert_alloc_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(ucnt, b.seg_start, b.seg_key, b.kb, f, g->rec, g->arena,
+    insert_alloc_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(ucnt, b.seg_start, b.seg_key, ubk, f, g->rec,
                                                  g->alloc_view(), b.ti, b.sc);
     HN_CHECK_LAUNCH();
     alloc_fix_kernel<<<1, 1, 0, st>>>(g->alloc, b.sc);
@@ -1153,7 +1153,7 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     // 4. merge the moving old entries and the batch segment
     HN_TRY(exclusive_scan<int64_t>(InsertWorkIn{b.ti, ucnt, b.sc}, b.wpos, n, b.tmp, st));
     const int64_t grid = (int64_t)num_cus() * 8;
-    insert_merge_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.wpos, b.ti, b.seg_start, b.kb, f, g->arena, stage,
+    insert_merge_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.wpos, b.ti, b.seg_start, b.kb, ubk, f, g->arena, stage,
                                                           b.sc, g->meta, b.sm, bm.m, b.ib);
     HN_CHECK_LAUNCH();
     // 5. publish
@@ -1192,7 +1192,7 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
     // 1. optional in-batch dedup -> kb
     HN_TRY(exclusive_scan<uint32_t>(FirstKeepIn{keys, (flags & HN_DEDUP_BATCH) ? 1 : 0}, b.pos, n, b.tmp, st));
-    compact_keys_kernel<<<gridn, BLOCK, 0, st>>>(n, keys, b.pos, b.kb, nullptr, nullptr);
+    compact_keys_kernel<<<gridn, BLOCK, 0, st>>>(n, keys, b.pos, b.kb, nullptr, nullptr, nullptr, nullptr);
     HN_CHECK_LAUNCH();
     uint32_t* nb = b.pos + n;
     // 2. distinct keys with multiplicities
