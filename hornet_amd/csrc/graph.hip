@@ -276,6 +276,7 @@ static void destroy_graph(hn_graph* g) {
     g->s_upd.release();
     g->s_tc.release();
     g->s_mstage.release();
+    g->s_stamp.release();
     g->s_misc.release();
     if (g->h_pinned) (void)hipHostFree(g->h_pinned);
     delete g;

@@ -102,23 +102,21 @@ __device__ __forceinline__ int64_t alloc_block_group(const AllocView& A, int k, 
     return res;
 }
 
-// Push freed blocks onto their class stacks: one atomicAdd per (wave, class)
-// on the stack depth.  Every active lane of the wave calls it (lanes that free
-// nothing pass want = false).
-__device__ __forceinline__ void free_block_wave(const AllocView& A, int k, int64_t slot_off, bool want) {
-    const int lane = __lane_id();
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    uint64_t pending = (uint64_t)__ballot(want);
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const int kl = __shfl(k, leader, 64);
-        const uint64_t grp = (uint64_t)__ballot(want && k == kl);
-        unsigned base = 0;
-        if (lane == leader) base = atomicAdd(&A.st->free_count[kl], (unsigned)__popcll(grp));
-        base = __shfl(base, leader, 64);
-        if (want && k == kl) A.stacks[A.stack_off[kl] + base + (unsigned)__popcll(grp & lt)] = (uint32_t)(slot_off / UNIT_SLOTS);
-        pending &= ~grp;
-    }
+// Push freed blocks onto their class stacks with one atomicAdd per class for
+// the whole workgroup (counts in LDS), as alloc_block_group takes them.  Every
+// thread of the workgroup calls it (threads that free nothing pass want =
+// false).
+__device__ __forceinline__ void free_block_group(const AllocView& A, int k, int64_t slot_off, bool want,
+                                                 AllocShared& sh) {
+    for (int i = threadIdx.x; i < NCLASS; i += blockDim.x) sh.cnt[i] = 0;
+    __syncthreads();
+    unsigned int lt = 0;
+    if (want) lt = atomicAdd(&sh.cnt[k], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < NCLASS; i += blockDim.x)
+        sh.base[i] = sh.cnt[i] ? atomicAdd(&A.st->free_count[i], sh.cnt[i]) : 0u;
+    __syncthreads();
+    if (want) A.stacks[A.stack_off[k] + sh.base[k] + lt] = (uint32_t)(slot_off / UNIT_SLOTS);
 }
 
 }  // namespace hn
@@ -177,6 +175,7 @@ struct hn_graph {
     hn::Scratch s_tc;                 // triangle counting
     hn::Scratch s_misc;
     hn::Scratch s_mstage;             // staging of edge meta-data during updates
+    hn::Scratch s_stamp;              // per vertex: its source segment in the current batch (unchecked)
     uint64_t* h_pinned = nullptr;     // small pinned readback area
     void* ms = nullptr;               // multi-source BFS engine (msbfs.hip)
     void (*ms_release)(void*) = nullptr;

@@ -249,12 +249,13 @@ constexpr int ALLOC_BLOCK = 1024;
 __global__ __launch_bounds__(ALLOC_BLOCK) void insert_alloc_kernel(
     const uint32_t* __restrict__ ucnt, const uint32_t* __restrict__ seg_start, const uint64_t* __restrict__ seg_key,
     const uint32_t* __restrict__ ubk, KeyFmt f, const hn_vrec* __restrict__ rec, AllocView A,
-    TouchInfo* __restrict__ ti, const UpdScal* sc) {
+    TouchInfo* __restrict__ ti, uint32_t* __restrict__ stamp, const UpdScal* sc) {
     __shared__ AllocShared ash;
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = j < (int64_t)*ucnt && !sc->err;   // every lane reaches the allocation below
     TouchInfo t{};
     if (active) insert_touch(j, seg_start, seg_key, ubk, f, rec, t);
+    if (active && stamp) stamp[f.src(seg_key[j])] = (uint32_t)j;
     const bool need = active && !t.inplace;
     const int64_t blk = alloc_block_group(A, need ? t.new_cap : 0, need, ash);
     if (!active) return;
@@ -495,22 +496,30 @@ __global__ __launch_bounds__(BLOCK) void insert_merge_kernel(int64_t nseg, const
 }
 
 // publish the new records and release the blocks that were left
-__global__ void commit_kernel(const uint32_t* __restrict__ ucnt, const uint64_t* __restrict__ seg_key, KeyFmt f,
-                              const TouchInfo* __restrict__ ti, const uint8_t* __restrict__ skip,
-                              hn_vrec* __restrict__ rec, AllocView A, const UpdScal* sc) {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= (int64_t)*ucnt || sc->err) return;   // a suffix of the wave
-    const bool live = !(skip && skip[j]);
-    const TouchInfo t = ti[j];
-    if (live) {
-        hn_vrec r;
-        r.off = t.new_cap >= 0 ? t.new_off : 0;
-        r.deg = t.new_deg;
-        r.cap_log2 = t.new_cap;
-        rec[f.src(seg_key[j])] = r;
+__global__ __launch_bounds__(ALLOC_BLOCK) void commit_kernel(const uint32_t* __restrict__ ucnt,
+                                                             const uint64_t* __restrict__ seg_key, KeyFmt f,
+                                                             const TouchInfo* __restrict__ ti,
+                                                             const uint8_t* __restrict__ skip,
+                                                             hn_vrec* __restrict__ rec, AllocView A,
+                                                             const UpdScal* sc) {
+    __shared__ AllocShared ash;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = j < (int64_t)*ucnt && !sc->err;   // every thread reaches the release below
+    bool rel = false;
+    TouchInfo t{};
+    if (active) {
+        const bool live = !(skip && skip[j]);
+        t = ti[j];
+        if (live) {
+            hn_vrec r;
+            r.off = t.new_cap >= 0 ? t.new_off : 0;
+            r.deg = t.new_deg;
+            r.cap_log2 = t.new_cap;
+            rec[f.src(seg_key[j])] = r;
+        }
+        rel = live && !t.inplace && t.old_cap >= 0;
     }
-    const bool rel = live && !t.inplace && t.old_cap >= 0;
-    free_block_wave(A, rel ? t.old_cap : 0, t.old_off, rel);
+    free_block_group(A, rel ? t.old_cap : 0, t.old_off, rel, ash);
 }
 
 // ---------------------------------------------------------------- erase
@@ -595,13 +604,14 @@ __global__ __launch_bounds__(ALLOC_BLOCK) void erase_alloc_kernel(
     const uint32_t* __restrict__ ucnt, const uint32_t* __restrict__ ustart, const uint64_t* __restrict__ ukey,
     KeyFmt f, const uint32_t* __restrict__ dpos, const uint32_t* __restrict__ dlen, const uint32_t* __restrict__ dlp,
     const hn_vrec* __restrict__ rec, AllocView A, TouchInfo* __restrict__ ti, uint8_t* __restrict__ skip,
-    const UpdScal* sc) {
+    uint32_t* __restrict__ stamp, const UpdScal* sc) {
     __shared__ AllocShared ash;
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = u < (int64_t)*ucnt && !sc->err;   // every lane reaches the allocation below
     TouchInfo t{};
     bool need = false;
     if (active) need = erase_touch(u, ustart, ukey, f, dpos, dlen, dlp, rec, t, skip);
+    if (active && stamp) stamp[f.src(ukey[u])] = (uint32_t)u;
     const int64_t blk = alloc_block_group(A, need ? t.new_cap : 0, need, ash);
     if (!active) return;
     if (need) t.new_off = blk;
@@ -989,18 +999,34 @@ __global__ void store_changed_kernel(const uint32_t* __restrict__ total, UpdScal
 // their membership, so the adjacency stays symmetric iff every batch pair
 // (u,w) has (u,w) and (w,u) both stored or both absent.
 //
+// The batch entries with source w are found through stamp[w], the index of
+// w's source segment written by this batch's allocation kernel: valid when it
+// is below the segment count and that segment's key has source w (stale
+// values from earlier batches fail the test, so the array is never cleared).
+__device__ __forceinline__ bool batch_has(const uint32_t* __restrict__ stamp, const uint32_t* __restrict__ seg_start,
+                                          const uint64_t* __restrict__ seg_key, const uint32_t* __restrict__ nseg,
+                                          const uint64_t* __restrict__ keys, KeyFmt f, int32_t w, int32_t u,
+                                          int64_t* at) {
+    const uint32_t s = stamp[w];
+    if (s >= *nseg || f.src(seg_key[s]) != w) return false;
+    const int64_t lo = seg_start[s];
+    const bool found = has_key_u64(keys + lo, (int64_t)seg_start[s + 1] - lo, ((uint64_t)w << f.sbits) | (uint64_t)u, at);
+    *at += lo;
+    return found;
+}
+
 // Insert: every kept pair (u,w) is stored.  (w,u) is stored too when the kept
-// batch holds it (the sorted keys, searched in cache); only otherwise is it
-// looked up in w's list.
+// batch holds it; only otherwise is it looked up in w's list.
 __global__ void insert_symmetry_kernel(const uint32_t* __restrict__ count, const uint64_t* __restrict__ keys,
-                                       KeyFmt f, const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena,
-                                       UpdScal* sc) {
+                                       KeyFmt f, const uint32_t* __restrict__ stamp,
+                                       const uint32_t* __restrict__ seg_start, const uint64_t* __restrict__ seg_key,
+                                       const uint32_t* __restrict__ nseg, const hn_vrec* __restrict__ rec,
+                                       const int32_t* __restrict__ arena, UpdScal* sc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = (int64_t)*count;
-    if (i >= n || sc->err) return;
+    if (i >= (int64_t)*count || sc->err) return;
     const int32_t u = f.src(keys[i]), w = f.dst(keys[i]);
     int64_t at;
-    if (has_key_u64(keys, n, ((uint64_t)w << f.sbits) | (uint64_t)u, &at)) return;
+    if (batch_has(stamp, seg_start, seg_key, nseg, keys, f, w, u, &at)) return;
     const hn_vrec rw = rec[w];
     const int64_t pw = lower_bound_i32(arena + rw.off, rw.deg, u);
     if (!(pw < rw.deg && arena[rw.off + pw] == u)) sc->asym = 1u;   // every writer stores the same value
@@ -1012,14 +1038,25 @@ __global__ void insert_symmetry_kernel(const uint32_t* __restrict__ count, const
 // (u,w) was stored before, i.e. iff (u,w) deleted something.  No list is read.
 __global__ void erase_symmetry_kernel(const uint32_t* __restrict__ dcnt, const uint64_t* __restrict__ dkey, KeyFmt f,
                                       const uint32_t* __restrict__ dlen, const uint8_t* __restrict__ live,
+                                      const uint32_t* __restrict__ stamp, const uint32_t* __restrict__ ustart,
+                                      const uint64_t* __restrict__ ukey, const uint32_t* __restrict__ ucnt,
                                       UpdScal* sc) {
     const int64_t jj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = (int64_t)*dcnt;
-    if (jj >= n || sc->err) return;
+    if (jj >= (int64_t)*dcnt || sc->err) return;
     const int32_t u = f.src(dkey[jj]), w = f.dst(dkey[jj]);
     int64_t at;
-    const bool b = has_key_u64(dkey, n, ((uint64_t)w << f.sbits) | (uint64_t)u, &at) ? live[at] != 0 : dlen[jj] > 0;
+    const bool b = batch_has(stamp, ustart, ukey, ucnt, dkey, f, w, u, &at) ? live[at] != 0 : dlen[jj] > 0;
     if ((live[jj] != 0) != b) sc->asym = 1u;
+}
+
+// the per-vertex segment stamps, when the graph is known symmetric (the only
+// case whose batches check symmetry); null otherwise
+static hn_status symmetry_stamp(hn_graph* g, uint32_t** stamp) {
+    *stamp = nullptr;
+    if (g->sym != 1) return HN_OK;
+    HN_TRY(g->s_stamp.reserve(sizeof(uint32_t) * (size_t)(g->nv > 0 ? g->nv : 1)));
+    *stamp = g->s_stamp.as<uint32_t>();
+    return HN_OK;
 }
 
 // symmetry bookkeeping after a successful batch (was_sym: known symmetric before)
@@ -1143,8 +1180,10 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     HN_CHECK_LAUNCH();
     uint32_t* ucnt = b.spos + n;
     // 3. in place or new blocks
-    insert_alloc_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(ucnt, b.seg_start, b.seg_key, ubk, f, g->rec,
-                                                 g->alloc_view(), b.ti, b.sc);
+    uint32_t* stamp = nullptr;
+    HN_TRY(symmetry_stamp(g, &stamp));
+    insert_alloc_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(
+        ucnt, b.seg_start, b.seg_key, ubk, f, g->rec, g->alloc_view(), b.ti, stamp, b.sc);
     HN_CHECK_LAUNCH();
     alloc_fix_kernel<<<1, 1, 0, st>>>(g->alloc, b.sc);
     HN_CHECK_LAUNCH();
@@ -1157,13 +1196,14 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
                                                           b.sc, g->meta, b.sm, bm.m, b.ib);
     HN_CHECK_LAUNCH();
     // 5. publish
-    commit_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg_key, f, b.ti, nullptr, g->rec, g->alloc_view(), b.sc);
+    commit_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(ucnt, b.seg_key, f, b.ti, nullptr, g->rec, g->alloc_view(), b.sc);
     HN_CHECK_LAUNCH();
     store_changed_kernel<<<1, 1, 0, st>>>(nb, b.sc);
     HN_CHECK_LAUNCH();
     const bool was_sym = g->sym == 1;
     if (was_sym) {
-        insert_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, g->rec, g->arena, b.sc);
+        insert_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, stamp, b.seg_start, b.seg_key, ucnt, g->rec,
+                                                        g->arena, b.sc);
         HN_CHECK_LAUNCH();
     }
     UpdScal h;
@@ -1211,8 +1251,10 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     HN_CHECK_LAUNCH();
     uint32_t* ucnt = b.tpos + n;
     // 5. in place, smaller block, or no block
-    erase_alloc_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(ucnt, b.seg2_start, b.seg2_key, f, b.dpos, b.dlen, b.dlp, g->rec,
-                                                g->alloc_view(), b.ti, b.skip, b.sc);
+    uint32_t* stamp = nullptr;
+    HN_TRY(symmetry_stamp(g, &stamp));
+    erase_alloc_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(
+        ucnt, b.seg2_start, b.seg2_key, f, b.dpos, b.dlen, b.dlp, g->rec, g->alloc_view(), b.ti, b.skip, stamp, b.sc);
     HN_CHECK_LAUNCH();
     alloc_fix_kernel<<<1, 1, 0, st>>>(g->alloc, b.sc);
     HN_CHECK_LAUNCH();
@@ -1245,13 +1287,14 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
         HN_CHECK_LAUNCH();
     }
     // 8. publish
-    commit_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.seg2_key, f, b.ti, b.skip, g->rec, g->alloc_view(), b.sc);
+    commit_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(ucnt, b.seg2_key, f, b.ti, b.skip, g->rec, g->alloc_view(), b.sc);
     HN_CHECK_LAUNCH();
     store_changed_kernel<<<1, 1, 0, st>>>(b.dlp + n, b.sc);
     HN_CHECK_LAUNCH();
     const bool was_sym = g->sym == 1;
     if (was_sym) {
-        erase_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, b.seg_key, f, b.dlen, b.live, b.sc);
+        erase_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, b.seg_key, f, b.dlen, b.live, stamp, b.seg2_start,
+                                                       b.seg2_key, ucnt, b.sc);
         HN_CHECK_LAUNCH();
     }
     UpdScal h;
