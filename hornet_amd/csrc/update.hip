@@ -18,17 +18,19 @@
 // Redesign for the device: one 64-bit LSD radix sort of packed (src,dst) keys,
 // flag + scan compactions, and every touched vertex rewritten in place (when
 // its block still fits) or into a block from the device allocator:
-//   insert  each element of old list A and batch segment B finds its place in
-//           merge(A, B) with one binary search in the other list (stored
+//   insert  every batch entry is located once in its stored list A; the old
+//           entries from the first one that moves go to i + #(B < A[i]), in
+//           place in one pass, then batch entry q to q + #(A <= B[q]) (stored
 //           copies first on ties), so adjacency stays sorted;
 //   erase   survivors are compacted in order, moved as runs between deleted
 //           ranges (one run per distinct deleted key plus a tail run per
-//           vertex), in place in one pass; the few entries whose slot an
-//           earlier copy tile may not have read yet go through a side buffer.
-// A device error word (bad vertex id, arena exhausted) turns every later
-// kernel into a no-op and rolls the allocator back, so a failed batch leaves
-// the graph unchanged.  Three host synchronisations per batch (staging or
-// side-buffer size, error word, final status).
+//           vertex), in place in one pass.
+// In both passes the few entries whose slot a neighbouring copy tile may not
+// have read yet go through a small side buffer.  A device error word (bad
+// vertex id, arena exhausted) turns every later kernel into a no-op and rolls
+// the allocator back, so a failed batch leaves the graph unchanged.  Three
+// host synchronisations per batch (side-buffer size, error word, final
+// status).
 #include "graph.hpp"
 #include "prims.hpp"
 
@@ -37,8 +39,6 @@ namespace {
 
 constexpr uint32_t ERR_BAD_ID = 1u;
 constexpr uint32_t ERR_OOM = 2u;     // AllocState::error bit 0 is mirrored here
-constexpr int UPD_IPT = 8;
-constexpr int UPD_TILE = BLOCK * UPD_IPT;
 // Segmented copies: 4 items per lane, at most BLOCK + 1 runs staged in LDS
 // per tile (a tile with more runs searches them in global memory), 8
 // workgroups per CU striding over the tiles.  A one-entry-per-item LDS stage
@@ -184,12 +184,10 @@ __global__ void seg_write_kernel(int64_t n, const uint64_t* __restrict__ k, cons
 // fits: insert new_deg <= capacity, the reference's `new_degree > limit`
 // test inverted, BatchUpdateKernels.cuh:180-186; erase keeps the size class)
 // or relocated into a block from the device allocator.  In place, only the
-// suffix of the old list that moves -- from `p` on -- is first staged in
-// scratch, so the rewrite never reads a slot it has already overwritten.
+// part of the old list from `p` on moves.
 struct TouchInfo {
     int64_t old_off;
     int64_t new_off;   // == old_off in place
-    int64_t soff;      // staging offset of A[p..old_deg) (in place only)
     int32_t old_deg;
     int32_t new_deg;
     int32_t old_cap;
@@ -197,23 +195,6 @@ struct TouchInfo {
     int32_t p;         // first old slot that moves (0 when relocating)
     int32_t inplace;
 };
-
-// A[i] of a touched vertex during the rewrite: the unchanged prefix from the
-// block, the moving suffix from the staging copy (or the old block)
-struct AView {
-    const int32_t* lo;   // i < p
-    const int32_t* hi;   // i >= p, indexed by i
-    int64_t p;
-    __device__ __forceinline__ int32_t operator[](int64_t i) const { return i < p ? lo[i] : hi[i]; }
-};
-
-__device__ __forceinline__ AView a_view(const TouchInfo& t, const int32_t* arena, const int32_t* stage) {
-    AView a;
-    a.lo = arena + t.old_off;
-    a.hi = t.inplace ? stage + t.soff - t.p : arena + t.old_off;
-    a.p = t.p;
-    return a;
-}
 
 // insert: per touched source j -- new degree, in place or a new block
 // per touched source j: new degree, in place or a new block (no allocation)
@@ -228,7 +209,6 @@ __device__ __forceinline__ void insert_touch(int64_t j, const uint32_t* __restri
     t.old_deg = r.deg;
     t.old_cap = r.cap_log2;
     t.new_deg = r.deg + kj;
-    t.soff = 0;
     t.inplace = (r.cap_log2 >= 0 && (int64_t)t.new_deg <= (1ll << r.cap_log2)) ? 1 : 0;
     if (t.inplace) {
         t.new_cap = r.cap_log2;
@@ -286,19 +266,6 @@ __global__ void alloc_fix_kernel(AllocState* st, UpdScal* sc) {
     st->saved_bump = st->bump_units;
 }
 
-// staged length of touched vertex j (moving suffix of an in-place rewrite)
-struct StageLenIn {
-    const TouchInfo* ti;
-    const uint32_t* ucnt;
-    const UpdScal* sc;
-    const uint8_t* skip;
-    __device__ __forceinline__ int64_t operator()(int64_t j) const {
-        if (j >= (int64_t)*ucnt || sc->err || (skip && skip[j])) return 0;
-        const TouchInfo& t = ti[j];
-        return t.inplace ? (int64_t)(t.old_deg - t.p) : 0;
-    }
-};
-
 // Scan output of the segmented copies' run lengths: P[i] = first item of run
 // i, and tile_lo[t] = the run holding item t * COPY_TILE, so a copy tile finds
 // its runs without a search (a run spanning several tiles writes them all).
@@ -318,26 +285,14 @@ struct RunOut {
 // GAP_STAGE set into the staging buffer.
 constexpr int64_t GAP_STAGE = (int64_t)1 << 62;
 
-// the staged suffix of touched vertex j starts at spre[j]; its copy run
-// reads the old list from slot p
-__global__ void set_soff_kernel(const uint32_t* __restrict__ ucnt, const int64_t* __restrict__ spre,
-                                TouchInfo* __restrict__ ti, int64_t* __restrict__ rsrc, int64_t* __restrict__ rdst,
-                                const UpdScal* sc) {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= (int64_t)*ucnt || sc->err) return;
-    ti[j].soff = spre[j];
-    const TouchInfo t = ti[j];
-    rsrc[j] = t.old_off + t.p;
-    rdst[j] = GAP_STAGE | spre[j];
-}
-
-// stage[soff_j + i] = A_j[p_j + i]
 // Segmented copy: run s moves P[s+1] - P[s] entries from index src[s] on to
 // index dst[s] on.  A lane's items are located first, then all loads issue
 // (unconditionally: a lane past the end reads arena[0]), then all stores.
+constexpr int COPY_CAP = LbShared<COPY_CAP_IPT, int64_t>::CAP;   // runs staged per copy tile
+
 struct CopyShared {
     LbShared<COPY_CAP_IPT, int64_t> lb;   // base = src[s] - P[s]
-    int64_t dst[LbShared<COPY_CAP_IPT, int64_t>::CAP];
+    int64_t dst[COPY_CAP];
 };
 
 // a tile's run range [lo, hi] (hi at or past the run holding its last item)
@@ -430,71 +385,6 @@ __global__ __launch_bounds__(BLOCK) void copy_runs_kernel(int64_t nruns, const i
     }
 }
 
-// items of the insert rewrite: moving old entries and the batch segment
-struct InsertWorkIn {
-    const TouchInfo* ti;
-    const uint32_t* ucnt;
-    const UpdScal* sc;
-    __device__ __forceinline__ int64_t operator()(int64_t j) const {
-        if (j >= (int64_t)*ucnt || sc->err) return 0;
-        return (int64_t)(ti[j].new_deg - ti[j].p);
-    }
-};
-
-// merge(A, B): A[i] -> i + #(B < A[i]); B[q] -> q + #(A <= B[q]) (stored
-// copies first on ties); only A[p..) and B are moved
-__global__ __launch_bounds__(BLOCK) void insert_merge_kernel(int64_t nseg, const int64_t* __restrict__ P,
-                                                             const TouchInfo* __restrict__ ti,
-                                                             const uint32_t* __restrict__ seg_start,
-                                                             const uint64_t* __restrict__ keys,
-                                                             const uint32_t* __restrict__ ubk, KeyFmt f,
-                                                             int32_t* __restrict__ arena,
-                                                             const int32_t* __restrict__ stage, const UpdScal* sc,
-                                                             MetaSet am, MetaSet sm, MetaSet bm,
-                                                             const uint32_t* __restrict__ bidx) {
-    __shared__ LbShared<UPD_IPT, int64_t> sh;
-    if (sc->err) return;
-    const int64_t W = P[nseg];
-    for (int64_t w0 = (int64_t)blockIdx.x * UPD_TILE; w0 < W; w0 += (int64_t)gridDim.x * UPD_TILE) {
-        lb_tile<UPD_IPT>(
-            P, nseg, w0, W, sh, [&](int64_t s) { return P[s]; },
-            [&](int64_t j, int64_t w, int64_t pj) {
-                const TouchInfo t = ti[j];
-                const AView A = a_view(t, arena, stage);
-                const int64_t na = t.old_deg;
-                const uint64_t* B = keys + seg_start[j];
-                const int64_t nb = t.new_deg - t.old_deg;
-                const int64_t r = t.p + (w - pj);
-                int64_t dest;
-                int32_t val;
-                if (r < na) {
-                    val = A[r];
-                    int64_t lo = 0, hi = nb;   // #(B < val)
-                    while (lo < hi) {
-                        int64_t mid = (lo + hi) >> 1;
-                        if (f.dst(B[mid]) < val) lo = mid + 1; else hi = mid;
-                    }
-                    dest = r + lo;
-                } else {
-                    const int64_t q = r - na;
-                    val = f.dst(B[q]);
-                    dest = q + ubk[seg_start[j] + q];   // q + #(A <= val)
-                }
-                arena[t.new_off + dest] = val;
-                if (am.any()) {
-                    // the entry's meta-data: an old entry from where A read it, a
-                    // batch entry from its batch position
-                    if (r < na) {
-                        if (r >= t.p && t.inplace) meta_put(am, t.new_off + dest, sm, t.soff + (r - t.p));
-                        else meta_put(am, t.new_off + dest, am, t.old_off + r);
-                    } else {
-                        meta_put(am, t.new_off + dest, bm, bidx[seg_start[j] + (r - na)]);
-                    }
-                }
-            });
-    }
-}
-
 // publish the new records and release the blocks that were left
 __global__ __launch_bounds__(ALLOC_BLOCK) void commit_kernel(const uint32_t* __restrict__ ucnt,
                                                              const uint64_t* __restrict__ seg_key, KeyFmt f,
@@ -575,7 +465,6 @@ __device__ __forceinline__ bool erase_touch(int64_t u, const uint32_t* __restric
     t.old_deg = r.deg;
     t.old_cap = r.cap_log2;
     t.new_deg = r.deg - (int32_t)del;
-    t.soff = 0;
     t.p = 0;
     t.inplace = 0;
     t.new_off = 0;
@@ -824,6 +713,203 @@ __global__ __launch_bounds__(BLOCK) void erase_copy_kernel(int64_t nruns, const 
     }
 }
 
+// ---------------------------------------------------------------- insert, moves
+// The old entries that move: A_j[p, old_deg) of touched vertex j, items
+// aP[j] .. aP[j + 1] in slot order.  Entry r goes to slot r + #(B_j < A_j[r])
+// of j's block (the same block in place; the new one when relocating, where
+// p = 0); the batch entries fill the slots left between them.
+struct MoveLenIn {
+    const TouchInfo* ti;
+    const uint32_t* ucnt;
+    const UpdScal* sc;
+    __device__ __forceinline__ int64_t operator()(int64_t j) const {
+        if (j >= (int64_t)*ucnt || sc->err) return 0;
+        return (int64_t)(ti[j].old_deg - ti[j].p);
+    }
+};
+
+// #(B < x) over nb sorted batch keys
+__device__ __forceinline__ int64_t count_below(const uint64_t* __restrict__ B, int64_t nb, KeyFmt f, int32_t x) {
+    int64_t lo = 0, hi = nb;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (f.dst(B[mid]) < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// The move mirrors the erase compaction: every moving entry goes up
+// (destination > source), so its destination can only be the source of a
+// later item of its vertex.  A copy tile reads all its sources before it
+// writes; the items of its last vertex in later tiles have sources from
+// source(i1) on (i1 = the next tile's first item).  The tile's last items
+// whose destination reaches source(i1) are deferred: at most
+// c = #(B < A[r_last]) of them, since r + #(B < A[r]) > r_last needs
+// r > r_last - c.  They are placed once every tile has read.
+__global__ void insert_defer_kernel(int64_t ntile_max, int64_t nseg, const int64_t* __restrict__ aP,
+                                    const uint32_t* __restrict__ tile_lo, const TouchInfo* __restrict__ ti,
+                                    const uint32_t* __restrict__ seg_start, const uint64_t* __restrict__ keys,
+                                    KeyFmt f, const int32_t* __restrict__ arena, uint32_t* __restrict__ tcnt,
+                                    const UpdScal* sc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntile_max) return;
+    const int64_t W = aP[nseg];
+    const int64_t ntiles = (W + COPY_TILE - 1) / COPY_TILE;
+    int64_t c = 0;
+    if (!sc->err && t + 1 < ntiles) {
+        const int64_t i1 = (t + 1) * COPY_TILE;
+        const int64_t j = tile_lo[t + 1];   // holds i1
+        if (aP[j] < i1) {                   // and i1 - 1, the tile's last item
+            const TouchInfo tj = ti[j];
+            if (tj.inplace) {
+                const int64_t r_last = tj.p + (i1 - 1 - aP[j]);
+                c = count_below(keys + seg_start[j], tj.new_deg - tj.old_deg, f, arena[tj.old_off + r_last]);
+                const int64_t first = aP[j] > t * COPY_TILE ? aP[j] : t * COPY_TILE;
+                c = c < i1 - first ? c : i1 - first;
+            }
+        }
+    }
+    tcnt[t] = (uint32_t)c;
+}
+
+struct MoveShared {
+    LbShared<COPY_CAP_IPT, int64_t> lb;   // pref = aP[j], base = old_off + p - aP[j]
+    int64_t dbase[COPY_CAP];               // new_off + p - aP[j]
+    uint32_t bstart[COPY_CAP];             // j's batch segment
+    uint32_t bn[COPY_CAP];
+};
+
+// One pass of the move: tile t's last tcnt[t] items go to the side buffer at
+// doff[t] (entry and destination), every other item to its destination.
+// Sources are read (values, then each meta-data field) by the whole tile
+// before any of its writes.
+__global__ __launch_bounds__(BLOCK) void insert_move_kernel(int64_t nseg, const int64_t* __restrict__ aP,
+                                                            const uint32_t* __restrict__ tile_lo,
+                                                            const TouchInfo* __restrict__ ti,
+                                                            const uint32_t* __restrict__ seg_start,
+                                                            const uint64_t* __restrict__ keys, KeyFmt f,
+                                                            const uint32_t* __restrict__ tcnt,
+                                                            const int64_t* __restrict__ doff,
+                                                            int32_t* __restrict__ arena, int32_t* __restrict__ side,
+                                                            int64_t* __restrict__ side_dst, const UpdScal* sc,
+                                                            MetaSet am, MetaSet sm) {
+    __shared__ MoveShared sh;
+    if (sc->err) return;
+    const int64_t W = aP[nseg];
+    const int64_t ntiles = (W + COPY_TILE - 1) / COPY_TILE;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t w0 = tile * COPY_TILE;
+        const int64_t wend = (w0 + (int64_t)COPY_TILE) < W ? (w0 + (int64_t)COPY_TILE) : W;
+        const CopyRange rg = copy_range(aP, nseg, W, ntiles, tile, tile_lo[tile],
+                                        tile + 1 < ntiles ? (int64_t)tile_lo[tile + 1] : 0);
+        if (rg.count > 0)
+            for (int i = threadIdx.x; i < rg.count; i += BLOCK) {
+                const int64_t j = rg.lo + i;
+                const TouchInfo t = ti[j];
+                sh.lb.pref[i] = aP[j];
+                sh.lb.base[i] = t.old_off + t.p - aP[j];
+                sh.dbase[i] = t.new_off + t.p - aP[j];
+                sh.bstart[i] = seg_start[j];
+                sh.bn[i] = (uint32_t)(t.new_deg - t.old_deg);
+            }
+        const int64_t first_def = wend - (int64_t)tcnt[tile];   // items from here on are deferred
+        const int64_t o = doff[tile];
+        __syncthreads();
+        int64_t si[COPY_IPT], di[COPY_IPT];
+        uint32_t bs[COPY_IPT], bn[COPY_IPT];
+#pragma unroll
+        for (int j = 0; j < COPY_IPT; j++) {
+            const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+            si[j] = 0;
+            di[j] = -1;
+            bs[j] = 0;
+            bn[j] = 0;
+            if (w < wend) {
+                if (rg.count > 0) {
+                    const int s = lb_locate<COPY_CAP_IPT>(sh.lb, rg.count, w);
+                    si[j] = sh.lb.base[s] + w;
+                    di[j] = sh.dbase[s] + w;
+                    bs[j] = sh.bstart[s];
+                    bn[j] = sh.bn[s];
+                } else {
+                    const int64_t s = upper_bound_m1<int64_t>(aP, rg.lo, rg.lo + (int64_t)(-rg.count), w);
+                    const TouchInfo t = ti[s];
+                    si[j] = t.old_off + t.p - aP[s] + w;
+                    di[j] = t.new_off + t.p - aP[s] + w;
+                    bs[j] = seg_start[s];
+                    bn[j] = (uint32_t)(t.new_deg - t.old_deg);
+                }
+            }
+        }
+        int32_t v[COPY_IPT];
+#pragma unroll
+        for (int j = 0; j < COPY_IPT; j++) v[j] = arena[si[j]];
+#pragma unroll
+        for (int j = 0; j < COPY_IPT; j++)
+            if (di[j] >= 0) di[j] += count_below(keys + bs[j], bn[j], f, v[j]);
+        // the tile's sources are all read before any slot is rewritten
+        wait_loads();
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < COPY_IPT; j++) {
+            if (di[j] < 0) continue;
+            const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+            if (w >= first_def) {
+                side[o + (w - first_def)] = v[j];
+                side_dst[o + (w - first_def)] = di[j];
+            } else {
+                arena[di[j]] = v[j];
+            }
+        }
+        for (int fl = 0; fl < am.n; fl++) {
+            const bool wide = am.bytes[fl] == 8;
+            uint64_t mv[COPY_IPT];
+#pragma unroll
+            for (int j = 0; j < COPY_IPT; j++)
+                mv[j] = wide ? reinterpret_cast<const uint64_t*>(am.p[fl])[si[j]]
+                             : reinterpret_cast<const uint32_t*>(am.p[fl])[si[j]];
+            wait_loads();
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < COPY_IPT; j++) {
+                if (di[j] < 0) continue;
+                const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+                char* b = w >= first_def ? sm.p[fl] : am.p[fl];
+                const int64_t x = w >= first_def ? o + (w - first_def) : di[j];
+                if (wide) reinterpret_cast<uint64_t*>(b)[x] = mv[j];
+                else reinterpret_cast<uint32_t*>(b)[x] = (uint32_t)mv[j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// the deferred entries (and their fields) to their destinations
+__global__ void insert_place_kernel(int64_t m, const int32_t* __restrict__ side, const int64_t* __restrict__ side_dst,
+                                    int32_t* __restrict__ arena, MetaSet am, MetaSet sm) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int64_t d = side_dst[k];
+    arena[d] = side[k];
+    if (am.any()) meta_put(am, d, sm, k);
+}
+
+// kept batch entry i (the q-th of its source segment j) to slot
+// q + #(A_j <= value) of j's block (stored copies first on ties), with its
+// fields from batch position bidx[i]
+__global__ void insert_batch_kernel(const uint32_t* __restrict__ nb, const uint64_t* __restrict__ keys,
+                                    const uint32_t* __restrict__ spos, const uint32_t* __restrict__ seg_start,
+                                    const uint32_t* __restrict__ ubk, const TouchInfo* __restrict__ ti, KeyFmt f,
+                                    int32_t* __restrict__ arena, const UpdScal* sc, MetaSet am, MetaSet bm,
+                                    const uint32_t* __restrict__ bidx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)*nb || sc->err) return;
+    const int64_t j = (int64_t)spos[i + 1] - 1;
+    const int64_t dest = ti[j].new_off + (i - (int64_t)seg_start[j]) + ubk[i];
+    arena[dest] = f.dst(keys[i]);
+    if (am.any()) meta_put(am, dest, bm, bidx[i]);
+}
+
 }  // namespace
 
 // scratch for a batch of n entries
@@ -1067,12 +1153,16 @@ static void note_symmetry(hn_graph* g, bool was_sym, const UpdScal& h) {
 
 // a staging buffer of `total` entries (and one per meta-data field, in b.sm),
 // after the batch's error word is checked
-static hn_status reserve_stage(hn_graph* g, UpdBufs& b, int64_t total, hipStream_t st, int32_t** stage) {
+static hn_status reserve_stage(hn_graph* g, UpdBufs& b, int64_t total, hipStream_t st, int32_t** stage,
+                               int64_t** dst = nullptr) {
     UpdScal h;
     HN_TRY(check_scal(g, b, st, &h));
     const size_t cnt = (size_t)(total > 0 ? total : 1);
-    HN_TRY(g->s_misc.reserve(sizeof(int32_t) * cnt));
+    // entries, then (when asked for) one 64-bit destination per entry
+    const size_t vbytes = (sizeof(int32_t) * cnt + 255) & ~size_t(255);
+    HN_TRY(g->s_misc.reserve(vbytes + (dst ? sizeof(int64_t) * cnt : 0)));
     *stage = g->s_misc.as<int32_t>();
+    if (dst) *dst = reinterpret_cast<int64_t*>(g->s_misc.as<char>() + vbytes);
     if (g->meta.any()) {
         // one staging array per meta-data field, 256-byte aligned, in s_mstage
         size_t bytes = 0;
@@ -1084,28 +1174,6 @@ static hn_status reserve_stage(hn_graph* g, UpdBufs& b, int64_t total, hipStream
             at += (cnt * g->meta.bytes[f] + 255) & ~size_t(255);
         }
     }
-    return HN_OK;
-}
-
-// size the staging buffer (one readback) and copy the moving suffixes
-static hn_status stage_suffixes(hn_graph* g, UpdBufs& b, int64_t n, const uint32_t* ucnt, const uint8_t* skip,
-                                hipStream_t st, int32_t** stage_out) {
-    const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
-    HN_TRY(exclusive_scan_fn<int64_t>(StageLenIn{b.ti, ucnt, b.sc, skip}, RunOut{b.stpos, b.tile_lo}, b.stpos + n, n,
-                                      b.tmp, st));
-    set_soff_kernel<<<gridn, BLOCK, 0, st>>>(ucnt, b.stpos, b.ti, b.gsrc, b.gdst, b.sc);
-    HN_CHECK_LAUNCH();
-    int64_t total = 0;
-    HN_TRY(read_back(g, b.stpos + n, 8, &total, st));
-    int32_t* stage = nullptr;
-    HN_TRY(reserve_stage(g, b, total, st, &stage));
-    if (total > 0) {
-        const int64_t grid = std::min<int64_t>(ceil_div(total, COPY_TILE), (int64_t)num_cus() * COPY_GRID_PER_CU);
-        copy_runs_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.stpos, b.tile_lo, b.gsrc, b.gdst, g->arena, stage, b.sc,
-                                                           g->meta, b.sm);
-        HN_CHECK_LAUNCH();
-    }
-    *stage_out = stage;
     return HN_OK;
 }
 
@@ -1187,13 +1255,32 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     HN_CHECK_LAUNCH();
     alloc_fix_kernel<<<1, 1, 0, st>>>(g->alloc, b.sc);
     HN_CHECK_LAUNCH();
-    int32_t* stage = nullptr;
-    HN_TRY(stage_suffixes(g, b, n, ucnt, nullptr, st, &stage));
-    // 4. merge the moving old entries and the batch segment
-    HN_TRY(exclusive_scan<int64_t>(InsertWorkIn{b.ti, ucnt, b.sc}, b.wpos, n, b.tmp, st));
-    const int64_t grid = (int64_t)num_cus() * 8;
-    insert_merge_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.wpos, b.ti, b.seg_start, b.kb, ubk, f, g->arena, stage,
-                                                          b.sc, g->meta, b.sm, bm.m, b.ib);
+    // 4. the old entries that move, in place in one pass (the few that could
+    // overwrite a slot a later copy tile has not read go through a side
+    // buffer), then the batch entries into the slots left between them
+    HN_TRY(exclusive_scan_fn<int64_t>(MoveLenIn{b.ti, ucnt, b.sc}, RunOut{b.wpos, b.tile_lo}, b.wpos + n, n, b.tmp,
+                                      st));
+    const int64_t nt = b.ntile_max;
+    insert_defer_kernel<<<(unsigned)ceil_div(nt, BLOCK), BLOCK, 0, st>>>(nt, n, b.wpos, b.tile_lo, b.ti, b.seg_start,
+                                                                        b.kb, f, g->arena, b.tcnt, b.sc);
+    HN_CHECK_LAUNCH();
+    HN_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, ArrayOut<int64_t>{b.doff}, b.doff + nt, nt, b.tmp, st));
+    int64_t ndefer = 0;
+    HN_TRY(read_back(g, b.doff + nt, 8, &ndefer, st));
+    int32_t* side = nullptr;
+    int64_t* side_dst = nullptr;
+    HN_TRY(reserve_stage(g, b, ndefer, st, &side, &side_dst));
+    const int64_t grid = (int64_t)num_cus() * COPY_GRID_PER_CU;
+    insert_move_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.wpos, b.tile_lo, b.ti, b.seg_start, b.kb, f, b.tcnt,
+                                                         b.doff, g->arena, side, side_dst, b.sc, g->meta, b.sm);
+    HN_CHECK_LAUNCH();
+    if (ndefer > 0) {
+        insert_place_kernel<<<(unsigned)ceil_div(ndefer, BLOCK), BLOCK, 0, st>>>(ndefer, side, side_dst, g->arena,
+                                                                                 g->meta, b.sm);
+        HN_CHECK_LAUNCH();
+    }
+    insert_batch_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, b.spos, b.seg_start, ubk, b.ti, f, g->arena, b.sc, g->meta,
+                                                 bm.m, b.ib);
     HN_CHECK_LAUNCH();
     // 5. publish
     commit_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(ucnt, b.seg_key, f, b.ti, nullptr, g->rec, g->alloc_view(), b.sc);

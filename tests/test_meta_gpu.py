@@ -139,3 +139,18 @@ def test_erase_hub_rows_in_place_with_meta():
         off, adj, f, erem = O.erase_meta(off, adj, f, es, ed)
         assert removed == erem
         check(h, off, adj, f)
+
+
+def test_insert_hub_rows_in_place_with_meta():
+    """Fields follow their entries when long rows grow in place (old entries
+    moving up across many copy tiles) and batch fields land beside them."""
+    print("case meta insert hub rows", flush=True)
+    from test_update_gpu import hub_graph, hub_insert_batch
+    off, adj, hubs = hub_graph(6)
+    f = fields_for(adj.size, 7)
+    h = build(off, adj, f)
+    bs, bd = hub_insert_batch(off, hubs, 31)
+    bf = fields_for(bs.size, 8)
+    h.insert(BatchUpdate(bs, bd, meta=bf))
+    off, adj, f, _ = O.insert_meta(off, adj, f, bs, bd, bf)
+    check(h, off, adj, f)

@@ -342,3 +342,29 @@ def test_erase_hub_rows_in_place(hn):
         off, adj, removed = O.erase(off, adj, es, ed)
         assert got == removed
         check_equal(h, off, adj)
+
+
+def hub_insert_batch(off, hubs, seed, per_hub=2000):
+    """per_hub new entries into every hub row (30000 -> 32000 of a 32768-entry
+    block: the rows grow in place, their old entries moving up by up to
+    per_hub slots across many copy tiles), repeats included, and uniform
+    background pairs."""
+    rng = np.random.default_rng(seed)
+    n = off.size - 1
+    s = [np.full(per_hub, v, np.int32) for v in hubs]
+    d = [rng.integers(0, n, per_hub).astype(np.int32) for _ in hubs]
+    bs, bd = O.uniform_pairs(n, 20000, seed=seed)
+    return np.concatenate(s + [bs]), np.concatenate(d + [bd])
+
+
+@pytest.mark.parametrize("dedup", [False, True])
+def test_insert_hub_rows_in_place(hn, dedup):
+    print(f"case insert hub rows in place dedup={dedup}", flush=True)
+    off, adj, hubs = hub_graph(5)
+    h = hn.Hornet.from_csr(off, adj)
+    bs, bd = hub_insert_batch(off, hubs, 30)
+    got = h.insert(hn.BatchUpdate(bs, bd), dedup, dedup)
+    off, adj, added = O.insert(off, adj, bs, bd, dedup, dedup)
+    assert got == added
+    assert all(30000 < off[v + 1] - off[v] <= 32768 for v in hubs)
+    check_equal(h, off, adj)
