@@ -28,9 +28,9 @@
 // In both passes the few entries whose slot a neighbouring copy tile may not
 // have read yet go through a small side buffer.  A device error word (bad
 // vertex id, arena exhausted) turns every later kernel into a no-op and rolls
-// the allocator back, so a failed batch leaves the graph unchanged.  Three
-// host synchronisations per batch (side-buffer size, error word, final
-// status).
+// the allocator back, so a failed batch leaves the graph unchanged.  Two
+// host synchronisations per batch (error word with the side-buffer size,
+// final status).
 #include "graph.hpp"
 #include "prims.hpp"
 
@@ -61,6 +61,7 @@ struct UpdScal {
     uint32_t err;
     uint32_t asym;                // a batch pair left (u,w) and (w,u) with different membership
     unsigned long long changed;   // entries inserted / erased
+    long long side;               // entries deferred to the side buffer
 };
 
 // idx (non-null with edge meta-data) records each entry's batch position
@@ -617,13 +618,28 @@ struct TileCountIn {
 };
 
 // scan output of the deferred counts: P (the second copy's run starts), its
-// tile_lo, and each run's source in the side buffer
+// tile_lo, and each run's source in the side buffer; the last tile also
+// records the total in the batch's status word, read back with it
 struct DeferOut {
     RunOut runs;
     int64_t* src;
+    int64_t last;
+    UpdScal* sc;
     __device__ __forceinline__ void operator()(int64_t i, int64_t ex, int64_t x) const {
         runs(i, ex, x);
         src[i] = GAP_STAGE | ex;
+        if (i == last) sc->side = ex + x;
+    }
+};
+
+// the same for the insert move: offsets and the total
+struct SideOut {
+    int64_t* P;
+    int64_t last;
+    UpdScal* sc;
+    __device__ __forceinline__ void operator()(int64_t i, int64_t ex, int64_t x) const {
+        P[i] = ex;
+        if (i == last) sc->side = ex + x;
     }
 };
 
@@ -1151,13 +1167,15 @@ static void note_symmetry(hn_graph* g, bool was_sym, const UpdScal& h) {
     g->sym_version = g->version;
 }
 
-// a staging buffer of `total` entries (and one per meta-data field, in b.sm),
-// after the batch's error word is checked
-static hn_status reserve_stage(hn_graph* g, UpdBufs& b, int64_t total, hipStream_t st, int32_t** stage,
-                               int64_t** dst = nullptr) {
+// the side buffer for the batch's deferred entries (their count is read back
+// with the error word, which is checked first), one per meta-data field in
+// b.sm, and when asked one 64-bit destination per entry
+static hn_status reserve_side(hn_graph* g, UpdBufs& b, hipStream_t st, int64_t* total, int32_t** stage,
+                              int64_t** dst = nullptr) {
     UpdScal h;
     HN_TRY(check_scal(g, b, st, &h));
-    const size_t cnt = (size_t)(total > 0 ? total : 1);
+    *total = (int64_t)h.side;
+    const size_t cnt = (size_t)(*total > 0 ? *total : 1);
     // entries, then (when asked for) one 64-bit destination per entry
     const size_t vbytes = (sizeof(int32_t) * cnt + 255) & ~size_t(255);
     HN_TRY(g->s_misc.reserve(vbytes + (dst ? sizeof(int64_t) * cnt : 0)));
@@ -1264,12 +1282,12 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     insert_defer_kernel<<<(unsigned)ceil_div(nt, BLOCK), BLOCK, 0, st>>>(nt, n, b.wpos, b.tile_lo, b.ti, b.seg_start,
                                                                         b.kb, f, g->arena, b.tcnt, b.sc);
     HN_CHECK_LAUNCH();
-    HN_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, ArrayOut<int64_t>{b.doff}, b.doff + nt, nt, b.tmp, st));
+    HN_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, SideOut{b.doff, nt - 1, b.sc}, b.doff + nt, nt, b.tmp,
+                                      st));
     int64_t ndefer = 0;
-    HN_TRY(read_back(g, b.doff + nt, 8, &ndefer, st));
     int32_t* side = nullptr;
     int64_t* side_dst = nullptr;
-    HN_TRY(reserve_stage(g, b, ndefer, st, &side, &side_dst));
+    HN_TRY(reserve_side(g, b, st, &ndefer, &side, &side_dst));
     const int64_t grid = (int64_t)num_cus() * COPY_GRID_PER_CU;
     insert_move_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.wpos, b.tile_lo, b.ti, b.seg_start, b.kb, f, b.tcnt,
                                                          b.doff, g->arena, side, side_dst, b.sc, g->meta, b.sm);
@@ -1357,12 +1375,11 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     erase_defer_kernel<<<(unsigned)ceil_div(nt, BLOCK), BLOCK, 0, st>>>(nt, 2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst,
                                                                        b.rtail, b.tcnt, b.tdst, b.sc);
     HN_CHECK_LAUNCH();
-    HN_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, DeferOut{RunOut{b.doff, b.tile_lo2}, b.dsrc}, b.doff + nt,
-                                      nt, b.tmp, st));
+    HN_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, DeferOut{RunOut{b.doff, b.tile_lo2}, b.dsrc, nt - 1, b.sc},
+                                      b.doff + nt, nt, b.tmp, st));
     int64_t ndefer = 0;
-    HN_TRY(read_back(g, b.doff + nt, 8, &ndefer, st));
     int32_t* side = nullptr;
-    HN_TRY(reserve_stage(g, b, ndefer, st, &side));
+    HN_TRY(reserve_side(g, b, st, &ndefer, &side));
     const int64_t grid = (int64_t)num_cus() * COPY_GRID_PER_CU;
     erase_copy_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst, b.tcnt, b.doff,
                                                         g->arena, side, b.sc, g->meta, b.sm);
