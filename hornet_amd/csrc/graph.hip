@@ -74,41 +74,124 @@ __global__ void set_records_kernel(int32_t nv, const int64_t* __restrict__ off,
 constexpr int COPY_IPT = 8;
 constexpr int COPY_TILE = BLOCK * COPY_IPT;
 
-// arena[rec[v].off + j] = adj[off[v] + j] for all edges, load balanced by edge
-__global__ __launch_bounds__(BLOCK) void copy_in_kernel(int32_t nv, int64_t ne,
-                                                        const int64_t* __restrict__ off,
+// Segmented copies between the blocks and a CSR (build, flatten / export,
+// snapshots).  tile_lo[t] is the segment holding item t * COPY_TILE of the
+// prefix P (one search per tile up front, so a tile finds its segments without
+// block-wide searches); a tile stages up to COPY_CAP segments in LDS, locates
+// all of its items, issues all loads (unconditionally: a lane past the end
+// reads slot 0), then all stores.
+constexpr int COPY_CAP_IPT = 1;
+constexpr int COPY_CAP = LbShared<COPY_CAP_IPT, int64_t>::CAP;
+
+__global__ void tile_starts_kernel(int64_t ntiles, const int64_t* __restrict__ P, int64_t nseg,
+                                   uint32_t* __restrict__ tile_lo) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntiles) tile_lo[t] = (uint32_t)upper_bound_m1<int64_t>(P, 0, nseg + 1, t * COPY_TILE);
+}
+
+// A tile's items [w0, wend) as (segment, block slot): base[s] + w is item w's
+// slot in its vertex block.
+struct SegCopy {
+    int64_t seg[COPY_IPT];
+    int64_t slot[COPY_IPT];
+};
+
+template <typename BaseFn>
+__device__ __forceinline__ SegCopy seg_locate(const int64_t* __restrict__ P, int64_t nseg, int64_t ntiles,
+                                              const uint32_t* __restrict__ tile_lo, int64_t t, int64_t w0,
+                                              int64_t wend, LbShared<COPY_CAP_IPT, int64_t>& sh, BaseFn base) {
+    const int64_t lo = tile_lo[t];
+    // the segment holding the next tile's first item bounds this tile's
+    // segments; for the last tile, every segment from lo on
+    const int64_t hi = t + 1 < ntiles ? (int64_t)tile_lo[t + 1] : nseg - 1;
+    const int64_t span = hi - lo + 1;
+    const int count = span <= COPY_CAP ? (int)span : -1;
+    if (count > 0)
+        for (int i = threadIdx.x; i < count; i += BLOCK) {
+            sh.pref[i] = P[lo + i];
+            sh.base[i] = base(lo + i);
+        }
+    __syncthreads();
+    SegCopy c;
+#pragma unroll
+    for (int j = 0; j < COPY_IPT; j++) {
+        const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+        c.seg[j] = -1;
+        c.slot[j] = 0;
+        if (w < wend) {
+            if (count > 0) {
+                const int s = lb_locate<COPY_CAP_IPT>(sh, count, w);
+                c.seg[j] = lo + s;
+                c.slot[j] = sh.base[s] + w;
+            } else {
+                const int64_t s = upper_bound_m1<int64_t>(P, lo, hi + 1, w);
+                c.seg[j] = s;
+                c.slot[j] = base(s) + w;
+            }
+        }
+    }
+    return c;
+}
+
+// arena[rec[v].off + j] = adj[off[v] + j] for all edges
+__global__ __launch_bounds__(BLOCK) void copy_in_kernel(int32_t nv, int64_t ne, const int64_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ tile_lo,
                                                         const int32_t* __restrict__ adj,
                                                         const hn_vrec* __restrict__ rec,
                                                         int32_t* __restrict__ arena, MetaSet am, MetaSet in) {
-    __shared__ LbShared<COPY_IPT, int64_t> sh;
-    for (int64_t w0 = (int64_t)blockIdx.x * COPY_TILE; w0 < ne; w0 += (int64_t)gridDim.x * COPY_TILE) {
-        lb_tile<COPY_IPT>(
-            off, (int64_t)nv, w0, ne, sh,
-            [&](int64_t s) { return rec[s].off - off[s]; },
-            [&](int64_t, int64_t w, int64_t base) {
-                arena[base + w] = adj[w];
-                meta_put(am, base + w, in, w);
-            });
+    __shared__ LbShared<COPY_CAP_IPT, int64_t> sh;
+    const int64_t ntiles = (ne + COPY_TILE - 1) / COPY_TILE;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t w0 = t * COPY_TILE;
+        const int64_t wend = (w0 + (int64_t)COPY_TILE) < ne ? (w0 + (int64_t)COPY_TILE) : ne;
+        const SegCopy c = seg_locate(off, (int64_t)nv, ntiles, tile_lo, t, w0, wend, sh,
+                                     [&](int64_t s) { return rec[s].off - off[s]; });
+        int32_t v[COPY_IPT];
+#pragma unroll
+        for (int j = 0; j < COPY_IPT; j++) {
+            const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+            v[j] = adj[c.seg[j] >= 0 ? w : 0];
+        }
+#pragma unroll
+        for (int j = 0; j < COPY_IPT; j++)
+            if (c.seg[j] >= 0) arena[c.slot[j]] = v[j];
+        if (am.any())
+#pragma unroll
+            for (int j = 0; j < COPY_IPT; j++)
+                if (c.seg[j] >= 0) meta_put(am, c.slot[j], in, w0 + (int64_t)j * BLOCK + threadIdx.x);
+        __syncthreads();
     }
 }
 
 // COO/CSR flatten: out positions from the degree prefix eoff
-__global__ __launch_bounds__(BLOCK) void flatten_kernel(int32_t nv, int64_t ne,
-                                                        const int64_t* __restrict__ eoff,
+__global__ __launch_bounds__(BLOCK) void flatten_kernel(int32_t nv, int64_t ne, const int64_t* __restrict__ eoff,
+                                                        const uint32_t* __restrict__ tile_lo,
                                                         const hn_vrec* __restrict__ rec,
                                                         const int32_t* __restrict__ arena,
                                                         int32_t* __restrict__ src,
                                                         int32_t* __restrict__ dst, MetaSet out, MetaSet am) {
-    __shared__ LbShared<COPY_IPT, int64_t> sh;
-    for (int64_t w0 = (int64_t)blockIdx.x * COPY_TILE; w0 < ne; w0 += (int64_t)gridDim.x * COPY_TILE) {
-        lb_tile<COPY_IPT>(
-            eoff, (int64_t)nv, w0, ne, sh,
-            [&](int64_t s) { return rec[s].off - eoff[s]; },
-            [&](int64_t s, int64_t w, int64_t base) {
-                dst[w] = arena[base + w];
-                if (src) src[w] = (int32_t)s;
-                meta_put(out, w, am, base + w);
-            });
+    __shared__ LbShared<COPY_CAP_IPT, int64_t> sh;
+    const int64_t ntiles = (ne + COPY_TILE - 1) / COPY_TILE;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t w0 = t * COPY_TILE;
+        const int64_t wend = (w0 + (int64_t)COPY_TILE) < ne ? (w0 + (int64_t)COPY_TILE) : ne;
+        const SegCopy c = seg_locate(eoff, (int64_t)nv, ntiles, tile_lo, t, w0, wend, sh,
+                                     [&](int64_t s) { return rec[s].off - eoff[s]; });
+        int32_t v[COPY_IPT];
+#pragma unroll
+        for (int j = 0; j < COPY_IPT; j++) v[j] = arena[c.slot[j]];
+#pragma unroll
+        for (int j = 0; j < COPY_IPT; j++) {
+            if (c.seg[j] < 0) continue;
+            const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+            dst[w] = v[j];
+            if (src) src[w] = (int32_t)c.seg[j];
+        }
+        if (out.any())
+#pragma unroll
+            for (int j = 0; j < COPY_IPT; j++)
+                if (c.seg[j] >= 0) meta_put(out, w0 + (int64_t)j * BLOCK + threadIdx.x, am, c.slot[j]);
+        __syncthreads();
     }
 }
 
@@ -454,13 +537,18 @@ hn_status graph_from_device_csr(int32_t nv, int64_t ne, const int64_t* d_off, co
     if (nv > 0) {
         set_records_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(nv, d_off, uoff, g->rec);
     }
+    uint32_t* tile_lo = nullptr;
     if (ne > 0) {
-        int64_t tiles = ceil_div(ne, COPY_TILE);
-        int64_t grid = std::min<int64_t>(tiles, (int64_t)num_cus() * 16);
-        copy_in_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(nv, ne, d_off, d_adj, g->rec, g->arena, g->meta, d_meta);
+        const int64_t tiles = ceil_div(ne, COPY_TILE);
+        if ((s = dalloc(&tile_lo, (size_t)tiles)) != HN_OK) { dfree(uoff); return fail(s); }
+        tile_starts_kernel<<<(unsigned)ceil_div(tiles, BLOCK), BLOCK, 0, st>>>(tiles, d_off, nv, tile_lo);
+        const int64_t grid = std::min<int64_t>(tiles, (int64_t)num_cus() * 8);
+        copy_in_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(nv, ne, d_off, tile_lo, d_adj, g->rec, g->arena, g->meta,
+                                                         d_meta);
     }
     hipError_t e = hipStreamSynchronize(st);
     dfree(uoff);
+    dfree(tile_lo);
     if (e != hipSuccess) {
         set_error(std::string("graph build: ") + hipGetErrorString(e));
         return fail(HN_E_HIP);
@@ -761,15 +849,20 @@ hn_status hn_graph_memory(hn_graph* g, int64_t* used_slots, int64_t* capacity_sl
 
 static hn_status flatten(hn_graph* g, int32_t* src, int32_t* dst, int64_t* eoff_out, hipStream_t st,
                          const MetaSet& out = MetaSet()) {
-    int64_t* eoff = eoff_out;
-    if (!eoff) HN_TRY(g->s_misc.reserve(sizeof(int64_t) * ((size_t)g->nv + 1)));
-    if (!eoff) eoff = g->s_misc.as<int64_t>();
+    // s_misc: the degree prefix (unless the caller's offsets take it), then
+    // the tiles' first segments
+    const int64_t tiles = ceil_div(g->ne, COPY_TILE);
+    const size_t eoff_bytes = eoff_out ? 0 : ((sizeof(int64_t) * ((size_t)g->nv + 1) + 255) & ~size_t(255));
+    HN_TRY(g->s_misc.reserve(eoff_bytes + sizeof(uint32_t) * (size_t)(tiles + 1)));
+    int64_t* eoff = eoff_out ? eoff_out : g->s_misc.as<int64_t>();
+    uint32_t* tile_lo = reinterpret_cast<uint32_t*>(g->s_misc.as<char>() + eoff_bytes);
     HN_TRY(degree_prefix(g, eoff, st));
     if (g->ne > 0) {
-        int64_t tiles = ceil_div(g->ne, COPY_TILE);
-        int64_t grid = std::min<int64_t>(tiles, (int64_t)num_cus() * 16);
-        flatten_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(g->nv, g->ne, eoff, g->rec, g->arena, src, dst, out,
-                                                         g->meta);
+        tile_starts_kernel<<<(unsigned)ceil_div(tiles, BLOCK), BLOCK, 0, st>>>(tiles, eoff, g->nv, tile_lo);
+        HN_CHECK_LAUNCH();
+        int64_t grid = std::min<int64_t>(tiles, (int64_t)num_cus() * 8);
+        flatten_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(g->nv, g->ne, eoff, tile_lo, g->rec, g->arena, src, dst,
+                                                         out, g->meta);
         HN_CHECK_LAUNCH();
     }
     HN_HIP(hipStreamSynchronize(st));
