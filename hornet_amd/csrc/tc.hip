@@ -17,8 +17,8 @@
 //     atomics): keep (u,v) iff (deg u, u) < (deg v, v); the oriented offsets
 //     are the scan positions at the symmetric offsets;
 //   * intersection work min(d(u), d(v)) splits every edge into one of three
-//     wave64 classes: one lane per edge (<= 16 probes), eight lanes per edge
-//     (<= 512), or a whole wave per edge; each lane binary-searches elements
+//     wave64 classes: one lane per edge (<= 16 probes), sixteen lanes per edge
+//     (<= 256), or a whole wave per edge; each lane binary-searches elements
 //     of the shorter list in the longer one (equal to the merge count on
 //     simple graphs);
 //   * counts are reduced per wave; the total takes one 64-bit atomicAdd per
@@ -32,8 +32,11 @@ namespace {
 constexpr int TC_IPT = 4;
 constexpr int TC_TILE = BLOCK * TC_IPT;
 constexpr int LIGHT_MAX = 16;
-constexpr int MEDIUM_MAX = 512;
-constexpr int MEDIUM_LANES = 8;
+// class limits: one lane per edge up to LIGHT_MAX probes, MEDIUM_LANES lanes up
+// to MEDIUM_MAX, a wave above; measured on R-MAT s24 (16 / 256: 783 ms per
+// recount; 8 / 512: 819 ms; 8 / 256: 805 ms; 32 / 512: 784 ms; 4 / 512: 1057 ms)
+constexpr int MEDIUM_MAX = 256;
+constexpr int MEDIUM_LANES = 16;
 
 __device__ __forceinline__ bool bsearch_has(const int32_t* __restrict__ a, int64_t n, int32_t x) {
     int64_t lo = 0, hi = n;
