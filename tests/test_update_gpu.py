@@ -368,3 +368,32 @@ def test_insert_hub_rows_in_place(hn, dedup):
     assert got == added
     assert all(30000 < off[v + 1] - off[v] <= 32768 for v in hubs)
     check_equal(h, off, adj)
+
+
+def test_updates_on_many_short_rows(hn):
+    """Rows of 4-8 entries, each losing or gaining about one entry in place:
+    a 1024-item copy tile then spans several hundred runs, more than its LDS
+    stage holds (257), so the copies search their runs in global memory."""
+    print("case updates on many short rows", flush=True)
+    rng = np.random.default_rng(17)
+    n = 1 << 14
+    deg = rng.integers(4, 9, n)
+    s = np.repeat(np.arange(n), deg).astype(np.int32)
+    d = rng.integers(0, n, s.size).astype(np.int32)
+    off, adj = O.coo_to_csr(n, s, d, symmetrize=False, drop_self_loops=False, dedup=True)
+    h = hn.Hornet.from_csr(off, adj)
+    s0, d0 = coo(off, adj)
+    # one stored entry of (almost) every row, from anywhere in the row
+    first = off[:-1][np.diff(off) > 0]
+    pick = first + (rng.integers(0, 1 << 30, first.size) % np.diff(off)[np.diff(off) > 0])
+    es, ed = s0[pick].astype(np.int32), d0[pick].astype(np.int32)
+    assert h.erase(hn.BatchUpdate(es, ed)) == es.size
+    off, adj, _ = O.erase(off, adj, es, ed)
+    check_equal(h, off, adj)
+    # one or two new entries per row, interleaved with the stored ones
+    bs = np.concatenate([np.arange(n), np.arange(0, n, 2)]).astype(np.int32)
+    bd = rng.integers(0, n, bs.size).astype(np.int32)
+    got = h.insert(hn.BatchUpdate(bs, bd), True, True)
+    off, adj, added = O.insert(off, adj, bs, bd, True, True)
+    assert got == added
+    check_equal(h, off, adj)
