@@ -282,13 +282,9 @@ struct RunOut {
     }
 };
 
-// Buffers of the segmented copies: an index into the arena, or with
-// GAP_STAGE set into the staging buffer.
-constexpr int64_t GAP_STAGE = (int64_t)1 << 62;
-
-// Segmented copy: run s moves P[s+1] - P[s] entries from index src[s] on to
-// index dst[s] on.  A lane's items are located first, then all loads issue
-// (unconditionally: a lane past the end reads arena[0]), then all stores.
+// Segmented copies: run s moves P[s+1] - P[s] entries from slot src[s] on to
+// slot dst[s] on.  A lane's items are located first, then all loads issue
+// (unconditionally: a lane past the end reads slot 0), then all stores.
 constexpr int COPY_CAP = LbShared<COPY_CAP_IPT, int64_t>::CAP;   // runs staged per copy tile
 
 struct CopyShared {
@@ -309,81 +305,7 @@ __device__ __forceinline__ CopyRange copy_range(const int64_t* __restrict__ P, i
     // last tile searches (the whole workgroup takes this branch together)
     const int64_t hi = tile + 1 < ntiles ? next_lo : block_upper_bound_m1(P, lo, nruns + 1, W - 1);
     const int64_t span = hi - lo + 1;
-    constexpr int cap = LbShared<COPY_CAP_IPT, int64_t>::CAP;
-    return {lo, span <= cap ? (int)span : -(int)(span < 0x7fffffff ? span : 0x7fffffff)};
-}
-
-// Edge meta-data (am: parallel to the arena, sm: to the staging buffer) move
-// with their entries.
-__global__ __launch_bounds__(BLOCK) void copy_runs_kernel(int64_t nruns, const int64_t* __restrict__ P,
-                                                          const uint32_t* __restrict__ tile_lo,
-                                                          const int64_t* __restrict__ src,
-                                                          const int64_t* __restrict__ dst,
-                                                          int32_t* __restrict__ arena, int32_t* __restrict__ stage,
-                                                          const UpdScal* sc, MetaSet am, MetaSet sm) {
-    __shared__ CopyShared sh;
-    if (sc->err) return;
-    const int64_t W = P[nruns];
-    const int64_t ntiles = (W + COPY_TILE - 1) / COPY_TILE;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int64_t w0 = tile * COPY_TILE;
-        const int64_t wend = (w0 + (int64_t)COPY_TILE) < W ? (w0 + (int64_t)COPY_TILE) : W;
-        const int64_t lo = tile_lo[tile];
-        // the run holding the next tile's first item bounds this tile's runs
-        const int64_t hi = tile + 1 < ntiles ? (int64_t)tile_lo[tile + 1] : block_upper_bound_m1(P, lo, nruns + 1, wend - 1);
-        const int count = lb_stage<COPY_CAP_IPT>(P, lo, hi, sh.lb, [&](int64_t s) { return src[s] - P[s]; });
-        if (count > 0)
-            for (int i = threadIdx.x; i < count; i += BLOCK) sh.dst[i] = dst[lo + i] - P[lo + i];
-        __syncthreads();
-        int64_t si[COPY_IPT], di[COPY_IPT];
-#pragma unroll
-        for (int j = 0; j < COPY_IPT; j++) {
-            const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
-            if (count > 0) {
-                const int s = lb_locate<COPY_CAP_IPT>(sh.lb, count, w);
-                si[j] = sh.lb.base[s] + w;
-                di[j] = sh.dst[s] + w;
-            } else {
-                const int64_t s = upper_bound_m1<int64_t>(P, lo, lo + (int64_t)(-count), w);
-                si[j] = src[s] - P[s] + w;
-                di[j] = dst[s] - P[s] + w;
-            }
-            if (w >= wend) si[j] = 0, di[j] = -1;
-        }
-        int32_t v[COPY_IPT];
-#pragma unroll
-        for (int j = 0; j < COPY_IPT; j++) {
-            const int32_t* b = (si[j] & GAP_STAGE) ? stage : arena;
-            v[j] = b[si[j] & ~GAP_STAGE];
-        }
-#pragma unroll
-        for (int j = 0; j < COPY_IPT; j++) {
-            if (di[j] < 0) continue;
-            int32_t* b = (di[j] & GAP_STAGE) ? stage : arena;
-            b[di[j] & ~GAP_STAGE] = v[j];
-        }
-        // meta-data: per field, all loads first (unconditional: a lane past the
-        // end reads slot 0), then the stores
-        for (int f = 0; f < am.n; f++) {
-            const bool wide = am.bytes[f] == 8;
-            uint64_t mv[COPY_IPT];
-#pragma unroll
-            for (int j = 0; j < COPY_IPT; j++) {
-                const char* b = (si[j] & GAP_STAGE) ? sm.p[f] : am.p[f];
-                const int64_t x = si[j] & ~GAP_STAGE;
-                mv[j] = wide ? reinterpret_cast<const uint64_t*>(b)[x] : reinterpret_cast<const uint32_t*>(b)[x];
-            }
-#pragma unroll
-            for (int j = 0; j < COPY_IPT; j++) {
-                if (di[j] < 0) continue;
-                char* b = (di[j] & GAP_STAGE) ? sm.p[f] : am.p[f];
-                const int64_t x = di[j] & ~GAP_STAGE;
-                if (wide) reinterpret_cast<uint64_t*>(b)[x] = mv[j];
-                else reinterpret_cast<uint32_t*>(b)[x] = (uint32_t)mv[j];
-            }
-        }
-        __syncthreads();
-    }
+    return {lo, span <= COPY_CAP ? (int)span : -(int)(span < 0x7fffffff ? span : 0x7fffffff)};
 }
 
 // publish the new records and release the blocks that were left
@@ -577,28 +499,28 @@ struct GapLenIn {
 // the earlier items of its first vertex that lie in earlier tiles -- read in
 // any order by other workgroups -- have sources up to s = source(i0 - 1).  The
 // tile's first items whose destination is <= s are deferred: written to a
-// side buffer and placed by a second copy once every tile has read.  All other
-// destinations lie above every source of an earlier tile and below every
-// source of a later one.  Deferred entries are a prefix of the tile, within
-// its first vertex (their destinations are consecutive), at most the number
-// of entries deleted before them in that vertex; measured at 1-2 % of the
-// moved entries on R-MAT graphs, where the staged alternative copies every
-// moved entry twice.
+// side buffer with their destinations and placed by a second launch once
+// every tile has read.  All other destinations lie above every source of an
+// earlier tile and below every source of a later one.  Deferred entries are a
+// prefix of the tile, within its first vertex, at most the number of entries
+// deleted before them in that vertex; measured at 1-2 % of the moved entries
+// on R-MAT graphs, where the staged alternative copies every moved entry
+// twice.
 __global__ void erase_defer_kernel(int64_t ntile_max, int64_t nruns, const int64_t* __restrict__ P,
                                    const uint32_t* __restrict__ tile_lo, const int64_t* __restrict__ src,
                                    const int64_t* __restrict__ dst, const uint32_t* __restrict__ rtail,
-                                   uint32_t* __restrict__ tcnt, int64_t* __restrict__ tdst, const UpdScal* sc) {
+                                   uint32_t* __restrict__ tcnt, const UpdScal* sc) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntile_max) return;
     const int64_t W = P[nruns];
     const int64_t ntiles = (W + COPY_TILE - 1) / COPY_TILE;
-    int64_t c = 0, d0 = 0;
+    int64_t c = 0;
     if (!sc->err && t > 0 && t < ntiles) {
         const int64_t i0 = t * COPY_TILE;
         const int64_t r = tile_lo[t];                                   // holds i0
         const int64_t rp = upper_bound_m1<int64_t>(P, 0, r + 1, i0 - 1);  // holds i0 - 1
         if ((src[r] & RUN_INPLACE) && rtail[r] == rtail[rp]) {
-            d0 = dst[r] - P[r] + i0;
+            const int64_t d0 = dst[r] - P[r] + i0;
             const int64_t sprev = (src[rp] & ~RUN_INPLACE) - P[rp] + (i0 - 1);
             if (sprev >= d0) {
                 c = sprev - d0 + 1;
@@ -609,7 +531,6 @@ __global__ void erase_defer_kernel(int64_t ntile_max, int64_t nruns, const int64
         }
     }
     tcnt[t] = (uint32_t)c;
-    tdst[t] = d0;
 }
 
 struct TileCountIn {
@@ -617,22 +538,9 @@ struct TileCountIn {
     __device__ __forceinline__ int64_t operator()(int64_t t) const { return (int64_t)c[t]; }
 };
 
-// scan output of the deferred counts: P (the second copy's run starts), its
-// tile_lo, and each run's source in the side buffer; the last tile also
-// records the total in the batch's status word, read back with it
-struct DeferOut {
-    RunOut runs;
-    int64_t* src;
-    int64_t last;
-    UpdScal* sc;
-    __device__ __forceinline__ void operator()(int64_t i, int64_t ex, int64_t x) const {
-        runs(i, ex, x);
-        src[i] = GAP_STAGE | ex;
-        if (i == last) sc->side = ex + x;
-    }
-};
-
-// the same for the insert move: offsets and the total
+// scan output of the deferred counts: each tile's offset in the side buffer;
+// the last tile also records the total in the batch's status word, read back
+// with it
 struct SideOut {
     int64_t* P;
     int64_t last;
@@ -647,9 +555,9 @@ struct SideOut {
 __device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // One pass of the in-place compaction: tile t's first tcnt[t] items go to the
-// side buffer at doff[t], every other item to its destination.  Sources are
-// read (values, then each meta-data field) by the whole tile before any of
-// its writes.
+// side buffer at doff[t] (entry and destination), every other item to its
+// destination.  Sources are read (values, then each meta-data field) by the
+// whole tile before any of its writes.
 __global__ __launch_bounds__(BLOCK) void erase_copy_kernel(int64_t nruns, const int64_t* __restrict__ P,
                                                            const uint32_t* __restrict__ tile_lo,
                                                            const int64_t* __restrict__ src,
@@ -657,7 +565,8 @@ __global__ __launch_bounds__(BLOCK) void erase_copy_kernel(int64_t nruns, const 
                                                            const uint32_t* __restrict__ tcnt,
                                                            const int64_t* __restrict__ doff,
                                                            int32_t* __restrict__ arena, int32_t* __restrict__ side,
-                                                           const UpdScal* sc, MetaSet am, MetaSet sm) {
+                                                           int64_t* __restrict__ side_dst, const UpdScal* sc,
+                                                           MetaSet am, MetaSet sm) {
     __shared__ CopyShared sh;
     if (sc->err) return;
     const int64_t W = P[nruns];
@@ -691,8 +600,6 @@ __global__ __launch_bounds__(BLOCK) void erase_copy_kernel(int64_t nruns, const 
                 di[j] = dst[s] - P[s] + w;
             }
             if (w >= wend) si[j] = 0, di[j] = -1;
-            // deferred: the destination becomes a slot of the side buffer
-            if (di[j] >= 0 && w - w0 < ndef) di[j] = GAP_STAGE | (o + (w - w0));
         }
         int32_t v[COPY_IPT];
 #pragma unroll
@@ -704,8 +611,13 @@ __global__ __launch_bounds__(BLOCK) void erase_copy_kernel(int64_t nruns, const 
 #pragma unroll
         for (int j = 0; j < COPY_IPT; j++) {
             if (di[j] < 0) continue;
-            if (di[j] & GAP_STAGE) side[di[j] & ~GAP_STAGE] = v[j];
-            else arena[di[j]] = v[j];
+            const int64_t k = (int64_t)j * BLOCK + threadIdx.x;   // item within the tile
+            if (k < ndef) {
+                side[o + k] = v[j];
+                side_dst[o + k] = di[j];
+            } else {
+                arena[di[j]] = v[j];
+            }
         }
         for (int f = 0; f < am.n; f++) {
             const bool wide = am.bytes[f] == 8;
@@ -719,8 +631,9 @@ __global__ __launch_bounds__(BLOCK) void erase_copy_kernel(int64_t nruns, const 
 #pragma unroll
             for (int j = 0; j < COPY_IPT; j++) {
                 if (di[j] < 0) continue;
-                char* b = (di[j] & GAP_STAGE) ? sm.p[f] : am.p[f];
-                const int64_t x = di[j] & ~GAP_STAGE;
+                const int64_t k = (int64_t)j * BLOCK + threadIdx.x;   // item within the tile
+                char* b = k < ndef ? sm.p[f] : am.p[f];
+                const int64_t x = k < ndef ? o + k : di[j];
                 if (wide) reinterpret_cast<uint64_t*>(b)[x] = mv[j];
                 else reinterpret_cast<uint32_t*>(b)[x] = (uint32_t)mv[j];
             }
@@ -900,8 +813,9 @@ __global__ __launch_bounds__(BLOCK) void insert_move_kernel(int64_t nseg, const 
     }
 }
 
-// the deferred entries (and their fields) to their destinations
-__global__ void insert_place_kernel(int64_t m, const int32_t* __restrict__ side, const int64_t* __restrict__ side_dst,
+// the deferred entries of either in-place pass (and their fields) to their
+// destinations, once every copy tile has read its sources
+__global__ void place_side_kernel(int64_t m, const int32_t* __restrict__ side, const int64_t* __restrict__ side_dst,
                                     int32_t* __restrict__ arena, MetaSet am, MetaSet sm) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
@@ -953,16 +867,13 @@ struct UpdBufs {
     int64_t* gP;         // 2n+1
     uint32_t* tile_lo;   // first run of each copy tile
     uint32_t* rtail;     // 2n: erase, the tail run of each run's vertex
-    uint32_t* tcnt;      // per copy tile: erase, entries deferred
-    int64_t* tdst;       // per copy tile: erase, destination of the first deferred entry
-    int64_t* doff;       // per copy tile + 1: erase, side-buffer offset of the deferred entries
-    int64_t* dsrc;       // per copy tile: erase, GAP_STAGE | doff
-    uint32_t* tile_lo2;  // first deferred run of each copy tile
+    uint32_t* tcnt;      // per copy tile: entries deferred
+    int64_t* doff;       // per copy tile + 1: side-buffer offset of the deferred entries
     UpdScal* sc;
     void* tmp;
     uint32_t* ia;        // n+1: batch positions (edge meta-data only)
     uint32_t* ib;        // n+1
-    MetaSet sm;          // staging area of the edge meta-data
+    MetaSet sm;          // side buffer of the edge meta-data (deferred entries)
     int64_t ntile_max;   // copy tiles the buffers above are sized for
 };
 
@@ -996,10 +907,7 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     int64_t o_tl = o; o = al(o + ntile_max * 4);
     int64_t o_rt = o; o = al(o + 2 * n1 * 4);
     int64_t o_tc = o; o = al(o + ntile_max * 4);
-    int64_t o_td = o; o = al(o + ntile_max * 8);
     int64_t o_do = o; o = al(o + (ntile_max + 1) * 8);
-    int64_t o_ds = o; o = al(o + ntile_max * 8);
-    int64_t o_tl2 = o; o = al(o + ntile_max * 4);
     int64_t o_sc = o; o = al(o + 64);
     const bool meta = g->meta.any();
     int64_t o_ia = o; o = al(o + (meta ? n1 * 4 : 0));
@@ -1032,10 +940,7 @@ static hn_status upd_buffers(hn_graph* g, int64_t n, UpdBufs* b) {
     b->tile_lo = reinterpret_cast<uint32_t*>(base + o_tl);
     b->rtail = reinterpret_cast<uint32_t*>(base + o_rt);
     b->tcnt = reinterpret_cast<uint32_t*>(base + o_tc);
-    b->tdst = reinterpret_cast<int64_t*>(base + o_td);
     b->doff = reinterpret_cast<int64_t*>(base + o_do);
-    b->dsrc = reinterpret_cast<int64_t*>(base + o_ds);
-    b->tile_lo2 = reinterpret_cast<uint32_t*>(base + o_tl2);
     b->sc = reinterpret_cast<UpdScal*>(base + o_sc);
     b->tmp = base + o_tmp;
     b->ia = meta ? reinterpret_cast<uint32_t*>(base + o_ia) : nullptr;
@@ -1182,7 +1087,7 @@ static hn_status reserve_side(hn_graph* g, UpdBufs& b, hipStream_t st, int64_t* 
     *stage = g->s_misc.as<int32_t>();
     if (dst) *dst = reinterpret_cast<int64_t*>(g->s_misc.as<char>() + vbytes);
     if (g->meta.any()) {
-        // one staging array per meta-data field, 256-byte aligned, in s_mstage
+        // one side array per meta-data field, 256-byte aligned, in s_mstage
         size_t bytes = 0;
         for (int f = 0; f < g->meta.n; f++) bytes += (cnt * g->meta.bytes[f] + 255) & ~size_t(255);
         HN_TRY(g->s_mstage.reserve(bytes));
@@ -1293,7 +1198,7 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
                                                          b.doff, g->arena, side, side_dst, b.sc, g->meta, b.sm);
     HN_CHECK_LAUNCH();
     if (ndefer > 0) {
-        insert_place_kernel<<<(unsigned)ceil_div(ndefer, BLOCK), BLOCK, 0, st>>>(ndefer, side, side_dst, g->arena,
+        place_side_kernel<<<(unsigned)ceil_div(ndefer, BLOCK), BLOCK, 0, st>>>(ndefer, side, side_dst, g->arena,
                                                                                  g->meta, b.sm);
         HN_CHECK_LAUNCH();
     }
@@ -1373,21 +1278,21 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     // a source of an earlier tile go through a side buffer
     const int64_t nt = b.ntile_max;
     erase_defer_kernel<<<(unsigned)ceil_div(nt, BLOCK), BLOCK, 0, st>>>(nt, 2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst,
-                                                                       b.rtail, b.tcnt, b.tdst, b.sc);
+                                                                       b.rtail, b.tcnt, b.sc);
     HN_CHECK_LAUNCH();
-    HN_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, DeferOut{RunOut{b.doff, b.tile_lo2}, b.dsrc, nt - 1, b.sc},
-                                      b.doff + nt, nt, b.tmp, st));
+    HN_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, SideOut{b.doff, nt - 1, b.sc}, b.doff + nt, nt, b.tmp,
+                                      st));
     int64_t ndefer = 0;
     int32_t* side = nullptr;
-    HN_TRY(reserve_side(g, b, st, &ndefer, &side));
+    int64_t* side_dst = nullptr;
+    HN_TRY(reserve_side(g, b, st, &ndefer, &side, &side_dst));
     const int64_t grid = (int64_t)num_cus() * COPY_GRID_PER_CU;
     erase_copy_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst, b.tcnt, b.doff,
-                                                        g->arena, side, b.sc, g->meta, b.sm);
+                                                        g->arena, side, side_dst, b.sc, g->meta, b.sm);
     HN_CHECK_LAUNCH();
     if (ndefer > 0) {
-        const int64_t g2 = std::min<int64_t>(ceil_div(ndefer, COPY_TILE), grid);
-        copy_runs_kernel<<<(unsigned)g2, BLOCK, 0, st>>>(nt, b.doff, b.tile_lo2, b.dsrc, b.tdst, g->arena, side,
-                                                         b.sc, g->meta, b.sm);
+        place_side_kernel<<<(unsigned)ceil_div(ndefer, BLOCK), BLOCK, 0, st>>>(ndefer, side, side_dst, g->arena,
+                                                                               g->meta, b.sm);
         HN_CHECK_LAUNCH();
     }
     // 8. publish
