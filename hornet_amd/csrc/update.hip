@@ -49,6 +49,10 @@ constexpr int COPY_TILE = BLOCK * COPY_IPT;
 constexpr int COPY_CAP_IPT = 1;
 constexpr int COPY_GRID_PER_CU = 8;
 
+// bits per endpoint of a packed (src, dst) key: ids are below nv, and no key
+// needs a sentinel, so 2^24 vertices pack into 48 bits (6 radix passes)
+inline int key_bits_for(int32_t nv) { return std::max(1, ceil_log2_host((int64_t)nv)); }
+
 struct KeyFmt {
     int sbits;
     __device__ __forceinline__ int32_t src(uint64_t k) const { return (int32_t)(k >> sbits); }
@@ -1149,7 +1153,7 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     }
     UpdBufs b;
     HN_TRY(upd_buffers(g, n, &b));
-    KeyFmt f{std::max(1, ceil_log2_host((int64_t)g->nv + 1))};
+    KeyFmt f{key_bits_for(g->nv)};
     uint64_t* keys = nullptr;
     HN_TRY(load_and_sort(g, src, dst, n, flags, b, f, st, &keys));
     const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
@@ -1236,7 +1240,7 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     hipStream_t st = as_stream(stream);
     UpdBufs b;
     HN_TRY(upd_buffers(g, n, &b));
-    KeyFmt f{std::max(1, ceil_log2_host((int64_t)g->nv + 1))};
+    KeyFmt f{key_bits_for(g->nv)};
     uint64_t* keys = nullptr;
     HN_TRY(load_and_sort(g, src, dst, n, flags, b, f, st, &keys));
     const unsigned gridn = (unsigned)ceil_div(n + 1, BLOCK);
