@@ -22,6 +22,7 @@ constexpr int RS_BUCKETS = 1 << RS_BITS;
 constexpr int RS_IPT = 16;                      // keys per lane
 constexpr int RS_TILE = BLOCK * RS_IPT;         // 4096 keys
 constexpr int RS_WAVE_KEYS = WAVE * RS_IPT;     // 1024 keys per wave
+static_assert(RS_BUCKETS == BLOCK, "one thread per digit in the per-digit steps");
 
 __global__ __launch_bounds__(BLOCK) void rs_hist_kernel(const uint64_t* __restrict__ keys, int64_t n,
                                                         int shift, int64_t ntiles,
@@ -39,11 +40,32 @@ __global__ __launch_bounds__(BLOCK) void rs_hist_kernel(const uint64_t* __restri
     hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Per digit (one workgroup each): exclusive prefix of its per-tile counts
+// across the tiles, and the digit's total.  The scatter adds the digit's start
+// (prefix of the totals over smaller digits), so a pass needs no global scan.
+__global__ __launch_bounds__(BLOCK) void rs_rowscan_kernel(const uint32_t* __restrict__ hist, int64_t ntiles,
+                                                           uint32_t* __restrict__ hoff, uint32_t* __restrict__ dtot) {
+    __shared__ uint32_t sc[NWAVES + 1];
+    const uint32_t* row = hist + (int64_t)blockIdx.x * ntiles;
+    uint32_t* out = hoff + (int64_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (int64_t t0 = 0; t0 < ntiles; t0 += BLOCK) {
+        const int64_t t = t0 + threadIdx.x;
+        const uint32_t x = t < ntiles ? row[t] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(x, sc, &tot);
+        if (t < ntiles) out[t] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
+}
+
 // KV: a 32-bit value rides with each key (staged beside it in LDS)
 template <bool KV>
 __global__ __launch_bounds__(BLOCK) void rs_scatter_kernel(const uint64_t* __restrict__ keys, int64_t n,
                                                            int shift, int64_t ntiles,
                                                            const uint32_t* __restrict__ hoff,
+                                                           const uint32_t* __restrict__ dtot,
                                                            uint64_t* __restrict__ out,
                                                            const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ vout) {
@@ -96,7 +118,9 @@ __global__ __launch_bounds__(BLOCK) void rs_scatter_kernel(const uint64_t* __res
         uint32_t tot;
         uint32_t ex = block_exclusive_scan(run, sc, &tot);
         dstart[d] = ex;
-        gbase[d] = hoff[(int64_t)d * ntiles + blockIdx.x];
+        // digit d starts after every key of the smaller digits
+        const uint32_t dbase = block_exclusive_scan(dtot[d], sc, &tot);
+        gbase[d] = dbase + hoff[(int64_t)d * ntiles + blockIdx.x];
     }
     __syncthreads();
 #pragma unroll
@@ -123,7 +147,7 @@ __global__ __launch_bounds__(BLOCK) void rs_scatter_kernel(const uint64_t* __res
 size_t sort_temp_bytes(int64_t n) {
     int64_t tiles = ceil_div(n > 0 ? n : 1, RS_TILE);
     int64_t hn_ = tiles * RS_BUCKETS;
-    return (size_t)(hn_ + 1) * 4 * 2 + scan_temp_bytes(hn_) + 512;
+    return (size_t)(hn_ + 1) * 4 * 2 + scan_temp_bytes(hn_) + RS_BUCKETS * 4 + 512;
 }
 
 static hn_status radix_sort_impl(uint64_t** keys, uint64_t** alt, uint32_t** vals, uint32_t** valt, int64_t n,
@@ -136,15 +160,17 @@ static hn_status radix_sort_impl(uint64_t** keys, uint64_t** alt, uint32_t** val
     uint32_t* hoff = hist + hn_ + 1;
     void* stmp = reinterpret_cast<char*>(tmp) + (size_t)(hn_ + 1) * 4 * 2 + 256;
     stmp = reinterpret_cast<void*>((reinterpret_cast<uintptr_t>(stmp) + 255) & ~uintptr_t(255));
+    uint32_t* dtot = reinterpret_cast<uint32_t*>(stmp);   // RS_BUCKETS digit totals
     for (int shift = 0; shift < key_bits; shift += RS_BITS) {
         rs_hist_kernel<<<(unsigned)tiles, BLOCK, 0, st>>>(*keys, n, shift, tiles, hist);
         HN_CHECK_LAUNCH();
-        HN_TRY((exclusive_scan<uint32_t>(ArrayIn<uint32_t>{hist}, hoff, hn_, stmp, st)));
+        rs_rowscan_kernel<<<RS_BUCKETS, BLOCK, 0, st>>>(hist, tiles, hoff, dtot);
+        HN_CHECK_LAUNCH();
         if (vals)
-            rs_scatter_kernel<true><<<(unsigned)tiles, BLOCK, 0, st>>>(*keys, n, shift, tiles, hoff, *alt, *vals,
-                                                                       *valt);
+            rs_scatter_kernel<true><<<(unsigned)tiles, BLOCK, 0, st>>>(*keys, n, shift, tiles, hoff, dtot, *alt,
+                                                                       *vals, *valt);
         else
-            rs_scatter_kernel<false><<<(unsigned)tiles, BLOCK, 0, st>>>(*keys, n, shift, tiles, hoff, *alt,
+            rs_scatter_kernel<false><<<(unsigned)tiles, BLOCK, 0, st>>>(*keys, n, shift, tiles, hoff, dtot, *alt,
                                                                         nullptr, nullptr);
         HN_CHECK_LAUNCH();
         std::swap(*keys, *alt);
