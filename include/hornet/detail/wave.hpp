@@ -2,8 +2,9 @@
 // libhornet_hip and the header-only operator layer (forAllEdges & co.).
 //
 // * wave / block scans over 64-lane wavefronts;
-// * exclusive_scan_dev(): reduce-then-scan over an input functor (three
-//   launches), the replacement for cub::DeviceScan::ExclusiveSum
+// * exclusive_scan_dev(): reduce-then-scan over an input functor (two
+//   launches up to SCAN_SELF_TILES tiles, three above), the replacement for
+//   cub::DeviceScan::ExclusiveSum
 //   (xlib/src/Device/Primitives/CubWrapper.cu:583-617);
 // * lb_tile(): edge-granular load balancing for one workgroup tile, the
 //   wave64 redesign of xlib::binarySearchLB
@@ -93,6 +94,10 @@ constexpr int SCAN_IPT = 16;                   // values per lane per tile
 constexpr int SCAN_TILE = BLOCK * SCAN_IPT;    // 4096
 constexpr int SCAN_WAVE_CHUNK = WAVE * SCAN_IPT;
 
+// up to this many tiles a down-sweep tile sums the totals before it itself
+// (at most SCAN_SELF_TILES loads each) instead of a separate partials launch
+constexpr int64_t SCAN_SELF_TILES = 1024;
+
 inline size_t scan_temp_bytes(int64_t n) {
     int64_t tiles = ceil_div(n > 0 ? n : 1, SCAN_TILE);
     return (size_t)(tiles + 1) * 8 + 256;
@@ -152,15 +157,34 @@ __global__ __launch_bounds__(BLOCK) void scan_partials_kernel(Out* __restrict__ 
     if (threadIdx.x == 0) *total = carry;
 }
 
-// down-sweep that hands (index, exclusive prefix, value) to a functor
-template <typename Out, typename InFn, typename OutFn, typename Cnt>
+// down-sweep that hands (index, exclusive prefix, value) to a functor.  With
+// SELF the tile adds up the totals of the tiles before it itself (used when
+// there are few tiles, saving the partials launch), and the last tile writes
+// the grand total.
+template <typename Out, typename InFn, typename OutFn, typename Cnt, bool SELF = false>
 __global__ __launch_bounds__(BLOCK) void scan_down_fn_kernel(InFn in, Cnt cnt, const Out* __restrict__ partial,
-                                                             OutFn out) {
+                                                             OutFn out, Out* __restrict__ total = nullptr) {
     __shared__ Out wtot[NWAVES];
     const int64_t n = cnt.get();
-    if ((int64_t)blockIdx.x * SCAN_TILE >= n) return;
     const int w = threadIdx.x / WAVE;
     const int l = __lane_id();
+    Out toff = 0;   // prefix of the tile totals before this tile
+    if constexpr (SELF) {
+        const int64_t last = n > 0 ? (n - 1) / SCAN_TILE : 0;
+        if ((int64_t)blockIdx.x > last) return;
+        Out s = 0;
+        for (int64_t k = threadIdx.x; k < (int64_t)blockIdx.x; k += BLOCK) s += partial[k];
+        s = wave_reduce_sum(s);
+        if (l == 0) wtot[w] = s;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NWAVES; q++) toff += wtot[q];
+        if ((int64_t)blockIdx.x == last && threadIdx.x == 0) *total = toff + (n > 0 ? partial[last] : (Out)0);
+        __syncthreads();   // wtot is reused below
+        if (n <= 0) return;
+    } else {
+        if ((int64_t)blockIdx.x * SCAN_TILE >= n) return;
+    }
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)w * SCAN_WAVE_CHUNK;
     Out pre[SCAN_IPT];
     Out val[SCAN_IPT];
@@ -176,7 +200,7 @@ __global__ __launch_bounds__(BLOCK) void scan_down_fn_kernel(InFn in, Cnt cnt, c
     }
     if (l == 0) wtot[w] = carry;
     __syncthreads();
-    Out off = partial[blockIdx.x];
+    Out off = SELF ? toff : partial[blockIdx.x];
     for (int k = 0; k < w; k++) off += wtot[k];
 #pragma unroll
     for (int j = 0; j < SCAN_IPT; j++) {
@@ -211,6 +235,11 @@ hipError_t exclusive_scan_dev(InFn in, OutFn out, Out* total, int64_t n, void* t
     int64_t tiles = ceil_div(n, SCAN_TILE);
     Out* partial = reinterpret_cast<Out*>(tmp);
     scan_reduce_kernel<Out, InFn, HostCount><<<(unsigned)tiles, BLOCK, 0, st>>>(in, HostCount{n}, partial);
+    if (tiles <= SCAN_SELF_TILES) {
+        scan_down_fn_kernel<Out, InFn, OutFn, HostCount, true><<<(unsigned)tiles, BLOCK, 0, st>>>(
+            in, HostCount{n}, partial, out, total);
+        return hipGetLastError();
+    }
     scan_partials_kernel<Out, HostCount><<<1, BLOCK, 0, st>>>(partial, HostCount{n}, total);
     scan_down_fn_kernel<Out, InFn, OutFn, HostCount><<<(unsigned)tiles, BLOCK, 0, st>>>(in, HostCount{n}, partial,
                                                                                         out);
@@ -225,6 +254,11 @@ hipError_t exclusive_scan_dev_count(InFn in, OutFn out, Out* total, const uint32
     const int64_t tiles = ceil_div(max_n > 0 ? max_n : 1, SCAN_TILE);
     Out* partial = reinterpret_cast<Out*>(tmp);
     scan_reduce_kernel<Out, InFn, DevCount><<<(unsigned)tiles, BLOCK, 0, st>>>(in, DevCount{n_dev}, partial);
+    if (tiles <= SCAN_SELF_TILES) {
+        scan_down_fn_kernel<Out, InFn, OutFn, DevCount, true><<<(unsigned)tiles, BLOCK, 0, st>>>(
+            in, DevCount{n_dev}, partial, out, total);
+        return hipGetLastError();
+    }
     scan_partials_kernel<Out, DevCount><<<1, BLOCK, 0, st>>>(partial, DevCount{n_dev}, total);
     scan_down_fn_kernel<Out, InFn, OutFn, DevCount><<<(unsigned)tiles, BLOCK, 0, st>>>(in, DevCount{n_dev}, partial,
                                                                                        out);
