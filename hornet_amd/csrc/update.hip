@@ -92,16 +92,6 @@ __device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t n, 
     }
     return lo;
 }
-// whether the sorted keys a[0, n) hold x
-__device__ __forceinline__ bool has_key_u64(const uint64_t* a, int64_t n, uint64_t x, int64_t* at) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (a[mid] < x) lo = mid + 1; else hi = mid;
-    }
-    *at = lo;
-    return lo < n && a[lo] == x;
-}
 
 // Per sorted batch entry (u, d) of an insert: ub = #(A_u <= d) on u's stored
 // list A_u, found once and used three times -- graph dedup (stored iff
@@ -240,7 +230,7 @@ __global__ __launch_bounds__(ALLOC_BLOCK) void insert_alloc_kernel(
     const bool active = j < (int64_t)*ucnt && !sc->err;   // every lane reaches the allocation below
     TouchInfo t{};
     if (active) insert_touch(j, seg_start, seg_key, ubk, f, rec, t);
-    if (active && stamp) stamp[f.src(seg_key[j])] = (uint32_t)j;
+    if (active && stamp) stamp[f.src(seg_key[j])] = seg_start[j];
     const bool need = active && !t.inplace;
     const int64_t blk = alloc_block_group(A, need ? t.new_cap : 0, need, ash);
     if (!active) return;
@@ -427,7 +417,7 @@ __global__ __launch_bounds__(ALLOC_BLOCK) void erase_alloc_kernel(
     TouchInfo t{};
     bool need = false;
     if (active) need = erase_touch(u, ustart, ukey, f, dpos, dlen, dlp, rec, t, skip);
-    if (active && stamp) stamp[f.src(ukey[u])] = (uint32_t)u;
+    if (active && stamp) stamp[f.src(ukey[u])] = ustart[u];
     const int64_t blk = alloc_block_group(A, need ? t.new_cap : 0, need, ash);
     if (!active) return;
     if (need) t.new_off = blk;
@@ -1010,34 +1000,51 @@ __global__ void store_changed_kernel(const uint32_t* __restrict__ total, UpdScal
 // their membership, so the adjacency stays symmetric iff every batch pair
 // (u,w) has (u,w) and (w,u) both stored or both absent.
 //
-// The batch entries with source w are found through stamp[w], the index of
-// w's source segment written by this batch's allocation kernel: valid when it
-// is below the segment count and that segment's key has source w (stale
-// values from earlier batches fail the test, so the array is never cleared).
-__device__ __forceinline__ bool batch_has(const uint32_t* __restrict__ stamp, const uint32_t* __restrict__ seg_start,
-                                          const uint64_t* __restrict__ seg_key, const uint32_t* __restrict__ nseg,
-                                          const uint64_t* __restrict__ keys, KeyFmt f, int32_t w, int32_t u,
-                                          int64_t* at) {
-    const uint32_t s = stamp[w];
-    if (s >= *nseg || f.src(seg_key[s]) != w) return false;
-    const int64_t lo = seg_start[s];
-    const bool found = has_key_u64(keys + lo, (int64_t)seg_start[s + 1] - lo, ((uint64_t)w << f.sbits) | (uint64_t)u, at);
-    *at += lo;
-    return found;
+// The batch entries with source w are found through stamp[w], the position
+// of w's first key in the sorted keys, written by this batch's allocation
+// kernel.  A stamp is valid when the key there has source w and the key before
+// it does not (stale values from earlier batches fail the test, so the array
+// is never cleared); the search for (w, u) then gallops from there, within
+// w's keys.
+__device__ __forceinline__ bool batch_has(const uint32_t* __restrict__ stamp, const uint64_t* __restrict__ keys,
+                                          int64_t n, KeyFmt f, int32_t w, int32_t u, int64_t* at) {
+    const int64_t s = stamp[w];
+    if (s >= n) return false;
+    const uint64_t k0 = keys[s];
+    if (f.src(k0) != w || (s > 0 && f.src(keys[s - 1]) == w)) return false;
+    const uint64_t x = ((uint64_t)w << f.sbits) | (uint64_t)u;
+    if (k0 >= x) {
+        *at = s;
+        return k0 == x;
+    }
+    // keys[lo] < x; gallop until keys[hi] >= x (keys past w's are larger)
+    int64_t lo = s, hi = s + 1, step = 1;
+    while (hi < n && keys[hi] < x) {
+        lo = hi;
+        step <<= 1;
+        hi = s + step;
+    }
+    if (hi > n) hi = n;
+    while (hi - lo > 1) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if (keys[mid] < x) lo = mid; else hi = mid;
+    }
+    *at = hi;
+    return hi < n && keys[hi] == x;
 }
 
 // Insert: every kept pair (u,w) is stored.  (w,u) is stored too when the kept
 // batch holds it; only otherwise is it looked up in w's list.
 __global__ void insert_symmetry_kernel(const uint32_t* __restrict__ count, const uint64_t* __restrict__ keys,
                                        KeyFmt f, const uint32_t* __restrict__ stamp,
-                                       const uint32_t* __restrict__ seg_start, const uint64_t* __restrict__ seg_key,
-                                       const uint32_t* __restrict__ nseg, const hn_vrec* __restrict__ rec,
-                                       const int32_t* __restrict__ arena, UpdScal* sc) {
+                                       const hn_vrec* __restrict__ rec, const int32_t* __restrict__ arena,
+                                       UpdScal* sc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)*count || sc->err) return;
+    const int64_t n = (int64_t)*count;
+    if (i >= n || sc->err) return;
     const int32_t u = f.src(keys[i]), w = f.dst(keys[i]);
     int64_t at;
-    if (batch_has(stamp, seg_start, seg_key, nseg, keys, f, w, u, &at)) return;
+    if (batch_has(stamp, keys, n, f, w, u, &at)) return;
     const hn_vrec rw = rec[w];
     const int64_t pw = lower_bound_i32(arena + rw.off, rw.deg, u);
     if (!(pw < rw.deg && arena[rw.off + pw] == u)) sc->asym = 1u;   // every writer stores the same value
@@ -1049,14 +1056,13 @@ __global__ void insert_symmetry_kernel(const uint32_t* __restrict__ count, const
 // (u,w) was stored before, i.e. iff (u,w) deleted something.  No list is read.
 __global__ void erase_symmetry_kernel(const uint32_t* __restrict__ dcnt, const uint64_t* __restrict__ dkey, KeyFmt f,
                                       const uint32_t* __restrict__ dlen, const uint8_t* __restrict__ live,
-                                      const uint32_t* __restrict__ stamp, const uint32_t* __restrict__ ustart,
-                                      const uint64_t* __restrict__ ukey, const uint32_t* __restrict__ ucnt,
-                                      UpdScal* sc) {
+                                      const uint32_t* __restrict__ stamp, UpdScal* sc) {
     const int64_t jj = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (jj >= (int64_t)*dcnt || sc->err) return;
+    const int64_t n = (int64_t)*dcnt;
+    if (jj >= n || sc->err) return;
     const int32_t u = f.src(dkey[jj]), w = f.dst(dkey[jj]);
     int64_t at;
-    const bool b = batch_has(stamp, ustart, ukey, ucnt, dkey, f, w, u, &at) ? live[at] != 0 : dlen[jj] > 0;
+    const bool b = batch_has(stamp, dkey, n, f, w, u, &at) ? live[at] != 0 : dlen[jj] > 0;
     if ((live[jj] != 0) != b) sc->asym = 1u;
 }
 
@@ -1216,8 +1222,7 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     HN_CHECK_LAUNCH();
     const bool was_sym = g->sym == 1;
     if (was_sym) {
-        insert_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, stamp, b.seg_start, b.seg_key, ucnt, g->rec,
-                                                        g->arena, b.sc);
+        insert_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, f, stamp, g->rec, g->arena, b.sc);
         HN_CHECK_LAUNCH();
     }
     UpdScal h;
@@ -1306,8 +1311,7 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     HN_CHECK_LAUNCH();
     const bool was_sym = g->sym == 1;
     if (was_sym) {
-        erase_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, b.seg_key, f, b.dlen, b.live, stamp, b.seg2_start,
-                                                       b.seg2_key, ucnt, b.sc);
+        erase_symmetry_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, b.seg_key, f, b.dlen, b.live, stamp, b.sc);
         HN_CHECK_LAUNCH();
     }
     UpdScal h;
