@@ -558,7 +558,10 @@ __device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70
 // One pass of the in-place compaction: tile t's first tcnt[t] items go to the
 // side buffer at doff[t] (entry and destination), every other item to its
 // destination.  Sources are read (values, then each meta-data field) by the
-// whole tile before any of its writes.
+// whole tile before any of its writes.  META = false is the variant for
+// graphs without edge meta-data (am.n == 0): without the field loop it fits
+// in 64 VGPRs, so twice as many waves per SIMD are resident.
+template <bool META>
 __global__ __launch_bounds__(BLOCK) void erase_copy_kernel(int64_t nruns, const int64_t* __restrict__ P,
                                                            const uint32_t* __restrict__ tile_lo,
                                                            const int64_t* __restrict__ src,
@@ -620,7 +623,7 @@ __global__ __launch_bounds__(BLOCK) void erase_copy_kernel(int64_t nruns, const 
                 arena[di[j]] = v[j];
             }
         }
-        for (int f = 0; f < am.n; f++) {
+        for (int f = 0; META && f < am.n; f++) {
             const bool wide = am.bytes[f] == 8;
             uint64_t mv[ERASE_IPT];
 #pragma unroll
@@ -1303,8 +1306,14 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     int64_t* side_dst = nullptr;
     HN_TRY(reserve_side(g, b, st, &ndefer, &side, &side_dst));
     const int64_t grid = (int64_t)num_cus() * COPY_GRID_PER_CU;
-    erase_copy_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst, b.tcnt, b.doff,
-                                                        g->arena, side, side_dst, b.sc, g->meta, b.sm);
+    if (g->meta.n > 0)
+        erase_copy_kernel<true><<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst, b.tcnt,
+                                                                  b.doff, g->arena, side, side_dst, b.sc, g->meta,
+                                                                  b.sm);
+    else
+        erase_copy_kernel<false><<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst, b.tcnt,
+                                                                   b.doff, g->arena, side, side_dst, b.sc, g->meta,
+                                                                   b.sm);
     HN_CHECK_LAUNCH();
     if (ndefer > 0) {
         place_side_kernel<<<(unsigned)ceil_div(ndefer, BLOCK), BLOCK, 0, st>>>(ndefer, side, side_dst, g->arena,
