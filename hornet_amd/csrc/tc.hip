@@ -60,7 +60,7 @@ struct TcOut {
     unsigned long long* per_vertex;   // may be null
     uint32_t* med_q;                  // edge ids for the medium class
     uint32_t* heavy_q;                // edge ids for the heavy class
-    uint32_t* cnt;                    // [0] medium, [1] heavy
+    uint32_t* cnt;                    // [0] medium, [1] heavy, [2] heavy-chunk ticket
     int32_t* src_of;                  // edge -> source (written by the light pass)
 };
 
@@ -112,6 +112,7 @@ __global__ __launch_bounds__(BLOCK) void tc_light_kernel(int32_t nv, int64_t ne,
 }
 
 // `lanes` lanes cooperate on one deferred edge
+constexpr int64_t TC_CHUNK = 16;
 template <int LANES>
 __global__ __launch_bounds__(BLOCK) void tc_group_kernel(TcCsr g, const uint32_t* __restrict__ q,
                                                          const uint32_t* __restrict__ qcount,
@@ -121,30 +122,43 @@ __global__ __launch_bounds__(BLOCK) void tc_group_kernel(TcCsr g, const uint32_t
     const int sub = threadIdx.x % LANES;
     const int64_t ngroups = (int64_t)gridDim.x * BLOCK / LANES;
     unsigned long long acc = 0;
-    // the LANES lanes of a group share k, so a group is either wholly active or
-    // wholly done and the group shuffles below never read an exited lane
-    for (int64_t k = gid; k < (int64_t)n; k += ngroups) {
+    // one deferred edge; the LANES lanes of a group share k, so a group is
+    // either wholly active or wholly done and the group shuffles never read an
+    // exited lane
+    auto one = [&](int64_t k) {
+        const int64_t e = q[k];
+        const int32_t u0 = src_of[e];
+        const int32_t v0 = g.adj[e];
+        const int64_t su = g.off[u0], du = g.off[u0 + 1] - su;
+        const int64_t sv = g.off[v0], dv = g.off[v0 + 1] - sv;
+        const bool src_smaller = du < dv;
+        const int32_t* sp = g.adj + (src_smaller ? su : sv);
+        const int64_t sn = src_smaller ? du : dv;
+        const int32_t* lp = g.adj + (src_smaller ? sv : su);
+        const int64_t ln = src_smaller ? dv : du;
         uint32_t c = 0;
-        int32_t credit = -1;
-        if (k < (int64_t)n) {
-            const int64_t e = q[k];
-            const int32_t u0 = src_of[e];
-            const int32_t v0 = g.adj[e];
-            const int64_t su = g.off[u0], du = g.off[u0 + 1] - su;
-            const int64_t sv = g.off[v0], dv = g.off[v0 + 1] - sv;
-            const bool src_smaller = du < dv;
-            const int32_t* sp = g.adj + (src_smaller ? su : sv);
-            const int64_t sn = src_smaller ? du : dv;
-            const int32_t* lp = g.adj + (src_smaller ? sv : su);
-            const int64_t ln = src_smaller ? dv : du;
-            for (int64_t i = sub; i < sn; i += LANES) c += bsearch_has(lp, ln, sp[i]) ? 1u : 0u;
-            credit = src_smaller ? u0 : v0;
-        }
+        for (int64_t i = sub; i < sn; i += LANES) c += bsearch_has(lp, ln, sp[i]) ? 1u : 0u;
+        const int32_t credit = src_smaller ? u0 : v0;
         // reduce over the group
 #pragma unroll
         for (int d = LANES / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, WAVE);
-        if (sub == 0 && c && o.per_vertex && credit >= 0) atomicAdd(o.per_vertex + credit, (unsigned long long)c);
+        if (sub == 0 && c && o.per_vertex) atomicAdd(o.per_vertex + credit, (unsigned long long)c);
         if (sub == 0) acc += c;
+    };
+    if constexpr (LANES == WAVE) {
+        // heavy edges differ widely in work: each wave claims TC_CHUNK of them
+        // at a time from a ticket until the queue is drained
+        for (;;) {
+            uint32_t t = 0;
+            if (__lane_id() == 0) t = atomicAdd(o.cnt + 2, 1u);
+            t = __shfl(t, 0, WAVE);
+            const int64_t k0 = (int64_t)t * TC_CHUNK;
+            if (k0 >= (int64_t)n) break;
+            const int64_t k1 = k0 + TC_CHUNK < (int64_t)n ? k0 + TC_CHUNK : (int64_t)n;
+            for (int64_t k = k0; k < k1; k++) one(k);
+        }
+    } else {
+        for (int64_t k = gid; k < (int64_t)n; k += ngroups) one(k);
     }
     acc = wave_reduce_sum(acc);
     if (__lane_id() == 0 && acc) atomicAdd(o.total, acc);
