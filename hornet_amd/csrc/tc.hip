@@ -60,7 +60,7 @@ struct TcOut {
     unsigned long long* per_vertex;   // may be null
     uint32_t* med_q;                  // edge ids for the medium class
     uint32_t* heavy_q;                // edge ids for the heavy class
-    uint32_t* cnt;                    // [0] medium, [1] heavy, [2] heavy-chunk ticket
+    uint32_t* cnt;                    // [0] medium, [1] heavy, [2] heavy ticket, [3] medium ticket
     int32_t* src_of;                  // edge -> source (written by the light pass)
 };
 
@@ -118,9 +118,7 @@ __global__ __launch_bounds__(BLOCK) void tc_group_kernel(TcCsr g, const uint32_t
                                                          const uint32_t* __restrict__ qcount,
                                                          const int32_t* __restrict__ src_of, TcOut o) {
     const uint32_t n = *qcount;
-    const int64_t gid = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) / LANES;
     const int sub = threadIdx.x % LANES;
-    const int64_t ngroups = (int64_t)gridDim.x * BLOCK / LANES;
     unsigned long long acc = 0;
     // one deferred edge; the LANES lanes of a group share k, so a group is
     // either wholly active or wholly done and the group shuffles never read an
@@ -145,20 +143,20 @@ __global__ __launch_bounds__(BLOCK) void tc_group_kernel(TcCsr g, const uint32_t
         if (sub == 0 && c && o.per_vertex) atomicAdd(o.per_vertex + credit, (unsigned long long)c);
         if (sub == 0) acc += c;
     };
-    if constexpr (LANES == WAVE) {
-        // heavy edges differ widely in work: each wave claims TC_CHUNK of them
-        // at a time from a ticket until the queue is drained
-        for (;;) {
-            uint32_t t = 0;
-            if (__lane_id() == 0) t = atomicAdd(o.cnt + 2, 1u);
-            t = __shfl(t, 0, WAVE);
-            const int64_t k0 = (int64_t)t * TC_CHUNK;
-            if (k0 >= (int64_t)n) break;
-            const int64_t k1 = k0 + TC_CHUNK < (int64_t)n ? k0 + TC_CHUNK : (int64_t)n;
-            for (int64_t k = k0; k < k1; k++) one(k);
-        }
-    } else {
-        for (int64_t k = gid; k < (int64_t)n; k += ngroups) one(k);
+    // edges differ widely in work: each wave claims TC_CHUNK edges per group
+    // at a time from its class's ticket until the queue is drained; the
+    // groups of a wave take the claimed edges in turn
+    constexpr int G = WAVE / LANES;
+    const int grp = (int)(__lane_id() / LANES);
+    uint32_t* ticket = o.cnt + (LANES == WAVE ? 2 : 3);
+    for (;;) {
+        uint32_t t = 0;
+        if (__lane_id() == 0) t = atomicAdd(ticket, 1u);
+        t = __shfl(t, 0, WAVE);
+        const int64_t k0 = (int64_t)t * TC_CHUNK * G;
+        if (k0 >= (int64_t)n) break;
+        const int64_t k1 = k0 + TC_CHUNK * G < (int64_t)n ? k0 + TC_CHUNK * G : (int64_t)n;
+        for (int64_t k = k0 + grp; k < k1; k += G) one(k);
     }
     acc = wave_reduce_sum(acc);
     if (__lane_id() == 0 && acc) atomicAdd(o.total, acc);
