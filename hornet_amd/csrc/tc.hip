@@ -21,6 +21,9 @@
 //     (<= 256), or a whole wave per edge; each lane binary-searches elements
 //     of the shorter list in the longer one (equal to the merge count on
 //     simple graphs);
+//   * the two group classes are queued by the one-lane pass; their waves
+//     claim 16 queued edges per group at a time from a ticket, which keeps
+//     the waves busy to the end of the skewed queues;
 //   * counts are reduced per wave; the total takes one 64-bit atomicAdd per
 //     wave, per-vertex credits (optional) one per edge with a non-zero count.
 #include "graph.hpp"
