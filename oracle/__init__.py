@@ -17,7 +17,15 @@ Pinning status (what independently checks each function, see
 * ``bfs``             -- networkx single-source shortest path lengths; BFS
   invariants (|d(u)-d(v)| <= 1 across edges, parent at d-1).
 * ``host_tc``         -- networkx triangle count; dense trace(A^3)/6.
-* ``tc2_per_vertex``  -- sum equals host_tc on the degree-oriented graph.
+* ``tc2_per_vertex``  -- sum equals host_tc on the degree-oriented graph; the
+  credit rule (smaller-degree endpoint, ties to the destination, skips) by a
+  hand-worked stored graph.
+* ``directed_by_degree`` -- hand cases, including equal-degree ties (lower id
+  keeps the edge); ``max_degree_id`` -- the hand-derived tie on G.mtx.
+* the ``threads`` forms (``rmat_pairs``, ``chung_lu_pairs``, ``coo_to_csr``,
+  ``host_tc`` / ``triangles``, ``bfs_many``) -- equal to the sequential
+  functions on the same inputs (they only split the work; the triangle count
+  intersects against a mark array instead of merging).
 * ``insert``/``erase`` -- the reference's own identities
   (hornet/test/HornetInsertTest.cu:52-75: sorted(after) == sorted(before ++
   batch); hornet/test/HornetDeleteTest.cu:97-120: sorted(after ++ batch) ==
@@ -110,8 +118,28 @@ def lib():
         L.orc_chung_lu_pairs.restype = None
         L.orc_chung_lu_pairs.argtypes = [c.c_int32, u64p, c.c_int64, c.c_int64, c.c_uint64,
                                          i32p, i32p]
+        L.orc_rmat_pairs_mt.restype = None
+        L.orc_rmat_pairs_mt.argtypes = [c.c_int, c.c_int64, c.c_int64, c.c_uint64, c.c_int, i32p, i32p]
+        L.orc_chung_lu_pairs_mt.restype = None
+        L.orc_chung_lu_pairs_mt.argtypes = [c.c_int32, u64p, c.c_int64, c.c_int64, c.c_uint64, c.c_int,
+                                            i32p, i32p]
+        L.orc_coo_to_csr_mt.restype = c.c_int64
+        L.orc_coo_to_csr_mt.argtypes = [c.c_int32, c.c_int64, i32p, i32p, c.c_int, c.c_int, c.c_int,
+                                        i64p, i32p]
+        L.orc_host_tc_mark_mt.restype = c.c_int64
+        L.orc_host_tc_mark_mt.argtypes = [c.c_int32, i64p, i32p, c.c_int]
         _lib = L
     return _lib
+
+
+def checker_threads() -> int:
+    """Threads the parallel checker functions use by default: the CPUs this
+    process may run on, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 # --------------------------------------------------------------------------
@@ -122,13 +150,16 @@ def stream_key(seed: int) -> int:
 
 
 def rmat_pairs(scale: int, seed: int, edge_factor: int = 16, first: int = 0,
-               count: int | None = None):
+               count: int | None = None, threads: int = 1):
     m = edge_factor << scale
     if count is None:
         count = m - first
     src = np.empty(count, np.int32)
     dst = np.empty(count, np.int32)
-    lib().orc_rmat_pairs(scale, first, count, seed, src, dst)
+    if threads > 1:
+        lib().orc_rmat_pairs_mt(scale, first, count, seed, threads, src, dst)
+    else:
+        lib().orc_rmat_pairs(scale, first, count, seed, src, dst)
     return src, dst
 
 
@@ -145,12 +176,15 @@ def chung_lu_table(n: int, gamma: float = 2.1):
     return cum
 
 
-def chung_lu_pairs(n: int, count: int, seed: int, gamma: float = 2.1, cum=None):
+def chung_lu_pairs(n: int, count: int, seed: int, gamma: float = 2.1, cum=None, threads: int = 1):
     if cum is None:
         cum = chung_lu_table(n, gamma)
     src = np.empty(count, np.int32)
     dst = np.empty(count, np.int32)
-    lib().orc_chung_lu_pairs(n, cum, 0, count, seed, src, dst)
+    if threads > 1:
+        lib().orc_chung_lu_pairs_mt(n, cum, 0, count, seed, threads, src, dst)
+    else:
+        lib().orc_chung_lu_pairs(n, cum, 0, count, seed, src, dst)
     return src, dst
 
 
@@ -167,21 +201,37 @@ def pick_sources(off: np.ndarray, k: int, seed: int) -> np.ndarray:
 # graph construction
 # --------------------------------------------------------------------------
 def coo_to_csr(n: int, src, dst, symmetrize=True, drop_self_loops=True, dedup=True,
-               sort=True):
+               sort=True, threads: int = 1):
+    """threads > 1 (sorted and deduplicated output only) splits the work over
+    threads with the same result."""
     src = np.ascontiguousarray(src, np.int32)
     dst = np.ascontiguousarray(dst, np.int32)
     m = src.size
     off = np.empty(n + 1, np.int64)
     adj = np.empty(max(2 * m, 1), np.int32)
-    k = lib().orc_coo_to_csr(n, m, src, dst, int(symmetrize), int(drop_self_loops),
-                             int(dedup), int(sort), off, adj)
+    if threads > 1 and dedup and sort:
+        k = lib().orc_coo_to_csr_mt(n, m, src, dst, int(symmetrize), int(drop_self_loops), threads, off, adj)
+        if k < 0:
+            raise MemoryError("orc_coo_to_csr_mt: out of host memory")
+    else:
+        k = lib().orc_coo_to_csr(n, m, src, dst, int(symmetrize), int(drop_self_loops),
+                                 int(dedup), int(sort), off, adj)
+    if k == adj.size:
+        return off, adj
     return off, adj[:k].copy()
 
 
-def rmat_graph(scale: int, seed: int, edge_factor: int = 16):
+def rmat_graph(scale: int, seed: int, edge_factor: int = 16, threads: int = 1):
     """R-MAT(a=.57,b=.19,c=.19), self loops dropped, symmetrized, deduped, sorted."""
-    s, d = rmat_pairs(scale, seed, edge_factor)
-    return coo_to_csr(1 << scale, s, d)
+    s, d = rmat_pairs(scale, seed, edge_factor, threads=threads)
+    return coo_to_csr(1 << scale, s, d, threads=threads)
+
+
+def chung_lu_graph(n: int, avg_degree: int, gamma: float, seed: int, threads: int = 1):
+    """Chung-Lu graph as the device builds it: n * avg_degree / 2 pairs, self
+    loops dropped, symmetrized, deduped, sorted."""
+    s, d = chung_lu_pairs(n, n * avg_degree // 2, seed, gamma, threads=threads)
+    return coo_to_csr(n, s, d, threads=threads)
 
 
 def directed_by_degree(off, adj):
@@ -205,16 +255,33 @@ def bfs(off, adj, source: int):
     return dist, int(scanned.value)
 
 
-def host_tc(off, adj) -> int:
+def bfs_many(off, adj, sources, threads: int | None = None):
+    """orc_bfs from each source, the sources spread over threads (the C code
+    runs without the interpreter lock); returns [(dist, edges_scanned)]."""
+    from concurrent.futures import ThreadPoolExecutor
+    off = np.ascontiguousarray(off, np.int64)
+    adj = np.ascontiguousarray(adj, np.int32)
+    srcs = [int(s) for s in sources]
+    with ThreadPoolExecutor(max_workers=threads or checker_threads()) as ex:
+        return list(ex.map(lambda s: bfs(off, adj, s), srcs))
+
+
+def host_tc(off, adj, threads: int = 1) -> int:
     n = off.size - 1
-    return int(lib().orc_host_tc(n, np.ascontiguousarray(off, np.int64),
-                                 np.ascontiguousarray(adj, np.int32)))
+    off = np.ascontiguousarray(off, np.int64)
+    adj = np.ascontiguousarray(adj, np.int32)
+    if threads > 1:
+        r = int(lib().orc_host_tc_mark_mt(n, off, adj, threads))
+        if r < 0:
+            raise MemoryError("orc_host_tc_mark_mt: out of host memory")
+        return r
+    return int(lib().orc_host_tc(n, off, adj))
 
 
-def triangles(off, adj) -> int:
+def triangles(off, adj, threads: int = 1) -> int:
     """Triangle count of the undirected simple graph stored symmetrically."""
     o, a = directed_by_degree(off, adj)
-    return host_tc(o, a)
+    return host_tc(o, a, threads)
 
 
 def tc2_per_vertex(off, adj):

@@ -27,6 +27,9 @@
  *                     hornet/include/Core/BatchUpdate/BatchUpdate.i.cuh:337-392,485-524
  *                     and BatchUpdateKernels.cuh:27-58,263-300,335-393
  *   orc_two_hop       |{v} ∪ N(v) ∪ N(N(v))| (definition owned by this build)
+ *   orc_*_mt          the same results split over POSIX threads (generators,
+ *                     COO -> CSR, host triangle count), so the configured graph
+ *                     sizes (2^22..2^25 vertices) are checked in seconds
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -504,4 +507,252 @@ void orc_chung_lu_pairs(int32_t n, const uint64_t* cum, int64_t first, int64_t c
         src[k] = cdf_pick(cum, n, (uint64_t)(((unsigned __int128)r0 * total) >> 64));
         dst[k] = cdf_pick(cum, n, (uint64_t)(((unsigned __int128)r1 * total) >> 64));
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Parallel forms of the checker.  Each returns exactly what its sequential */
+/* counterpart above returns (tests/test_oracle.py compares them); they only */
+/* split the work over POSIX threads so that graphs of 2^22..2^25 vertices   */
+/* can be checked in seconds.  Not a restatement of anything new.            */
+/* ------------------------------------------------------------------------ */
+#include <pthread.h>
+
+typedef void (*orc_range_fn)(void* ctx, int64_t lo, int64_t hi);
+
+typedef struct {
+    orc_range_fn fn;
+    void* ctx;
+    int64_t n, chunk;
+    int64_t next;   /* shared cursor, taken with an atomic add */
+} orc_pool;
+
+static void* orc_pool_worker(void* arg) {
+    orc_pool* p = (orc_pool*)arg;
+    for (;;) {
+        int64_t lo = __atomic_fetch_add(&p->next, p->chunk, __ATOMIC_RELAXED);
+        if (lo >= p->n) break;
+        int64_t hi = lo + p->chunk < p->n ? lo + p->chunk : p->n;
+        p->fn(p->ctx, lo, hi);
+    }
+    return NULL;
+}
+
+/* run fn over [0, n) in chunks taken dynamically by `threads` threads */
+static void orc_parallel(int threads, int64_t n, int64_t chunk, orc_range_fn fn, void* ctx) {
+    orc_pool p = {fn, ctx, n, chunk > 0 ? chunk : 1, 0};
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t tid[256];
+    int started = 0;
+    for (int t = 1; t < threads; t++)
+        if (pthread_create(&tid[started], NULL, orc_pool_worker, &p) == 0) started++;
+    orc_pool_worker(&p);
+    for (int t = 0; t < started; t++) pthread_join(tid[t], NULL);
+}
+
+typedef struct { int scale; int64_t first; uint64_t seed; int32_t* src; int32_t* dst; } rmat_ctx;
+static void rmat_range(void* c, int64_t lo, int64_t hi) {
+    rmat_ctx* x = (rmat_ctx*)c;
+    orc_rmat_pairs(x->scale, x->first + lo, hi - lo, x->seed, x->src + lo, x->dst + lo);
+}
+/* orc_rmat_pairs over threads (pairs are independent draws of the stream) */
+void orc_rmat_pairs_mt(int scale, int64_t first, int64_t count, uint64_t seed, int threads,
+                       int32_t* src, int32_t* dst) {
+    rmat_ctx c = {scale, first, seed, src, dst};
+    orc_parallel(threads, count, 1 << 18, rmat_range, &c);
+}
+
+typedef struct { int32_t n; const uint64_t* cum; int64_t first; uint64_t seed; int32_t* src; int32_t* dst; } cl_ctx;
+static void cl_range(void* c, int64_t lo, int64_t hi) {
+    cl_ctx* x = (cl_ctx*)c;
+    orc_chung_lu_pairs(x->n, x->cum, x->first + lo, hi - lo, x->seed, x->src + lo, x->dst + lo);
+}
+/* orc_chung_lu_pairs over threads */
+void orc_chung_lu_pairs_mt(int32_t n, const uint64_t* cum, int64_t first, int64_t count, uint64_t seed,
+                           int threads, int32_t* src, int32_t* dst) {
+    cl_ctx c = {n, cum, first, seed, src, dst};
+    orc_parallel(threads, count, 1 << 18, cl_range, &c);
+}
+
+/* COO -> CSR with symmetrize / self-loop drop, always sorted and deduplicated
+ * (orc_coo_to_csr with dedup = sort = 1), as a parallel MSD pass: the entries
+ * are scattered into buckets of 2^bshift consecutive source ids (per-thread
+ * histograms, so each thread writes its own run of every bucket), then every
+ * bucket is radix-sorted and deduplicated on its own and written out. */
+#define CSR_MAX_THREADS 64
+typedef struct {
+    const int32_t* src; const int32_t* dst; int sym, drop;
+    int64_t m; int T; int bshift; int32_t n; int64_t nb;
+    int64_t* hist;        /* T x nb: entries thread t puts in bucket b, then its write cursor */
+    int64_t* boff;        /* nb + 1 bucket starts */
+    uint64_t* keys;       /* 2m entries in bucket order */
+    uint64_t* tmp;        /* radix scratch, 2m */
+    int64_t* bcnt;        /* per bucket: distinct entries */
+    int64_t* off;
+    int32_t* adj;
+    int key_bits;
+} csrb_ctx;
+
+static inline int64_t csrb_lo(const csrb_ctx* x, int t) { return x->m * t / x->T; }
+
+static void csrb_hist(void* c, int64_t lo, int64_t hi) {
+    csrb_ctx* x = (csrb_ctx*)c;
+    for (int64_t t = lo; t < hi; t++) {
+        int64_t* h = x->hist + t * x->nb;
+        for (int64_t i = csrb_lo(x, (int)t); i < csrb_lo(x, (int)t + 1); i++) {
+            int32_t s = x->src[i], d = x->dst[i];
+            if (s == d && x->drop) continue;
+            h[s >> x->bshift]++;
+            if (x->sym && s != d) h[d >> x->bshift]++;
+        }
+    }
+}
+static void csrb_scatter(void* c, int64_t lo, int64_t hi) {
+    csrb_ctx* x = (csrb_ctx*)c;
+    for (int64_t t = lo; t < hi; t++) {
+        int64_t* h = x->hist + t * x->nb;
+        for (int64_t i = csrb_lo(x, (int)t); i < csrb_lo(x, (int)t + 1); i++) {
+            int32_t s = x->src[i], d = x->dst[i];
+            if (s == d && x->drop) continue;
+            x->keys[h[s >> x->bshift]++] = pack(s, d);
+            if (x->sym && s != d) x->keys[h[d >> x->bshift]++] = pack(d, s);
+        }
+    }
+}
+/* LSD radix sort of a[0..len) on the key bits (source low bits and
+ * destination), 11-bit digits, scratch t of len entries */
+static void csrb_sort(uint64_t* a, uint64_t* t, int64_t len, int bshift, int dbits) {
+    /* sort key: (src & (2^bshift - 1)) << dbits | dst */
+    const int bits = bshift + dbits;
+    int64_t cnt[2048];
+    uint64_t* from = a;
+    uint64_t* to = t;
+    for (int sh = 0; sh < bits; sh += 11) {
+        memset(cnt, 0, sizeof(cnt));
+        for (int64_t i = 0; i < len; i++) {
+            uint64_t k = ((from[i] >> 32) & ((1ull << bshift) - 1)) << dbits | (from[i] & 0xFFFFFFFFull);
+            cnt[(k >> sh) & 2047]++;
+        }
+        int64_t s = 0;
+        for (int d = 0; d < 2048; d++) { int64_t c = cnt[d]; cnt[d] = s; s += c; }
+        for (int64_t i = 0; i < len; i++) {
+            uint64_t k = ((from[i] >> 32) & ((1ull << bshift) - 1)) << dbits | (from[i] & 0xFFFFFFFFull);
+            to[cnt[(k >> sh) & 2047]++] = from[i];
+        }
+        uint64_t* x = from; from = to; to = x;
+    }
+    if (from != a) memcpy(a, from, sizeof(uint64_t) * (size_t)len);
+}
+static void csrb_sort_buckets(void* c, int64_t lo, int64_t hi) {
+    csrb_ctx* x = (csrb_ctx*)c;
+    for (int64_t b = lo; b < hi; b++) {
+        uint64_t* a = x->keys + x->boff[b];
+        int64_t len = x->boff[b + 1] - x->boff[b];
+        if (len > 1) csrb_sort(a, x->tmp + x->boff[b], len, x->bshift, x->key_bits);
+        int64_t w = 0;
+        for (int64_t i = 0; i < len; i++)
+            if (w == 0 || a[i] != a[w - 1]) a[w++] = a[i];
+        x->bcnt[b] = w;
+        /* degrees of the bucket's vertices (off[v + 1] is filled here, scanned later) */
+        for (int64_t i = 0; i < w; i++) x->off[(a[i] >> 32) + 1]++;
+    }
+}
+static void csrb_write(void* c, int64_t lo, int64_t hi) {
+    csrb_ctx* x = (csrb_ctx*)c;
+    for (int64_t b = lo; b < hi; b++) {
+        const uint64_t* a = x->keys + x->boff[b];
+        int64_t v0 = b << x->bshift;
+        int64_t o = x->off[v0 < x->n ? v0 : x->n];
+        for (int64_t i = 0; i < x->bcnt[b]; i++) x->adj[o + i] = (int32_t)(uint32_t)a[i];
+    }
+}
+
+/* Returns the number of entries (adj must hold 2m); -1 when out of memory. */
+int64_t orc_coo_to_csr_mt(int32_t n, int64_t m, const int32_t* src, const int32_t* dst, int symmetrize,
+                          int drop_self_loops, int threads, int64_t* off, int32_t* adj) {
+    int T = threads < 1 ? 1 : (threads > CSR_MAX_THREADS ? CSR_MAX_THREADS : threads);
+    int nbits = 0;
+    while ((1ll << nbits) < (int64_t)n) nbits++;
+    int bshift = nbits > 12 ? nbits - 12 : 0;
+    int64_t nb = (((int64_t)n + (1ll << bshift) - 1) >> bshift) + 1;
+    csrb_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.src = src; c.dst = dst; c.sym = symmetrize; c.drop = drop_self_loops;
+    c.m = m; c.T = T; c.bshift = bshift; c.n = n; c.nb = nb; c.off = off; c.adj = adj;
+    c.key_bits = nbits > 0 ? nbits : 1;
+    c.hist = (int64_t*)calloc((size_t)(T * nb), sizeof(int64_t));
+    c.boff = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
+    c.bcnt = (int64_t*)malloc(sizeof(int64_t) * (size_t)nb);
+    c.keys = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(2 * m + 1));
+    c.tmp = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(2 * m + 1));
+    if (!c.hist || !c.boff || !c.bcnt || !c.keys || !c.tmp) {
+        free(c.hist); free(c.boff); free(c.bcnt); free(c.keys); free(c.tmp);
+        return -1;
+    }
+    orc_parallel(T, T, 1, csrb_hist, &c);
+    /* bucket starts, and each thread's cursor inside every bucket */
+    int64_t s = 0;
+    for (int64_t b = 0; b < nb; b++) {
+        c.boff[b] = s;
+        for (int t = 0; t < T; t++) {
+            int64_t h = c.hist[t * nb + b];
+            c.hist[t * nb + b] = s;
+            s += h;
+        }
+    }
+    c.boff[nb] = s;
+    orc_parallel(T, T, 1, csrb_scatter, &c);
+    memset(off, 0, sizeof(int64_t) * ((size_t)n + 1));
+    orc_parallel(T, nb, 1, csrb_sort_buckets, &c);
+    for (int32_t v = 0; v < n; v++) off[v + 1] += off[v];
+    orc_parallel(T, nb, 16, csrb_write, &c);
+    int64_t k = off[n];
+    free(c.hist); free(c.boff); free(c.bcnt); free(c.keys); free(c.tmp);
+    return k;
+}
+
+/* orc_host_tc (the sum over stored edges (u,v) of |N(u) ∩ N(v)|) with the
+ * intersections taken against a per-thread byte mark of N(u) instead of merges:
+ * sum over u, over v in N(u), of #{w in N(v) : w in N(u)}.  Same value, fewer
+ * operations on lists of very different lengths. */
+typedef struct { int32_t n; const int64_t* off; const int32_t* adj; int64_t sum; int64_t next; } tcm_ctx;
+static void* tcm_worker(void* arg) {
+    tcm_ctx* x = (tcm_ctx*)arg;
+    uint8_t* mark = (uint8_t*)calloc((size_t)x->n + 1, 1);
+    int64_t s = 0;
+    if (!mark) return (void*)1;
+    for (;;) {
+        int64_t lo = __atomic_fetch_add(&x->next, 256, __ATOMIC_RELAXED);
+        if (lo >= x->n) break;
+        int64_t hi = lo + 256 < x->n ? lo + 256 : x->n;
+        for (int64_t u = lo; u < hi; u++) {
+            const int64_t a = x->off[u], b = x->off[u + 1];
+            if (b - a < 1) continue;
+            for (int64_t j = a; j < b; j++) mark[x->adj[j]] = 1;
+            for (int64_t j = a; j < b; j++) {
+                const int32_t v = x->adj[j];
+                for (int64_t t = x->off[v]; t < x->off[v + 1]; t++) s += mark[x->adj[t]];
+            }
+            for (int64_t j = a; j < b; j++) mark[x->adj[j]] = 0;
+        }
+    }
+    free(mark);
+    __atomic_fetch_add(&x->sum, s, __ATOMIC_RELAXED);
+    return NULL;
+}
+/* Returns -1 when a thread could not allocate its mark array. */
+int64_t orc_host_tc_mark_mt(int32_t n, const int64_t* off, const int32_t* adj, int threads) {
+    tcm_ctx c = {n, off, adj, 0, 0};
+    int T = threads < 1 ? 1 : (threads > 256 ? 256 : threads);
+    pthread_t tid[256];
+    int started = 0, failed = 0;
+    for (int t = 1; t < T; t++)
+        if (pthread_create(&tid[started], NULL, tcm_worker, &c) == 0) started++;
+    if (tcm_worker(&c)) failed = 1;
+    for (int t = 0; t < started; t++) {
+        void* r = NULL;
+        pthread_join(tid[t], &r);
+        if (r) failed = 1;
+    }
+    return failed ? -1 : c.sum;
 }

@@ -368,3 +368,77 @@ def test_meta_batch_duplicates_keep_first():
     o, a, (w,), added = O.insert_meta(off, adj, [np.array([], np.int64)], [1, 1], [2, 2],
                                       [np.array([7, 8])])
     assert a.tolist() == [2, 2] and w.tolist() == [7, 8]
+
+
+# ------------------------------------------------- hand-derived rule pins
+def test_tc2_credit_rule_hand_case():
+    """TriangleCounting2 credit rule on a stored directed graph, worked out by
+    hand from primitives/Operator++.i.cuh:120-131 and triangle2.cu:121: skip
+    src == dst and deg(src) < 2; credit |N(s) ∩ N(d)| to the endpoint of
+    smaller degree, ties to the destination.
+
+    rows: 0:[1,2] 1:[2,3] 2:[3] 3:[3] 4:[4,5] 5:[]
+      (0,1) deg 2/2 tie -> 1 gets |{1,2}∩{2,3}| = 1
+      (0,2) 2 > 1 -> 2 gets 0;  (1,2) -> 2 gets 1;  (1,3) -> 3 gets 1
+      (2,3) deg(2) = 1 -> skipped (else 3 would get 1 more)
+      (3,3), (4,4) src == dst -> skipped (else 4 would get 2)
+      (4,5) -> 5 gets 0
+    Ties to the source would give [1,0,1,1,0,0]; crediting the larger
+    endpoint [*,2,0,0,..]."""
+    off = np.array([0, 2, 4, 5, 6, 8, 8], np.int64)
+    adj = np.array([1, 2, 2, 3, 3, 3, 4, 5], np.int32)
+    assert O.tc2_per_vertex(off, adj).tolist() == [0, 1, 1, 1, 0, 0]
+
+
+def test_directed_by_degree_equal_degree_tie():
+    """GraphStd.cpp:265 keeps u->v iff deg u < deg v, or deg u == deg v and
+    u < v.  Cycle 0-1-2-3-0 plus the pendant 4-2 (degrees 2,2,3,2,1), by hand:
+    0->1 and 0->3 (ties, lower id first), 1->2, 3->2, 4->2."""
+    pairs = [(0, 1), (1, 2), (2, 3), (3, 0), (2, 4)]
+    s = np.array([p[0] for p in pairs], np.int32)
+    d = np.array([p[1] for p in pairs], np.int32)
+    off, adj = O.coo_to_csr(5, s, d)
+    assert np.diff(off).tolist() == [2, 2, 3, 2, 1]
+    do, da = O.directed_by_degree(off, adj)
+    assert do.tolist() == [0, 2, 3, 3, 4, 5] and da.tolist() == [1, 3, 2, 2, 2]
+
+
+def test_gmtx_max_degree_tie_hand():
+    """example/G.mtx read UNDIRECTED (0-based): degrees worked out by hand from
+    its 12 arcs are [3,1,4,4,3,1,2,2,2]; vertices 2 and 3 tie at 4 and
+    std::max_element takes the first (GraphStd.i.hpp:296-299): 2."""
+    n, s, d, _ = O.read_market(os.path.join(GOLD, "G.mtx"))
+    off, adj = O.coo_to_csr(n, s, d, symmetrize=True, drop_self_loops=False, dedup=True)
+    assert np.diff(off).tolist() == [3, 1, 4, 4, 3, 1, 2, 2, 2]
+    assert O.max_degree_id(off) == 2
+
+
+# --------------------------------- parallel checker == sequential checker
+@pytest.mark.parametrize("threads", [2, 5])
+def test_parallel_checker_matches_sequential(threads):
+    s1, d1 = O.rmat_pairs(13, seed=5)
+    s2, d2 = O.rmat_pairs(13, seed=5, threads=threads)
+    assert np.array_equal(s1, s2) and np.array_equal(d1, d2)
+    n = 1 << 13
+    # keep some self loops and repeats in the input
+    s1 = np.concatenate([s1, np.arange(50, dtype=np.int32), s1[:1000]])
+    d1 = np.concatenate([d1, np.arange(50, dtype=np.int32), d1[:1000]])
+    for sym in (True, False):
+        for drop in (True, False):
+            a = O.coo_to_csr(n, s1, d1, symmetrize=sym, drop_self_loops=drop)
+            b = O.coo_to_csr(n, s1, d1, symmetrize=sym, drop_self_loops=drop, threads=threads)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    off, adj = O.rmat_graph(13, seed=5)
+    o2, a2 = O.rmat_graph(13, seed=5, threads=threads)
+    assert np.array_equal(off, o2) and np.array_equal(adj, a2)
+    do, da = O.directed_by_degree(off, adj)
+    assert O.host_tc(do, da) == O.host_tc(do, da, threads=threads)
+    assert O.triangles(off, adj) == O.triangles(off, adj, threads=threads)
+    cs, cd = O.chung_lu_pairs(1 << 12, 1 << 15, seed=3)
+    cs2, cd2 = O.chung_lu_pairs(1 << 12, 1 << 15, seed=3, threads=threads)
+    assert np.array_equal(cs, cs2) and np.array_equal(cd, cd2)
+    srcs = O.pick_sources(off, 6, seed=1)
+    many = O.bfs_many(off, adj, srcs, threads=threads)
+    for s, (dist, sc) in zip(srcs, many):
+        ed, esc = O.bfs(off, adj, int(s))
+        assert np.array_equal(dist, ed) and sc == esc
