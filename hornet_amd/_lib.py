@@ -63,6 +63,12 @@ class HnBfsStats(ctypes.Structure):
                 ("pad_", ctypes.c_int32)]
 
 
+class HnUpdateStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("applied", ctypes.c_int64), ("touched", ctypes.c_int64),
+                ("relocated", ctypes.c_int64), ("touched_deg", ctypes.c_int64), ("relocated_deg", ctypes.c_int64),
+                ("moved", ctypes.c_int64), ("kind", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
 class HnMsbfsStats(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_int32), ("push_levels", ctypes.c_int32),
                 ("pull_levels", ctypes.c_int32), ("pad_", ctypes.c_int32), ("edges", ctypes.c_int64)]
@@ -117,16 +123,20 @@ SIGNATURES = {
     "hn_graph_memory": [P, PI64, PI64, P],
     "hn_graph_export_coo": [P, P, P, I64, PI64, P],
     "hn_graph_export_csr": [P, P, P, I64, P],
+    "hn_graph_export_csr_meta": [P, P, P, P, I64, P],
     "hn_graph_degrees": [P, P, P],
     "hn_graph_snapshot": [P, D, P, PP],
     "hn_graph_max_degree": [P, PI32, PI32, P],
     "hn_graph_insert": [P, P, P, I64, U32, PI64, P],
     "hn_graph_erase": [P, P, P, I64, U32, PI64, P],
+    "hn_graph_update_stats": [P, ctypes.POINTER(HnUpdateStats), P],
     "hn_bfs_topdown": [P, I32, P, ctypes.POINTER(HnBfsStats), P],
     "hn_bfs_multi": [P, P, I32, P, ctypes.POINTER(HnMsbfsStats), P],
     "hn_bfs_multi_depth": [P, P, P, I32, P, ctypes.POINTER(HnMsbfsStats), P],
     "hn_bfs_multi_params": [P, I32, I32],
     "hn_bfs_multi_tune": [P, P, I32, P, PI32, PI32, P],
+    "hn_bfs_multi_lb_tile": [P, I32],
+    "hn_bfs_multi_get_params": [P, PI32, PI32, PI32],
     "hn_tc_count": [P, PU64, P, P],
     "hn_tc2_stored": [P, PU64, P, P],
     "hn_gen_rmat_pairs": [I32, I64, I64, U64, P, P, P],

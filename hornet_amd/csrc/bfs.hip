@@ -39,7 +39,7 @@
 //      so every vertex enters a frontier once;
 //   5. one 12-byte readback (level totals + next frontier size).
 // Distances: source 0, its neighbours 1, unreached INT32_MAX (TopDown.cu:42,81).
-#include "graph.hpp"
+#include "bfs.hpp"
 #include "prims.hpp"
 
 namespace hn {
@@ -87,9 +87,11 @@ __global__ void bfs_init_kernel(int32_t nv, int32_t* __restrict__ dist, uint32_t
 }
 
 // (degree, adjacency offset) of every frontier vertex, one record gather each
+// (the next-frontier counter is cleared here, before this level's collect)
 __global__ void bfs_gather_kernel(int64_t F, const int32_t* __restrict__ q, const hn_vrec* __restrict__ rec,
-                                  int32_t* __restrict__ qdeg, int64_t* __restrict__ qoff) {
+                                  int32_t* __restrict__ qdeg, int64_t* __restrict__ qoff, uint32_t* __restrict__ cnt) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *cnt = 0u;
     if (i >= F) return;
     const hn_vrec r = rec[q[i]];
     qdeg[i] = r.deg;
@@ -272,70 +274,58 @@ __global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, const
 
 }  // namespace
 
-// scratch layout inside g->s_bfs
-// The frontier needs no ping-pong: q is gathered into (qdeg, qoff) and
-// scanned into P / tile_lo before collect overwrites q with level L+1.
-struct BfsBufs {
-    uint32_t* bitmap;   // visited as of the start of the level
-    int32_t* q;
-    int32_t* qdeg;
-    int64_t* qoff;
-    uint32_t* P;
-    uint32_t* tile_lo;
-    uint8_t* state;     // one byte per vertex, padded to whole bitmap words
-    uint64_t* tot;      // [0] level totals (edges << 32 | groups)
-    uint32_t* cnt;      // [0] next frontier size
-};
-
-static hn_status bfs_buffers(hn_graph* g, BfsBufs* b) {
-    const int64_t nv = g->nv > 0 ? g->nv : 1;
+// Lays the buffers of a traversal of a graph with nv vertices and ne edges
+// out from `base` (nullptr: sizes only); returns the bytes they take.
+int64_t bfs_layout(int64_t nv_in, int64_t ne, char* base, BfsBufs* b) {
+    const int64_t nv = nv_in > 0 ? nv_in : 1;
     const int64_t words = (nv + 31) / 32;
-    const int64_t tiles = (g->ne / EXP_G + nv) / EXP_TILE + 2;
+    const int64_t tiles = (ne / EXP_G + nv) / EXP_TILE + 2;
     auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
-    int64_t o_bitmap = 0;
-    int64_t o_q = al(o_bitmap + words * 4);
-    int64_t o_qdeg = al(o_q + nv * 4);
-    int64_t o_qoff = al(o_qdeg + nv * 4);
-    int64_t o_P = al(o_qoff + nv * 8);
-    int64_t o_tile = al(o_P + (nv + 1) * 4);
-    int64_t o_state = al(o_tile + tiles * 4);
-    int64_t o_tot = al(o_state + words * 32);
-    int64_t o_cnt = al(o_tot + 8);
-    int64_t total = al(o_cnt + 64);
-    HN_TRY(g->s_bfs.reserve((size_t)total));
-    char* base = g->s_bfs.as<char>();
-    b->bitmap = reinterpret_cast<uint32_t*>(base + o_bitmap);
-    b->q = reinterpret_cast<int32_t*>(base + o_q);
-    b->qdeg = reinterpret_cast<int32_t*>(base + o_qdeg);
-    b->qoff = reinterpret_cast<int64_t*>(base + o_qoff);
-    b->P = reinterpret_cast<uint32_t*>(base + o_P);
-    b->tile_lo = reinterpret_cast<uint32_t*>(base + o_tile);
-    b->state = reinterpret_cast<uint8_t*>(base + o_state);
-    b->tot = reinterpret_cast<uint64_t*>(base + o_tot);
-    b->cnt = reinterpret_cast<uint32_t*>(base + o_cnt);
-    HN_TRY(g->s_scan.reserve(scan_temp_bytes(nv)));
-    return HN_OK;
+    const int64_t o_bitmap = 0;
+    const int64_t o_q = al(o_bitmap + words * 4);
+    const int64_t o_qdeg = al(o_q + nv * 4);
+    const int64_t o_qoff = al(o_qdeg + nv * 4);
+    const int64_t o_P = al(o_qoff + nv * 8);
+    const int64_t o_tile = al(o_P + (nv + 1) * 4);
+    const int64_t o_state = al(o_tile + tiles * 4);
+    const int64_t o_tot = al(o_state + words * 32);
+    const int64_t o_cnt = al(o_tot + 8);
+    const int64_t o_scan = al(o_cnt + 64);
+    const int64_t total = al(o_scan + (int64_t)scan_temp_bytes(nv));
+    if (base) {
+        b->bitmap = reinterpret_cast<uint32_t*>(base + o_bitmap);
+        b->q = reinterpret_cast<int32_t*>(base + o_q);
+        b->qdeg = reinterpret_cast<int32_t*>(base + o_qdeg);
+        b->qoff = reinterpret_cast<int64_t*>(base + o_qoff);
+        b->P = reinterpret_cast<uint32_t*>(base + o_P);
+        b->tile_lo = reinterpret_cast<uint32_t*>(base + o_tile);
+        b->state = reinterpret_cast<uint8_t*>(base + o_state);
+        b->tot = reinterpret_cast<uint64_t*>(base + o_tot);
+        b->cnt = reinterpret_cast<uint32_t*>(base + o_cnt);
+        b->scan_tmp = base + o_scan;
+    }
+    return total;
 }
 
-}  // namespace hn
+__global__ void zero_u32_kernel(int64_t n, uint32_t* __restrict__ p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
 
-using namespace hn;
-
-extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats,
-                                    void* stream) {
-    HN_ARG(g && dist, "null argument");
-    HN_ARG(source >= 0 && source < g->nv, "source outside [0, nv)");
-    HN_ARG(g->ne < (int64_t)0xFFFFFFFF, "edge count exceeds the 32-bit work prefix");
-    hipStream_t st = as_stream(stream);
-    BfsBufs b;
-    HN_TRY(bfs_buffers(g, &b));
+// The level loop over laid-out buffers.  Nothing here allocates (kernel
+// timing, which creates two events, only with `may_time`).
+hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats, hipStream_t st,
+                  const BfsBufs& b, bool may_time) {
     const int64_t nv = g->nv;
     const int64_t words = (nv + 31) / 32;
-    HN_HIP(hipMemsetAsync(b.state, 0, (size_t)words * 32, st));
+    // state starts at zero (a fill kernel, not a memset node, so a captured
+    // copy of this sequence replays correctly)
+    zero_u32_kernel<<<(unsigned)ceil_div(words * 8, BLOCK), BLOCK, 0, st>>>(words * 8,
+                                                                          reinterpret_cast<uint32_t*>(b.state));
     bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(g->nv, dist, b.bitmap, b.state, source, b.q);
     HN_CHECK_LAUNCH();
     const int64_t grid_cap = (int64_t)num_cus() * 64;
-    const bool timed = stats && stats->time_kernels;
+    const bool timed = may_time && stats && stats->time_kernels;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (timed) {
         HN_HIP(hipEventCreate(&ev0));
@@ -350,11 +340,10 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
     while (F > 0) {
         visited += F;
         nlevels++;
-        bfs_gather_kernel<<<(unsigned)ceil_div(F, BLOCK), BLOCK, 0, st>>>(F, b.q, g->rec, b.qdeg, b.qoff);
-        rc = exclusive_scan_fn<uint64_t>(FrontierIn{b.qdeg}, FrontierOut{b.P, b.tile_lo}, b.tot, F,
-                                         g->s_scan.ptr, st);
+        bfs_gather_kernel<<<(unsigned)ceil_div(F, BLOCK), BLOCK, 0, st>>>(F, b.q, g->rec, b.qdeg, b.qoff, b.cnt);
+        rc = exclusive_scan_fn<uint64_t>(FrontierIn{b.qdeg}, FrontierOut{b.P, b.tile_lo}, b.tot, F, b.scan_tmp, st);
         if (rc != HN_OK) break;
-        hipError_t e = hipMemsetAsync(b.cnt, 0, 4, st);
+        hipError_t e = hipGetLastError();
         if (timed && e == hipSuccess) e = hipEventRecord(ev0, st);
         if (e == hipSuccess) {
             bfs_expand_kernel<EXP_G><<<(unsigned)grid_cap, BLOCK, 0, st>>>(b.tot, b.P, b.tile_lo, b.qdeg, b.qoff,
@@ -399,3 +388,19 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
     }
     return HN_OK;
 }
+
+}  // namespace hn
+
+using namespace hn;
+
+extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats,
+                                    void* stream) {
+    HN_ARG(g && dist, "null argument");
+    HN_ARG(source >= 0 && source < g->nv, "source outside [0, nv)");
+    HN_ARG(g->ne < (int64_t)0xFFFFFFFF, "edge count exceeds the 32-bit work prefix");
+    BfsBufs b;
+    HN_TRY(g->s_bfs.reserve((size_t)bfs_layout(g->nv, g->ne, nullptr, &b)));
+    bfs_layout(g->nv, g->ne, g->s_bfs.as<char>(), &b);
+    return bfs_run(g, source, dist, stats, as_stream(stream), b, true);
+}
+

@@ -259,6 +259,7 @@ __global__ void init_alloc_kernel(AllocState* st, unsigned long long bump, unsig
     }
     st->error = 0;
     st->pad = 0;
+    st->want_units = 0;
 }
 
 // ---- COO -> CSR on the device
@@ -386,6 +387,69 @@ static hn_status setup_arena(hn_graph* g, uint64_t used_units, double slack, hip
     HN_TRY(dalloc(&g->alloc, 1));
     init_alloc_kernel<<<1, 1, 0, st>>>(g->alloc, used_units, cap);
     HN_CHECK_LAUNCH();
+    return HN_OK;
+}
+
+hn_status grow_arena(hn_graph* g, uint64_t need_units, hipStream_t st) {
+    AllocState a;
+    HN_TRY(read_back(g, g->alloc, sizeof(AllocState), &a, st));
+    need_units = std::max<uint64_t>(need_units, a.want_units);
+    const uint64_t cap = (uint64_t)g->cap_units;
+    uint64_t want = std::max<uint64_t>(2 * cap, a.bump_units + need_units + 4096);
+    if (want > 0xFFFFFFFFull) want = 0xFFFFFFFFull;   // unit offsets are 32-bit
+    if (want <= cap || want < a.bump_units + need_units) {
+        set_error("edge arena at its limit of 2^32 units of 128 B");
+        return HN_E_OOM;
+    }
+    int32_t* arena = nullptr;
+    uint32_t* stacks = nullptr;
+    char* meta[HN_MAX_EDGE_META] = {};
+    auto fail = [&](hn_status s) {
+        dfree(arena);
+        dfree(stacks);
+        for (int f = 0; f < g->meta.n; f++) dfree(meta[f]);
+        set_error("edge arena growth to " + std::to_string(want) + " units: " + hn_last_error());
+        return s;
+    };
+    hn_status s;
+    if ((s = dalloc(&arena, (size_t)want * UNIT_SLOTS)) != HN_OK) return fail(s);
+    for (int f = 0; f < g->meta.n; f++)
+        if ((s = dmalloc(reinterpret_cast<void**>(&meta[f]), (size_t)want * UNIT_SLOTS * (size_t)g->meta.bytes[f])) !=
+            HN_OK)
+            return fail(s);
+    int64_t off[NCLASS];
+    int64_t tot = 0;
+    for (int k = 0; k < NCLASS; k++) {
+        off[k] = tot;
+        if (k >= MIN_CLASS) tot += (int64_t)(want >> (k - MIN_CLASS)) + 1;
+    }
+    if ((s = dalloc(&stacks, (size_t)tot)) != HN_OK) return fail(s);
+    // blocks below the bump pointer keep their offsets; the free stacks keep
+    // their entries, at their classes' new starts
+    const size_t used = (size_t)a.bump_units * UNIT_SLOTS;
+    if (used) {
+        HN_HIP(hipMemcpyAsync(arena, g->arena, used * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+        for (int f = 0; f < g->meta.n; f++)
+            HN_HIP(hipMemcpyAsync(meta[f], g->meta.p[f], used * (size_t)g->meta.bytes[f], hipMemcpyDeviceToDevice, st));
+    }
+    for (int k = MIN_CLASS; k < NCLASS; k++)
+        if (a.free_count[k])
+            HN_HIP(hipMemcpyAsync(stacks + off[k], g->stacks + g->h_stack_off[k], sizeof(uint32_t) * a.free_count[k],
+                                  hipMemcpyDeviceToDevice, st));
+    HN_HIP(hipMemcpyAsync(g->stack_off, off, sizeof(off), hipMemcpyHostToDevice, st));
+    const unsigned long long cap64 = want;
+    HN_HIP(hipMemcpyAsync(&g->alloc->cap_units, &cap64, sizeof(cap64), hipMemcpyHostToDevice, st));
+    HN_HIP(hipStreamSynchronize(st));
+    dfree(g->arena);
+    dfree(g->stacks);
+    for (int f = 0; f < g->meta.n; f++) {
+        dfree(g->meta.p[f]);
+        g->meta.p[f] = meta[f];
+    }
+    g->arena = arena;
+    g->stacks = stacks;
+    g->cap_units = (int64_t)want;
+    std::memcpy(g->h_stack_off, off, sizeof(off));
     return HN_OK;
 }
 
@@ -638,6 +702,9 @@ hn_status degree_prefix(hn_graph* g, int64_t* d_eoff, hipStream_t st) {
 }
 
 hn_status graph_is_symmetric(hn_graph* g, hipStream_t st, bool* yes) {
+    // the serving loop's slot threads call this concurrently on one graph: the
+    // check (which reserves the shared s_misc scratch) runs under a lock
+    std::lock_guard<std::mutex> lock(g->sym_mu);
     if (g->sym == 1 || (g->sym == -1 && g->sym_version == g->version)) {
         *yes = g->sym == 1;
         return HN_OK;
@@ -717,7 +784,6 @@ hn_status hn_graph_create_csr_meta(int32_t nv, int64_t ne, const void* offsets, 
         in_meta.n = layout->nfields;
         for (int f = 0; f < in_meta.n; f++) {
             HN_ARG(layout->bytes[f] == 4 || layout->bytes[f] == 8, "meta-data fields are 4 or 8 bytes");
-            HN_ARG(ne == 0 || meta[f] != nullptr, "a meta-data array is null");
             in_meta.bytes[f] = layout->bytes[f];
         }
     }
@@ -757,12 +823,14 @@ hn_status hn_graph_create_csr_meta(int32_t nv, int64_t ne, const void* offsets, 
     }
     const int32_t* adj_in = edges;
     MetaSet dmeta = in_meta;
+    // a null field array reads as zeros
     for (int f = 0; f < in_meta.n; f++) dmeta.p[f] = const_cast<char*>(static_cast<const char*>(meta[f]));
     if (!dev) {
         if ((s = dalloc(&d_adj, (size_t)(ne > 0 ? ne : 1))) != HN_OK) { release(); return s; }
         if (ne > 0) HN_HIP(hipMemcpyAsync(d_adj, edges, sizeof(int32_t) * (size_t)ne, hipMemcpyHostToDevice, st));
         adj_in = d_adj;
         for (int f = 0; f < up.n; f++) {
+            if (!meta[f]) continue;
             const size_t bytes = (size_t)(ne > 0 ? ne : 1) * (size_t)up.bytes[f];
             if ((s = dmalloc(reinterpret_cast<void**>(&up.p[f]), bytes)) != HN_OK) { release(); return s; }
             if (ne > 0)
@@ -899,6 +967,17 @@ hn_status hn_graph_export_csr(hn_graph* g, int64_t* offsets, int32_t* adj, int64
         return HN_E_CAPACITY;
     }
     return flatten(g, nullptr, adj, offsets, as_stream(stream));
+}
+
+hn_status hn_graph_export_csr_meta(hn_graph* g, int64_t* offsets, int32_t* adj, void* const* meta, int64_t cap,
+                                   void* stream) {
+    HN_ARG(g && offsets && adj, "null argument");
+    if (cap < g->ne) {
+        set_error("export_csr: capacity too small");
+        return HN_E_CAPACITY;
+    }
+    MetaSet out = meta_like(g->meta, meta ? reinterpret_cast<char* const*>(meta) : nullptr);
+    return flatten(g, nullptr, adj, offsets, as_stream(stream), meta ? out : MetaSet());
 }
 
 hn_status hn_graph_snapshot(hn_graph* g, double slack, void* stream, hn_graph** out) {

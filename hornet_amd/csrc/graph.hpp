@@ -19,6 +19,8 @@
 // commits the stack depths, and frees are pushed after the data has moved.
 // No host round trip per vertex.
 #pragma once
+#include <mutex>
+
 #include "common.hpp"
 
 namespace hn {
@@ -35,6 +37,7 @@ struct AllocState {
     unsigned int take[NCLASS];        // tickets handed out in the current batch
     unsigned int error;               // bit 0: arena exhausted
     unsigned int pad;
+    unsigned long long want_units;    // fresh units the last batch that exhausted the arena asked for
 };
 
 struct AllocView {
@@ -160,6 +163,7 @@ struct hn_graph {
     // -1 known not (for sym_version), 0 unknown
     int32_t sym = 0;
     uint64_t sym_version = 0;
+    std::mutex sym_mu;                // serialises the symmetry check (serving slots call it concurrently)
     hn_vrec* rec = nullptr;           // device, nv records
     int32_t* arena = nullptr;         // device, cap_units * 32 slots
     hn::MetaSet meta;                 // device, edge meta-data parallel to the arena
@@ -177,6 +181,15 @@ struct hn_graph {
     hn::Scratch s_mstage;             // staging of edge meta-data during updates
     hn::Scratch s_stamp;              // per vertex: its source segment in the current batch (unchecked)
     uint64_t* h_pinned = nullptr;     // small pinned readback area
+    // the last applied batch, for hn_graph_update_stats (pointers into s_upd,
+    // valid until the next batch)
+    int32_t last_kind = 0;            // 0 none, 1 insert, 2 erase
+    int64_t last_submitted = 0;
+    int64_t last_applied = 0;
+    const void* last_ti = nullptr;    // per touched vertex (update.hip TouchInfo)
+    const uint32_t* last_ucnt = nullptr;
+    const uint8_t* last_skip = nullptr;
+    const int64_t* last_moved = nullptr;
     void* ms = nullptr;               // multi-source BFS engine (msbfs.hip)
     void (*ms_release)(void*) = nullptr;
 
@@ -209,6 +222,17 @@ hn_status csr_from_device_coo(int32_t nv, int64_t m, const int32_t* src, const i
                               int64_t* ne_out);
 // exclusive prefix of the record degrees into d_eoff[nv+1] (int64)
 hn_status degree_prefix(hn_graph* g, int64_t* d_eoff, hipStream_t st);
+// Internal status (never returned through the C ABI): a batch found the edge
+// arena full and changed nothing; the caller grows the arena and retries.
+constexpr hn_status HN_E_ARENA_FULL = 100;
+// Grow the edge arena (and the meta-data arrays and free stacks) so that at
+// least `need_units` fresh units (and at least what the last batch that
+// exhausted the arena asked for) fit beyond the bump pointer; capacity at
+// least doubles.  Blocks keep their unit offsets, so the vertex records stay
+// valid; the arena (and every pointer into it, such as a device view) moves.
+// HN_E_OOM when device memory or the 2^32-unit offset range is exhausted
+// (the graph is then unchanged).  Synchronises.
+hn_status grow_arena(hn_graph* g, uint64_t need_units, hipStream_t st);
 // synchronous readback of a few bytes via the handle's pinned buffer
 hn_status read_back(hn_graph* g, const void* dptr, size_t bytes, void* host, hipStream_t st);
 // batched BFS of k <= 64 sources (host array), optional per-source depth

@@ -43,6 +43,8 @@
 #include <cstring>
 #include <vector>
 
+#include <type_traits>
+
 #include "graph.hpp"
 #include "prims.hpp"
 
@@ -51,9 +53,11 @@ namespace {
 
 constexpr int32_t INF = 0x7fffffff;
 constexpr int MS_G = 4;                          // adjacency slots per work unit (16 bytes)
-constexpr int MS_IPT = 4;                        // work units per lane per tile
-constexpr int MS_TILE = BLOCK * MS_IPT;          // work units per tile
-constexpr int MS_CAP = LbShared<MS_IPT, uint32_t>::CAP;
+// Expansion tiles hold BLOCK * IPT work units; IPT (units per lane) is a
+// load-balancing parameter the tuner picks per graph from MS_IPTS.
+constexpr int MS_IPT_DEFAULT = 4;
+constexpr int MS_IPT_MIN = 2;                    // buffers are sized for the most tiles
+constexpr int MS_IPTS[] = {2, 4, 6};
 constexpr int MS_COLLECT_PER_CU = 8;             // collect workgroups per CU
 constexpr int32_t MS_NO_HEAVY = 1 << 30;         // T: every open vertex goes through the edge-granular pull
 static_assert(UNIT_SLOTS % MS_G == 0, "groups must not straddle a block start");
@@ -80,6 +84,7 @@ struct MsState {
 struct MsParams {
     int32_t T;          // pull: open vertices of degree > T take one wave each
     int32_t alpha;      // push while active_deg * alpha < open_deg
+    int32_t ipt;        // expansion work units per lane per tile (one of MS_IPTS)
     int32_t push_only;  // adjacency not symmetric: pulling over out-edges would be wrong
     int32_t digest;     // no distance matrix: per-source digests only
     int32_t depth_limit[64];
@@ -272,23 +277,26 @@ struct GroupIn {
 struct GroupOut {
     uint32_t* P;
     uint32_t* tile_lo;
+    uint32_t tile;   // work units per expansion tile
     __device__ __forceinline__ void operator()(int64_t i, uint64_t ex, uint64_t x) const {
         const uint32_t g0 = (uint32_t)ex, ng = (uint32_t)x;
         P[i] = g0;
         if (ng) {
-            const uint32_t t0 = (uint32_t)(((uint64_t)g0 + MS_TILE - 1) / MS_TILE);
-            const uint32_t t1 = (uint32_t)(((uint64_t)g0 + ng + MS_TILE - 1) / MS_TILE);
+            const uint32_t t0 = (uint32_t)(((uint64_t)g0 + tile - 1) / tile);
+            const uint32_t t1 = (uint32_t)(((uint64_t)g0 + ng + tile - 1) / tile);
             for (uint32_t t = t0; t < t1; t++) tile_lo[t] = (uint32_t)i;
         }
     }
 };
 
+template <int IPT>
 struct MsShared {
-    LbShared<MS_IPT, uint32_t> lb;   // pref = group prefix, base = adjacency offset
-    int32_t deg[MS_CAP];
-    uint64_t mask[MS_CAP];           // push: frontier mask; pull: missing bits
-    unsigned long long acc[MS_CAP];  // pull: bits found this tile
-    uint64_t crange[MS_CAP];         // window: range of the vertex's changed pairs (lo << 32 | hi), 0 if none
+    static constexpr int CAP = LbShared<IPT, uint32_t>::CAP;
+    LbShared<IPT, uint32_t> lb;      // pref = group prefix, base = adjacency offset
+    int32_t deg[CAP];
+    uint64_t mask[CAP];              // push: frontier mask; pull: missing bits
+    unsigned long long acc[CAP];     // pull: bits found this tile
+    uint64_t crange[CAP];            // window: range of the vertex's changed pairs (lo << 32 | hi), 0 if none
 };
 
 // Multi-version window (served stream): the changed pairs of the window,
@@ -311,7 +319,7 @@ __device__ __forceinline__ uint64_t changed_range(const uint64_t* __restrict__ c
 }
 
 // Edge-granular expansion of this level's work list (see the file comment).
-template <bool PULL>
+template <bool PULL, int IPT>
 __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, const uint32_t* __restrict__ P,
                                                           const uint32_t* __restrict__ tile_lo,
                                                           const int32_t* __restrict__ qdeg,
@@ -327,17 +335,18 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
                                                           const uint32_t* __restrict__ nchg, int sbits) {
     if (S->done || S->mode != (PULL ? 1 : 0)) return;
     typedef int32_t Vec __attribute__((ext_vector_type(MS_G)));
-    __shared__ MsShared sh;
+    __shared__ MsShared<IPT> sh;
+    constexpr int TILE = BLOCK * IPT;
     const uint32_t nc = *nchg;
     const int64_t F = S->wn;
     const int64_t W = (uint32_t)S->tot;
-    const int64_t ntiles = (W + MS_TILE - 1) / MS_TILE;
+    const int64_t ntiles = (W + TILE - 1) / TILE;
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int64_t lo = tile_lo[t];
         const int64_t hi = (t + 1 < ntiles) ? (int64_t)tile_lo[t + 1] : F - 1;
-        const int64_t w0 = t * MS_TILE;
+        const int64_t w0 = t * TILE;
         const int64_t span = hi - lo + 1;
-        const bool staged = span <= MS_CAP;
+        const bool staged = span <= MsShared<IPT>::CAP;
         if (staged) {
             for (int i = threadIdx.x; i < (int)span; i += BLOCK) {
                 sh.lb.pref[i] = P[lo + i];
@@ -351,16 +360,16 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
         }
         __syncthreads();
         // phase 1: every unit's segment, slot address, valid count and mask
-        int64_t addr[MS_IPT];
-        int nval[MS_IPT];
-        int64_t seg[MS_IPT];
-        uint64_t msk[MS_IPT];
-        uint64_t crg[MS_IPT];
+        int64_t addr[IPT];
+        int nval[IPT];
+        int64_t seg[IPT];
+        uint64_t msk[IPT];
+        uint64_t crg[IPT];
         if (staged) {
 #pragma unroll
-            for (int j = 0; j < MS_IPT; j++) {
+            for (int j = 0; j < IPT; j++) {
                 const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
-                const int s = lb_locate<MS_IPT>(sh.lb, (int)span, w);
+                const int s = lb_locate<IPT>(sh.lb, (int)span, w);
                 const int64_t k = (w - (int64_t)sh.lb.pref[s]) * MS_G;
                 addr[j] = sh.lb.base[s] + k;
                 nval[j] = w < W ? (int)min((int64_t)MS_G, (int64_t)sh.deg[s] - k) : 0;
@@ -369,7 +378,7 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
                 crg[j] = sh.crange[s];
             }
         } else {   // slice wider than LDS (runs of work-free list entries)
-            for (int j = 0; j < MS_IPT; j++) {
+            for (int j = 0; j < IPT; j++) {
                 const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
                 addr[j] = 0;
                 nval[j] = 0;
@@ -389,20 +398,20 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
             }
         }
         // phase 2: adjacency groups, all in flight (a unit without work reads arena[0..4))
-        Vec v[MS_IPT];
+        Vec v[IPT];
 #pragma unroll
-        for (int j = 0; j < MS_IPT; j++)
+        for (int j = 0; j < IPT; j++)
             v[j] = __builtin_nontemporal_load(reinterpret_cast<const Vec*>(arena + (nval[j] > 0 ? addr[j] : 0)));
         // phase 3: neighbour masks, all in flight (masked slots read probe[0])
-        uint64_t pr[MS_IPT][MS_G];
+        uint64_t pr[IPT][MS_G];
 #pragma unroll
-        for (int j = 0; j < MS_IPT; j++)
+        for (int j = 0; j < IPT; j++)
 #pragma unroll
             for (int e = 0; e < MS_G; e++) pr[j][e] = probe[e < nval[j] ? v[j][e] : 0];
         // window: a changed pair carries only the sources whose version holds
         // it (push: the others count as visited; pull: their bits are dropped)
 #pragma unroll
-        for (int j = 0; j < MS_IPT; j++) {
+        for (int j = 0; j < IPT; j++) {
             if (!crg[j]) continue;
             const uint32_t clo = (uint32_t)(crg[j] >> 32), chi = (uint32_t)crg[j];
             const uint64_t ub = ckey[clo] & ~((1ull << sbits) - 1);   // u << sbits
@@ -421,7 +430,7 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
         if (!PULL) {
             // push: bits of the frontier vertex its neighbour has not seen
 #pragma unroll
-            for (int j = 0; j < MS_IPT; j++)
+            for (int j = 0; j < IPT; j++)
 #pragma unroll
                 for (int e = 0; e < MS_G; e++) {
                     const uint64_t add = msk[j] & ~pr[j][e];
@@ -430,7 +439,7 @@ __global__ __launch_bounds__(BLOCK) void ms_expand_kernel(const MsState* S, cons
         } else {
             // pull: bits the open vertex misses, found in its neighbours' frontiers
 #pragma unroll
-            for (int j = 0; j < MS_IPT; j++) {
+            for (int j = 0; j < IPT; j++) {
                 uint64_t a = 0;
 #pragma unroll
                 for (int e = 0; e < MS_G; e++) a |= e < nval[j] ? pr[j][e] : 0ull;
@@ -813,7 +822,7 @@ __global__ void ms_degrees_kernel(int64_t nv, const hn_vrec* __restrict__ rec, i
 // engine buffer sizes
 int64_t ms_words(int64_t nv) { return (nv + 31) / 32; }
 int64_t ms_bit_words(int64_t nv) { return ((nv + 63) / 64) * 2; }   // whole 64-vertex pairs
-int64_t ms_tiles(int64_t nv, int64_t ne) { return (ne / MS_G + nv) / MS_TILE + 2; }
+int64_t ms_tiles(int64_t nv, int64_t ne) { return (ne / MS_G + nv) / (BLOCK * MS_IPT_MIN) + 2; }
 int64_t ms_collect_grid(int64_t nv) {
     return std::max<int64_t>(1, std::min<int64_t>(ceil_div(nv, BLOCK), (int64_t)num_cus() * MS_COLLECT_PER_CU));
 }
@@ -849,6 +858,7 @@ struct MsEngine {
     hipGraphExec_t exec = nullptr;
     hipStream_t exec_stream = nullptr;
     int32_t* exec_dist = nullptr;
+    const int32_t* exec_arena = nullptr;   // edge arena the captured level step reads (it moves when it grows)
     MsParams exec_params{};
     MsParams params{};
     hipStream_t own = nullptr;        // non-blocking stream the level graph is captured on
@@ -995,6 +1005,7 @@ hn_status ms_engine_create(hn_graph* g, MsEngine** out) {
     e->g = g;
     e->params.T = MS_NO_HEAVY;
     e->params.alpha = 16;
+    e->params.ipt = MS_IPT_DEFAULT;
     hn_status s = ms_reserve(e);
     if (s != HN_OK) {
         e->release();
@@ -1055,20 +1066,25 @@ static hipError_t ms_level(MsEngine* e, int32_t* dist, const MsParams& P, hipStr
     ms_emit_kernel<<<(unsigned)ceil_div(words, BLOCK), BLOCK, 0, st>>>(words, e->S, e->fbits, e->obits, e->wpre,
                                                                        g->rec, e->visited, e->frontier, e->dirty,
                                                                        e->list, e->qdeg, e->qoff, e->qmask);
-    err = exclusive_scan_dev_count<uint64_t>(GroupIn{e->qdeg}, GroupOut{e->P, e->tile_lo}, &e->S->tot, &e->S->wn, nv,
+    err = exclusive_scan_dev_count<uint64_t>(GroupIn{e->qdeg}, GroupOut{e->P, e->tile_lo, (uint32_t)(BLOCK * P.ipt)},
+                                             &e->S->tot, &e->S->wn, nv,
                                              e->scan_tmp, st);
     if (err != hipSuccess) return err;
     const int sbits = std::max(1, ceil_log2_host((int64_t)nv + 1));
-    ms_expand_kernel<false><<<xgrid, BLOCK, 0, st>>>(e->S, e->P, e->tile_lo, e->qdeg, e->qoff, e->qmask, e->list,
-                                                     g->arena, e->visited,
-                                                     reinterpret_cast<unsigned long long*>(e->next), e->dirty,
-                                                     e->w_ckey.as<uint64_t>(), e->w_cS.as<uint64_t>(), e->d_nchg,
-                                                     sbits);
-    ms_expand_kernel<true><<<xgrid, BLOCK, 0, st>>>(e->S, e->P, e->tile_lo, e->qdeg, e->qoff, e->qmask, e->list,
-                                                    g->arena, e->frontier,
-                                                    reinterpret_cast<unsigned long long*>(e->next), e->dirty,
-                                                    e->w_ckey.as<uint64_t>(), e->w_cS.as<uint64_t>(), e->d_nchg,
-                                                    sbits);
+    auto expand = [&](auto ipt_c) {
+        constexpr int IPT = decltype(ipt_c)::value;
+        ms_expand_kernel<false, IPT><<<xgrid, BLOCK, 0, st>>>(
+            e->S, e->P, e->tile_lo, e->qdeg, e->qoff, e->qmask, e->list, g->arena, e->visited,
+            reinterpret_cast<unsigned long long*>(e->next), e->dirty, e->w_ckey.as<uint64_t>(), e->w_cS.as<uint64_t>(),
+            e->d_nchg, sbits);
+        ms_expand_kernel<true, IPT><<<xgrid, BLOCK, 0, st>>>(
+            e->S, e->P, e->tile_lo, e->qdeg, e->qoff, e->qmask, e->list, g->arena, e->frontier,
+            reinterpret_cast<unsigned long long*>(e->next), e->dirty, e->w_ckey.as<uint64_t>(), e->w_cS.as<uint64_t>(),
+            e->d_nchg, sbits);
+    };
+    if (P.ipt == 2) expand(std::integral_constant<int, 2>());
+    else if (P.ipt == 6) expand(std::integral_constant<int, 6>());
+    else expand(std::integral_constant<int, 4>());
     e->exec_cS = e->w_cS.ptr;
     e->exec_ckey = e->w_ckey.ptr;
     ms_pull_heavy_kernel<<<wgrid, BLOCK, 0, st>>>(words, e->S, e->hbits, g->rec, g->arena, e->frontier, e->visited,
@@ -1131,7 +1147,7 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
     ms_seed_kernel<<<1, WAVE, 0, st>>>(k, e->d_src, nv, e->visited, e->frontier, e->fbits, dist, g->rec, e->S, P);
     HN_CHECK_LAUNCH();
     // capture the level step once per (stream, dist buffer, parameters, buffers)
-    const bool same = e->exec && e->exec_stream == st && e->exec_dist == dist &&
+    const bool same = e->exec && e->exec_stream == st && e->exec_dist == dist && e->exec_arena == g->arena &&
                       std::memcmp(&e->exec_params, &P, sizeof(P)) == 0;
     if (!same) {
         if (e->exec) (void)hipGraphExecDestroy(e->exec);
@@ -1154,6 +1170,7 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
         }
         e->exec_stream = st;
         e->exec_dist = dist;
+        e->exec_arena = g->arena;
         e->exec_params = P;
     }
     // replay; the done flag is read every `chunk` levels
@@ -1355,6 +1372,28 @@ hn_status hn_bfs_multi_params(hn_graph* g, int32_t degree_threshold, int32_t pus
     return HN_OK;
 }
 
+hn_status hn_bfs_multi_lb_tile(hn_graph* g, int32_t units_per_lane) {
+    HN_ARG(g, "null graph");
+    bool ok = false;
+    for (int v : MS_IPTS) ok |= v == units_per_lane;
+    HN_ARG(ok, "units per lane must be 2, 4 or 6");
+    MsEngine* e = nullptr;
+    HN_TRY(ms_engine(g, &e));
+    e->params.ipt = units_per_lane;
+    return HN_OK;
+}
+
+hn_status hn_bfs_multi_get_params(hn_graph* g, int32_t* degree_threshold, int32_t* push_alpha,
+                                  int32_t* units_per_lane) {
+    HN_ARG(g, "null graph");
+    MsEngine* e = nullptr;
+    HN_TRY(ms_engine(g, &e));
+    if (degree_threshold) *degree_threshold = e->params.T;
+    if (push_alpha) *push_alpha = e->params.alpha;
+    if (units_per_lane) *units_per_lane = e->params.ipt;
+    return HN_OK;
+}
+
 hn_status hn_bfs_multi_tune(hn_graph* g, const int32_t* sources, int32_t k, int32_t* dist, int32_t* best_threshold,
                             int32_t* best_alpha, void* stream) {
     HN_ARG(g && sources && dist, "null argument");
@@ -1368,20 +1407,29 @@ hn_status hn_bfs_multi_tune(hn_graph* g, const int32_t* sources, int32_t k, int3
     HN_HIP(hipEventCreate(&a));
     HN_HIP(hipEventCreate(&b));
     float best = 1e30f;
-    int32_t bt = e->params.T, ba = e->params.alpha;
+    int32_t bt = e->params.T, ba = e->params.alpha, bi = MS_IPT_DEFAULT;
     hn_status s = HN_OK;
+    // one warm run (captures the level graph), one timed run
+    auto timed = [&](float* ms) -> hn_status {
+        hn_status r = ms_run(g, sources, k, nullptr, dist, nullptr, st);
+        if (r != HN_OK) return r;
+        (void)hipEventRecord(a, st);
+        if ((r = ms_run(g, sources, k, nullptr, dist, nullptr, st)) != HN_OK) return r;
+        (void)hipEventRecord(b, st);
+        (void)hipEventSynchronize(b);
+        *ms = 0.f;
+        (void)hipEventElapsedTime(ms, a, b);
+        return HN_OK;
+    };
+    // the degree threshold and switch point over a grid at the default tile,
+    // then the tile size at the best pair
+    e->params.ipt = MS_IPT_DEFAULT;
     for (int32_t T : Ts) {
         for (int32_t A : As) {
             e->params.T = T;
             e->params.alpha = A;
-            // one warm run (captures the level graph), one timed run
-            if ((s = ms_run(g, sources, k, nullptr, dist, nullptr, st)) != HN_OK) break;
-            (void)hipEventRecord(a, st);
-            if ((s = ms_run(g, sources, k, nullptr, dist, nullptr, st)) != HN_OK) break;
-            (void)hipEventRecord(b, st);
-            (void)hipEventSynchronize(b);
             float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, a, b);
+            if ((s = timed(&ms)) != HN_OK) break;
             if (ms < best) {
                 best = ms;
                 bt = T;
@@ -1390,10 +1438,21 @@ hn_status hn_bfs_multi_tune(hn_graph* g, const int32_t* sources, int32_t k, int3
         }
         if (s != HN_OK) break;
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     e->params.T = bt;
     e->params.alpha = ba;
+    for (int ipt : MS_IPTS) {
+        if (s != HN_OK || ipt == MS_IPT_DEFAULT) continue;
+        e->params.ipt = ipt;
+        float ms = 0.f;
+        if ((s = timed(&ms)) != HN_OK) break;
+        if (ms < best) {
+            best = ms;
+            bi = ipt;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    e->params.ipt = bi;
     if (best_threshold) *best_threshold = bt;
     if (best_alpha) *best_alpha = ba;
     return s;

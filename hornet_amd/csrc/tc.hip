@@ -208,19 +208,24 @@ struct U8In {
 
 }  // namespace
 
-// Triangle counting core over a device CSR (sorted adjacency).
-static hn_status tc_core(hn_graph* g, int32_t nv, int64_t ne, const int64_t* off, const int32_t* adj,
-                         uint64_t* total_out, uint64_t* per_vertex, hipStream_t st) {
+static int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+// bytes of tc_core's buffers for ne stored edges
+static int64_t tc_core_bytes(int64_t ne) {
     const int64_t ne1 = ne > 0 ? ne : 1;
-    auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
+    return al256(512 + ne1 * 4) + al256(ne1 * 4) + al256(ne1 * 4);
+}
+
+// Triangle counting core over a device CSR (sorted adjacency); its buffers
+// start at `base` (tc_core_bytes(ne) bytes of the handle's TC scratch).
+static hn_status tc_core(hn_graph* g, int32_t nv, int64_t ne, const int64_t* off, const int32_t* adj,
+                         uint64_t* total_out, uint64_t* per_vertex, hipStream_t st, char* base) {
+    const int64_t ne1 = ne > 0 ? ne : 1;
     int64_t o_tot = 0;
     int64_t o_cnt = 256;
     int64_t o_src = 512;
-    int64_t o_med = al(o_src + ne1 * 4);
-    int64_t o_heavy = al(o_med + ne1 * 4);
-    int64_t total = al(o_heavy + ne1 * 4);
-    HN_TRY(g->s_tc.reserve((size_t)total));
-    char* base = g->s_tc.as<char>();
+    int64_t o_med = al256(o_src + ne1 * 4);
+    int64_t o_heavy = al256(o_med + ne1 * 4);
     TcOut o;
     o.total = reinterpret_cast<unsigned long long*>(base + o_tot);
     o.cnt = reinterpret_cast<uint32_t*>(base + o_cnt);
@@ -253,21 +258,23 @@ using namespace hn;
 
 extern "C" {
 
+// The TC entry points keep every buffer in the handle's grow-only scratch
+// (s_tc): the CSR they count on, then tc_core's buffers.
+
 hn_status hn_tc2_stored(hn_graph* g, uint64_t* total, uint64_t* per_vertex, void* stream) {
     HN_ARG(g && total, "null argument");
     HN_ARG(g->ne < (int64_t)0xFFFFFFFF, "edge count exceeds 32-bit queue ids");
     hipStream_t st = as_stream(stream);
-    const int64_t nv = g->nv, ne = g->ne;
-    int64_t* off = nullptr;
-    int32_t* adj = nullptr;
-    HN_TRY(dalloc(&off, (size_t)nv + 1));
-    hn_status s = dalloc(&adj, (size_t)(ne > 0 ? ne : 1));
-    if (s == HN_OK) s = hn_graph_export_csr(g, off, adj, ne, stream);
-    if (s == HN_OK) s = tc_core(g, g->nv, ne, off, adj, total, per_vertex, st);
-    (void)hipStreamSynchronize(st);
-    dfree(off);
-    dfree(adj);
-    return s;
+    const int64_t nv = g->nv, ne = g->ne, ne1 = ne > 0 ? ne : 1;
+    const int64_t o_off = 0;
+    const int64_t o_adj = al256(o_off + (nv + 1) * 8);
+    const int64_t o_core = al256(o_adj + ne1 * 4);
+    HN_TRY(g->s_tc.reserve((size_t)(o_core + tc_core_bytes(ne))));
+    char* base = g->s_tc.as<char>();
+    int64_t* off = reinterpret_cast<int64_t*>(base + o_off);
+    int32_t* adj = reinterpret_cast<int32_t*>(base + o_adj);
+    HN_TRY(hn_graph_export_csr(g, off, adj, ne, stream));
+    return tc_core(g, g->nv, ne, off, adj, total, per_vertex, st, base + o_core);
 }
 
 hn_status hn_tc_count(hn_graph* g, uint64_t* total, uint64_t* per_vertex, void* stream) {
@@ -276,30 +283,32 @@ hn_status hn_tc_count(hn_graph* g, uint64_t* total, uint64_t* per_vertex, void* 
     hipStream_t st = as_stream(stream);
     const int32_t nv = g->nv;
     const int64_t ne = g->ne, ne1 = ne > 0 ? ne : 1;
-    int64_t *eoff = nullptr, *offp = nullptr;
-    int32_t *adj_tmp = nullptr, *adjp = nullptr;
-    uint8_t* keep = nullptr;
-    uint32_t* pos = nullptr;
-    void* tmp = nullptr;
-    auto cleanup = [&]() {
-        (void)hipStreamSynchronize(st);
-        dfree(eoff); dfree(offp); dfree(adj_tmp); dfree(adjp); dfree(keep); dfree(pos); dfree(tmp);
-    };
-    hn_status s = HN_OK;
-    if ((s = dalloc(&eoff, (size_t)nv + 1)) != HN_OK || (s = dalloc(&offp, (size_t)nv + 1)) != HN_OK ||
-        (s = dalloc(&adj_tmp, (size_t)ne1)) != HN_OK || (s = dalloc(&adjp, (size_t)ne1)) != HN_OK ||
-        (s = dalloc(&keep, (size_t)ne1)) != HN_OK || (s = dalloc(&pos, (size_t)ne1 + 1)) != HN_OK ||
-        (s = dmalloc(&tmp, scan_temp_bytes(ne1))) != HN_OK) {
-        cleanup();
-        return s;
-    }
-    if ((s = degree_prefix(g, eoff, st)) != HN_OK) { cleanup(); return s; }
+    // the oriented CSR holds at most half the entries of the symmetric one
+    // (a multigraph's repeated entries may all point up: size for ne)
+    const int64_t o_eoff = 0;
+    const int64_t o_offp = al256(o_eoff + ((int64_t)nv + 1) * 8);
+    const int64_t o_tmp = al256(o_offp + ((int64_t)nv + 1) * 8);
+    const int64_t o_adjp = al256(o_tmp + ne1 * 4);
+    const int64_t o_keep = al256(o_adjp + ne1 * 4);
+    const int64_t o_pos = al256(o_keep + ne1);
+    const int64_t o_scan = al256(o_pos + (ne1 + 1) * 4);
+    const int64_t o_core = al256(o_scan + (int64_t)scan_temp_bytes(ne1));
+    HN_TRY(g->s_tc.reserve((size_t)(o_core + tc_core_bytes(ne))));
+    char* base = g->s_tc.as<char>();
+    int64_t* eoff = reinterpret_cast<int64_t*>(base + o_eoff);
+    int64_t* offp = reinterpret_cast<int64_t*>(base + o_offp);
+    int32_t* adj_tmp = reinterpret_cast<int32_t*>(base + o_tmp);
+    int32_t* adjp = reinterpret_cast<int32_t*>(base + o_adjp);
+    uint8_t* keep = reinterpret_cast<uint8_t*>(base + o_keep);
+    uint32_t* pos = reinterpret_cast<uint32_t*>(base + o_pos);
+    void* tmp = base + o_scan;
+    HN_TRY(degree_prefix(g, eoff, st));
     if (ne > 0) {
         int64_t grid = std::min<int64_t>(ceil_div(ne, OR_TILE), (int64_t)num_cus() * 16);
         orient_flags_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(nv, ne, eoff, g->rec, g->arena, adj_tmp, keep);
         HN_CHECK_LAUNCH();
     }
-    if ((s = exclusive_scan<uint32_t>(U8In{keep}, pos, ne, tmp, st)) != HN_OK) { cleanup(); return s; }
+    HN_TRY(exclusive_scan<uint32_t>(U8In{keep}, pos, ne, tmp, st));
     if (ne > 0) {
         orient_compact_kernel<<<(unsigned)ceil_div(ne, BLOCK), BLOCK, 0, st>>>(ne, keep, pos, adj_tmp, adjp);
         HN_CHECK_LAUNCH();
@@ -307,10 +316,8 @@ hn_status hn_tc_count(hn_graph* g, uint64_t* total, uint64_t* per_vertex, void* 
     orient_offsets_kernel<<<(unsigned)ceil_div((int64_t)nv + 1, BLOCK), BLOCK, 0, st>>>(nv, eoff, pos, offp);
     HN_CHECK_LAUNCH();
     uint32_t nep = 0;
-    if ((s = read_back(g, pos + ne, 4, &nep, st)) != HN_OK) { cleanup(); return s; }
-    s = tc_core(g, nv, (int64_t)nep, offp, adjp, total, per_vertex, st);
-    cleanup();
-    return s;
+    HN_TRY(read_back(g, pos + ne, 4, &nep, st));
+    return tc_core(g, nv, (int64_t)nep, offp, adjp, total, per_vertex, st, base + o_core);
 }
 
 }  // extern "C"

@@ -31,6 +31,8 @@
 // the allocator back, so a failed batch leaves the graph unchanged.  Two
 // host synchronisations per batch (error word with the side-buffer size,
 // final status).
+#include <cstdlib>
+
 #include "graph.hpp"
 #include "prims.hpp"
 
@@ -247,6 +249,7 @@ __global__ __launch_bounds__(ALLOC_BLOCK) void insert_alloc_kernel(
 // commit or roll back the tickets of this batch
 __global__ void alloc_fix_kernel(AllocState* st, UpdScal* sc) {
     if (st->error) {
+        st->want_units = st->bump_units - st->saved_bump;   // what the batch asked for (the arena grows by it)
         st->bump_units = st->saved_bump;
         for (int k = 0; k < NCLASS; k++) st->take[k] = 0;
         st->error = 0;
@@ -334,6 +337,26 @@ __global__ __launch_bounds__(ALLOC_BLOCK) void commit_kernel(const uint32_t* __r
         rel = live && !t.inplace && t.old_cap >= 0;
     }
     free_block_group(A, rel ? t.old_cap : 0, t.old_off, rel, ash);
+}
+
+// Undo the allocation of a batch that fails after alloc_fix_kernel committed
+// its tickets and before commit_kernel: the blocks handed to relocating
+// vertices go back to the free stacks (a no-op when the batch already failed
+// on the device, where the allocation was rolled back).
+__global__ __launch_bounds__(ALLOC_BLOCK) void release_new_blocks_kernel(const uint32_t* __restrict__ ucnt,
+                                                                         const TouchInfo* __restrict__ ti,
+                                                                         const uint8_t* __restrict__ skip,
+                                                                         AllocView A, const UpdScal* sc) {
+    __shared__ AllocShared ash;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = j < (int64_t)*ucnt && !sc->err;   // every thread reaches the release below
+    bool rel = false;
+    TouchInfo t{};
+    if (active) {
+        t = ti[j];
+        rel = !(skip && skip[j]) && !t.inplace && t.new_cap >= 0;
+    }
+    free_block_group(A, rel ? t.new_cap : 0, t.new_off, rel, ash);
 }
 
 // ---------------------------------------------------------------- erase
@@ -996,8 +1019,8 @@ static hn_status check_scal(hn_graph* g, UpdBufs& b, hipStream_t st, UpdScal* h)
         return HN_E_INVALID_ARG;
     }
     if (h->err & ERR_OOM) {
-        set_error("edge arena exhausted (graph unchanged); create the graph with a larger arena_slack");
-        return HN_E_OOM;
+        set_error("edge arena exhausted (graph unchanged)");
+        return HN_E_ARENA_FULL;   // the entry point grows the arena and runs the batch again
     }
     return HN_OK;
 }
@@ -1099,6 +1122,13 @@ static hn_status reserve_side(hn_graph* g, UpdBufs& b, hipStream_t st, int64_t* 
                               int64_t** dst = nullptr) {
     UpdScal h;
     HN_TRY(check_scal(g, b, st, &h));
+    // test hook: HORNET_HIP_FAIL_SIDE_RESERVE=1 makes this reservation fail as
+    // a device allocation would, after the batch's blocks were allocated
+    if (const char* e = std::getenv("HORNET_HIP_FAIL_SIDE_RESERVE"))
+        if (e[0] == '1') {
+            set_error("side buffer: allocation failure forced by HORNET_HIP_FAIL_SIDE_RESERVE");
+            return HN_E_OOM;
+        }
     *total = (int64_t)h.side;
     const size_t cnt = (size_t)(*total > 0 ? *total : 1);
     // entries, then (when asked for) one 64-bit destination per entry
@@ -1120,16 +1150,66 @@ static hn_status reserve_side(hn_graph* g, UpdBufs& b, hipStream_t st, int64_t* 
     return HN_OK;
 }
 
+static void record_last(hn_graph* g, int32_t kind, int64_t submitted, int64_t applied, const TouchInfo* ti,
+                        const uint32_t* ucnt, const uint8_t* skip, const int64_t* moved) {
+    g->last_kind = kind;
+    g->last_submitted = submitted;
+    g->last_applied = applied;
+    g->last_ti = ti;
+    g->last_ucnt = ucnt;
+    g->last_skip = skip;
+    g->last_moved = moved;
+}
+
+// [0] touched, [1] relocated, [2] sum of old degrees over the touched, [3] sum
+// over the relocated of the old (insert) or new (erase) degree
+__global__ void update_stats_kernel(const uint32_t* __restrict__ ucnt, const TouchInfo* __restrict__ ti,
+                                    const uint8_t* __restrict__ skip, int erase,
+                                    unsigned long long* __restrict__ acc) {
+    unsigned long long rel = 0, td = 0, rd = 0;
+    const int64_t n = *ucnt;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        const TouchInfo t = ti[j];
+        td += (unsigned long long)t.old_deg;
+        if (!(skip && skip[j]) && !t.inplace && t.new_cap >= 0) {
+            rel++;
+            rd += (unsigned long long)(erase ? t.new_deg : t.old_deg);
+        }
+    }
+    rel = wave_reduce_sum(rel);
+    td = wave_reduce_sum(td);
+    rd = wave_reduce_sum(rd);
+    if (__lane_id() == 0) {
+        if (rel) atomicAdd(acc + 1, rel);
+        if (td) atomicAdd(acc + 2, td);
+        if (rd) atomicAdd(acc + 3, rd);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(acc, (unsigned long long)n);
+}
+
+// A batch that finds the edge arena full changes nothing; the arena then
+// grows (BlockArrayManager adds a BlockArray when a size class is full,
+// hornet/include/Core/MemoryManager/BlockArray/BlockArray.i.cuh:138-161) and
+// the batch runs again.  HN_E_OOM only when device memory is exhausted.
+template <typename Fn>
+static hn_status with_growth(hn_graph* g, void* stream, Fn once) {
+    for (int attempt = 0;; attempt++) {
+        const hn_status s = once();
+        if (s != HN_E_ARENA_FULL) return s;
+        if (attempt >= 16) {
+            set_error("edge arena: batch still does not fit after growing");
+            return HN_E_OOM;
+        }
+        HN_TRY(grow_arena(g, 0, as_stream(stream)));
+    }
+}
+
 }  // namespace hn
 
 using namespace hn;
 
 extern "C" {
 
-hn_status hn_graph_insert(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n, uint32_t flags,
-                          int64_t* inserted, void* stream) {
-    return hn_graph_insert_meta(g, src, dst, nullptr, n, flags, inserted, stream);
-}
 
 // device copies of a host batch's meta-data fields, released with the object
 struct BatchMeta {
@@ -1141,8 +1221,8 @@ struct BatchMeta {
     }
 };
 
-hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* dst, const void* const* meta,
-                               int64_t n, uint32_t flags, int64_t* inserted, void* stream) {
+static hn_status insert_once(hn_graph* g, const int32_t* src, const int32_t* dst, const void* const* meta,
+                             int64_t n, uint32_t flags, int64_t* inserted, void* stream) {
     HN_ARG(g, "null graph");
     HN_ARG(n >= 0, "negative batch size");
     if (inserted) *inserted = 0;
@@ -1198,33 +1278,48 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     HN_CHECK_LAUNCH();
     alloc_fix_kernel<<<1, 1, 0, st>>>(g->alloc, b.sc);
     HN_CHECK_LAUNCH();
+    // from here to the commit, a failure hands the new blocks back
+    auto undo = [&](hn_status e) {
+        release_new_blocks_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(
+            ucnt, b.ti, nullptr, g->alloc_view(), b.sc);
+        (void)hipStreamSynchronize(st);
+        return e;
+    };
+#define HN_UNDO_TRY(expr)                  \
+    do {                                   \
+        hn_status s_ = (expr);             \
+        if (s_ != HN_OK) return undo(s_);  \
+    } while (0)
+#define HN_UNDO_LAUNCH() HN_UNDO_TRY(hipGetLastError() == hipSuccess ? HN_OK : HN_E_HIP)
     // 4. the old entries that move, in place in one pass (the few that could
     // overwrite a slot a later copy tile has not read go through a side
     // buffer), then the batch entries into the slots left between them
-    HN_TRY(exclusive_scan_fn<int64_t>(MoveLenIn{b.ti, ucnt, b.sc}, RunOut{b.wpos, b.tile_lo}, b.wpos + n, n, b.tmp,
-                                      st));
+    HN_UNDO_TRY(exclusive_scan_fn<int64_t>(MoveLenIn{b.ti, ucnt, b.sc}, RunOut{b.wpos, b.tile_lo}, b.wpos + n, n,
+                                           b.tmp, st));
     const int64_t nt = b.ntile_max;
     insert_defer_kernel<<<(unsigned)ceil_div(nt, BLOCK), BLOCK, 0, st>>>(nt, n, b.wpos, b.tile_lo, b.ti, b.seg_start,
                                                                         b.kb, f, g->arena, b.tcnt, b.sc);
-    HN_CHECK_LAUNCH();
-    HN_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, SideOut{b.doff, nt - 1, b.sc}, b.doff + nt, nt, b.tmp,
-                                      st));
+    HN_UNDO_LAUNCH();
+    HN_UNDO_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, SideOut{b.doff, nt - 1, b.sc}, b.doff + nt, nt,
+                                           b.tmp, st));
     int64_t ndefer = 0;
     int32_t* side = nullptr;
     int64_t* side_dst = nullptr;
-    HN_TRY(reserve_side(g, b, st, &ndefer, &side, &side_dst));
+    HN_UNDO_TRY(reserve_side(g, b, st, &ndefer, &side, &side_dst));
     const int64_t grid = (int64_t)num_cus() * COPY_GRID_PER_CU;
     insert_move_kernel<<<(unsigned)grid, BLOCK, 0, st>>>(n, b.wpos, b.tile_lo, b.ti, b.seg_start, b.kb, f, b.tcnt,
                                                          b.doff, g->arena, side, side_dst, b.sc, g->meta, b.sm);
-    HN_CHECK_LAUNCH();
+    HN_UNDO_LAUNCH();
     if (ndefer > 0) {
         place_side_kernel<<<(unsigned)ceil_div(ndefer, BLOCK), BLOCK, 0, st>>>(ndefer, side, side_dst, g->arena,
                                                                                  g->meta, b.sm);
-        HN_CHECK_LAUNCH();
+        HN_UNDO_LAUNCH();
     }
     insert_batch_kernel<<<gridn, BLOCK, 0, st>>>(nb, b.kb, b.spos, b.seg_start, ubk, b.ti, f, g->arena, b.sc, g->meta,
                                                  bm.m, b.ib);
-    HN_CHECK_LAUNCH();
+    HN_UNDO_LAUNCH();
+#undef HN_UNDO_LAUNCH
+#undef HN_UNDO_TRY
     // 5. publish
     commit_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(ucnt, b.seg_key, f, b.ti, nullptr, g->rec, g->alloc_view(), b.sc);
     HN_CHECK_LAUNCH();
@@ -1240,12 +1335,13 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
     g->ne += (int64_t)h.changed;
     g->version++;
     note_symmetry(g, was_sym, h);
+    record_last(g, 1, n, (int64_t)h.changed, b.ti, ucnt, nullptr, b.wpos + n);
     if (inserted) *inserted = (int64_t)h.changed;
     return HN_OK;
 }
 
-hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n, uint32_t flags,
-                         int64_t* erased, void* stream) {
+static hn_status erase_once(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n, uint32_t flags,
+                            int64_t* erased, void* stream) {
     HN_ARG(g, "null graph");
     HN_ARG(n >= 0, "negative batch size");
     if (erased) *erased = 0;
@@ -1287,24 +1383,37 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     HN_CHECK_LAUNCH();
     alloc_fix_kernel<<<1, 1, 0, st>>>(g->alloc, b.sc);
     HN_CHECK_LAUNCH();
+    // from here to the commit, a failure hands the new blocks back
+    auto undo = [&](hn_status e) {
+        release_new_blocks_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(
+            ucnt, b.ti, b.skip, g->alloc_view(), b.sc);
+        (void)hipStreamSynchronize(st);
+        return e;
+    };
+#define HN_UNDO_TRY(expr)                  \
+    do {                                   \
+        hn_status s_ = (expr);             \
+        if (s_ != HN_OK) return undo(s_);  \
+    } while (0)
+#define HN_UNDO_LAUNCH() HN_UNDO_TRY(hipGetLastError() == hipSuccess ? HN_OK : HN_E_HIP)
     // 6. the survivors as runs between deleted ranges, grouped by vertex
     erase_gaps_kernel<<<gridn, BLOCK, 0, st>>>(dcnt, ucnt, b.seg2_start, b.tpos, b.dpos, b.dlen, b.dlp, b.ti, b.skip,
                                                b.glen, b.gsrc, b.gdst, b.rtail, b.sc);
-    HN_CHECK_LAUNCH();
-    HN_TRY(exclusive_scan_fn<int64_t>(GapLenIn{b.glen, dcnt, ucnt}, RunOutT<ERASE_TILE>{b.gP, b.tile_lo},
-                                      b.gP + 2 * n, 2 * n, b.tmp, st));
+    HN_UNDO_LAUNCH();
+    HN_UNDO_TRY(exclusive_scan_fn<int64_t>(GapLenIn{b.glen, dcnt, ucnt}, RunOutT<ERASE_TILE>{b.gP, b.tile_lo},
+                                           b.gP + 2 * n, 2 * n, b.tmp, st));
     // 7. in-place compaction in one pass; the few entries that would overwrite
     // a source of an earlier tile go through a side buffer
     const int64_t nt = b.ntile_max;
     erase_defer_kernel<<<(unsigned)ceil_div(nt, BLOCK), BLOCK, 0, st>>>(nt, 2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst,
                                                                        b.rtail, b.tcnt, b.sc);
-    HN_CHECK_LAUNCH();
-    HN_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, SideOut{b.doff, nt - 1, b.sc}, b.doff + nt, nt, b.tmp,
-                                      st));
+    HN_UNDO_LAUNCH();
+    HN_UNDO_TRY(exclusive_scan_fn<int64_t>(TileCountIn{b.tcnt}, SideOut{b.doff, nt - 1, b.sc}, b.doff + nt, nt,
+                                           b.tmp, st));
     int64_t ndefer = 0;
     int32_t* side = nullptr;
     int64_t* side_dst = nullptr;
-    HN_TRY(reserve_side(g, b, st, &ndefer, &side, &side_dst));
+    HN_UNDO_TRY(reserve_side(g, b, st, &ndefer, &side, &side_dst));
     const int64_t grid = (int64_t)num_cus() * COPY_GRID_PER_CU;
     if (g->meta.n > 0)
         erase_copy_kernel<true><<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst, b.tcnt,
@@ -1314,12 +1423,14 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
         erase_copy_kernel<false><<<(unsigned)grid, BLOCK, 0, st>>>(2 * n, b.gP, b.tile_lo, b.gsrc, b.gdst, b.tcnt,
                                                                    b.doff, g->arena, side, side_dst, b.sc, g->meta,
                                                                    b.sm);
-    HN_CHECK_LAUNCH();
+    HN_UNDO_LAUNCH();
     if (ndefer > 0) {
         place_side_kernel<<<(unsigned)ceil_div(ndefer, BLOCK), BLOCK, 0, st>>>(ndefer, side, side_dst, g->arena,
                                                                                g->meta, b.sm);
-        HN_CHECK_LAUNCH();
+        HN_UNDO_LAUNCH();
     }
+#undef HN_UNDO_LAUNCH
+#undef HN_UNDO_TRY
     // 8. publish
     commit_kernel<<<(unsigned)ceil_div(n + 1, ALLOC_BLOCK), ALLOC_BLOCK, 0, st>>>(ucnt, b.seg2_key, f, b.ti, b.skip, g->rec, g->alloc_view(), b.sc);
     HN_CHECK_LAUNCH();
@@ -1335,7 +1446,49 @@ hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, in
     g->ne -= (int64_t)h.changed;
     g->version++;
     note_symmetry(g, was_sym, h);
+    record_last(g, 2, n, (int64_t)h.changed, b.ti, ucnt, b.skip, b.gP + 2 * n);
     if (erased) *erased = (int64_t)h.changed;
+    return HN_OK;
+}
+
+hn_status hn_graph_insert(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n, uint32_t flags,
+                          int64_t* inserted, void* stream) {
+    return hn_graph_insert_meta(g, src, dst, nullptr, n, flags, inserted, stream);
+}
+
+hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* dst, const void* const* meta,
+                               int64_t n, uint32_t flags, int64_t* inserted, void* stream) {
+    return with_growth(g, stream, [&]() { return insert_once(g, src, dst, meta, n, flags, inserted, stream); });
+}
+
+hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n, uint32_t flags,
+                         int64_t* erased, void* stream) {
+    return with_growth(g, stream, [&]() { return erase_once(g, src, dst, n, flags, erased, stream); });
+}
+
+hn_status hn_graph_update_stats(hn_graph* g, hn_update_stats* out, void* stream) {
+    HN_ARG(g && out, "null argument");
+    *out = hn_update_stats{};
+    out->kind = g->last_kind;
+    if (!g->last_kind) return HN_OK;
+    hipStream_t st = as_stream(stream);
+    HN_TRY(g->s_misc.reserve(64));
+    unsigned long long* acc = g->s_misc.as<unsigned long long>();
+    HN_HIP(hipMemsetAsync(acc, 0, 32, st));
+    update_stats_kernel<<<(unsigned)num_cus(), BLOCK, 0, st>>>(
+        g->last_ucnt, static_cast<const TouchInfo*>(g->last_ti), g->last_skip, g->last_kind == 2 ? 1 : 0, acc);
+    HN_CHECK_LAUNCH();
+    unsigned long long h[4];
+    int64_t moved = 0;
+    HN_TRY(read_back(g, acc, sizeof(h), h, st));
+    HN_TRY(read_back(g, g->last_moved, sizeof(moved), &moved, st));
+    out->submitted = g->last_submitted;
+    out->applied = g->last_applied;
+    out->touched = (int64_t)h[0];
+    out->relocated = (int64_t)h[1];
+    out->touched_deg = (int64_t)h[2];
+    out->relocated_deg = (int64_t)h[3];
+    out->moved = moved;
     return HN_OK;
 }
 

@@ -398,6 +398,12 @@ class Hornet:
         call("hn_graph_export_csr", self._h, _ptr(off), _ptr(adj), ne, _stream())
         return off, adj[:ne]
 
+    def copy(self, slack: float = 2.0) -> "Hornet":
+        """An independent, updatable device copy of the graph (hn_graph_snapshot)."""
+        h = ctypes.c_void_p()
+        call("hn_graph_snapshot", self._h, float(slack), _stream(), ctypes.byref(h))
+        return Hornet(_handle=h)
+
     def snapshot(self, slack: float = 1.0) -> "HornetStatic":
         """A frozen device copy of the graph as it is now (hn_graph_snapshot):
         later updates of this graph do not reach it."""
@@ -429,6 +435,14 @@ class Hornet:
         call("hn_graph_erase", self._h, _ptr(batch.src), _ptr(batch.dst), batch.nE(), flags,
              ctypes.byref(n), _stream())
         return n.value
+
+    def update_stats(self) -> dict:
+        """What the last insert / erase did (hn_graph_update_stats): submitted
+        and applied entries, touched and relocated vertices with their degree
+        sums, and stored entries moved."""
+        v = _lib.HnUpdateStats()
+        call("hn_graph_update_stats", self._h, ctypes.byref(v), _stream())
+        return {f: getattr(v, f) for f, _ in v._fields_ if f != "pad_"}
 
     # ------------------------------------------------------------------ cleanup
     def release(self):
@@ -588,7 +602,19 @@ class BfsMulti:
     def set_params(self, degree_threshold: int = 0, push_alpha: int = 0):
         call("hn_bfs_multi_params", self.hornet.handle, int(degree_threshold), int(push_alpha))
 
+    def set_lb_tile(self, units_per_lane: int):
+        """Expansion tile of BLOCK * units_per_lane work units (2, 4 or 6)."""
+        call("hn_bfs_multi_lb_tile", self.hornet.handle, int(units_per_lane))
+
+    def params(self):
+        """(degree threshold, push alpha, units per lane) in use."""
+        t, a, i = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        call("hn_bfs_multi_get_params", self.hornet.handle, ctypes.byref(t), ctypes.byref(a), ctypes.byref(i))
+        return t.value, a.value, i.value
+
     def tune(self, sources):
+        """Autotune the degree threshold, the push/pull switch and the LB tile;
+        returns (threshold, alpha) (params() gives the tile as well)."""
         src = np.ascontiguousarray(sources, dtype=np.int32)
         t, a = ctypes.c_int32(), ctypes.c_int32()
         call("hn_bfs_multi_tune", self.hornet.handle, src.ctypes.data_as(ctypes.c_void_p), src.size,

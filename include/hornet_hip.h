@@ -128,8 +128,9 @@ hn_status hn_graph_create_csr(int32_t nv, int64_t ne, const void* offsets,
 /* Hornet(HornetInit&) after HornetInit::insertEdgeData(meta...)
  * (HornetInit.cuh:75-97): as hn_graph_create_csr, with `layout->nfields`
  * meta-data arrays meta[f] of ne elements of layout->bytes[f] bytes, in the
- * memory space the flags give the edges; they follow their edges through
- * every update.  layout NULL or nfields 0 = hn_graph_create_csr. */
+ * memory space the flags give the edges (a NULL meta[f] reads as zeros); they
+ * follow their edges through every update.  layout NULL or nfields 0 =
+ * hn_graph_create_csr. */
 hn_status hn_graph_create_csr_meta(int32_t nv, int64_t ne, const void* offsets, const int32_t* edges,
                                    const hn_edge_meta* layout, const void* const* meta, uint32_t flags,
                                    double arena_slack, void* stream, hn_graph** out);
@@ -165,6 +166,13 @@ hn_status hn_graph_export_coo_meta(hn_graph* g, int32_t* src, int32_t* dst, void
  * adj[cap] int32, device buffers.  Synchronises. */
 hn_status hn_graph_export_csr(hn_graph* g, int64_t* offsets, int32_t* adj, int64_t cap,
                               void* stream);
+/* Hornet::getCSR(sortAdjacencyList) -- hornet/include/Core/HornetOperations/HornetQuery.i.cuh:69-107
+ * (CSR<DEVICE, vid_t, TypeList<EdgeMetaTypes...>>, hornet/include/Core/Static/Static.cuh:92-129):
+ * hn_graph_export_csr plus meta[f] (device, cap elements of field f's size)
+ * for each edge meta-data field; meta may be NULL.  Rows are sorted, as the
+ * storage keeps them.  Synchronises. */
+hn_status hn_graph_export_csr_meta(hn_graph* g, int64_t* offsets, int32_t* adj, void* const* meta, int64_t cap,
+                                   void* stream);
 /* An independent device copy of g (no reference counterpart: the reference's
  * immutable graph, HornetStatic, hornet/include/Core/Static/HornetStatic.cuh:16-66,
  * is built from a HornetInit).  Updates to either graph do not reach the other,
@@ -206,6 +214,22 @@ hn_status hn_graph_insert_meta(hn_graph* g, const int32_t* src, const int32_t* d
 hn_status hn_graph_erase(hn_graph* g, const int32_t* src, const int32_t* dst, int64_t n,
                          uint32_t flags, int64_t* erased, void* stream);
 
+/* What the last insert / erase on g did (no reference counterpart; the
+ * measurement hook for the batch-update roofline).  Valid until the next
+ * update of g.  Runs one small reduction; synchronises. */
+typedef struct hn_update_stats {
+    int64_t submitted;      /* entries in the batch                                   */
+    int64_t applied;        /* b': entries inserted / erased                          */
+    int64_t touched;        /* |T|: distinct sources of the (kept) batch entries      */
+    int64_t relocated;      /* |R|: touched vertices moved to a new block             */
+    int64_t touched_deg;    /* sum over T of the degree before the batch              */
+    int64_t relocated_deg;  /* sum over R of the degree before (insert) / after (erase) */
+    int64_t moved;          /* stored entries rewritten in place or copied            */
+    int32_t kind;           /* 0 none yet, 1 insert, 2 erase                          */
+    int32_t pad_;
+} hn_update_stats;
+hn_status hn_graph_update_stats(hn_graph* g, hn_update_stats* out, void* stream);
+
 /* --------------------------------------------------------- algorithms */
 /* BfsTopDown{set_parameters(source); run()} --
  * hornetsnest/src/Static/BreadthFirstSearch/TopDown.cu:88-106.
@@ -230,9 +254,16 @@ hn_status hn_bfs_multi_depth(hn_graph* g, const int32_t* sources, const int32_t*
  * switch (push while frontier degree * alpha < unvisited degree).  0 keeps a
  * value. */
 hn_status hn_bfs_multi_params(hn_graph* g, int32_t degree_threshold, int32_t push_alpha);
+/* Load-balancing tile of the batched BFS expansion: work units (16-byte
+ * adjacency groups) per lane per tile, 2, 4 (default) or 6. */
+hn_status hn_bfs_multi_lb_tile(hn_graph* g, int32_t units_per_lane);
+/* The batched BFS parameters in use (NULL outputs are skipped). */
+hn_status hn_bfs_multi_get_params(hn_graph* g, int32_t* degree_threshold, int32_t* push_alpha,
+                                  int32_t* units_per_lane);
 /* Autotuner: times the batched BFS of `sources` over a grid of thresholds and
- * switch points and keeps the fastest for this graph.  dist is scratch of
- * k * nv int32.  Synchronises. */
+ * switch points, then over the load-balancing tile sizes at the best pair,
+ * and keeps the fastest for this graph (hn_bfs_multi_get_params reads the
+ * tile chosen).  dist is scratch of k * nv int32.  Synchronises. */
 hn_status hn_bfs_multi_tune(hn_graph* g, const int32_t* sources, int32_t k, int32_t* dist,
                             int32_t* best_threshold, int32_t* best_alpha, void* stream);
 

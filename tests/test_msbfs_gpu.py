@@ -49,15 +49,29 @@ def test_k_sources_with_repeats_and_isolated(hn, g14, k):
     assert np.array_equal(got, expected(off, adj, srcs))
 
 
-@pytest.mark.parametrize("T,alpha", [(1, 1), (4096, 1), (8, 1 << 20), (64, 16)])
-def test_lb_parameters_do_not_change_results(hn, g14, T, alpha):
-    print(f"case msbfs T={T} alpha={alpha}", flush=True)
+@pytest.mark.parametrize("T,alpha,ipt", [(1, 1, 4), (4096, 1, 2), (8, 1 << 20, 6), (64, 16, 2), (1 << 30, 4, 6)])
+def test_lb_parameters_do_not_change_results(hn, g14, T, alpha, ipt):
+    print(f"case msbfs T={T} alpha={alpha} units/lane={ipt}", flush=True)
     off, adj, h = g14
     srcs = O.pick_sources(off, 64, seed=5)
     ms = hn.BfsMulti(h)
     ms.set_params(T, alpha)
+    ms.set_lb_tile(ipt)
+    assert ms.params() == (T, alpha, ipt)
     got = ms.run(srcs).cpu().numpy()
     assert np.array_equal(got, expected(off, adj, srcs))
+    ms.set_lb_tile(4)
+
+
+def test_tuner_picks_a_tile_and_keeps_results(hn, g14):
+    print("case msbfs tuner", flush=True)
+    off, adj, h = g14
+    srcs = O.pick_sources(off, 64, seed=6)
+    ms = hn.BfsMulti(h)
+    T, A = ms.tune(srcs)
+    t2, a2, ipt = ms.params()
+    assert (t2, a2) == (T, A) and ipt in (2, 4, 6)
+    assert np.array_equal(ms.run(srcs).cpu().numpy(), expected(off, adj, srcs))
     ms.set_params(32, 16)
 
 

@@ -109,3 +109,25 @@ def test_tc_no_triangles_and_empty(hn):
     assert total == 0 and not per.any()
     total, _ = tc_count(hn, hn.Hornet.from_csr(np.zeros(5, np.int64), np.zeros(0, np.int32)))
     assert total == 0
+
+
+def test_tc2_credit_rule_hand_case(hn):
+    """The hand-worked stored graph of the oracle pin (primitives/Operator++.i.cuh:120-131,
+    triangle2.cu:121): smaller-degree endpoint credited, ties to the destination,
+    src == dst and deg(src) < 2 skipped -> per-vertex [0, 1, 1, 1, 0, 0]."""
+    print("case tc2 hand-worked credit rule", flush=True)
+    off = np.array([0, 2, 4, 5, 6, 8, 8], np.int64)
+    adj = np.array([1, 2, 2, 3, 3, 3, 4, 5], np.int32)
+    h = hn.Hornet.from_csr(off, adj)
+    total, per = tc_stored(hn, h)
+    assert per.tolist() == [0, 1, 1, 1, 0, 0] and total == 3
+
+
+def test_gmtx_max_degree_tie_hand(hn):
+    """G.mtx read UNDIRECTED: vertices 2 and 3 tie at degree 4; the lowest id, 2,
+    is Hornet::max_degree_id (hand value, GraphStd.i.hpp:296-299)."""
+    print("case G.mtx max-degree tie", flush=True)
+    n, s, d, _ = O.read_market(os.path.join(GOLD, "G.mtx"))
+    off, adj = O.coo_to_csr(n, s, d, symmetrize=True, drop_self_loops=False, dedup=True)
+    h = hn.Hornet.from_csr(off, adj)
+    assert h.max_degree_id() == 2 and h.max_degree() == 4

@@ -130,24 +130,85 @@ def test_invalid_batch_leaves_graph_unchanged(hn, g12):
     check_equal(h, off, adj)
 
 
-def test_arena_exhaustion_is_reported_and_harmless(hn):
-    print("case arena exhaustion", flush=True)
-    from hornet_amd import HornetError
+def test_arena_grows_on_demand(hn):
+    """An insert that does not fit the arena grows it and runs again
+    (BlockArrayManager adds BlockArrays, BlockArray.i.cuh:138-161)."""
+    print("case arena growth", flush=True)
     off, adj = O.rmat_graph(12, seed=9)
     h = hn.Hornet.from_csr(off, adj, slack=1.0)
-    used0, cap = h.memory()
+    _, cap0 = h.memory()
     n = off.size - 1
     bs, bd = O.uniform_pairs(n, 400000, seed=1)
-    with pytest.raises(HornetError) as ei:
-        h.insert(hn.BatchUpdate(bs, bd))
-    assert ei.value.status == 2
-    check_equal(h, off, adj)
-    assert h.memory() == (used0, cap)
+    added = h.insert(hn.BatchUpdate(bs, bd))
+    o2, a2, e_added = O.insert(off, adj, bs, bd)
+    assert added == e_added
+    check_equal(h, o2, a2)
+    used, cap = h.memory()
+    assert cap > cap0 and used <= cap
     # a batch that fits still works afterwards
     bs2, bd2 = O.uniform_pairs(n, 50, seed=2)
     h.insert(hn.BatchUpdate(bs2, bd2))
-    o2, a2, _ = O.insert(off, adj, bs2, bd2)
+    o3, a3, _ = O.insert(o2, a2, bs2, bd2)
+    check_equal(h, o3, a3)
+
+
+def test_empty_graph_grows_past_ten_times_its_arena(hn):
+    """Hornet(nV) with no edges (4096 units = 131072 slots) filled with an
+    R-MAT s15 graph and more pairs, in batches, through several growths; the
+    batched BFS (whose captured level step reads the arena) stays exact."""
+    print("case empty graph growth", flush=True)
+    n = 1 << 15
+    h = hn.Hornet.from_csr(np.zeros(n + 1, np.int64), np.zeros(0, np.int32))
+    _, cap0 = h.memory()
+    off = np.zeros(n + 1, np.int64)
+    adj = np.zeros(0, np.int32)
+    gs, gd = O.rmat_pairs(15, seed=4)
+    us, ud = O.uniform_pairs(n, 500000, seed=5)
+    ms = hn.BfsMulti(h)
+    srcs = None
+    for i, (bs, bd) in enumerate([(gs[:100000], gd[:100000]), (gs[100000:], gd[100000:]), (us, ud),
+                                  (gd, gs)]):
+        h.insert(hn.BatchUpdate(bs, bd), True, True)
+        off, adj, _ = O.insert(off, adj, bs, bd, True, True)
+        check_equal(h, off, adj)
+        if srcs is None:
+            srcs = O.pick_sources(off, 8, seed=1)
+        dist = ms.run(srcs).cpu().numpy()
+        for j, s in enumerate(srcs):
+            assert np.array_equal(dist[j], O.bfs(off, adj, int(s))[0]), f"batch {i} source {s}"
+    used, cap = h.memory()
+    assert adj.size >= 10 * cap0 and cap >= used >= adj.size
+
+
+def test_failed_batch_returns_its_blocks(hn, monkeypatch):
+    """A batch that fails after its blocks were allocated (a device allocation
+    failure, forced from the host) hands them back: the arena bookkeeping and
+    the graph are as before, and the next batch works."""
+    print("case failed batch rollback", flush=True)
+    from hornet_amd import HornetError
+    off, adj = O.rmat_graph(12, seed=9)
+    h = hn.Hornet.from_csr(off, adj, slack=1.5)
+    mem0 = h.memory()
+    n = off.size - 1
+    bs, bd = O.uniform_pairs(n, 20000, seed=3)
+    s0, d0 = coo(off, adj)
+    es, ed = s0[::2], d0[::2]
+    monkeypatch.setenv("HORNET_HIP_FAIL_SIDE_RESERVE", "1")
+    with pytest.raises(HornetError) as ei:
+        h.insert(hn.BatchUpdate(bs, bd))
+    assert ei.value.status == 2
+    with pytest.raises(HornetError) as ei:
+        h.erase(hn.BatchUpdate(es, ed))
+    assert ei.value.status == 2
+    monkeypatch.delenv("HORNET_HIP_FAIL_SIDE_RESERVE")
+    assert h.memory() == mem0
+    check_equal(h, off, adj)
+    h.insert(hn.BatchUpdate(bs, bd))
+    o2, a2, _ = O.insert(off, adj, bs, bd)
     check_equal(h, o2, a2)
+    h.erase(hn.BatchUpdate(es, ed))
+    o3, a3, _ = O.erase(o2, a2, es, ed)
+    check_equal(h, o3, a3)
 
 
 def test_free_lists_recycle_blocks(hn):
@@ -397,3 +458,32 @@ def test_updates_on_many_short_rows(hn):
     off, adj, added = O.insert(off, adj, bs, bd, True, True)
     assert got == added
     check_equal(h, off, adj)
+
+
+def test_update_stats_match_host_model(hn):
+    """hn_graph_update_stats after an insert and an erase: the touched vertices,
+    their degree sum and the applied count follow from the batch and the host
+    model; relocations are those whose block no longer fits."""
+    print("case update stats", flush=True)
+    off, adj = O.rmat_graph(12, seed=4)
+    h = hn.Hornet.from_csr(off, adj)
+    n = off.size - 1
+    deg = np.diff(off)
+    bs, bd = O.uniform_pairs(n, 3000, seed=8)
+    added = h.insert(hn.BatchUpdate(bs, bd), True, True)
+    st = h.update_stats()
+    o2, a2, e_added = O.insert(off, adj, bs, bd, True, True)
+    assert st["kind"] == 1 and st["submitted"] == bs.size and st["applied"] == added == e_added
+    # touched = distinct sources of the kept entries
+    keep_src = np.repeat(np.arange(n), np.diff(o2) - deg)
+    T = np.unique(keep_src)
+    assert st["touched"] == T.size and st["touched_deg"] == int(deg[T].sum())
+    cap = np.where(deg > 0, np.maximum(32, 2 ** np.ceil(np.log2(np.maximum(deg, 1)))), 0)
+    R = T[np.diff(o2)[T] > cap[T]]
+    assert st["relocated"] == R.size and st["relocated_deg"] == int(deg[R].sum())
+    s2, d2 = coo(o2, a2)
+    es, ed = s2[::3], d2[::3]
+    removed = h.erase(hn.BatchUpdate(es, ed))
+    st = h.update_stats()
+    assert st["kind"] == 2 and st["applied"] == removed == es.size
+    assert st["touched"] == np.unique(es).size
