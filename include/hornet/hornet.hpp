@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -98,6 +99,12 @@ public:
         static_assert(sizeof...(Ps) == (size_t)num_edge_meta, "one array per edge meta-data field");
         int i = 0;
         ((_meta[i++] = static_cast<const void*>(edge_meta_data)), ...);
+    }
+    // one field at a time (HornetInit.cuh:78-79); fields not given read as zero
+    template <unsigned N>
+    void insertEdgeData(const typename detail::FieldAt<N, EMeta>::type* edge_meta_data) noexcept {
+        static_assert(N < (unsigned)num_edge_meta, "no such edge meta-data field");
+        _meta[N] = static_cast<const void*>(edge_meta_data);
     }
     VID nV() const noexcept { return _nV; }
     DEG nE() const noexcept { return _nE; }
@@ -201,6 +208,68 @@ private:
     void* _meta[HN_MAX_EDGE_META] = {};
 };
 
+namespace detail {
+template <typename DEG>
+__global__ void narrow_offsets_kernel(int64_t n, const int64_t* __restrict__ in, DEG* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (DEG)in[i];
+}
+}  // namespace detail
+
+// CSR (hornet/include/Core/Static/Static.cuh:92-129): offsets[nV+1] and the
+// index (destinations and edge meta-data) in device memory owned by the
+// object.  offset() holds degree_t offsets as in the reference (valid while
+// nE fits a degree_t); offset64() holds the same offsets as int64.
+template <DeviceType device = DeviceType::DEVICE, typename VID = vid_t, typename EMeta = EMPTY,
+          typename DEG = degree_t>
+class CSR {
+    static_assert(device == DeviceType::DEVICE, "getCSR returns device storage");
+    static constexpr int NM = detail::MetaTraits<EMeta>::n;
+
+public:
+    CSR() = default;
+    CSR(int64_t nv, int64_t ne) : _nv(nv), _ne(ne), _index(ne) {
+        check(hipMalloc(&_off64, sizeof(int64_t) * (size_t)(nv + 1)) == hipSuccess ? HN_OK : HN_E_OOM, "CSR");
+        check(hipMalloc(&_off, sizeof(DEG) * (size_t)(nv + 1)) == hipSuccess ? HN_OK : HN_E_OOM, "CSR");
+    }
+    CSR(const CSR&) = delete;
+    CSR& operator=(const CSR&) = delete;
+    CSR(CSR&& o) noexcept : _nv(o._nv), _ne(o._ne), _off(o._off), _off64(o._off64), _index(std::move(o._index)) {
+        o._off = nullptr;
+        o._off64 = nullptr;
+        o._nv = o._ne = 0;
+    }
+    ~CSR() {
+        if (_off) (void)hipFree(_off);
+        if (_off64) (void)hipFree(_off64);
+    }
+    DEG* offset() noexcept { return _off; }
+    int64_t* offset64() noexcept { return _off64; }
+    VID* index() noexcept { return _index.dstPtr(); }
+    template <unsigned N>
+    typename detail::FieldAt<N, EMeta>::type* edgeMetaPtr() const noexcept {
+        return _index.template edgeMetaPtr<N>();
+    }
+    int64_t nV() const noexcept { return _nv; }
+    int64_t nE() const noexcept { return _ne; }
+    // copy to host vectors (int64 offsets)
+    void toHost(std::vector<int64_t>& off, std::vector<VID>& adj) const {
+        off.resize((size_t)_nv + 1);
+        adj.resize((size_t)_ne);
+        if (_off64) (void)hipMemcpy(off.data(), _off64, sizeof(int64_t) * off.size(), hipMemcpyDeviceToHost);
+        if (_ne > 0) (void)hipMemcpy(adj.data(), _index.dstPtr(), sizeof(VID) * (size_t)_ne, hipMemcpyDeviceToHost);
+    }
+    // the index as a COO whose source array is unused (destinations + meta-data)
+    COO<device, VID, EMeta, DEG>& index_storage() noexcept { return _index; }
+
+private:
+    int64_t _nv = 0;
+    int64_t _ne = 0;
+    DEG* _off = nullptr;
+    int64_t* _off64 = nullptr;
+    COO<device, VID, EMeta, DEG> _index;
+};
+
 namespace gpu {
 
 // A batch as seen by insert/erase: the library sorts and deduplicates it in
@@ -301,6 +370,32 @@ public:
                   "Hornet::getCOO");
         return coo;
     }
+    // getCSR (HornetQuery.i.cuh:69-107; the reference's flatten leaves the
+    // index unsorted unless asked): rows are always sorted here
+    CSR<DeviceType::DEVICE, VID, EMeta, DEG> getCSR(bool sortAdjacencyList = false) const {
+        (void)sortAdjacencyList;
+        const int64_t nv = nV(), ne = nE64();
+        CSR<DeviceType::DEVICE, VID, EMeta, DEG> csr(nv, ne);
+        check(hn_graph_export_csr_meta(_g, csr.offset64(), csr.index(), csr.index_storage().edge_meta(), ne,
+                                       _stream),
+              "Hornet::getCSR");
+        const int64_t n = nv + 1;
+        detail::narrow_offsets_kernel<DEG><<<(unsigned)((n + 255) / 256), 256, 0, _stream>>>(n, csr.offset64(),
+                                                                                           csr.offset());
+        check(hipStreamSynchronize(_stream) == hipSuccess ? HN_OK : HN_E_HIP, "Hornet::getCSR");
+        return csr;
+    }
+    // print (HornetInitialize.i.cuh:152-170): "v : degree | neighbours" per vertex
+    void print() const {
+        std::vector<int64_t> off;
+        std::vector<VID> adj;
+        getCSR().toHost(off, adj);
+        for (int64_t v = 0; v + 1 < (int64_t)off.size(); v++) {
+            std::printf("%lld : %lld | ", (long long)v, (long long)(off[v + 1] - off[v]));
+            for (int64_t j = off[v]; j < off[v + 1]; j++) std::printf("%d ", (int)adj[j]);
+            std::printf("\n");
+        }
+    }
     hn_graph* handle() const noexcept { return _g; }
     hipStream_t stream() const noexcept { return _stream; }
 
@@ -326,10 +421,20 @@ class Hornet : public HornetStatic<VID, VMeta, EMeta, DEG> {
 public:
     using typename Base::HInitT;
 
+    // Hornet(void) / Hornet(degree_t nV) (HornetInitialize.i.cuh:72-84): a
+    // graph of nV vertices and no edges, which inserts then fill
+    Hornet() : Hornet(DEG(0)) {}
+    explicit Hornet(DEG nV, hipStream_t stream = nullptr) : Base(empty_graph(nV, stream), stream) {}
     // Hornet(HornetInit&) (HornetInitialize.i.cuh:86-146); `arena_slack` sizes
     // the edge arena as a multiple of the blocks the input needs
     explicit Hornet(HInitT& init, double arena_slack = 0.0, hipStream_t stream = nullptr)
         : Base(init, arena_slack, stream) {}
+    // reset(HornetInit&) (HornetInitialize.i.cuh:188-204): the graph becomes
+    // the one the init describes; on failure it is left unchanged
+    void reset(HInitT& init, double arena_slack = 0.0) {
+        Hornet fresh(init, arena_slack, _stream);
+        std::swap(_g, fresh._g);
+    }
     // adopt a handle created through the C ABI
     explicit Hornet(hn_graph* g, hipStream_t stream = nullptr) noexcept : Base(g, stream) {}
 
@@ -356,6 +461,19 @@ public:
         hn_graph* c = nullptr;
         check(hn_graph_snapshot(_g, arena_slack, _stream, &c), "Hornet::snapshot");
         return Base(c, _stream);
+    }
+
+private:
+    static hn_graph* empty_graph(DEG nV, hipStream_t stream) {
+        if (nV < 0) throw Error(HN_E_INVALID_ARG, "Hornet(nV)", "negative vertex count");
+        std::vector<int64_t> off((size_t)nV + 1, 0);
+        hn_edge_meta layout;
+        detail::MetaTraits<EMeta>::layout(layout);
+        const void* none[HN_MAX_EDGE_META] = {};
+        hn_graph* g = nullptr;
+        check(hn_graph_create_csr_meta(nV, 0, off.data(), nullptr, &layout, none, 0, 0.0, stream, &g),
+              "Hornet(nV)");
+        return g;
     }
 };
 

@@ -35,6 +35,9 @@
 //                                       max_out_degree_id; writes int32 dist[nv]
 //   tcfile  <file>                      TriangleTest2.cu:86-100 shape: read
 //                                       UNDIRECTED, DIRECTED_BY_DEGREE | SORT
+//   api     <graph> <out.bin>           Hornet(), Hornet(nV) + insert, getCSR, reset,
+//                                       print, insertEdgeData<N>, TriangleCounting2::validate;
+//                                       writes the CSR after the insert and after the reset
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -416,6 +419,76 @@ int main(int argc, char** argv) {
         };
         put(frozen);
         put(resumed);
+        return 0;
+    }
+    if (cmd == "api" && argc >= 4) {
+        // Hornet(void); Hornet(nV) filled by one insert; getCSR; reset(HornetInit&);
+        // print; HornetInit::insertEdgeData<N>; TriangleCounting2::validate
+        hornet::gpu::Hornet<int> none;
+        std::printf("api default nv=%d ne=%d\n", none.nV(), none.nE());
+        HornetGraph grown((int)g.nv);
+        std::vector<int> s, d;
+        for (int64_t v = 0; v < g.nv; v++)
+            for (int j = g.off[(size_t)v]; j < g.off[(size_t)v + 1]; j++) s.push_back((int)v), d.push_back(g.adj[(size_t)j]);
+        hornet::BatchUpdatePtr<int, hornet::EMPTY, hornet::DeviceType::HOST> ptr((int)s.size(), s.data(), d.data());
+        hornet::gpu::BatchUpdate<int> batch(ptr);
+        grown.insert(batch);
+        std::ofstream out(argv[3], std::ios::binary);
+        auto put_csr = [&](const HornetGraph& h, const char* tag) {
+            auto csr = h.getCSR();
+            std::vector<int64_t> o64;
+            std::vector<int> a;
+            csr.toHost(o64, a);
+            std::vector<int> o32(o64.size());
+            (void)hipMemcpy(o32.data(), csr.offset(), sizeof(int) * o32.size(), hipMemcpyDeviceToHost);
+            int same = 1;
+            for (size_t i = 0; i < o64.size(); i++) same &= (int64_t)o32[i] == o64[i] ? 1 : 0;
+            std::printf("api %s nv=%lld ne=%lld offsets32_equal=%d\n", tag, (long long)csr.nV(), (long long)csr.nE(),
+                        same);
+            int64_t n[2] = {(int64_t)o64.size() - 1, (int64_t)a.size()};
+            out.write(reinterpret_cast<const char*>(n), 16);
+            out.write(reinterpret_cast<const char*>(o64.data()), (std::streamsize)(8 * o64.size()));
+            out.write(reinterpret_cast<const char*>(a.data()), (std::streamsize)(4 * a.size()));
+        };
+        std::printf("api grown inserted=%lld\n", (long long)batch.applied());
+        put_csr(grown, "grown");
+        // reset to the rows of the first half of the vertices
+        const int64_t half = g.nv / 2;
+        std::vector<int> hoff(g.off.begin(), g.off.end());
+        for (int64_t v = half + 1; v <= g.nv; v++) hoff[(size_t)v] = g.off[(size_t)half];
+        HornetInit hinit((int)g.nv, g.off[(size_t)half], hoff.data(), g.adj.data());
+        grown.reset(hinit);
+        put_csr(grown, "reset");
+        TriangleCounting2 tc(hornet_graph);
+        tc.init();
+        tc.run_undirected();
+        const int v_und = tc.validate() ? 1 : 0;
+        const unsigned long long t_und = tc.countTriangles();
+        tc.run();
+        std::printf("api tc undirected=%llu validate_undirected=%d stored=%llu validate_stored=%d\n", t_und, v_und,
+                    tc.total_count(), tc.validate() ? 1 : 0);
+        // one meta-data field given, the other read as zero
+        using W = hornet::TypeList<int, double>;
+        std::vector<double> w1((size_t)g.ne);
+        for (size_t j = 0; j < w1.size(); j++) w1[j] = 0.25 * (double)j;
+        hornet::HornetInit<int, hornet::EMPTY, W> winit((int)g.nv, (int)g.ne, g.off.data(), g.adj.data());
+        winit.insertEdgeData<1>(w1.data());
+        hornet::gpu::Hornet<int, hornet::EMPTY, W> wh(winit);
+        auto wcsr = wh.getCSR();
+        std::vector<int> f0((size_t)g.ne);
+        std::vector<double> f1((size_t)g.ne);
+        if (g.ne > 0) {
+            (void)hipMemcpy(f0.data(), wcsr.edgeMetaPtr<0>(), 4 * f0.size(), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(f1.data(), wcsr.edgeMetaPtr<1>(), 8 * f1.size(), hipMemcpyDeviceToHost);
+        }
+        long long z = 0;
+        double sw = 0;
+        for (size_t j = 0; j < f0.size(); j++) z += f0[j] != 0, sw += f1[j];
+        std::printf("api meta nonzero0=%lld sum1_x4=%lld\n", z, (long long)(sw * 4.0));
+        std::fflush(stdout);
+        // print() (HornetInitialize.i.cuh:152-170) of the reset graph
+        grown.print();
+        std::fflush(stdout);
         return 0;
     }
     if (cmd == "update" && argc >= 5) {

@@ -219,3 +219,36 @@ def test_edge_meta_through_headers(graph, tmp_path):
     g1 = np.frombuffer(raw, np.float64, m, pos)
     assert np.array_equal(src, np.repeat(np.arange(n), np.diff(o1))) and np.array_equal(dst, a1)
     assert np.array_equal(g0, f0) and np.array_equal(g1, f1)
+
+
+def test_hornet_api_members(graph, tmp_path):
+    """Hornet(), Hornet(nV) filled by an insert, getCSR (int32 and int64
+    offsets), reset(HornetInit&), print(), HornetInit::insertEdgeData<N> and
+    TriangleCounting2::validate (Hornet.cuh:101-129, HornetInit.cuh:78-79,
+    HornetInitialize.i.cuh:152-204)."""
+    off, adj, p = graph
+    out = run("api", p, str(tmp_path / "api.bin"))
+    assert "api default nv=0 ne=0" in out
+    assert field(out, "inserted") == adj.size
+    with open(tmp_path / "api.bin", "rb") as f:
+        def csr():
+            nv, ne = np.fromfile(f, np.int64, 2)
+            return np.fromfile(f, np.int64, nv + 1), np.fromfile(f, np.int32, ne)
+        g_off, g_adj = csr()
+        r_off, r_adj = csr()
+    assert np.array_equal(g_off, off) and np.array_equal(g_adj, adj)
+    half = (off.size - 1) // 2
+    exp_off = np.minimum(off, off[half])
+    assert np.array_equal(r_off, exp_off) and np.array_equal(r_adj, adj[: off[half]])
+    assert out.count("offsets32_equal=1") == 2
+    tri = O.triangles(off, adj)
+    assert field(out, "undirected") == tri and field(out, "validate_undirected") == 1
+    assert field(out, "stored") == int(O.tc2_per_vertex(off, adj).sum()) and field(out, "validate_stored") == 1
+    assert field(out, "nonzero0") == 0 and field(out, "sum1_x4") == adj.size * (adj.size - 1) // 2
+    # print(): "v : degree | neighbours" for every vertex of the reset graph
+    rows = [ln for ln in out.splitlines() if re.match(r"^\d+ : \d+ \|", ln)]
+    assert len(rows) == off.size - 1
+    for v in (0, 1, half - 1, half, off.size - 2):
+        head, tail = rows[v].split("|")
+        assert head.split() == [str(v), ":", str(exp_off[v + 1] - exp_off[v])]
+        assert [int(x) for x in tail.split()] == adj[exp_off[v]:exp_off[v + 1]].tolist()
