@@ -30,6 +30,15 @@ HBM_PEAK_GBS = 8000.0
 
 
 def cpu_model() -> str:
+    """The full CPU model string (lscpu "Model name", else /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.strip().startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
@@ -38,6 +47,43 @@ def cpu_model() -> str:
     except OSError:
         pass
     return platform.processor() or "unknown"
+
+
+class one_core:
+    """Pins the calling thread to one CPU for a timed host baseline (restored
+    on exit)."""
+
+    def __enter__(self):
+        self.saved = os.sched_getaffinity(0)
+        self.core = min(self.saved)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.saved)
+        return False
+
+
+def median_time(fn, reps=3):
+    """(median seconds, last result) over `reps` calls of fn()."""
+    ts = []
+    out = None
+    for _ in range(reps):
+        t = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t)
+    ts.sort()
+    return ts[len(ts) // 2], out
+
+
+def update_bytes(st):
+    """Algorithmic HBM bytes of one batch (b submitted entries, b' applied,
+    T touched, R relocated): insert 8b + 16b + 16|T| + 4 sum_T d_old + 4b' +
+    8 sum_R d_old + 16|R|; erase the same with 8b' and sum_R d_new."""
+    b = st["submitted"]
+    per_applied = 4 if st["kind"] == 1 else 8
+    return (24 * b + 16 * st["touched"] + 4 * st["touched_deg"] + per_applied * st["applied"]
+            + 8 * st["relocated_deg"] + 16 * st["relocated"])
 
 
 def load_traffic(name: str):
@@ -63,6 +109,8 @@ def update_rounds(h, args, torch):
     ins_t = era_t = 0.0
     ins_n = era_n = 0
     ins_added = era_removed = 0
+    ins_bytes = era_bytes = 0
+    moved = 0
     tc_t = 0.0
     tris = []
     tc = TriangleCounting2(h)
@@ -77,6 +125,9 @@ def update_rounds(h, args, torch):
         torch.cuda.synchronize()
         ins_t += time.perf_counter() - t
         ins_n += bi.nE()
+        st = h.update_stats()
+        ins_bytes += update_bytes(st)
+        moved += st["moved"]
         # deletions: positions uniform over the current directed entries
         src, dst = h.getCOO()
         pos, _ = uniform_pairs_device(src.numel(), args.batch, seed=100 + r)
@@ -89,6 +140,9 @@ def update_rounds(h, args, torch):
         torch.cuda.synchronize()
         era_t += time.perf_counter() - t
         era_n += be.nE()
+        st = h.update_stats()
+        era_bytes += update_bytes(st)
+        moved += st["moved"]
         t = time.perf_counter()
         tc.run_undirected()
         torch.cuda.synchronize()
@@ -104,6 +158,19 @@ def update_rounds(h, args, torch):
         "erase_ms_avg": round(era_t / args.rounds * 1e3, 3),
         "inserted": ins_added, "erased": era_removed, "final_ne": h.nE(),
         "unit": "directed edge-updates/s (submitted entries)",
+        # whole insert / erase pipelines against HBM: the algorithmic bytes of
+        # each batch (touched and relocated vertices measured per batch) over
+        # its wall time
+        "roofline": {
+            "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "achieved": round((ins_bytes + era_bytes) / (ins_t + era_t) / 1e9, 1),
+            "frac": round((ins_bytes + era_bytes) / (ins_t + era_t) / 1e9 / HBM_PEAK_GBS, 4),
+            "insert_GBps": round(ins_bytes / ins_t / 1e9, 1), "erase_GBps": round(era_bytes / era_t / 1e9, 1),
+            "alg_bytes_per_insert": round(ins_bytes / max(args.rounds, 1)),
+            "alg_bytes_per_erase": round(era_bytes / max(args.rounds, 1)),
+            "entries_moved_per_batch": round(moved / max(2 * args.rounds, 1)),
+            "traffic": (load_traffic("update_traffic.json") or {}).get("bytes_per_batch"),
+        },
     }
     tc_info = {"triangles_per_round": tris, "ms_avg": round(tc_t / args.rounds * 1e3, 3),
                "triangles_per_s": round(sum(tris) / tc_t, 1)}
@@ -297,6 +364,7 @@ def served(args, torch):
     trace = serve_trace(h, args.serve_requests, seed=5, torch=torch)
     out = {"graph": f"R-MAT s{args.serve_scale} ef16", "nv": h.nV(), "ne": h.nE()}
     h.release()
+    results = {}
     for label, plain, n in (("served", False, len(trace)), ("plain", True, min(args.serve_plain, len(trace)))):
         # every replay starts from the same graph: the trace's updates change it
         sub = trace[:n]
@@ -316,13 +384,175 @@ def served(args, torch):
         el = time.perf_counter() - t
         out[label] = {"saturating_rps": round(sat, 1), "offered_rps": round(rate, 1), "slots": srv.slots(),
                       **latency_stats(res, el)}
+        results[label] = res
         srv.close()
         h.release()
         torch.cuda.empty_cache()
         print(f"# serve {label}: {out[label]}", file=sys.stderr, flush=True)
     out["saturating_speedup_vs_plain"] = round(out["served"]["saturating_rps"] / out["plain"]["saturating_rps"], 2)
-    out["plain_requests"] = min(args.serve_plain, len(trace))
+    n_plain = min(args.serve_plain, len(trace))
+    out["plain_requests"] = n_plain
+    # every request of the plain prefix: same graph version and digest when served
+    p, q = results["plain"], results["served"][:n_plain]
+    for f in ("kind", "status", "version", "reached", "dist_sum", "max_depth", "applied"):
+        if not np.array_equal(p[f], q[f]):
+            raise CheckFailed(f"served and plain results differ in {f}")
+    out["served_equals_plain_on_prefix"] = True
+    # latency at equal load: the served form offered the plain form's rate
+    rate = out["plain"]["offered_rps"]
+    arrivals = np.cumsum(np.random.default_rng(5).exponential(1e9 / rate, n_plain)).astype(np.int64)
+    h = fresh()
+    srv = Server(h, max_batch=64, slots=args.serve_slots)
+    t = time.perf_counter()
+    res = srv.replay(trace[:n_plain], arrivals)
+    out["served_at_plain_rate"] = {"offered_rps": rate, **latency_stats(res, time.perf_counter() - t)}
+    srv.close()
+    h.release()
+    torch.cuda.empty_cache()
+    print(f"# serve served at the plain rate: {out['served_at_plain_rate']}", file=sys.stderr, flush=True)
     return out
+
+
+def arena_info(h):
+    """Edge-arena footprint of the resident graph: slots held by blocks and
+    slots reserved, per stored entry (4 bytes a slot)."""
+    used, cap = h.memory()
+    ne = max(h.nE(), 1)
+    return {"used_bytes_per_edge": round(4 * used / ne, 3), "capacity_bytes_per_edge": round(4 * cap / ne, 3),
+            "used_GB": round(4 * used / 1e9, 3), "capacity_GB": round(4 * cap / 1e9, 3)}
+
+
+def batch_sweep(h, args, torch):
+    """Insert and erase pipelines at 2 M, 8 M and 32 M directed entries (1, 4
+    and 16 M uniform pairs, symmetrized, removeBatchDuplicates +
+    removeGraphDuplicates; the erase takes the same entries back) on a copy of
+    the graph; each size runs twice and the second run is reported."""
+    from hornet_amd.graph import BatchUpdate, uniform_pairs_device
+
+    g = h.copy(slack=2.0)
+    nv = g.nV()
+    out = []
+    for pairs in args.sweep:
+        us, ud = uniform_pairs_device(nv, pairs, seed=7000 + pairs.bit_length())
+        keep = us != ud
+        b = BatchUpdate(torch.cat([us[keep], ud[keep]]), torch.cat([ud[keep], us[keep]]))
+        del us, ud, keep
+        for rep in range(2):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            g.insert(b, True, True)
+            torch.cuda.synchronize()
+            ti = time.perf_counter() - t
+            si = g.update_stats()
+            t = time.perf_counter()
+            g.erase(b, True)
+            torch.cuda.synchronize()
+            te = time.perf_counter() - t
+            se = g.update_stats()
+        bi, be = update_bytes(si), update_bytes(se)
+        out.append({"entries": b.nE(), "insert_ms": round(ti * 1e3, 3), "erase_ms": round(te * 1e3, 3),
+                    "insert_updates_per_s": round(b.nE() / ti, 1), "erase_updates_per_s": round(b.nE() / te, 1),
+                    "insert_hbm_frac": round(bi / ti / 1e9 / HBM_PEAK_GBS, 4),
+                    "erase_hbm_frac": round(be / te / 1e9 / HBM_PEAK_GBS, 4),
+                    "touched": si["touched"], "relocated": si["relocated"], "entries_moved": si["moved"]})
+        print(f"# batch sweep: {out[-1]}", file=sys.stderr, flush=True)
+        del b
+    g.release()
+    torch.cuda.empty_cache()
+    return out
+
+
+class CheckFailed(RuntimeError):
+    pass
+
+
+def cpu_baselines_and_checks(h, sources, updates, tc_info, args, torch):
+    """Host baselines on one pinned core (median of 3): the oracle's sequential
+    BFS (a restatement of xlib/src/Graph/BFS.cpp:54-76) from a bounded sample
+    of the s24 sources, the host set-model insert of one 2 M-entry batch, and
+    the reference's CPU configuration (R-MAT s16 seed 1: BFS from the
+    max-degree vertex and the host triangle count of
+    hornetsnest/test/TriangleTest2.cu:50-69).  The oracle outputs are then
+    compared with the device's on the same inputs (outside every timed
+    region); a mismatch stops the benchmark."""
+    import numpy as np
+
+    import oracle as O
+    from hornet_amd.graph import BatchUpdate, BfsTopDown, Hornet, TriangleCounting2, uniform_pairs_device
+
+    off, adj = h.getCSR()
+    off = off.cpu().numpy()
+    adj = adj.cpu().numpy()
+    nv = h.nV()
+    # s24 BFS: sources in order until the sample reaches about cpu_seconds
+    bfs_times, bfs_m, used = [], 0, []
+    with one_core() as pin:
+        for s in sources:
+            t, (dist, scanned) = median_time(lambda: O.bfs(off, adj, int(s)))
+            bfs_times.append(t)
+            bfs_m += scanned
+            used.append((int(s), dist))
+            if 3 * sum(bfs_times) >= args.cpu_seconds:
+                break
+        core = pin.core
+    gbfs = BfsTopDown(h)
+    for s, dist in used:
+        gbfs.set_parameters(s)
+        gbfs.run()
+        if not np.array_equal(gbfs.distances().cpu().numpy(), dist):
+            raise CheckFailed(f"BFS distances from source {s} differ from the oracle")
+    cpu = {"value": round(bfs_m / sum(bfs_times) / 1e9, 4), "unit": "GTEPS", "cores": 1, "kind": "port",
+           "sample": f"sequential host BFS (oracle restatement of xlib/src/Graph/BFS.cpp:54-76) from the first "
+                     f"{len(used)} of the {len(sources)} sources on the s24 graph after the update rounds, "
+                     f"median of 3 runs each, pinned to CPU {core}",
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(), "checked_against_device": True}
+    if updates is not None:
+        us, ud = uniform_pairs_device(nv, args.batch, seed=1000)
+        us, ud = us.cpu().numpy(), ud.cpu().numpy()
+        keep = us != ud
+        bs = np.concatenate([us[keep], ud[keep]])
+        bd = np.concatenate([ud[keep], us[keep]])
+        with one_core():
+            t, (o2, a2, added) = median_time(lambda: O.insert(off, adj, bs, bd, True, True))
+        g = h.copy(slack=1.2)
+        got = g.insert(BatchUpdate(bs, bd), True, True)
+        go, ga = g.getCSR()
+        same = got == added and np.array_equal(go.cpu().numpy(), o2) and np.array_equal(ga.cpu().numpy(), a2)
+        g.release()
+        del go, ga
+        if not same:
+            raise CheckFailed("the device insert differs from the host set model")
+        updates["cpu_baseline"] = {
+            "value": round(bs.size / t, 1), "unit": "edge-updates/s", "cores": 1, "kind": "port",
+            "sample": f"host set-model insert (oracle restatement of BatchUpdate semantics) of {bs.size} directed "
+                      f"entries into the s24 graph, median of 3, one pinned core; the device result of the same "
+                      f"insert is equal"}
+    # the reference's CPU configuration, in full
+    o1, a1 = O.rmat_graph(16, seed=1)
+    src1 = O.max_degree_id(o1)
+    with one_core():
+        tb, (d1, m1) = median_time(lambda: O.bfs(o1, a1, src1))
+        tt, tri1 = median_time(lambda: O.triangles(o1, a1))
+    g1 = Hornet.from_csr(o1, a1)
+    b1 = BfsTopDown(g1)
+    b1.set_parameters(src1)
+    b1.run()
+    t1 = TriangleCounting2(g1)
+    t1.init()
+    t1.run_undirected()
+    if not (np.array_equal(b1.distances().cpu().numpy(), d1) and t1.countTriangles() == tri1):
+        raise CheckFailed("R-MAT s16 BFS / triangle count differ from the oracle")
+    g1.release()
+    c1 = {"graph": "R-MAT s16 ef16 seed 1", "nv": int(o1.size - 1), "ne": int(a1.size), "bfs_source": int(src1),
+          "bfs_ms": round(tb * 1e3, 3), "bfs_gteps": round(m1 / tb / 1e9, 4), "triangles": int(tri1),
+          "tc_ms": round(tt * 1e3, 3), "triangles_per_s": round(tri1 / tt, 1), "cores": 1, "kind": "port",
+          "timing": "median of 3, one pinned core", "checked_against_device": True}
+    if tc_info is not None:
+        tc_info["cpu_baseline"] = {"value": c1["triangles_per_s"], "unit": "triangles/s", "cores": 1, "kind": "port",
+                                   "sample": "host triangle count (oracle restatement of "
+                                             "hornetsnest/test/TriangleTest2.cu:50-69 on the DIRECTED_BY_DEGREE "
+                                             "graph) of R-MAT s16 seed 1 in full, median of 3, one pinned core"}
+    return cpu, c1
 
 
 def main() -> int:
@@ -347,6 +577,8 @@ def main() -> int:
     ap.add_argument("--serve-plain", type=int, default=300, help="requests replayed in the plain form")
     ap.add_argument("--serve-scale", type=int, default=25)
     ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--sweep", type=lambda x: [int(v) for v in x.split(",") if v], default=[1 << 20, 1 << 22, 1 << 24],
+                    help="pairs per batch of the insert / erase size sweep (empty disables)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size reference "
                          "for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -438,41 +670,12 @@ def main() -> int:
         updates, tc_info = update_rounds(h, args, torch)
 
     cpu = None
+    c1 = None
     if args.cpu_seconds > 0:
-        import oracle as O
-        off, adj = h.getCSR()
-        off = off.cpu().numpy()
-        adj = adj.cpu().numpy()
-        if updates is not None:
-            # host set model on the same graph: one insert batch of the same size
-            from hornet_amd.graph import uniform_pairs_device
-            us, ud = uniform_pairs_device(nv, args.batch, seed=1000)
-            us, ud = us.cpu().numpy(), ud.cpu().numpy()
-            keep = us != ud
-            bs = np.concatenate([us[keep], ud[keep]])
-            bd = np.concatenate([ud[keep], us[keep]])
-            t = time.perf_counter()
-            O.insert(off, adj, bs, bd, True, True)
-            dt = time.perf_counter() - t
-            updates["cpu_baseline"] = {
-                "value": round(bs.size / dt, 1), "unit": "edge-updates/s", "cores": 1, "kind": "port",
-                "sample": f"one host set-model insert (oracle restatement of BatchUpdate semantics) of "
-                          f"{bs.size} directed entries into the same graph"}
-        cm = 0
-        used = 0
-        t_cpu = 0.0
-        for s in sources:
-            t = time.perf_counter()
-            _, scanned = O.bfs(off, adj, int(s))
-            t_cpu += time.perf_counter() - t
-            cm += scanned
-            used += 1
-            if t_cpu >= args.cpu_seconds:
-                break
-        cpu = {"value": round(cm / t_cpu / 1e9, 4), "unit": "GTEPS", "cores": 1, "kind": "port",
-               "sample": f"sequential host BFS (oracle restatement of xlib/src/Graph/BFS.cpp:54-76) "
-                         f"from the first {used} of the {len(sources)} sources on the same graph",
-               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+        cpu, c1 = cpu_baselines_and_checks(h, sources, updates, tc_info, args, torch)
+
+    arena = arena_info(h)
+    sweep = batch_sweep(h, args, torch) if args.sweep else None
 
     if args.meta_rounds > 0:
         meta_info = meta_rounds(h, args, torch)
@@ -507,7 +710,10 @@ def main() -> int:
         "gteps_graph500": round(gteps / 2, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_reference_config": c1,
+        "arena": arena,
         "updates": updates,
+        "update_batch_sweep": sweep,
         "triangle_counting": tc_info,
         "snapshot": snap_info,
         "updates_with_edge_meta": meta_info,
