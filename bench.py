@@ -27,6 +27,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BFS GTEPS + edge-updates/s at R-MAT s24; served p50/p99 latency vs plain"
 HBM_PEAK_GBS = 8000.0
+# L2 request ceiling: 8 XCDs x 16 L2 channels, one 128-byte request per
+# channel per clock at the 2.4 GHz peak engine clock
+L2_PEAK_GREQS = 8 * 16 * 2.4
 
 
 def cpu_model() -> str:
@@ -290,7 +293,7 @@ def batched_bfs(args, torch):
             "batched_gteps": round(edges / tb / 1e9, 2), "batched_s": round(tb, 4),
             "plain_gteps": round(pe / tp / 1e9, 2), "plain_sources_timed": plain_n,
             "speedup_vs_plain": round((edges / tb) / (pe / tp), 2),
-            "tuned_degree_threshold": T, "tuned_push_alpha": A,
+            "tuned_degree_threshold": T, "tuned_push_alpha": A, "tuned_lb_units_per_lane": ms.params()[2],
             "avg_levels_per_launch": round(levels / ((srcs.size + 63) // 64), 2),
         }
         print(f"# batched BFS {name}: {out[name]}", file=sys.stderr, flush=True)
@@ -649,6 +652,7 @@ def main() -> int:
     kernel_s = agg_ns * 1e-9
     achieved = alg_bytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
     traffic = load_traffic("bfs_expand_traffic.json")
+    l2req = load_traffic("bfs_expand_l2req.json")
     roofline = {
         "kernel": "bfs_expand_kernel",
         "bound": "hbm",
@@ -661,6 +665,17 @@ def main() -> int:
         "avg_launch_us": round(agg_ns / max(agg_l, 1) / 1e3, 2),
         "launches": agg_l,
     }
+    if l2req and l2req.get("read_req_per_entry"):
+        # the bound the random visited probes meet: L1 -> L2 read requests
+        # (TCP_TCC_READ_REQ per scanned entry, from the committed counter
+        # pass on the same workload) over the live kernel time
+        reqs = l2req["read_req_per_entry"] * agg_m
+        roofline["l2_requests"] = {
+            "bound": "l2 request rate", "unit": "G req/s", "peak": L2_PEAK_GREQS,
+            "achieved": round(reqs / kernel_s / 1e9, 1) if kernel_s > 0 else 0.0,
+            "frac": round(reqs / kernel_s / 1e9 / L2_PEAK_GREQS, 4) if kernel_s > 0 else 0.0,
+            "req_per_entry": l2req["read_req_per_entry"],
+        }
 
     updates = tc_info = ms_info = snap_info = meta_info = None
     if args.snapshot:

@@ -28,7 +28,8 @@ import pytest
 
 import oracle as O
 
-pytestmark = pytest.mark.gpu
+# the oracle side of C3 alone takes about two minutes on 16 host threads
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
 INF = 2**31 - 1
 
 

@@ -84,7 +84,7 @@ def test_tc_fixtures(hn, name):
 
 def test_tc_complete_graph_heavy_class(hn):
     # K_600: oriented out-degrees reach 599, so intersections use the
-    # eight-lane and whole-wave classes
+    # 16-lane and whole-wave classes
     print("case tc K600", flush=True)
     n = 600
     s, d = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
