@@ -131,6 +131,8 @@ SIGNATURES = {
     "hn_graph_erase": [P, P, P, I64, U32, PI64, P],
     "hn_graph_update_stats": [P, ctypes.POINTER(HnUpdateStats), P],
     "hn_bfs_topdown": [P, I32, P, ctypes.POINTER(HnBfsStats), P],
+    "hn_bfs_topdown_noalloc": [P, I32, P, ctypes.POINTER(HnBfsStats), P],
+    "hn_bfs_topdown_noalloc_limit": [PI64],
     "hn_bfs_multi": [P, P, I32, P, ctypes.POINTER(HnMsbfsStats), P],
     "hn_bfs_multi_depth": [P, P, P, I32, P, ctypes.POINTER(HnMsbfsStats), P],
     "hn_bfs_multi_params": [P, I32, I32],

@@ -404,3 +404,69 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
     return bfs_run(g, source, dist, stats, as_stream(stream), b, true);
 }
 
+
+// ---------------------------------------------------------------------------
+// BFS TopDown without allocation, for callers that cannot allocate (code under
+// HIP graph capture, tight serving loops).  Its traversal scratch is one
+// static device array per GPU, held until the process ends; its device address
+// is looked up once per device with hipGetSymbolAddress, and only that address
+// is handed to kernels (the array's name is a host-side stand-in).  Every call
+// on a GPU shares that GPU's array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace hn {
+namespace {
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+__device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch`, looked up on its first use.
+hn_status noalloc_scratch(char** out) {
+    int dev = -1;
+    HN_HIP(hipGetDevice(&dev));
+    HN_ARG(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device ordinal beyond the no-allocation table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("no-allocation scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return HN_E_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return HN_OK;
+}
+
+}  // namespace
+}  // namespace hn
+
+extern "C" hn_status hn_bfs_topdown_noalloc_limit(int64_t* scratch_bytes) {
+    HN_ARG(scratch_bytes, "null argument");
+    *scratch_bytes = (int64_t)NOALLOC_SCRATCH_BYTES;
+    return HN_OK;
+}
+
+extern "C" hn_status hn_bfs_topdown_noalloc(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats,
+                                            void* stream) {
+    HN_ARG(g && dist, "null argument");
+    HN_ARG(source >= 0 && source < g->nv, "source outside [0, nv)");
+    HN_ARG(g->ne < (int64_t)0xFFFFFFFF, "edge count exceeds the 32-bit work prefix");
+    BfsBufs b;
+    const int64_t need = bfs_layout(g->nv, g->ne, nullptr, &b);
+    if (need > (int64_t)NOALLOC_SCRATCH_BYTES) {
+        set_error("hn_bfs_topdown_noalloc: the graph needs " + std::to_string(need) +
+                  " bytes of traversal scratch, more than the static " +
+                  std::to_string(NOALLOC_SCRATCH_BYTES) + " (use hn_bfs_topdown)");
+        return HN_E_CAPACITY;
+    }
+    char* base = nullptr;
+    HN_TRY(noalloc_scratch(&base));
+    bfs_layout(g->nv, g->ne, base, &b);
+    return bfs_run(g, source, dist, stats, as_stream(stream), b, false);
+}

@@ -237,6 +237,18 @@ hn_status hn_graph_update_stats(hn_graph* g, hn_update_stats* out, void* stream)
  * the host with one 4-byte readback per level.  stats may be NULL. */
 hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats,
                          void* stream);
+/* hn_bfs_topdown without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * traversal scratch is one static device array per GPU of
+ * hn_bfs_topdown_noalloc_limit() bytes (64 MiB: R-MAT up to about 2.5 M
+ * vertices), held until the process ends; a graph needing more gets
+ * HN_E_CAPACITY.  All calls on one GPU share that array, so calls that overlap
+ * on one GPU (any streams) race.  Same results as hn_bfs_topdown; no kernel
+ * timing (stats->time_kernels is ignored). */
+hn_status hn_bfs_topdown_noalloc(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats,
+                                 void* stream);
+hn_status hn_bfs_topdown_noalloc_limit(int64_t* scratch_bytes);
 /* Batched BFS: k <= 64 sources (host array) in one traversal with 64-bit
  * per-vertex source masks; no reference counterpart (the reference runs one
  * BfsTopDown per source, TopDown.cu:94-106).  dist: device int32[k * nv],
