@@ -49,6 +49,10 @@ constexpr int EXP_G = 4;                       // adjacency slots per work unit
 constexpr int EXP_IPT = 4;                     // work units per lane per tile
 constexpr int EXP_TILE = BLOCK * EXP_IPT;      // work units per tile
 constexpr int32_t INF = 0x7fffffff;
+// The expansion keeps the first FULL_LDS_WORDS words of the full-word summary
+// in LDS (8 KB: vertices below 2^21).  A neighbour whose bitmap word is known
+// full was visited before the level, so its bitmap probe is skipped.
+constexpr int FULL_LDS_WORDS = 2048;
 static_assert(UNIT_SLOTS % EXP_G == 0, "groups must not straddle a block start");
 
 struct FrontierIn {
@@ -75,11 +79,13 @@ struct FrontierOut {
 };
 
 __global__ void bfs_init_kernel(int32_t nv, int32_t* __restrict__ dist, uint32_t* __restrict__ bitmap,
-                                uint8_t* __restrict__ state, int32_t source, int32_t* __restrict__ q) {
+                                uint32_t* __restrict__ full, uint8_t* __restrict__ state, int32_t source,
+                                int32_t* __restrict__ q) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t words = ((int64_t)nv + 31) / 32;
     if (i < nv) dist[i] = (i == source) ? 0 : INF;
     if (i < words) bitmap[i] = (i == (source >> 5)) ? (1u << (source & 31)) : 0u;
+    if (i < (words + 31) / 32) full[i] = 0u;
     if (i == 0) {
         q[0] = source;
         state[source] = 1;
@@ -101,6 +107,7 @@ __global__ void bfs_gather_kernel(int64_t F, const int32_t* __restrict__ q, cons
 struct ExpShared {
     LbShared<EXP_IPT, uint32_t> lb;   // pref = group prefix, base = adjacency offset
     int32_t deg[LbShared<EXP_IPT, uint32_t>::CAP];
+    uint32_t full[FULL_LDS_WORDS];     // head of the full-word summary
 };
 
 // Every neighbour of the frontier is tested against the visited bitmap and,
@@ -114,12 +121,15 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
                                                            const int64_t* __restrict__ qoff, int64_t F,
                                                            const int32_t* __restrict__ arena,
                                                            uint8_t* __restrict__ state,
-                                                           const uint32_t* __restrict__ bitmap) {
+                                                           const uint32_t* __restrict__ bitmap,
+                                                           const uint32_t* __restrict__ full, int64_t fwords) {
     typedef int32_t Vec __attribute__((ext_vector_type(G)));
     constexpr int CAP = LbShared<EXP_IPT, uint32_t>::CAP;
     __shared__ ExpShared sh;
     const int64_t W = (uint32_t)tot[0];
     const int64_t ntiles = (W + EXP_TILE - 1) / EXP_TILE;
+    if ((int64_t)blockIdx.x >= ntiles) return;
+    for (int i = threadIdx.x; i < FULL_LDS_WORDS; i += BLOCK) sh.full[i] = i < fwords ? full[i] : 0u;
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int64_t lo = tile_lo[t];
         const int64_t hi = (t + 1 < ntiles) ? (int64_t)tile_lo[t + 1] : F - 1;
@@ -169,17 +179,30 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
             v[j] = __builtin_nontemporal_load(reinterpret_cast<const Vec*>(arena + (nval[j] > 0 ? addr[j] : 0)));
         // phase 3a: all visited-bitmap probes in flight at once (n/8 bytes,
         // L2 resident), folded branch-free into one bit per candidate slot
+        // (a neighbour in a word the LDS summary knows full probes word 0 instead)
+        uint32_t known = 0;
+#pragma unroll
+        for (int j = 0; j < EXP_IPT; j++)
+#pragma unroll
+            for (int e = 0; e < G; e++) {
+                const uint32_t t = e < nval[j] ? (uint32_t)v[j][e] : 0u;
+                const uint32_t fw = t >> 10;
+                const uint32_t f = sh.full[fw < (uint32_t)FULL_LDS_WORDS ? fw : 0u];
+                known |= (uint32_t)((fw < (uint32_t)FULL_LDS_WORDS) & ((f >> ((t >> 5) & 31)) & 1u)) << (j * G + e);
+            }
         uint32_t wb[EXP_IPT][G];
 #pragma unroll
         for (int j = 0; j < EXP_IPT; j++)
 #pragma unroll
-            for (int e = 0; e < G; e++) wb[j][e] = bitmap[(e < nval[j] ? v[j][e] : 0) >> 5];
+            for (int e = 0; e < G; e++)
+                wb[j][e] = bitmap[((e < nval[j]) && !((known >> (j * G + e)) & 1u)) ? (v[j][e] >> 5) : 0];
         uint32_t cand = 0;
 #pragma unroll
         for (int j = 0; j < EXP_IPT; j++)
 #pragma unroll
             for (int e = 0; e < G; e++)
-                cand |= (uint32_t)((e < nval[j]) & (((wb[j][e] >> (v[j][e] & 31)) & 1u) == 0)) << (j * G + e);
+                cand |= (uint32_t)((e < nval[j]) & !((known >> (j * G + e)) & 1u) &
+                                   (((wb[j][e] >> (v[j][e] & 31)) & 1u) == 0)) << (j * G + e);
         // phase 3b: candidates (not visited before this level) probe their
         // state byte; the rest probe state[0], one shared line
         uint8_t s[EXP_IPT][G];
@@ -208,11 +231,13 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
 // get their distance, their bitmap bit and a slot in the next frontier.
 __global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, const uint32_t* __restrict__ state4,
                                                             uint32_t* __restrict__ bitmap,
+                                                            uint32_t* __restrict__ full,
                                                             int32_t* __restrict__ dist, int32_t level,
                                                             int32_t* __restrict__ qn,
                                                             uint32_t* __restrict__ qn_count) {
     const int64_t wi = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     uint32_t found = 0;
+    bool wfull = false;
     if (wi < words) {
         const uint4* p = reinterpret_cast<const uint4*>(state4 + wi * 8);
         const uint4 a = p[0], b = p[1];
@@ -232,6 +257,16 @@ __global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, const
             found = set & ~old;
             if (found) bitmap[wi] = old | found;
         }
+        // state bytes are never cleared during a traversal, so `set` is every
+        // vertex of the word visited so far: the word is full iff all are
+        wfull = set == 0xFFFFFFFFu;
+    }
+    {
+        // the full-word summary: a wave owns 64 consecutive words = 2 summary words
+        const uint64_t fb = __ballot(wfull);
+        const int64_t fw = (wi - __lane_id()) >> 5;
+        if (__lane_id() == 0 && fb && fw < (words + 31) / 32) full[fw] = (uint32_t)fb;
+        if (__lane_id() == 1 && (fb >> 32) && fw + 1 < (words + 31) / 32) full[fw + 1] = (uint32_t)(fb >> 32);
     }
     // next-frontier slots: one atomicAdd per workgroup (the counter is a
     // single device-scope address, so fewer of them is what matters)
@@ -282,7 +317,8 @@ int64_t bfs_layout(int64_t nv_in, int64_t ne, char* base, BfsBufs* b) {
     const int64_t tiles = (ne / EXP_G + nv) / EXP_TILE + 2;
     auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
     const int64_t o_bitmap = 0;
-    const int64_t o_q = al(o_bitmap + words * 4);
+    const int64_t o_full = al(o_bitmap + words * 4);
+    const int64_t o_q = al(o_full + (words + 31) / 32 * 4);
     const int64_t o_qdeg = al(o_q + nv * 4);
     const int64_t o_qoff = al(o_qdeg + nv * 4);
     const int64_t o_P = al(o_qoff + nv * 8);
@@ -294,6 +330,7 @@ int64_t bfs_layout(int64_t nv_in, int64_t ne, char* base, BfsBufs* b) {
     const int64_t total = al(o_scan + (int64_t)scan_temp_bytes(nv));
     if (base) {
         b->bitmap = reinterpret_cast<uint32_t*>(base + o_bitmap);
+        b->full = reinterpret_cast<uint32_t*>(base + o_full);
         b->q = reinterpret_cast<int32_t*>(base + o_q);
         b->qdeg = reinterpret_cast<int32_t*>(base + o_qdeg);
         b->qoff = reinterpret_cast<int64_t*>(base + o_qoff);
@@ -315,14 +352,14 @@ __global__ void zero_u32_kernel(int64_t n, uint32_t* __restrict__ p) {
 // The level loop over laid-out buffers.  Nothing here allocates (kernel
 // timing, which creates two events, only with `may_time`).
 hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats, hipStream_t st,
-                  const BfsBufs& b, bool may_time) {
+                  const BfsBufs& b, bool may_time, uint64_t* host) {
     const int64_t nv = g->nv;
     const int64_t words = (nv + 31) / 32;
     // state starts at zero (a fill kernel, not a memset node, so a captured
     // copy of this sequence replays correctly)
     zero_u32_kernel<<<(unsigned)ceil_div(words * 8, BLOCK), BLOCK, 0, st>>>(words * 8,
                                                                           reinterpret_cast<uint32_t*>(b.state));
-    bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(g->nv, dist, b.bitmap, b.state, source, b.q);
+    bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(g->nv, dist, b.bitmap, b.full, b.state, source, b.q);
     HN_CHECK_LAUNCH();
     const int64_t grid_cap = (int64_t)num_cus() * 64;
     const bool timed = may_time && stats && stats->time_kernels;
@@ -335,7 +372,6 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
     int32_t level = 1;
     int64_t visited = 0, scanned = 0, kns = 0;
     int32_t nlevels = 0;
-    uint64_t* host = g->h_pinned;
     hn_status rc = HN_OK;
     while (F > 0) {
         visited += F;
@@ -347,13 +383,14 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
         if (timed && e == hipSuccess) e = hipEventRecord(ev0, st);
         if (e == hipSuccess) {
             bfs_expand_kernel<EXP_G><<<(unsigned)grid_cap, BLOCK, 0, st>>>(b.tot, b.P, b.tile_lo, b.qdeg, b.qoff,
-                                                                          F, g->arena, b.state, b.bitmap);
+                                                                          F, g->arena, b.state, b.bitmap,
+                                                                          b.full, (words + 31) / 32);
             e = hipGetLastError();
         }
         if (timed && e == hipSuccess) e = hipEventRecord(ev1, st);
         if (e == hipSuccess) {
             bfs_collect_kernel<<<(unsigned)ceil_div(words, BLOCK), BLOCK, 0, st>>>(
-                words, reinterpret_cast<const uint32_t*>(b.state), b.bitmap, dist, level, b.q, b.cnt);
+                words, reinterpret_cast<const uint32_t*>(b.state), b.bitmap, b.full, dist, level, b.q, b.cnt);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipMemcpyAsync(host, b.tot, 8, hipMemcpyDeviceToHost, st);
@@ -401,7 +438,7 @@ extern "C" hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, 
     BfsBufs b;
     HN_TRY(g->s_bfs.reserve((size_t)bfs_layout(g->nv, g->ne, nullptr, &b)));
     bfs_layout(g->nv, g->ne, g->s_bfs.as<char>(), &b);
-    return bfs_run(g, source, dist, stats, as_stream(stream), b, true);
+    return bfs_run(g, source, dist, stats, as_stream(stream), b, true, g->h_pinned);
 }
 
 
@@ -468,5 +505,5 @@ extern "C" hn_status hn_bfs_topdown_noalloc(hn_graph* g, int32_t source, int32_t
     char* base = nullptr;
     HN_TRY(noalloc_scratch(&base));
     bfs_layout(g->nv, g->ne, base, &b);
-    return bfs_run(g, source, dist, stats, as_stream(stream), b, false);
+    return bfs_run(g, source, dist, stats, as_stream(stream), b, false, g->h_pinned);
 }

@@ -11,6 +11,7 @@ namespace hn {
 // scanned into P / tile_lo before collect overwrites q with level L+1.
 struct BfsBufs {
     uint32_t* bitmap;   // visited as of the start of the level
+    uint32_t* full;     // bit w: bitmap word w is all ones (every one of its 32 vertices visited)
     int32_t* q;
     int32_t* qdeg;
     int64_t* qoff;
@@ -25,10 +26,11 @@ struct BfsBufs {
 // Lays the buffers of a traversal of a graph with nv vertices and ne edges
 // out from `base` (nullptr: sizes only); returns the bytes they take.
 int64_t bfs_layout(int64_t nv, int64_t ne, char* base, BfsBufs* b);
-// BFS TopDown from `source` into dist[nv] over laid-out buffers.  Allocates
-// nothing, except the two timing events when `may_time` and
+// BFS TopDown from `source` into dist[nv] over laid-out buffers, reading the
+// per-level totals back through `host` (pinned, at least 16 bytes).
+// Allocates nothing, except the two timing events when `may_time` and
 // stats->time_kernels ask for kernel timing.
 hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats, hipStream_t st,
-                  const BfsBufs& b, bool may_time);
+                  const BfsBufs& b, bool may_time, uint64_t* host);
 
 }  // namespace hn
