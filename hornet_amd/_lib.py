@@ -88,7 +88,7 @@ class HnResult(ctypes.Structure):
 
 class HnServeConfig(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_int32), ("slots", ctypes.c_int32), ("mem_budget_bytes", ctypes.c_int64),
-                ("plain", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("plain", ctypes.c_int32), ("min_window_queries", ctypes.c_int32)]
 
 
 HN_REQ_BFS, HN_REQ_TWO_HOP, HN_REQ_INSERT, HN_REQ_ERASE = 0, 1, 2, 3

@@ -31,6 +31,7 @@
 #include <thread>
 #include <vector>
 
+#include "bfs.hpp"
 #include "graph.hpp"
 
 namespace hn {
@@ -82,9 +83,11 @@ struct ServeReq {
 
 struct ServeSlot {
     MsEngine* eng = nullptr;
-    int32_t* dist = nullptr;                 // max_batch x nv
+    int32_t* dist = nullptr;                 // nv: the single-source BFS's distances
+    char* bfs_scratch = nullptr;             // the single-source BFS's level buffers
+    int64_t bfs_scratch_bytes = 0;
     unsigned long long* digest = nullptr;    // 64 x 3
-    unsigned long long* h_digest = nullptr;  // pinned
+    unsigned long long* h_digest = nullptr;  // pinned; its first 16 bytes also serve the BFS level readback
     hipStream_t stream = nullptr;
     std::thread th;
     std::vector<ServeReq*> batch;
@@ -153,8 +156,16 @@ static void worker_loop(hn_server* s, ServeSlot* slot) {
             limits[i] = batch[i]->kind == HN_REQ_TWO_HOP ? 2 : 0;
         }
         hn_status st = HN_OK;
-        if (s->cfg.plain && k == 1 && batch[0]->kind == HN_REQ_BFS) {
-            st = hn_bfs_topdown(g, srcs[0], slot->dist, nullptr, slot->stream);
+        // a lone BFS query runs the single-source TopDown (about half the
+        // time of a one-source batched launch) over the slot's own buffers;
+        // a window's queries need the batched engine's version masks
+        BfsBufs bb;
+        const bool single = k == 1 && batch[0]->kind == HN_REQ_BFS && !window &&
+                            bfs_layout(g->nv, g->ne, nullptr, &bb) <= slot->bfs_scratch_bytes;
+        if (single) {
+            bfs_layout(g->nv, g->ne, slot->bfs_scratch, &bb);
+            st = bfs_run(g, srcs[0], slot->dist, nullptr, slot->stream, bb, false,
+                         reinterpret_cast<uint64_t*>(slot->h_digest));
         } else {
             // digest mode: the launch never materialises k x nv distances
             st = ms_run_engine(slot->eng, srcs, k, limits, nullptr, nullptr, slot->stream);
@@ -165,7 +176,7 @@ static void worker_loop(hn_server* s, ServeSlot* slot) {
             if (e == hipSuccess && st == HN_OK) e = hipStreamSynchronize(slot->stream);
             if (e != hipSuccess) st = HN_E_HIP;
         }
-        if (st == HN_OK && s->cfg.plain && k == 1 && batch[0]->kind == HN_REQ_BFS) {
+        if (st == HN_OK && single) {
             hipError_t e = hipMemsetAsync(slot->digest, 0, sizeof(unsigned long long) * 3 * 64, slot->stream);
             if (e == hipSuccess && nv > 0) {
                 dim3 grid((unsigned)std::min<int64_t>(ceil_div(nv, BLOCK), 1024), (unsigned)k);
@@ -367,6 +378,10 @@ static void dispatcher_loop(hn_server* s) {
     const int maxb = s->cfg.plain ? 1 : s->cfg.max_batch;
     const int sbits = std::max(1, ceil_log2_host((int64_t)s->g->nv + 1));
     const bool windows = !s->cfg.plain && 2 * sbits + 7 <= 64;
+    // a window pays for its set-up (sorting its update entries, per-pair
+    // existence masks, restricted expansion) only when it batches enough
+    // queries; with fewer waiting, the updates apply one by one
+    const int min_wq = std::max(1, s->cfg.min_window_queries > 0 ? s->cfg.min_window_queries : 8);
     int barrier_updates = 0;   // updates to apply one by one (a window was not symmetric)
     while (true) {
         s->cv_disp.wait(lk, [&] { return s->stop || !s->queue.empty(); });
@@ -374,7 +389,7 @@ static void dispatcher_loop(hn_server* s) {
         // queries across updates, in one launch (symmetric graphs)
         if (windows && barrier_updates == 0 && s->g->sym == 1) {
             Window w = scan_window(s, maxb);
-            if (w.nq > 0 && w.nu > 0) {
+            if (w.nq >= min_wq && w.nu > 0) {
                 s->cv_disp.wait(lk, [&] { return s->stop || s->inflight == 0; });
                 if (s->stop) break;
                 w = scan_window(s, maxb);   // the queue may have grown
@@ -425,12 +440,22 @@ static void dispatcher_loop(hn_server* s) {
     }
 }
 
+// the single-source BFS scratch a slot reserves: sized for twice the edges
+// the graph has now (or 2^26 more), so that inserts rarely outgrow it; a
+// slot whose scratch the graph has outgrown runs lone BFS queries through the
+// batched engine instead
+static int64_t slot_bfs_scratch(const hn_graph* g) {
+    BfsBufs bb;
+    return bfs_layout(g->nv, std::max<int64_t>(2 * g->ne, g->ne + ((int64_t)1 << 26)), nullptr, &bb);
+}
+
 // device bytes one launch slot holds: an engine and the digests (batched
-// launches run in digest mode and never hold k x nv distances); the plain
-// form adds one distance row for its single-source BFS
+// launches run in digest mode and never hold k x nv distances), plus one
+// distance row and the level buffers of the single-source BFS
 static int64_t slot_bytes(const hn_graph* g, bool plain) {
+    (void)plain;
     const int64_t nv = g->nv > 0 ? g->nv : 1;
-    return ms_engine_bytes(nv, g->ne) + (plain ? nv * 4 : 0) + 64 * 24;
+    return ms_engine_bytes(nv, g->ne) + nv * 4 + slot_bfs_scratch(g) + 64 * 24;
 }
 
 static void destroy_server(hn_server* s) {
@@ -445,6 +470,7 @@ static void destroy_server(hn_server* s) {
         if (sl->th.joinable()) sl->th.join();
         ms_engine_destroy(sl->eng);
         dfree(sl->dist);
+        dfree(sl->bfs_scratch);
         dfree(sl->digest);
         if (sl->h_digest) (void)hipHostFree(sl->h_digest);
         if (sl->stream) (void)hipStreamDestroy(sl->stream);
@@ -512,7 +538,11 @@ hn_status hn_server_create(hn_graph* g, const hn_serve_config* cfg, hn_server** 
     for (int i = 0; i < nslots; i++) {
         auto sl = std::make_unique<ServeSlot>();
         hn_status st = ms_engine_create(g, &sl->eng);
-        if (st == HN_OK) st = dalloc(&sl->dist, (size_t)(cfg->plain ? nv : 1));
+        if (st == HN_OK) st = dalloc(&sl->dist, (size_t)nv);
+        if (st == HN_OK) {
+            sl->bfs_scratch_bytes = slot_bfs_scratch(g);
+            st = dalloc(&sl->bfs_scratch, (size_t)sl->bfs_scratch_bytes);
+        }
         if (st == HN_OK) st = dalloc(&sl->digest, 64 * 3);
         if (st == HN_OK && (hipHostMalloc(reinterpret_cast<void**>(&sl->h_digest), 64 * 3 * 8, 0) != hipSuccess ||
                             hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess)) {

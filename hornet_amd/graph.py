@@ -643,9 +643,9 @@ class Server:
                                    _lib.HN_REQ_ERASE)
 
     def __init__(self, hornet: Hornet, max_batch: int = 64, slots: int = 2, mem_budget_bytes: int = 0,
-                 plain: bool = False):
+                 plain: bool = False, min_window_queries: int = 0):
         self.hornet = hornet
-        cfg = _lib.HnServeConfig(max_batch, slots, mem_budget_bytes, 1 if plain else 0, 0)
+        cfg = _lib.HnServeConfig(max_batch, slots, mem_budget_bytes, 1 if plain else 0, int(min_window_queries))
         self._s = ctypes.c_void_p()
         call("hn_server_create", hornet.handle, ctypes.byref(cfg), ctypes.byref(self._s))
 

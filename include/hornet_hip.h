@@ -335,7 +335,9 @@ typedef struct hn_serve_config {
     int32_t slots;            /* concurrent launches wanted                    */
     int64_t mem_budget_bytes; /* device memory for launches (0: 1/4 of free)   */
     int32_t plain;            /* 1: one query per launch, single-source BFS    */
-    int32_t pad_;
+    int32_t min_window_queries; /* queries a multi-version window must batch
+                                   across updates; fewer -> the updates apply
+                                   one by one (0: 8; > 64: never a window)   */
 } hn_serve_config;
 
 hn_status hn_server_create(hn_graph* g, const hn_serve_config* cfg, hn_server** out);
