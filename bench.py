@@ -33,13 +33,20 @@ L2_PEAK_GREQS = 8 * 16 * 2.4
 
 
 def cpu_model() -> str:
-    """The full CPU model string (lscpu "Model name", else /proc/cpuinfo)."""
+    """The CPU as lscpu reports it: "Model name", with the family / model /
+    stepping numbers when the model name alone is generic (virtual machines
+    often report only the vendor's family name); else /proc/cpuinfo."""
     import subprocess
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        f = {}
         for line in out.splitlines():
-            if line.strip().startswith("Model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                f.setdefault(k.strip(), v.strip())
+        if "Model name" in f:
+            ids = ", ".join(f"{k.lower()} {f[k]}" for k in ("CPU family", "Model", "Stepping") if k in f)
+            return f["Model name"] + (f" ({ids})" if ids else "")
     except (OSError, subprocess.SubprocessError):
         pass
     try:
@@ -152,6 +159,7 @@ def update_rounds(h, args, torch):
         tc_t += time.perf_counter() - t
         tris.append(tc.countTriangles())
         print(f"# round {r}: ne={h.nE()} triangles={tris[-1]}", file=sys.stderr, flush=True)
+    utr = load_traffic("update_traffic.json")
     updates = {
         "rounds": args.rounds, "pairs_per_round": args.batch,
         "edge_updates_per_s": round((ins_n + era_n) / (ins_t + era_t), 1),
@@ -172,7 +180,11 @@ def update_rounds(h, args, torch):
             "alg_bytes_per_insert": round(ins_bytes / max(args.rounds, 1)),
             "alg_bytes_per_erase": round(era_bytes / max(args.rounds, 1)),
             "entries_moved_per_batch": round(moved / max(2 * args.rounds, 1)),
-            "traffic": (load_traffic("update_traffic.json") or {}).get("bytes_per_batch"),
+            # HBM bytes per batch from the committed FETCH_SIZE / WRITE_SIZE
+            # passes on the same workload (round 1 of these update rounds)
+            "traffic": (utr or {}).get("bytes_per_batch"),
+            "insert_traffic": (utr or {}).get("insert_bytes_per_batch"),
+            "erase_traffic": (utr or {}).get("erase_bytes_per_batch"),
         },
     }
     tc_info = {"triangles_per_round": tris, "ms_avg": round(tc_t / args.rounds * 1e3, 3),
