@@ -8,9 +8,10 @@
 // Its operator is a non-atomic check-then-set (TopDown.cu:54-60), so a vertex
 // may be enqueued more than once; the distances are still exact.
 //
-// Here, per level:
-//   1. gather: (degree, adjacency offset) of every frontier vertex, one
-//      16-byte record read each;
+// Here, per level (no host round trip: counts live in a device control
+// block, and the host polls its done flag every few levels):
+//   1. the frontier arrives as (id, degree, adjacency offset) triples, written
+//      by the previous level's collect (one 16-byte record read each);
 //   2. one scan whose values pack (degree << 32 | groups), a group being four
 //      adjacency slots (16 bytes).  Its down-sweep writes the group prefix P
 //      and, per tile, the frontier index holding the tile's first group -- the
@@ -37,8 +38,13 @@
 //      bitmap bit is clear is a newly reached vertex: it gets its distance, its
 //      bitmap bit and one next-frontier slot (one atomicAdd per workgroup),
 //      so every vertex enters a frontier once;
-//   5. one 12-byte readback (level totals + next frontier size).
+//   5. a one-thread step that adds the level to the statistics and makes the
+//      next frontier current (done when it is empty).
 // Distances: source 0, its neighbours 1, unreached INT32_MAX (TopDown.cu:42,81).
+#include <algorithm>
+#include <cstddef>
+#include <cstring>
+
 #include "bfs.hpp"
 #include "prims.hpp"
 
@@ -78,30 +84,42 @@ struct FrontierOut {
     }
 };
 
-__global__ void bfs_init_kernel(int32_t nv, int32_t* __restrict__ dist, uint32_t* __restrict__ bitmap,
-                                uint32_t* __restrict__ full, uint8_t* __restrict__ state, int32_t source,
-                                int32_t* __restrict__ q) {
+__global__ void bfs_init_kernel(int32_t nv, const hn_vrec* __restrict__ rec, int32_t* __restrict__ dist,
+                                uint32_t* __restrict__ bitmap, uint32_t* __restrict__ full,
+                                uint8_t* __restrict__ state, int32_t source, int32_t* __restrict__ q,
+                                int32_t* __restrict__ qdeg, int64_t* __restrict__ qoff, BfsCtl* __restrict__ ctl) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t words = ((int64_t)nv + 31) / 32;
     if (i < nv) dist[i] = (i == source) ? 0 : INF;
     if (i < words) bitmap[i] = (i == (source >> 5)) ? (1u << (source & 31)) : 0u;
     if (i < (words + 31) / 32) full[i] = 0u;
     if (i == 0) {
+        const hn_vrec r = rec[source];
         q[0] = source;
+        qdeg[0] = r.deg;
+        qoff[0] = r.off;
         state[source] = 1;
+        BfsCtl c{};
+        c.F = 1;
+        c.level = 1;
+        *ctl = c;
     }
 }
 
-// (degree, adjacency offset) of every frontier vertex, one record gather each
-// (the next-frontier counter is cleared here, before this level's collect)
-__global__ void bfs_gather_kernel(int64_t F, const int32_t* __restrict__ q, const hn_vrec* __restrict__ rec,
-                                  int32_t* __restrict__ qdeg, int64_t* __restrict__ qoff, uint32_t* __restrict__ cnt) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *cnt = 0u;
-    if (i >= F) return;
-    const hn_vrec r = rec[q[i]];
-    qdeg[i] = r.deg;
-    qoff[i] = r.off;
+// End of a level: its frontier joins the statistics and the next frontier
+// becomes current; an empty one ends the traversal.  Levels launched after
+// the end find F == 0 and change nothing.
+__global__ void bfs_advance_kernel(BfsCtl* __restrict__ ctl, const uint64_t* __restrict__ tot) {
+    BfsCtl c = *ctl;
+    if (c.F == 0) return;
+    c.visited += c.F;
+    c.scanned += tot[0] >> 32;
+    c.levels += 1;
+    c.F = c.nextF;
+    c.nextF = 0;
+    c.level += 1;
+    c.done = c.F == 0 ? 1u : 0u;
+    *ctl = c;
 }
 
 struct ExpShared {
@@ -118,7 +136,8 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
                                                            const uint32_t* __restrict__ P,
                                                            const uint32_t* __restrict__ tile_lo,
                                                            const int32_t* __restrict__ qdeg,
-                                                           const int64_t* __restrict__ qoff, int64_t F,
+                                                           const int64_t* __restrict__ qoff,
+                                                           const BfsCtl* __restrict__ ctl,
                                                            const int32_t* __restrict__ arena,
                                                            uint8_t* __restrict__ state,
                                                            const uint32_t* __restrict__ bitmap,
@@ -126,7 +145,8 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
     typedef int32_t Vec __attribute__((ext_vector_type(G)));
     constexpr int CAP = LbShared<EXP_IPT, uint32_t>::CAP;
     __shared__ ExpShared sh;
-    const int64_t W = (uint32_t)tot[0];
+    const int64_t F = ctl->F;
+    const int64_t W = F > 0 ? (int64_t)(uint32_t)tot[0] : 0;
     const int64_t ntiles = (W + EXP_TILE - 1) / EXP_TILE;
     if ((int64_t)blockIdx.x >= ntiles) return;
     for (int i = threadIdx.x; i < FULL_LDS_WORDS; i += BLOCK) sh.full[i] = i < fwords ? full[i] : 0u;
@@ -232,9 +252,13 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
 __global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, const uint32_t* __restrict__ state4,
                                                             uint32_t* __restrict__ bitmap,
                                                             uint32_t* __restrict__ full,
-                                                            int32_t* __restrict__ dist, int32_t level,
-                                                            int32_t* __restrict__ qn,
-                                                            uint32_t* __restrict__ qn_count) {
+                                                            int32_t* __restrict__ dist,
+                                                            const hn_vrec* __restrict__ rec,
+                                                            int32_t* __restrict__ qn, int32_t* __restrict__ qdeg,
+                                                            int64_t* __restrict__ qoff, BfsCtl* __restrict__ ctl) {
+    if (ctl->F == 0) return;   // a level launched after the traversal ended
+    const int32_t level = (int32_t)ctl->level;
+    uint32_t* qn_count = &ctl->nextF;
     const int64_t wi = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     uint32_t found = 0;
     bool wfull = false;
@@ -301,8 +325,12 @@ __global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, const
         const uint32_t bo = __shfl(b, owner, WAVE);
         if ((fw >> bit) & 1u) {
             const int32_t v = (int32_t)((wbase + owner) * 32 + bit);
+            const uint32_t slot = bo + (uint32_t)__popc(fw & below);
+            const hn_vrec r = rec[v];
             dist[v] = level;
-            qn[bo + (uint32_t)__popc(fw & below)] = v;
+            qn[slot] = v;
+            qdeg[slot] = r.deg;
+            qoff[slot] = r.off;
         }
     }
 }
@@ -325,8 +353,8 @@ int64_t bfs_layout(int64_t nv_in, int64_t ne, char* base, BfsBufs* b) {
     const int64_t o_tile = al(o_P + (nv + 1) * 4);
     const int64_t o_state = al(o_tile + tiles * 4);
     const int64_t o_tot = al(o_state + words * 32);
-    const int64_t o_cnt = al(o_tot + 8);
-    const int64_t o_scan = al(o_cnt + 64);
+    const int64_t o_ctl = al(o_tot + 8);
+    const int64_t o_scan = al(o_ctl + (int64_t)sizeof(BfsCtl));
     const int64_t total = al(o_scan + (int64_t)scan_temp_bytes(nv));
     if (base) {
         b->bitmap = reinterpret_cast<uint32_t*>(base + o_bitmap);
@@ -338,7 +366,7 @@ int64_t bfs_layout(int64_t nv_in, int64_t ne, char* base, BfsBufs* b) {
         b->tile_lo = reinterpret_cast<uint32_t*>(base + o_tile);
         b->state = reinterpret_cast<uint8_t*>(base + o_state);
         b->tot = reinterpret_cast<uint64_t*>(base + o_tot);
-        b->cnt = reinterpret_cast<uint32_t*>(base + o_cnt);
+        b->ctl = reinterpret_cast<BfsCtl*>(base + o_ctl);
         b->scan_tmp = base + o_scan;
     }
     return total;
@@ -349,8 +377,11 @@ __global__ void zero_u32_kernel(int64_t n, uint32_t* __restrict__ p) {
     if (i < n) p[i] = 0u;
 }
 
-// The level loop over laid-out buffers.  Nothing here allocates (kernel
-// timing, which creates two events, only with `may_time`).
+// The level loop over laid-out buffers.  Levels are queued without a host
+// round trip; the host reads the done flag after as many levels as the last
+// traversal of this graph took, then after 1, 2, 4, 4, ... more (levels
+// queued past the end change nothing).  Nothing here allocates (kernel timing,
+// which creates its events, only with `may_time`).
 hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats, hipStream_t st,
                   const BfsBufs& b, bool may_time, uint64_t* host) {
     const int64_t nv = g->nv;
@@ -359,68 +390,92 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
     // copy of this sequence replays correctly)
     zero_u32_kernel<<<(unsigned)ceil_div(words * 8, BLOCK), BLOCK, 0, st>>>(words * 8,
                                                                           reinterpret_cast<uint32_t*>(b.state));
-    bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(g->nv, dist, b.bitmap, b.full, b.state, source, b.q);
+    bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(g->nv, g->rec, dist, b.bitmap, b.full, b.state,
+                                                                      source, b.q, b.qdeg, b.qoff, b.ctl);
     HN_CHECK_LAUNCH();
     const int64_t grid_cap = (int64_t)num_cus() * 64;
     const bool timed = may_time && stats && stats->time_kernels;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (timed) {
-        HN_HIP(hipEventCreate(&ev0));
-        HN_HIP(hipEventCreate(&ev1));
-    }
-    int64_t F = 1;
-    int32_t level = 1;
-    int64_t visited = 0, scanned = 0, kns = 0;
-    int32_t nlevels = 0;
+    // kernel timing: one event pair around each queued expansion of a chunk
+    constexpr int MAX_CHUNK = hn_graph::BFS_EVENTS / 2;
+    hipEvent_t* ev = g->bfs_ev;
     hn_status rc = HN_OK;
-    while (F > 0) {
-        visited += F;
-        nlevels++;
-        bfs_gather_kernel<<<(unsigned)ceil_div(F, BLOCK), BLOCK, 0, st>>>(F, b.q, g->rec, b.qdeg, b.qoff, b.cnt);
-        rc = exclusive_scan_fn<uint64_t>(FrontierIn{b.qdeg}, FrontierOut{b.P, b.tile_lo}, b.tot, F, b.scan_tmp, st);
-        if (rc != HN_OK) break;
-        hipError_t e = hipGetLastError();
-        if (timed && e == hipSuccess) e = hipEventRecord(ev0, st);
-        if (e == hipSuccess) {
-            bfs_expand_kernel<EXP_G><<<(unsigned)grid_cap, BLOCK, 0, st>>>(b.tot, b.P, b.tile_lo, b.qdeg, b.qoff,
-                                                                          F, g->arena, b.state, b.bitmap,
-                                                                          b.full, (words + 31) / 32);
-            e = hipGetLastError();
+    if (timed)
+        for (int i = 0; i < 2 * MAX_CHUNK && rc == HN_OK; i++)
+            if (!ev[i] && hipEventCreate(&ev[i]) != hipSuccess) {
+                (void)hipGetLastError();
+                ev[i] = nullptr;
+                set_error("bfs: event creation failed");
+                rc = HN_E_HIP;
+            }
+    const int last = g->bfs_last_levels.load(std::memory_order_relaxed);
+    int64_t next = std::min<int64_t>(last > 0 ? last : 4, MAX_CHUNK);
+    int64_t ramp = 1, queued = 0, kns = 0;
+    uint32_t levels_seen = 0;
+    static_assert(offsetof(BfsCtl, done) == offsetof(BfsCtl, levels) + 4, "levels, done adjacent");
+    while (rc == HN_OK) {
+        hipError_t e = hipSuccess;
+        for (int64_t c = 0; c < next && e == hipSuccess; c++) {
+            e = exclusive_scan_dev_count<uint64_t>(FrontierIn{b.qdeg}, FrontierOut{b.P, b.tile_lo}, b.tot,
+                                                    &b.ctl->F, nv, b.scan_tmp, st);
+            if (timed && e == hipSuccess) e = hipEventRecord(ev[2 * c], st);
+            if (e == hipSuccess) {
+                bfs_expand_kernel<EXP_G><<<(unsigned)grid_cap, BLOCK, 0, st>>>(
+                    b.tot, b.P, b.tile_lo, b.qdeg, b.qoff, b.ctl, g->arena, b.state, b.bitmap, b.full,
+                    (words + 31) / 32);
+                e = hipGetLastError();
+            }
+            if (timed && e == hipSuccess) e = hipEventRecord(ev[2 * c + 1], st);
+            if (e == hipSuccess) {
+                bfs_collect_kernel<<<(unsigned)ceil_div(words, BLOCK), BLOCK, 0, st>>>(
+                    words, reinterpret_cast<const uint32_t*>(b.state), b.bitmap, b.full, dist, g->rec, b.q, b.qdeg,
+                    b.qoff, b.ctl);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) {
+                bfs_advance_kernel<<<1, 1, 0, st>>>(b.ctl, b.tot);
+                e = hipGetLastError();
+            }
         }
-        if (timed && e == hipSuccess) e = hipEventRecord(ev1, st);
-        if (e == hipSuccess) {
-            bfs_collect_kernel<<<(unsigned)ceil_div(words, BLOCK), BLOCK, 0, st>>>(
-                words, reinterpret_cast<const uint32_t*>(b.state), b.bitmap, b.full, dist, level, b.q, b.cnt);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipMemcpyAsync(host, b.tot, 8, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(host + 1, b.cnt, 4, hipMemcpyDeviceToHost, st);
+        queued += next;
+        // [0] = levels | done << 32
+        if (e == hipSuccess) e = hipMemcpyAsync(host, &b.ctl->levels, 8, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (timed && e == hipSuccess) {
-            float ms = 0.f;
-            e = hipEventElapsedTime(&ms, ev0, ev1);
-            kns += (int64_t)((double)ms * 1e6);
-        }
+        const uint32_t levels_now = (uint32_t)host[0];
+        const bool done = (host[0] >> 32) != 0;
+        if (timed && e == hipSuccess)
+            // only the expansions of non-empty levels count
+            for (uint32_t c = 0; c < levels_now - levels_seen && e == hipSuccess; c++) {
+                float ms = 0.f;
+                e = hipEventElapsedTime(&ms, ev[2 * c], ev[2 * c + 1]);
+                kns += (int64_t)((double)ms * 1e6);
+            }
         if (e != hipSuccess) {
             set_error(std::string("bfs level: ") + hipGetErrorString(e));
             rc = HN_E_HIP;
             break;
         }
-        scanned += (int64_t)(host[0] >> 32);
-        F = (int64_t)(uint32_t)host[1];
-        level++;
-    }
-    if (timed) {
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
+        levels_seen = levels_now;
+        if (done) break;
+        if (queued > nv + 1) {   // cannot happen: every level reaches a new vertex
+            set_error("bfs: level loop did not end");
+            rc = HN_E_HIP;
+            break;
+        }
+        next = ramp;
+        ramp = std::min<int64_t>(2 * ramp, 4);
     }
     if (rc != HN_OK) return rc;
+    g->bfs_last_levels.store((int)levels_seen, std::memory_order_relaxed);
     if (stats) {
-        stats->levels = nlevels;
-        stats->visited = visited;
-        stats->edges_scanned = scanned;
+        BfsCtl c;
+        HN_HIP(hipMemcpyAsync(host, b.ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, st));
+        HN_HIP(hipStreamSynchronize(st));
+        std::memcpy(&c, host, sizeof(BfsCtl));
+        stats->levels = (int32_t)c.levels;
+        stats->visited = (int64_t)c.visited;
+        stats->edges_scanned = (int64_t)c.scanned;
         stats->expand_ns = kns;
-        stats->expand_launches = nlevels;
+        stats->expand_launches = (int32_t)c.levels;
         stats->pad_ = 0;
     }
     return HN_OK;

@@ -5,10 +5,24 @@
 
 namespace hn {
 
+// Level-loop control on the device: the loop runs without a host round trip
+// per level, and the host polls `done` every few levels.
+struct BfsCtl {
+    uint32_t F;        // frontier size of the level being expanded
+    uint32_t nextF;    // next frontier size (collect's slot counter)
+    uint32_t level;    // distance given to this level's discoveries
+    uint32_t levels;   // non-empty frontiers so far
+    uint32_t done;     // the frontier is empty
+    uint32_t pad_;
+    uint64_t visited;  // sum of the frontier sizes
+    uint64_t scanned;  // sum of the frontiers' degrees (m_C)
+};
+
 // Scratch of one traversal, carved from one device buffer: the handle's
 // s_bfs, or the static array of the no-allocation entry point.
-// The frontier needs no ping-pong: q is gathered into (qdeg, qoff) and
-// scanned into P / tile_lo before collect overwrites q with level L+1.
+// The frontier needs no ping-pong: collect writes each discovered vertex's
+// id, degree and adjacency offset into (q, qdeg, qoff), which the next
+// level's scan and expansion read before that level's collect overwrites them.
 struct BfsBufs {
     uint32_t* bitmap;   // visited as of the start of the level
     uint32_t* full;     // bit w: bitmap word w is all ones (every one of its 32 vertices visited)
@@ -19,7 +33,7 @@ struct BfsBufs {
     uint32_t* tile_lo;
     uint8_t* state;     // one byte per vertex, padded to whole bitmap words
     uint64_t* tot;      // [0] level totals (edges << 32 | groups)
-    uint32_t* cnt;      // [0] next frontier size
+    BfsCtl* ctl;
     void* scan_tmp;     // scan partials
 };
 

@@ -363,6 +363,8 @@ static void destroy_graph(hn_graph* g) {
     g->s_stamp.release();
     g->s_misc.release();
     if (g->h_pinned) (void)hipHostFree(g->h_pinned);
+    for (hipEvent_t& e : g->bfs_ev)
+        if (e) (void)hipEventDestroy(e);
     delete g;
 }
 

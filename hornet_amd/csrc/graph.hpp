@@ -19,6 +19,7 @@
 // commits the stack depths, and frees are pushed after the data has moved.
 // No host round trip per vertex.
 #pragma once
+#include <atomic>
 #include <mutex>
 
 #include "common.hpp"
@@ -181,6 +182,13 @@ struct hn_graph {
     hn::Scratch s_mstage;             // staging of edge meta-data during updates
     hn::Scratch s_stamp;              // per vertex: its source segment in the current batch (unchecked)
     uint64_t* h_pinned = nullptr;     // small pinned readback area
+    // levels + 1 of the last single-source BFS: when its level loop first
+    // polls for the end (serving slots run BFS concurrently, hence atomic)
+    std::atomic<int> bfs_last_levels{0};
+    // timing events of the BFS expansion (one pair per queued level), created
+    // on the first timed traversal
+    static constexpr int BFS_EVENTS = 64;
+    hipEvent_t bfs_ev[BFS_EVENTS] = {};
     // the last applied batch, for hn_graph_update_stats (pointers into s_upd,
     // valid until the next batch)
     int32_t last_kind = 0;            // 0 none, 1 insert, 2 erase

@@ -864,6 +864,7 @@ struct MsEngine {
     hipStream_t own = nullptr;        // non-blocking stream the level graph is captured on
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     int32_t* pinned = nullptr;        // engine-private readback buffer
+    int32_t last_steps = 0;           // level steps the last unlimited launch ran until done
     // multi-version window state (served stream)
     uint8_t* dirty = nullptr;         // nv flags (vertex has changed pairs), in `mem`
     uint32_t* d_nside = nullptr;      // side record count, in `mem`
@@ -1173,14 +1174,34 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
         e->exec_arena = g->arena;
         e->exec_params = P;
     }
-    // replay; the done flag is read every `chunk` levels
-    const int chunk = 4;
-    int32_t* host = e->pinned;
-    for (int64_t rep = 0; rep <= nv + 1; rep += chunk) {
-        for (int c = 0; c < chunk; c++) HN_HIP(hipGraphLaunch(e->exec, st));
-        HN_HIP(hipMemcpyAsync(host, &e->S->done, 4, hipMemcpyDeviceToHost, st));
+    // Replay.  The first poll of the done flag comes after as many steps as
+    // the last launch on this graph needed (a graph's depth barely changes
+    // between launches; depth-limited launches need at most limit + 2), the
+    // next ones after 1, 2, 4, 4, ... more steps: with a fixed poll every 4
+    // steps a launch ran up to 3 steps past its end, each one 11 kernels that
+    // return at once but still cost their launch gaps.
+    int32_t max_limit = 0;
+    bool limited = k > 0;
+    for (int b = 0; b < k; b++) {
+        limited = limited && P.depth_limit[b] > 0;
+        max_limit = std::max(max_limit, P.depth_limit[b]);
+    }
+    int64_t next = e->last_steps > 0 ? e->last_steps : 4;
+    if (limited) next = std::min<int64_t>(next, max_limit + 2);
+    int64_t ramp = 1;
+    int32_t* host = e->pinned;   // [0] done, [1] level steps that did work
+    static_assert(offsetof(MsState, levels_run) == offsetof(MsState, done) + 4, "done, levels_run adjacent");
+    for (int64_t rep = 0; rep <= nv + 1;) {
+        for (int64_t c = 0; c < next; c++) HN_HIP(hipGraphLaunch(e->exec, st));
+        rep += next;
+        HN_HIP(hipMemcpyAsync(host, &e->S->done, 8, hipMemcpyDeviceToHost, st));
         HN_HIP(hipStreamSynchronize(st));
-        if (host[0]) break;
+        if (host[0]) {
+            if (!limited) e->last_steps = host[1] + 1;
+            break;
+        }
+        next = ramp;
+        ramp = std::min<int64_t>(2 * ramp, 4);
     }
     if (!dist) {
         ms_digest_kernel<<<(unsigned)k, BLOCK, 0, st>>>(e->dg_part, e->nparts, e->digest);
