@@ -562,6 +562,34 @@ def cpu_baselines_and_checks(h, sources, updates, tc_info, args, torch):
           "bfs_ms": round(tb * 1e3, 3), "bfs_gteps": round(m1 / tb / 1e9, 4), "triangles": int(tri1),
           "tc_ms": round(tt * 1e3, 3), "triangles_per_s": round(tri1 / tt, 1), "cores": 1, "kind": "port",
           "timing": "median of 3, one pinned core", "checked_against_device": True}
+    # the s22 configuration (16 seeded sources + a triangle count) on all host
+    # cores with the oracle's parallel checker, labelled as such
+    T = O.checker_threads()
+    o2, a2 = O.rmat_graph(22, seed=2, threads=T)
+    srcs2 = O.pick_sources(o2, 16, seed=3)
+    tb2, exp2 = median_time(lambda: O.bfs_many(o2, a2, srcs2, threads=T))
+    tt2, tri2 = median_time(lambda: O.triangles(o2, a2, threads=T))
+    g2 = Hornet.rmat(22, seed=2)
+    b2 = BfsTopDown(g2)
+    for s, (d, _) in zip(srcs2, exp2):
+        b2.set_parameters(int(s))
+        b2.run()
+        if not np.array_equal(b2.distances().cpu().numpy(), d):
+            raise CheckFailed(f"R-MAT s22 BFS from {s} differs from the oracle")
+    t2 = TriangleCounting2(g2)
+    t2.init()
+    t2.run_undirected()
+    if t2.countTriangles() != tri2:
+        raise CheckFailed("R-MAT s22 triangle count differs from the oracle")
+    del b2, t2
+    g2.release()
+    m2 = sum(m for _, m in exp2)
+    c2 = {"graph": "R-MAT s22 ef16 seed 2", "nv": int(o2.size - 1), "ne": int(a2.size), "sources": 16,
+          "bfs_s": round(tb2, 3), "bfs_gteps": round(m2 / tb2 / 1e9, 4), "triangles": int(tri2),
+          "tc_s": round(tt2, 3), "triangles_per_s": round(tri2 / tt2, 1), "cores": T, "kind": "port",
+          "timing": f"median of 3, the oracle's parallel checker on {T} host threads (an all-core variant of the "
+                    f"sequential host path, not the one-core baseline)", "checked_against_device": True}
+    c1["all_cores_s22"] = c2
     if tc_info is not None:
         tc_info["cpu_baseline"] = {"value": c1["triangles_per_s"], "unit": "triangles/s", "cores": 1, "kind": "port",
                                    "sample": "host triangle count (oracle restatement of "

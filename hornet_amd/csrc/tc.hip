@@ -17,13 +17,14 @@
 //     atomics): keep (u,v) iff (deg u, u) < (deg v, v); the oriented offsets
 //     are the scan positions at the symmetric offsets;
 //   * intersection work min(d(u), d(v)) splits every edge into one of three
-//     wave64 classes: one lane per edge (<= 16 probes), sixteen lanes per edge
-//     (<= 256), or a whole wave per edge; each lane binary-searches elements
-//     of the shorter list in the longer one (equal to the merge count on
-//     simple graphs);
-//   * the two group classes are queued by the one-lane pass; their waves
-//     claim 16 queued edges per group at a time from a ticket, which keeps
-//     the waves busy to the end of the skewed queues;
+//     wave64 classes: one lane per edge (<= 16 entries in the shorter list),
+//     sixteen lanes per edge (<= 256), where each lane binary-searches
+//     entries of the shorter list in the longer one, or a whole wave per edge
+//     intersecting 64-entry windows of both lists (tc_window_kernel; both
+//     equal the merge count on simple graphs);
+//   * the two deferred classes are queued by the one-lane pass; their waves
+//     claim 16 queued edges at a time from a ticket, which keeps the waves
+//     busy to the end of the skewed queues;
 //   * counts are reduced per wave; the total takes one 64-bit atomicAdd per
 //     wave, per-vertex credits (optional) one per edge with a non-zero count.
 #include "graph.hpp"
@@ -165,6 +166,148 @@ __global__ __launch_bounds__(BLOCK) void tc_group_kernel(TcCsr g, const uint32_t
     if (__lane_id() == 0 && acc) atomicAdd(o.total, acc);
 }
 
+// A whole wave per deferred edge, intersecting by sorted windows: lane l
+// holds element l of a 64-entry window of the shorter list S and of the
+// longer list L; each lane finds its S element in the L window with six
+// cross-lane steps, then the S window moves on if its last element is not
+// above the L window's, else the L window does.  Every load is one coalesced
+// 256-byte window, where the binary searches scatter their lanes over a
+// list's cache lines.  Once the L window has moved past an S window, the
+// S entries up to the old L window's last element were already checked, so
+// they count only above it: each S entry counts once if L holds it, as with
+// the binary searches, repeated entries of a multigraph included (a host
+// model of this loop was checked against per-entry binary searches on random
+// lists with and without repeats).
+//
+// A wave runs TC_SLOTS edges at once, each in its own slot, so that the
+// slots' dependent cross-lane steps and window loads overlap; a slot whose
+// edge is done takes the next one.  Windows past a list's end repeat its last
+// entry (they stay sorted and their last lane holds their largest entry),
+// and every iteration loads the next window of both lists of every slot (the
+// same one again when a list does not move on), so the loads need no
+// selects and are in flight while the slots are matched.
+// (R-MAT s24 recount: 1 slot 472 ms, 2 slots 450 ms, 4 slots 490 ms, whose
+// registers leave half the waves per SIMD)
+constexpr int SLOTS = 2;
+
+struct WinSlot {
+    const int32_t* sp;   // shorter list
+    const int32_t* lp;   // longer list
+    int64_t sn, ln;      // their lengths (>= 1 while live)
+    int64_t i, j;        // window starts
+    int32_t a, b;        // this lane's entry of the current windows
+    int32_t an, bn;      // ... of the next windows
+    int32_t lim;         // S entries up to lim were checked against an earlier L window
+    int32_t credit;      // vertex credited with the count
+    uint32_t c;          // this lane's matches
+    bool live;
+};
+
+__device__ __forceinline__ int32_t window_entry(const int32_t* __restrict__ x, int64_t n, int64_t k) {
+    return x[k < n ? k : n - 1];
+}
+
+__global__ __launch_bounds__(BLOCK) void tc_window_kernel(TcCsr g, const uint32_t* __restrict__ q,
+                                                          const uint32_t* __restrict__ qcount,
+                                                          const int32_t* __restrict__ src_of, TcOut o) {
+    const uint32_t n = *qcount;
+    const int lane = __lane_id();
+    unsigned long long acc = 0;
+    uint32_t* ticket = o.cnt + 2;
+    int64_t k = 0, k1 = 0;   // the wave's claimed edges [k, k1)
+    bool drained = false;
+    // the next queued edge into slot w (or the slot goes idle)
+    auto fetch = [&](WinSlot& w) {
+        if (k >= k1 && !drained) {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(ticket, 1u);
+            t = __shfl(t, 0, WAVE);
+            k = (int64_t)t * TC_CHUNK;
+            k1 = k + TC_CHUNK < (int64_t)n ? k + TC_CHUNK : (int64_t)n;
+            drained = k >= (int64_t)n;
+        }
+        if (drained || k >= k1) {
+            w.live = false;
+            w.sp = w.lp = g.adj;   // an idle slot keeps loading one valid entry
+            w.sn = w.ln = 1;
+            w.i = w.j = 0;
+            w.a = w.b = w.an = w.bn = 0;
+            w.lim = -1;
+            w.c = 0;
+            w.credit = 0;
+            return;
+        }
+        const int64_t e = q[k++];
+        const int32_t u0 = src_of[e];
+        const int32_t v0 = g.adj[e];
+        const int64_t su = g.off[u0], du = g.off[u0 + 1] - su;
+        const int64_t sv = g.off[v0], dv = g.off[v0 + 1] - sv;
+        const bool src_smaller = du < dv;
+        w.sp = g.adj + (src_smaller ? su : sv);
+        w.sn = src_smaller ? du : dv;
+        w.lp = g.adj + (src_smaller ? sv : su);
+        w.ln = src_smaller ? dv : du;
+        w.credit = src_smaller ? u0 : v0;
+        w.live = true;   // a deferred edge has both lists above LIGHT_MAX entries
+        w.i = w.j = 0;
+        w.a = window_entry(w.sp, w.sn, lane);
+        w.b = window_entry(w.lp, w.ln, lane);
+        w.an = window_entry(w.sp, w.sn, WAVE + lane);
+        w.bn = window_entry(w.lp, w.ln, WAVE + lane);
+        w.lim = -1;
+        w.c = 0;
+    };
+    WinSlot ws[SLOTS];
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) fetch(ws[s]);
+    for (;;) {
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++) any = any || ws[s].live;
+        if (!any) break;
+        // first index p of each L window with b[p] >= a, the slots interleaved
+        int p[SLOTS];
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++) p[s] = 0;
+#pragma unroll
+        for (int st = 32; st > 0; st >>= 1)
+#pragma unroll
+            for (int s = 0; s < SLOTS; s++) {
+                const int32_t x = __shfl(ws[s].b, p[s] + st - 1, WAVE);
+                p[s] += x < ws[s].a ? st : 0;
+            }
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++) {
+            WinSlot& w = ws[s];
+            const int32_t y = __shfl(w.b, p[s] & (WAVE - 1), WAVE);
+            w.c += (w.live && p[s] < WAVE && y == w.a && w.i + lane < w.sn && w.a > w.lim) ? 1u : 0u;
+            const int32_t amax = __builtin_amdgcn_readlane(w.a, WAVE - 1);
+            const int32_t bmax = __builtin_amdgcn_readlane(w.b, WAVE - 1);
+            const bool adv_a = amax <= bmax;
+            w.i += adv_a ? WAVE : 0;
+            w.j += adv_a ? 0 : WAVE;
+            w.a = adv_a ? w.an : w.a;
+            w.b = adv_a ? w.b : w.bn;
+            w.lim = adv_a ? -1 : bmax;
+            w.an = window_entry(w.sp, w.sn, w.i + WAVE + lane);
+            w.bn = window_entry(w.lp, w.ln, w.j + WAVE + lane);
+        }
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++) {
+            WinSlot& w = ws[s];
+            if (w.live && (w.i >= w.sn || w.j >= w.ln)) {
+                const uint32_t tot = wave_reduce_sum(w.c);
+                if (lane == 0) {
+                    if (tot && o.per_vertex) atomicAdd(o.per_vertex + w.credit, (unsigned long long)tot);
+                    acc += tot;
+                }
+                fetch(w);
+            }
+        }
+    }
+    if (lane == 0 && acc) atomicAdd(o.total, acc);
+}
+
 // ---- orientation of the symmetric storage
 constexpr int OR_IPT = 8;
 constexpr int OR_TILE = BLOCK * OR_IPT;
@@ -243,7 +386,7 @@ static hn_status tc_core(hn_graph* g, int32_t nv, int64_t ne, const int64_t* off
         HN_CHECK_LAUNCH();
         tc_group_kernel<MEDIUM_LANES><<<(unsigned)cap, BLOCK, 0, st>>>(c, o.med_q, o.cnt, o.src_of, o);
         HN_CHECK_LAUNCH();
-        tc_group_kernel<WAVE><<<(unsigned)cap, BLOCK, 0, st>>>(c, o.heavy_q, o.cnt + 1, o.src_of, o);
+        tc_window_kernel<<<(unsigned)cap, BLOCK, 0, st>>>(c, o.heavy_q, o.cnt + 1, o.src_of, o);
         HN_CHECK_LAUNCH();
     }
     unsigned long long t = 0;
