@@ -142,6 +142,37 @@ def test_bfs_isolated_source_and_tiny_graphs(hn, rmat16):
     assert h1.getCOO()[0].numel() == 0
 
 
+def test_bfs_level_loop_depths(hn, rmat16):
+    """The level loop queues levels without a host round trip and reads the
+    end flag after as many levels as the graph's previous traversal took:
+    traversals of very different depth in a row (a 3000-vertex path from
+    either end and its middle, R-MAT s16 between them), with and without
+    kernel timing, each against the oracle."""
+    off, adj, h = rmat16
+    n = 3000
+    s = np.arange(n - 1, dtype=np.int32)
+    poff, padj = O.coo_to_csr(n, s, s + 1, symmetrize=True)
+    hp = hn.Hornet.from_csr(poff, padj)
+    bp = hn.BfsTopDown(hp)
+    br = hn.BfsTopDown(h)
+    src_r = O.max_degree_id(off)
+    for timed in (0, 1):
+        for graph, b, o, a, src in (("path", bp, poff, padj, 0), ("rmat16", br, off, adj, src_r),
+                                    ("path", bp, poff, padj, n - 1), ("path", bp, poff, padj, n // 2),
+                                    ("rmat16", br, off, adj, src_r)):
+            print(f"case bfs levels {graph} src={src} timed={timed}", flush=True)
+            exp, scanned = O.bfs(o, a, src)
+            b.set_parameters(src)
+            b.stats.time_kernels = timed
+            b.run()
+            assert np.array_equal(b.distances().cpu().numpy(), exp), (graph, src)
+            assert b.stats.levels == int(exp[exp != INF].max()) + 1
+            assert b.stats.edges_scanned == scanned
+            assert b.stats.expand_launches == b.stats.levels
+            assert (b.stats.expand_ns > 0) == bool(timed)
+    hp.release()
+
+
 def test_invalid_inputs_raise(hn):
     print("case invalid inputs", flush=True)
     from hornet_amd import HornetError
