@@ -56,9 +56,9 @@ constexpr int EXP_IPT = 4;                     // work units per lane per tile
 constexpr int EXP_TILE = BLOCK * EXP_IPT;      // work units per tile
 constexpr int32_t INF = 0x7fffffff;
 // The expansion keeps the first FULL_LDS_WORDS words of the full-word summary
-// in LDS (8 KB: vertices below 2^21).  A neighbour whose bitmap word is known
+// in LDS (12 KB: vertices below 3 * 2^20).  A neighbour whose bitmap word is known
 // full was visited before the level, so its bitmap probe is skipped.
-constexpr int FULL_LDS_WORDS = 2048;
+constexpr int FULL_LDS_WORDS = 3072;
 static_assert(UNIT_SLOTS % EXP_G == 0, "groups must not straddle a block start");
 
 struct FrontierIn {
@@ -122,9 +122,12 @@ __global__ void bfs_advance_kernel(BfsCtl* __restrict__ ctl, const uint64_t* __r
     *ctl = c;
 }
 
+// the frontier slice of a tile: group prefix, adjacency offset in 128-byte
+// units (every block starts on one), degree
+using ExpLb = LbShared<EXP_IPT, uint32_t, uint32_t>;
 struct ExpShared {
-    LbShared<EXP_IPT, uint32_t> lb;   // pref = group prefix, base = adjacency offset
-    int32_t deg[LbShared<EXP_IPT, uint32_t>::CAP];
+    ExpLb lb;
+    int32_t deg[ExpLb::CAP];
     uint32_t full[FULL_LDS_WORDS];     // head of the full-word summary
 };
 
@@ -143,7 +146,7 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
                                                            const uint32_t* __restrict__ bitmap,
                                                            const uint32_t* __restrict__ full, int64_t fwords) {
     typedef int32_t Vec __attribute__((ext_vector_type(G)));
-    constexpr int CAP = LbShared<EXP_IPT, uint32_t>::CAP;
+    constexpr int CAP = ExpLb::CAP;
     __shared__ ExpShared sh;
     const int64_t F = ctl->F;
     const int64_t W = F > 0 ? (int64_t)(uint32_t)tot[0] : 0;
@@ -159,7 +162,7 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
         if (staged) {
             for (int i = threadIdx.x; i < (int)span; i += BLOCK) {
                 sh.lb.pref[i] = P[lo + i];
-                sh.lb.base[i] = qoff[lo + i];
+                sh.lb.base[i] = (uint32_t)(qoff[lo + i] / UNIT_SLOTS);
                 sh.deg[i] = qdeg[lo + i];
             }
         }
@@ -173,7 +176,7 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
                 const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
                 const int s = lb_locate<EXP_IPT>(sh.lb, (int)span, w);
                 const int64_t k = (w - (int64_t)sh.lb.pref[s]) * G;
-                addr[j] = sh.lb.base[s] + k;
+                addr[j] = (int64_t)sh.lb.base[s] * UNIT_SLOTS + k;
                 nval[j] = w < W ? (int)min((int64_t)G, (int64_t)sh.deg[s] - k) : 0;
             }
         } else {   // slice wider than LDS (runs of zero-degree frontier vertices)

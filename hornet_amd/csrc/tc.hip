@@ -33,7 +33,7 @@
 namespace hn {
 namespace {
 
-constexpr int TC_IPT = 4;
+constexpr int TC_IPT = 2;
 constexpr int TC_TILE = BLOCK * TC_IPT;
 constexpr int LIGHT_MAX = 16;
 // class limits: one lane per edge up to LIGHT_MAX probes, MEDIUM_LANES lanes up
@@ -68,12 +68,32 @@ struct TcOut {
     int32_t* src_of;                  // edge -> source (written by the light pass)
 };
 
-// light pass over all stored edges; defers medium/heavy edges to queues
+// The light edges of one tile (at most LIGHT_MAX entries in the shorter
+// list): their lists, and one work item per entry of the shorter list.
+struct LightShared {
+    uint32_t s[TC_TILE];        // first entry of the shorter list in adj
+    uint32_t l[TC_TILE];        // first entry of the longer list
+    int32_t ln[TC_TILE];        // length of the longer list
+    int32_t credit[TC_TILE];    // the vertex credited
+    uint32_t pre[TC_TILE + 1];  // shorter-list lengths, then their exclusive prefix
+    uint32_t cnt[TC_TILE];      // matches per light edge
+    uint32_t sc[NWAVES + 1];
+    uint32_t n;
+};
+
+// Light pass over all stored edges: defers medium / heavy edges to queues and
+// counts the light ones.  A light edge's up to LIGHT_MAX binary searches are
+// spread over the workgroup (one search per lane, the tile's light edges
+// flattened into one item list), not run one after another by the lane that
+// found the edge: the slowest lane of a wave had chained up to
+// TC_IPT x LIGHT_MAX searches.
 __global__ __launch_bounds__(BLOCK) void tc_light_kernel(int32_t nv, int64_t ne, TcCsr g, TcOut o) {
     __shared__ LbShared<TC_IPT, int64_t> sh;
+    __shared__ LightShared ls;
     unsigned long long acc = 0;
     const uint64_t lt = lanemask_lt();
     for (int64_t w0 = (int64_t)blockIdx.x * TC_TILE; w0 < ne; w0 += (int64_t)gridDim.x * TC_TILE) {
+        if (threadIdx.x == 0) ls.n = 0;   // lb_tile synchronises before it calls the items
         lb_tile<TC_IPT>(
             g.off, (int64_t)nv, w0, ne, sh, [&](int64_t s) { return (int64_t)0; },
             [&](int64_t s, int64_t e, int64_t) {
@@ -82,22 +102,24 @@ __global__ __launch_bounds__(BLOCK) void tc_light_kernel(int32_t nv, int64_t ne,
                 const int64_t su = g.off[u0], du = g.off[u0 + 1] - su;
                 const int64_t sv = g.off[v0], dv = g.off[v0 + 1] - sv;
                 int cls = 0;   // 0 done here, 1 medium, 2 heavy
-                uint32_t c = 0;
                 o.src_of[e] = u0;
                 if (u0 != v0 && du >= 2) {
                     const bool src_smaller = du < dv;
-                    const int32_t* sp = g.adj + (src_smaller ? su : sv);
                     const int64_t sn = src_smaller ? du : dv;
-                    const int32_t* lp = g.adj + (src_smaller ? sv : su);
-                    const int64_t ln = src_smaller ? dv : du;
                     if (sn <= LIGHT_MAX) {
-                        for (int64_t i = 0; i < sn; i++) c += bsearch_has(lp, ln, sp[i]) ? 1u : 0u;
-                        if (c && o.per_vertex) atomicAdd(o.per_vertex + (src_smaller ? u0 : v0), (unsigned long long)c);
+                        if (sn > 0) {
+                            const uint32_t k = atomicAdd(&ls.n, 1u);
+                            ls.s[k] = (uint32_t)(src_smaller ? su : sv);
+                            ls.l[k] = (uint32_t)(src_smaller ? sv : su);
+                            ls.ln[k] = (int32_t)(src_smaller ? dv : du);
+                            ls.credit[k] = src_smaller ? u0 : v0;
+                            ls.pre[k] = (uint32_t)sn;
+                            ls.cnt[k] = 0;
+                        }
                     } else {
                         cls = sn <= MEDIUM_MAX ? 1 : 2;
                     }
                 }
-                acc += c;
                 // wave-aggregated queue appends for deferred edges
                 for (int q = 1; q <= 2; q++) {
                     const uint64_t m = __ballot(cls == q);
@@ -110,6 +132,45 @@ __global__ __launch_bounds__(BLOCK) void tc_light_kernel(int32_t nv, int64_t ne,
                     }
                 }
             });
+        __syncthreads();
+        // exclusive prefix of the shorter-list lengths (thread t owns entries
+        // t * TC_IPT .. t * TC_IPT + TC_IPT - 1)
+        const uint32_t n = ls.n;
+        uint32_t len[TC_IPT], sum = 0;
+#pragma unroll
+        for (int j = 0; j < TC_IPT; j++) {
+            const uint32_t k = (uint32_t)threadIdx.x * TC_IPT + j;
+            len[j] = k < n ? ls.pre[k] : 0u;
+            sum += len[j];
+        }
+        uint32_t items;
+        uint32_t run = block_exclusive_scan(sum, ls.sc, &items);
+#pragma unroll
+        for (int j = 0; j < TC_IPT; j++) {
+            const uint32_t k = (uint32_t)threadIdx.x * TC_IPT + j;
+            if (k < n) ls.pre[k] = run;
+            run += len[j];
+        }
+        __syncthreads();
+        // one binary search per item: item t is entry t - pre[k] of light edge k
+        for (uint32_t t = threadIdx.x; t < items; t += BLOCK) {
+            uint32_t lo = 0, hi = n;   // largest k with pre[k] <= t (pre strictly increasing)
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (ls.pre[mid] <= t) lo = mid; else hi = mid;
+            }
+            const int32_t x = g.adj[(int64_t)ls.s[lo] + (t - ls.pre[lo])];
+            if (bsearch_has(g.adj + ls.l[lo], ls.ln[lo], x)) atomicAdd(&ls.cnt[lo], 1u);
+        }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < n; k += BLOCK) {
+            const uint32_t c = ls.cnt[k];
+            if (c) {
+                if (o.per_vertex) atomicAdd(o.per_vertex + ls.credit[k], (unsigned long long)c);
+                acc += c;
+            }
+        }
+        __syncthreads();   // the next tile reuses ls
     }
     acc = wave_reduce_sum(acc);
     if (__lane_id() == 0 && acc) atomicAdd(o.total, acc);

@@ -266,11 +266,11 @@ hipError_t exclusive_scan_dev_count(InFn in, OutFn out, Out* total, const uint32
 }
 
 // ------------------------------------------------------------ load balancing
-template <int IPT, typename PrefT>
+template <int IPT, typename PrefT, typename BaseT = int64_t>
 struct LbShared {
     static constexpr int CAP = BLOCK * IPT + 1;   // segment entries staged per tile
     PrefT pref[CAP];
-    int64_t base[CAP];
+    BaseT base[CAP];
 };
 
 // Largest s in [lo, hi) with P[s] <= x (P[lo] <= x), found by the whole
@@ -354,9 +354,9 @@ __device__ __forceinline__ int lb_stage(const PrefT* __restrict__ P, int64_t lo,
 constexpr int pow2_floor(int x) { return x <= 1 ? 1 : 2 * pow2_floor(x / 2); }
 
 // largest i in [0, count) with pref[i] <= w (pref[0] <= w), log2(CAP) fixed steps
-template <int IPT, typename PrefT>
-__device__ __forceinline__ int lb_locate(const LbShared<IPT, PrefT>& sh, int count, int64_t w) {
-    constexpr int TOP = pow2_floor(LbShared<IPT, PrefT>::CAP - 1);
+template <int IPT, typename PrefT, typename BaseT>
+__device__ __forceinline__ int lb_locate(const LbShared<IPT, PrefT, BaseT>& sh, int count, int64_t w) {
+    constexpr int TOP = pow2_floor(LbShared<IPT, PrefT, BaseT>::CAP - 1);
     int pos = 0;
 #pragma unroll
     for (int step = TOP; step >= 1; step >>= 1) {
