@@ -1,7 +1,7 @@
 // hornet/detail/wave.hpp -- wave64 device building blocks shared by
 // libhornet_hip and the header-only operator layer (forAllEdges & co.).
 //
-// * wave / block scans over 64-lane wavefronts;
+// * wave / block scans over 64-lane wavefronts (DPP row operations);
 // * exclusive_scan_dev(): reduce-then-scan over an input functor (two
 //   launches up to SCAN_SELF_TILES tiles, three above), the replacement for
 //   cub::DeviceScan::ExclusiveSum
