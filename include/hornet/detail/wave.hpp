@@ -34,23 +34,48 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
-// inclusive wave scan (64 lanes) with shuffles
+// Move v across lanes with one DPP row operation: lanes whose source lies
+// outside their row (row_shr) or whose row is masked off (row_bcast) get 0.
+// 8-byte values move as two 4-byte halves from the same source lane.
+template <int CTRL, int ROW_MASK, typename T>
+__device__ __forceinline__ T dpp_move(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte integers");
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = (uint64_t)v;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+        return (T)(((uint64_t)hi << 32) | lo);
+    } else {
+        return (T)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+    }
+}
+
+// inclusive wave scan (64 lanes): row_shr 1, 2, 4, 8 scan each 16-lane row;
+// row_bcast:15 adds row r's total to row r+1 (rows 1 and 3), row_bcast:31
+// adds lane 31's running total to rows 2 and 3
 template <typename T>
 __device__ __forceinline__ T wave_inclusive_scan(T v) {
-    int l = __lane_id();
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        T o = __shfl_up(v, d, WAVE);
-        if (l >= d) v += o;
-    }
+    v += dpp_move<0x111, 0xf>(v);   // row_shr:1
+    v += dpp_move<0x112, 0xf>(v);   // row_shr:2
+    v += dpp_move<0x114, 0xf>(v);   // row_shr:4
+    v += dpp_move<0x118, 0xf>(v);   // row_shr:8
+    v += dpp_move<0x142, 0xa>(v);   // row_bcast:15 into rows 1, 3
+    v += dpp_move<0x143, 0xc>(v);   // row_bcast:31 into rows 2, 3
     return v;
 }
 
+// wave sum, uniform across the wave (lane 63 of the inclusive scan)
 template <typename T>
 __device__ __forceinline__ T wave_reduce_sum(T v) {
-#pragma unroll
-    for (int d = WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, WAVE);
-    return v;
+    const T inc = wave_inclusive_scan(v);
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = (uint64_t)inc;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, WAVE - 1);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), WAVE - 1);
+        return (T)(((uint64_t)hi << 32) | lo);
+    } else {
+        return (T)__builtin_amdgcn_readlane((int)inc, WAVE - 1);
+    }
 }
 
 // exclusive block scan for BLOCK threads; returns the exclusive prefix and the
