@@ -9,41 +9,43 @@
 // may be enqueued more than once; the distances are still exact.
 //
 // Here, per level (no host round trip: counts live in a device control
-// block, and the host polls its done flag every few levels):
-//   1. the frontier arrives as (id, degree, adjacency offset) triples, written
-//      by the previous level's collect (one 16-byte record read each);
-//   2. one scan whose values pack (degree << 32 | groups), a group being four
-//      adjacency slots (16 bytes).  Its down-sweep writes the group prefix P
-//      and, per tile, the frontier index holding the tile's first group -- the
-//      partition that BinarySearchLB recomputes with two global binary
-//      searches per block (BinarySearchLB.i.cuh:254-264); its total carries
-//      the level's edge count beside the group count;
-//   3. expand: every edge block is a power of two of at least 32 slots and
+// block, and the host polls its done flag every few levels), two launches:
+//   1. expand: the frontier arrives as (degree, adjacency offset, group
+//      prefix) triples plus, per tile of the expansion, the frontier index
+//      holding the tile's first group -- the partition that BinarySearchLB
+//      recomputes with two global binary searches per block
+//      (BinarySearchLB.i.cuh:254-264); a group is four adjacency slots (16
+//      bytes).  Every edge block is a power of two of at least 32 slots and
 //      128-byte aligned, so a vertex's ceil(deg/4) groups lie inside its block
 //      and each is one aligned 16-byte load (slots past deg in the last group
-//      are masked).  Each tile stages its frontier slice (prefix, offset,
-//      degree) in LDS and maps a group to its vertex with a fixed-step LDS
-//      search.  Every neighbour first probes the visited bitmap as of the
-//      start of the level (n/8 bytes, read-only here, so L2 resident); the
-//      survivors probe the byte-per-vertex state array (0 = never reached)
-//      and an unreached one is marked with an idempotent byte store of 1:
-//      probe and mark touch the same cache line, and marks made earlier in
-//      the level suppress later ones.  No atomics: with eight
-//      XCDs, each holding its own L2, device-scope atomics are not served from
-//      cache (claiming vertices with atomicOr measured 7x slower, and even
-//      XCD-private mark bitmaps with workgroup-scope atomics measured 1.5x
-//      slower than byte stores);
-//   4. collect: a lane owns 32 vertices (32 state bytes, one word of the
+//      are masked).  Each tile stages its frontier slice in LDS and maps a
+//      group to its vertex with a fixed-step LDS search.  Every neighbour first
+//      probes the visited bitmap as of the start of the level (n/8 bytes,
+//      read-only here, so L2 resident); the survivors probe the byte-per-vertex
+//      state array (0 = never reached) and an unreached one is marked with an
+//      idempotent byte store of 1: probe and mark touch the same cache line,
+//      and marks made earlier in the level suppress later ones.  No atomics:
+//      with eight XCDs, each holding its own L2, device-scope atomics are not
+//      served from cache (claiming vertices with atomicOr measured 7x slower,
+//      and even XCD-private mark bitmaps with workgroup-scope atomics measured
+//      1.5x slower than byte stores);
+//   2. collect: a lane owns 32 vertices (32 state bytes, one word of the
 //      visited bitmap as of the start of the level); a set state byte whose
-//      bitmap bit is clear is a newly reached vertex: it gets its distance, its
-//      bitmap bit and one next-frontier slot (one atomicAdd per workgroup),
-//      so every vertex enters a frontier once;
-//   5. a one-thread step that adds the level to the statistics and makes the
-//      next frontier current (done when it is empty).
+//      bitmap bit is clear is a newly reached vertex: it gets its distance and
+//      its bitmap bit, so every vertex enters a frontier once.  A workgroup
+//      sums its discoveries' degrees and groups, claims their frontier slots
+//      and group range with one 64-bit atomicAdd of (count << 32 | groups), and
+//      writes each discovery's triple and the tile starts inside its groups:
+//      the exclusive scan of the frontier's groups that the reference runs as
+//      a separate pass (cub ExclusiveSum) falls out of the slot claims.
+// The frontier counters alternate by level parity, so no launch is needed to
+// make the next frontier current: the host passes each launch its level.
 // Distances: source 0, its neighbours 1, unreached INT32_MAX (TopDown.cu:42,81).
 #include <algorithm>
 #include <cstddef>
 #include <cstring>
+
+#include <hip/hip_ext.h>
 
 #include "bfs.hpp"
 #include "prims.hpp"
@@ -61,65 +63,34 @@ constexpr int32_t INF = 0x7fffffff;
 constexpr int FULL_LDS_WORDS = 3072;
 static_assert(UNIT_SLOTS % EXP_G == 0, "groups must not straddle a block start");
 
-struct FrontierIn {
-    const int32_t* qdeg;
-    __device__ __forceinline__ uint64_t operator()(int64_t i) const {
-        const uint32_t d = (uint32_t)qdeg[i];
-        return ((uint64_t)d << 32) | ((d + EXP_G - 1) / EXP_G);
-    }
-};
-
-// per frontier vertex i with exclusive packed prefix `ex` and value `x`
-struct FrontierOut {
-    uint32_t* P;
-    uint32_t* tile_lo;
-    __device__ __forceinline__ void operator()(int64_t i, uint64_t ex, uint64_t x) const {
-        const uint32_t g0 = (uint32_t)ex, ng = (uint32_t)x;
-        P[i] = g0;
-        if (ng) {
-            uint32_t t0 = (uint32_t)(((uint64_t)g0 + EXP_TILE - 1) / EXP_TILE);
-            uint32_t t1 = (uint32_t)(((uint64_t)g0 + ng + EXP_TILE - 1) / EXP_TILE);
-            for (uint32_t t = t0; t < t1; t++) tile_lo[t] = (uint32_t)i;
-        }
-    }
-};
-
 __global__ void bfs_init_kernel(int32_t nv, const hn_vrec* __restrict__ rec, int32_t* __restrict__ dist,
                                 uint32_t* __restrict__ bitmap, uint32_t* __restrict__ full,
-                                uint8_t* __restrict__ state, int32_t source, int32_t* __restrict__ q,
-                                int32_t* __restrict__ qdeg, int64_t* __restrict__ qoff, BfsCtl* __restrict__ ctl) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t words = ((int64_t)nv + 31) / 32;
+                                uint32_t* __restrict__ state4, int32_t source, int32_t* __restrict__ qdeg,
+                                uint32_t* __restrict__ qbase, uint32_t* __restrict__ P,
+                                uint32_t* __restrict__ tile_lo, uint64_t* __restrict__ part,
+                                BfsCtl* __restrict__ ctl) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t words = ((int64_t)nv + 31) / 32;
+    if (i < (words + BLOCK - 1) / BLOCK) part[i] = 0u;
     if (i < nv) dist[i] = (i == source) ? 0 : INF;
     if (i < words) bitmap[i] = (i == (source >> 5)) ? (1u << (source & 31)) : 0u;
     if (i < (words + 31) / 32) full[i] = 0u;
+    // state bytes (4 per word, words * 8 words): the source's alone set
+    if (i < words * 8) state4[i] = (i == (source >> 2)) ? (1u << (8 * (source & 3))) : 0u;
+    // the first frontier: the source alone, owning every tile of its groups
+    const hn_vrec r = rec[source];
+    const int64_t ng = ((int64_t)r.deg + EXP_G - 1) / EXP_G;
+    for (int64_t t = i; t < (ng + EXP_TILE - 1) / EXP_TILE; t += (int64_t)gridDim.x * blockDim.x) tile_lo[t] = 0u;
     if (i == 0) {
-        const hn_vrec r = rec[source];
-        q[0] = source;
         qdeg[0] = r.deg;
-        qoff[0] = r.off;
-        state[source] = 1;
+        qbase[0] = (uint32_t)(r.off / UNIT_SLOTS);
+        P[0] = 0u;
         BfsCtl c{};
-        c.F = 1;
-        c.level = 1;
+        c.fc[1] = ((uint64_t)1 << 32) | (uint64_t)ng;
+        c.scanned = (uint64_t)r.deg;
+        c.src_deg = (uint64_t)r.deg;
         *ctl = c;
     }
-}
-
-// End of a level: its frontier joins the statistics and the next frontier
-// becomes current; an empty one ends the traversal.  Levels launched after
-// the end find F == 0 and change nothing.
-__global__ void bfs_advance_kernel(BfsCtl* __restrict__ ctl, const uint64_t* __restrict__ tot) {
-    BfsCtl c = *ctl;
-    if (c.F == 0) return;
-    c.visited += c.F;
-    c.scanned += tot[0] >> 32;
-    c.levels += 1;
-    c.F = c.nextF;
-    c.nextF = 0;
-    c.level += 1;
-    c.done = c.F == 0 ? 1u : 0u;
-    *ctl = c;
 }
 
 // the frontier slice of a tile: group prefix, adjacency offset in 128-byte
@@ -135,21 +106,42 @@ struct ExpShared {
 // if not visited before this level, against state[]; unreached ones are
 // marked.  state[] is only ever set to 1 here, so concurrent stores agree.
 template <int G>
-__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __restrict__ tot,
-                                                           const uint32_t* __restrict__ P,
+__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uint32_t* __restrict__ P,
                                                            const uint32_t* __restrict__ tile_lo,
                                                            const int32_t* __restrict__ qdeg,
-                                                           const int64_t* __restrict__ qoff,
-                                                           const BfsCtl* __restrict__ ctl,
+                                                           const uint32_t* __restrict__ qbase,
+                                                           BfsCtl* __restrict__ ctl,
                                                            const int32_t* __restrict__ arena,
                                                            uint8_t* __restrict__ state,
                                                            const uint32_t* __restrict__ bitmap,
-                                                           const uint32_t* __restrict__ full, int64_t fwords) {
+                                                           const uint32_t* __restrict__ full, int64_t fwords,
+                                                           const uint64_t* __restrict__ part, int64_t nparts) {
     typedef int32_t Vec __attribute__((ext_vector_type(G)));
     constexpr int CAP = ExpLb::CAP;
     __shared__ ExpShared sh;
-    const int64_t F = ctl->F;
-    const int64_t W = F > 0 ? (int64_t)(uint32_t)tot[0] : 0;
+    const uint64_t fcur = ctl->fc[L & 1];
+    const int64_t F = (int64_t)(fcur >> 32);
+    const int64_t W = F > 0 ? (int64_t)(uint32_t)fcur : 0;
+    // level bookkeeping by workgroup 0: this level's collect counts into the
+    // other counter (nothing else touches it during this launch), and the
+    // degrees of every discovery so far, one partial per collect workgroup,
+    // join the source's in `scanned`
+    if (blockIdx.x == 0 && F > 0) {
+        __shared__ uint64_t red[NWAVES];
+        uint64_t x = 0;
+        for (int64_t i = threadIdx.x; i < nparts; i += BLOCK) x += part[i];
+        x = wave_reduce_sum(x);
+        if (__lane_id() == 0) red[threadIdx.x / WAVE] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t t = ctl->src_deg;
+            for (int k = 0; k < NWAVES; k++) t += red[k];
+            ctl->scanned = t;
+            ctl->levels = L;
+            ctl->visited += (uint64_t)F;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->fc[(L + 1) & 1] = 0;
     const int64_t ntiles = (W + EXP_TILE - 1) / EXP_TILE;
     if ((int64_t)blockIdx.x >= ntiles) return;
     for (int i = threadIdx.x; i < FULL_LDS_WORDS; i += BLOCK) sh.full[i] = i < fwords ? full[i] : 0u;
@@ -162,7 +154,7 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
         if (staged) {
             for (int i = threadIdx.x; i < (int)span; i += BLOCK) {
                 sh.lb.pref[i] = P[lo + i];
-                sh.lb.base[i] = (uint32_t)(qoff[lo + i] / UNIT_SLOTS);
+                sh.lb.base[i] = qbase[lo + i];
                 sh.deg[i] = qdeg[lo + i];
             }
         }
@@ -187,7 +179,7 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
                 if (w < W) {
                     const int64_t s = upper_bound_m1<int64_t>(P, lo, hi + 1, w);
                     const int64_t k = (w - (int64_t)P[s]) * G;
-                    addr[j] = qoff[s] + k;
+                    addr[j] = (int64_t)qbase[s] * UNIT_SLOTS + k;
                     nval[j] = (int)min((int64_t)G, (int64_t)qdeg[s] - k);
                 }
             }
@@ -251,17 +243,20 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(const uint64_t* __res
 
 // One lane owns 32 vertices (32 state bytes and one visited-bitmap word):
 // set state bytes not yet in the bitmap are this level's discoveries; they
-// get their distance, their bitmap bit and a slot in the next frontier.
-__global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, const uint32_t* __restrict__ state4,
+// get their distance, their bitmap bit, a slot in the next frontier and a
+// range of its groups.
+constexpr int COL_CHUNK = 8;   // rounds whose record loads are in flight together
+__global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(uint32_t L, int64_t words,
+                                                            const uint32_t* __restrict__ state4,
                                                             uint32_t* __restrict__ bitmap,
                                                             uint32_t* __restrict__ full,
                                                             int32_t* __restrict__ dist,
                                                             const hn_vrec* __restrict__ rec,
-                                                            int32_t* __restrict__ qn, int32_t* __restrict__ qdeg,
-                                                            int64_t* __restrict__ qoff, BfsCtl* __restrict__ ctl) {
-    if (ctl->F == 0) return;   // a level launched after the traversal ended
-    const int32_t level = (int32_t)ctl->level;
-    uint32_t* qn_count = &ctl->nextF;
+                                                            int32_t* __restrict__ qdeg, uint32_t* __restrict__ qbase,
+                                                            uint32_t* __restrict__ P, uint32_t* __restrict__ tile_lo,
+                                                            uint64_t* __restrict__ part, BfsCtl* __restrict__ ctl) {
+    if ((ctl->fc[L & 1] >> 32) == 0) return;   // a level launched after the traversal ended
+    const int32_t level = (int32_t)L;
     const int64_t wi = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     uint32_t found = 0;
     bool wfull = false;
@@ -295,45 +290,99 @@ __global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(int64_t words, const
         if (__lane_id() == 0 && fb && fw < (words + 31) / 32) full[fw] = (uint32_t)fb;
         if (__lane_id() == 1 && (fb >> 32) && fw + 1 < (words + 31) / 32) full[fw + 1] = (uint32_t)(fb >> 32);
     }
-    // next-frontier slots: one atomicAdd per workgroup (the counter is a
-    // single device-scope address, so fewer of them is what matters)
-    __shared__ uint32_t wsum[NWAVES + 1];
-    const uint32_t cnt = (uint32_t)__popc(found);
-    const uint32_t inc = wave_inclusive_scan(cnt);
-    if (__lane_id() == WAVE - 1) wsum[threadIdx.x / WAVE] = inc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int k = 0; k < NWAVES; k++) {
-            const uint32_t x = wsum[k];
-            wsum[k] = acc;
-            acc += x;
+    // The wave's discoveries are visited transposed, so that consecutive lanes
+    // handle consecutive vertices: in round r lane l takes bit (l & 31) of the
+    // wave's word 2r + (l >> 5), fetching that word's found bits from its owner.
+    // A lane without a discovery in a round loads rec[0] (one shared line), so
+    // that a chunk's record loads are all in flight before any is used.
+    const int lane = __lane_id();
+    const int64_t vbase = (wi - lane + (lane >> 5)) * 32 + (lane & 31);   // vertex of round 0
+    const int bit = lane & 31;
+    const bool any = __ballot(found != 0) != 0;
+    // pass 1: the groups and adjacency entries of the wave's discoveries
+    uint32_t cg = 0;
+    uint64_t ce = 0;
+    if (any) {
+        for (int r0 = 0; r0 < WAVE / 2; r0 += COL_CHUNK) {
+            bool f[COL_CHUNK];
+            int32_t d[COL_CHUNK];
+#pragma unroll
+            for (int k = 0; k < COL_CHUNK; k++)
+                f[k] = (__shfl(found, 2 * (r0 + k) + (lane >> 5), WAVE) >> bit) & 1u;
+#pragma unroll
+            for (int k = 0; k < COL_CHUNK; k++) d[k] = rec[f[k] ? vbase + 64 * (r0 + k) : 0].deg;
+#pragma unroll
+            for (int k = 0; k < COL_CHUNK; k++) {
+                const uint32_t dk = f[k] ? (uint32_t)d[k] : 0u;
+                cg += (dk + EXP_G - 1) / EXP_G;
+                ce += dk;
+            }
         }
-        wsum[NWAVES] = acc ? atomicAdd(qn_count, acc) : 0u;
+    }
+    const uint32_t wc = wave_reduce_sum((uint32_t)__popc(found));
+    const uint32_t wg = wave_reduce_sum(cg);
+    const uint64_t we = wave_reduce_sum(ce);
+    // One atomicAdd per workgroup claims its frontier slots and groups
+    // (count << 32 | groups: a level's groups stay below 2^32, so the low half
+    // never carries); their entries add to the workgroup's own partial.
+    __shared__ uint64_t wsum[NWAVES + 1];
+    __shared__ uint64_t went[NWAVES];
+    const int w = threadIdx.x / WAVE;
+    if (lane == 0) {
+        wsum[w] = ((uint64_t)wc << 32) | wg;
+        went[w] = we;
     }
     __syncthreads();
-    if (__shfl(inc, WAVE - 1, WAVE) == 0) return;   // nothing found by this wave
-    const uint32_t b = wsum[NWAVES] + wsum[threadIdx.x / WAVE] + inc - cnt;
-    // Writes, transposed so that consecutive lanes write consecutive vertices:
-    // in round r lane l handles bit (l & 31) of the wave's word 2r + (l >> 5),
-    // fetching that word's found bits and first queue slot from its owner.
-    const int lane = __lane_id();
-    const int64_t wbase = wi - lane;
-    const int bit = lane & 31;
-    const uint32_t below = (1u << bit) - 1u;
-#pragma unroll 4
-    for (int r = 0; r < WAVE / 2; r++) {
-        const int owner = 2 * r + (lane >> 5);
-        const uint32_t fw = __shfl(found, owner, WAVE);
-        const uint32_t bo = __shfl(b, owner, WAVE);
-        if ((fw >> bit) & 1u) {
-            const int32_t v = (int32_t)((wbase + owner) * 32 + bit);
-            const uint32_t slot = bo + (uint32_t)__popc(fw & below);
-            const hn_vrec r = rec[v];
-            dist[v] = level;
-            qn[slot] = v;
-            qdeg[slot] = r.deg;
-            qoff[slot] = r.off;
+    if (threadIdx.x == 0) {
+        uint64_t acc = 0, acce = 0;
+        for (int k = 0; k < NWAVES; k++) {
+            const uint64_t x = wsum[k];
+            wsum[k] = acc;
+            acc += x;
+            acce += went[k];
+        }
+        wsum[NWAVES] = acc ? atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->fc[(L + 1) & 1]),
+                                       (unsigned long long)acc)
+                           : 0ull;
+        if (acce) part[blockIdx.x] += acce;
+    }
+    __syncthreads();
+    if (!any) return;
+    // pass 2: each discovery's distance, frontier triple and tile starts
+    const uint64_t b = wsum[NWAVES] + wsum[w];
+    uint32_t sb = (uint32_t)(b >> 32), gb = (uint32_t)b;
+    const uint64_t lt = lanemask_lt();
+    for (int r0 = 0; r0 < WAVE / 2; r0 += COL_CHUNK) {
+        bool f[COL_CHUNK];
+        uint64_t fm[COL_CHUNK];
+        uint64_t cm = 0;
+#pragma unroll
+        for (int k = 0; k < COL_CHUNK; k++) {
+            f[k] = (__shfl(found, 2 * (r0 + k) + (lane >> 5), WAVE) >> bit) & 1u;
+            fm[k] = __ballot(f[k]);
+            cm |= fm[k];
+        }
+        if (!cm) continue;
+        hn_vrec rv[COL_CHUNK];
+#pragma unroll
+        for (int k = 0; k < COL_CHUNK; k++) rv[k] = rec[f[k] ? vbase + 64 * (r0 + k) : 0];
+#pragma unroll
+        for (int k = 0; k < COL_CHUNK; k++) {
+            const uint32_t ng = f[k] ? ((uint32_t)rv[k].deg + EXP_G - 1) / EXP_G : 0u;
+            const uint32_t inc = wave_inclusive_scan(ng);
+            if (f[k]) {
+                const int64_t v = vbase + 64 * (r0 + k);
+                const uint32_t slot = sb + (uint32_t)__popcll(fm[k] & lt);
+                const uint64_t g0 = (uint64_t)gb + inc - ng;
+                dist[v] = level;
+                qdeg[slot] = rv[k].deg;
+                qbase[slot] = (uint32_t)(rv[k].off / UNIT_SLOTS);
+                P[slot] = (uint32_t)g0;
+                for (uint64_t t = (g0 + EXP_TILE - 1) / EXP_TILE; t * EXP_TILE < g0 + ng; t++)
+                    tile_lo[t] = slot;
+            }
+            sb += (uint32_t)__popcll(fm[k]);
+            gb += (uint32_t)__builtin_amdgcn_readlane((int)inc, WAVE - 1);
         }
     }
 }
@@ -349,52 +398,45 @@ int64_t bfs_layout(int64_t nv_in, int64_t ne, char* base, BfsBufs* b) {
     auto al = [](int64_t x) { return (x + 255) & ~int64_t(255); };
     const int64_t o_bitmap = 0;
     const int64_t o_full = al(o_bitmap + words * 4);
-    const int64_t o_q = al(o_full + (words + 31) / 32 * 4);
-    const int64_t o_qdeg = al(o_q + nv * 4);
-    const int64_t o_qoff = al(o_qdeg + nv * 4);
-    const int64_t o_P = al(o_qoff + nv * 8);
-    const int64_t o_tile = al(o_P + (nv + 1) * 4);
+    const int64_t o_qdeg = al(o_full + (words + 31) / 32 * 4);
+    const int64_t o_qbase = al(o_qdeg + nv * 4);
+    const int64_t o_P = al(o_qbase + nv * 4);
+    const int64_t o_tile = al(o_P + nv * 4);
     const int64_t o_state = al(o_tile + tiles * 4);
-    const int64_t o_tot = al(o_state + words * 32);
-    const int64_t o_ctl = al(o_tot + 8);
-    const int64_t o_scan = al(o_ctl + (int64_t)sizeof(BfsCtl));
-    const int64_t total = al(o_scan + (int64_t)scan_temp_bytes(nv));
+    const int64_t o_part = al(o_state + words * 32);
+    const int64_t o_ctl = al(o_part + ceil_div(words, BLOCK) * 8);
+    const int64_t total = al(o_ctl + (int64_t)sizeof(BfsCtl));
     if (base) {
         b->bitmap = reinterpret_cast<uint32_t*>(base + o_bitmap);
         b->full = reinterpret_cast<uint32_t*>(base + o_full);
-        b->q = reinterpret_cast<int32_t*>(base + o_q);
         b->qdeg = reinterpret_cast<int32_t*>(base + o_qdeg);
-        b->qoff = reinterpret_cast<int64_t*>(base + o_qoff);
+        b->qbase = reinterpret_cast<uint32_t*>(base + o_qbase);
         b->P = reinterpret_cast<uint32_t*>(base + o_P);
         b->tile_lo = reinterpret_cast<uint32_t*>(base + o_tile);
         b->state = reinterpret_cast<uint8_t*>(base + o_state);
-        b->tot = reinterpret_cast<uint64_t*>(base + o_tot);
+        b->part = reinterpret_cast<uint64_t*>(base + o_part);
         b->ctl = reinterpret_cast<BfsCtl*>(base + o_ctl);
-        b->scan_tmp = base + o_scan;
     }
     return total;
 }
 
-__global__ void zero_u32_kernel(int64_t n, uint32_t* __restrict__ p) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
-// The level loop over laid-out buffers.  Levels are queued without a host
-// round trip; the host reads the done flag after as many levels as the last
-// traversal of this graph took, then after 1, 2, 4, 4, ... more (levels
-// queued past the end change nothing).  Nothing here allocates (kernel timing,
+// The level loop over laid-out buffers: two launches per level, queued
+// without a host round trip.  The host reads the control block after as many
+// levels as the last traversal of this graph took, then after 1, 2, 4, 4, ...
+// more (levels queued past the end change nothing), and ends when the
+// frontier of the next level to launch is empty.  Nothing here allocates (kernel timing,
 // which creates its events, only with `may_time`).
 hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats, hipStream_t st,
                   const BfsBufs& b, bool may_time, uint64_t* host) {
     const int64_t nv = g->nv;
     const int64_t words = (nv + 31) / 32;
-    // state starts at zero (a fill kernel, not a memset node, so a captured
-    // copy of this sequence replays correctly)
-    zero_u32_kernel<<<(unsigned)ceil_div(words * 8, BLOCK), BLOCK, 0, st>>>(words * 8,
-                                                                          reinterpret_cast<uint32_t*>(b.state));
-    bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(g->nv, g->rec, dist, b.bitmap, b.full, b.state,
-                                                                      source, b.q, b.qdeg, b.qoff, b.ctl);
+    // frontier adjacency offsets are kept in 128-byte units as 32-bit words
+    // (2^32 units = 512 GiB of arena, more than one device holds)
+    HN_ARG(g->cap_units < ((int64_t)1 << 32), "edge arena beyond 2^32 128-byte units");
+    // kernels only (no memset node), so a captured copy of this sequence replays correctly
+    bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(
+        g->nv, g->rec, dist, b.bitmap, b.full, reinterpret_cast<uint32_t*>(b.state), source, b.qdeg, b.qbase, b.P,
+        b.tile_lo, b.part, b.ctl);
     HN_CHECK_LAUNCH();
     const int64_t grid_cap = (int64_t)num_cus() * 64;
     const bool timed = may_time && stats && stats->time_kernels;
@@ -412,44 +454,40 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
             }
     const int last = g->bfs_last_levels.load(std::memory_order_relaxed);
     int64_t next = std::min<int64_t>(last > 0 ? last : 4, MAX_CHUNK);
-    int64_t ramp = 1, queued = 0, kns = 0;
+    int64_t ramp = 1, kns = 0;
+    uint32_t L = 0;   // levels launched
     uint32_t levels_seen = 0;
-    static_assert(offsetof(BfsCtl, done) == offsetof(BfsCtl, levels) + 4, "levels, done adjacent");
+    BfsCtl c{};
     while (rc == HN_OK) {
         hipError_t e = hipSuccess;
-        for (int64_t c = 0; c < next && e == hipSuccess; c++) {
-            e = exclusive_scan_dev_count<uint64_t>(FrontierIn{b.qdeg}, FrontierOut{b.P, b.tile_lo}, b.tot,
-                                                    &b.ctl->F, nv, b.scan_tmp, st);
-            if (timed && e == hipSuccess) e = hipEventRecord(ev[2 * c], st);
-            if (e == hipSuccess) {
-                bfs_expand_kernel<EXP_G><<<(unsigned)grid_cap, BLOCK, 0, st>>>(
-                    b.tot, b.P, b.tile_lo, b.qdeg, b.qoff, b.ctl, g->arena, b.state, b.bitmap, b.full,
-                    (words + 31) / 32);
-                e = hipGetLastError();
-            }
-            if (timed && e == hipSuccess) e = hipEventRecord(ev[2 * c + 1], st);
+        for (int64_t k = 0; k < next && e == hipSuccess; k++) {
+            L++;
+            // timed: the events take the dispatch's own start and end
+            // (hipExtLaunchKernel), with no marker between the launches
+            hipExtLaunchKernelGGL(bfs_expand_kernel<EXP_G>, dim3((unsigned)grid_cap), dim3(BLOCK), 0, st,
+                                  timed ? ev[2 * k] : nullptr, timed ? ev[2 * k + 1] : nullptr, 0, L,
+                                  (const uint32_t*)b.P, (const uint32_t*)b.tile_lo, (const int32_t*)b.qdeg,
+                                  (const uint32_t*)b.qbase, b.ctl, (const int32_t*)g->arena, b.state,
+                                  (const uint32_t*)b.bitmap, (const uint32_t*)b.full, (int64_t)((words + 31) / 32),
+                                  (const uint64_t*)b.part, (int64_t)ceil_div(words, BLOCK));
+            e = hipGetLastError();
             if (e == hipSuccess) {
                 bfs_collect_kernel<<<(unsigned)ceil_div(words, BLOCK), BLOCK, 0, st>>>(
-                    words, reinterpret_cast<const uint32_t*>(b.state), b.bitmap, b.full, dist, g->rec, b.q, b.qdeg,
-                    b.qoff, b.ctl);
-                e = hipGetLastError();
-            }
-            if (e == hipSuccess) {
-                bfs_advance_kernel<<<1, 1, 0, st>>>(b.ctl, b.tot);
+                    L, words, reinterpret_cast<const uint32_t*>(b.state), b.bitmap, b.full, dist, g->rec, b.qdeg,
+                    b.qbase, b.P, b.tile_lo, b.part, b.ctl);
                 e = hipGetLastError();
             }
         }
-        queued += next;
-        // [0] = levels | done << 32
-        if (e == hipSuccess) e = hipMemcpyAsync(host, &b.ctl->levels, 8, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, b.ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
-        const uint32_t levels_now = (uint32_t)host[0];
-        const bool done = (host[0] >> 32) != 0;
+        if (e == hipSuccess) std::memcpy(&c, host, sizeof(BfsCtl));
+        // done: the frontier of the next level to launch is empty
+        const bool done = (c.fc[(L + 1) & 1] >> 32) == 0;
         if (timed && e == hipSuccess)
             // only the expansions of non-empty levels count
-            for (uint32_t c = 0; c < levels_now - levels_seen && e == hipSuccess; c++) {
+            for (uint32_t k = 0; k < c.levels - levels_seen && e == hipSuccess; k++) {
                 float ms = 0.f;
-                e = hipEventElapsedTime(&ms, ev[2 * c], ev[2 * c + 1]);
+                e = hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
                 kns += (int64_t)((double)ms * 1e6);
             }
         if (e != hipSuccess) {
@@ -457,9 +495,9 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
             rc = HN_E_HIP;
             break;
         }
-        levels_seen = levels_now;
+        levels_seen = c.levels;
         if (done) break;
-        if (queued > nv + 1) {   // cannot happen: every level reaches a new vertex
+        if (L > nv + 1) {   // cannot happen: every level reaches a new vertex
             set_error("bfs: level loop did not end");
             rc = HN_E_HIP;
             break;
@@ -470,10 +508,6 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
     if (rc != HN_OK) return rc;
     g->bfs_last_levels.store((int)levels_seen, std::memory_order_relaxed);
     if (stats) {
-        BfsCtl c;
-        HN_HIP(hipMemcpyAsync(host, b.ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, st));
-        HN_HIP(hipStreamSynchronize(st));
-        std::memcpy(&c, host, sizeof(BfsCtl));
         stats->levels = (int32_t)c.levels;
         stats->visited = (int64_t)c.visited;
         stats->edges_scanned = (int64_t)c.scanned;

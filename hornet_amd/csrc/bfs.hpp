@@ -6,42 +6,43 @@
 namespace hn {
 
 // Level-loop control on the device: the loop runs without a host round trip
-// per level, and the host polls `done` every few levels.
+// per level, and the host polls it every few levels.  Launch L (from 1)
+// expands the frontier of level L and collects level L's discoveries, whose
+// distance is L; the host passes L to both kernels.
 struct BfsCtl {
-    uint32_t F;        // frontier size of the level being expanded
-    uint32_t nextF;    // next frontier size (collect's slot counter)
-    uint32_t level;    // distance given to this level's discoveries
-    uint32_t levels;   // non-empty frontiers so far
-    uint32_t done;     // the frontier is empty
+    uint64_t fc[2];    // frontier of level L: fc[L & 1] = size << 32 | groups
+                       // (collect L adds into fc[(L + 1) & 1], which expand L clears)
+    uint32_t levels;   // last non-empty level = number of non-empty frontiers
     uint32_t pad_;
     uint64_t visited;  // sum of the frontier sizes
-    uint64_t scanned;  // sum of the frontiers' degrees (m_C)
+    uint64_t scanned;  // sum of the frontiers' degrees (m_C), as of the last expansion
+    uint64_t src_deg;  // the source's degree
 };
 
 // Scratch of one traversal, carved from one device buffer: the handle's
 // s_bfs, or the static array of the no-allocation entry point.
 // The frontier needs no ping-pong: collect writes each discovered vertex's
-// id, degree and adjacency offset into (q, qdeg, qoff), which the next
-// level's scan and expansion read before that level's collect overwrites them.
+// degree, adjacency offset (in 128-byte units) and group prefix into
+// (qdeg, qbase, P), and the first frontier index of every expansion tile into
+// tile_lo; the next level's expansion reads them before that level's collect
+// overwrites them.
 struct BfsBufs {
     uint32_t* bitmap;   // visited as of the start of the level
     uint32_t* full;     // bit w: bitmap word w is all ones (every one of its 32 vertices visited)
-    int32_t* q;
     int32_t* qdeg;
-    int64_t* qoff;
+    uint32_t* qbase;
     uint32_t* P;
     uint32_t* tile_lo;
     uint8_t* state;     // one byte per vertex, padded to whole bitmap words
-    uint64_t* tot;      // [0] level totals (edges << 32 | groups)
+    uint64_t* part;     // per collect workgroup: the degrees of its discoveries so far
     BfsCtl* ctl;
-    void* scan_tmp;     // scan partials
 };
 
 // Lays the buffers of a traversal of a graph with nv vertices and ne edges
 // out from `base` (nullptr: sizes only); returns the bytes they take.
 int64_t bfs_layout(int64_t nv, int64_t ne, char* base, BfsBufs* b);
 // BFS TopDown from `source` into dist[nv] over laid-out buffers, reading the
-// per-level totals back through `host` (pinned, at least 16 bytes).
+// level counters back through `host` (pinned, at least sizeof(BfsCtl) bytes).
 // Allocates nothing, except the two timing events when `may_time` and
 // stats->time_kernels ask for kernel timing.
 hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats, hipStream_t st,

@@ -241,8 +241,8 @@ hn_status hn_bfs_topdown(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stat
  * reference: it is for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  The
  * traversal scratch is one static device array per GPU of
- * hn_bfs_topdown_noalloc_limit() bytes (64 MiB: R-MAT up to about 2.5 M
- * vertices), held until the process ends; a graph needing more gets
+ * hn_bfs_topdown_noalloc_limit() bytes (64 MiB: about 13 bytes per vertex,
+ * R-MAT up to about 5 M vertices), held until the process ends; a graph needing more gets
  * HN_E_CAPACITY.  All calls on one GPU share that array, so calls that overlap
  * on one GPU (any streams) race.  Same results as hn_bfs_topdown; no kernel
  * timing (stats->time_kernels is ignored). */

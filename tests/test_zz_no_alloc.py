@@ -52,8 +52,8 @@ def test_noalloc_bfs_refuses_graphs_beyond_its_scratch():
 
     import hornet_amd.graph as G
     lib = G._lib.load()
-    # 4 M vertices need about 90 MB of traversal scratch, beyond the 64 MiB array
-    nv = 1 << 22
+    # 8 M vertices need about 105 MiB of traversal scratch, beyond the 64 MiB array
+    nv = 1 << 23
     e = torch.tensor([0, 1], dtype=torch.int32, device="cuda")
     h = G.Hornet.from_coo_device(nv, e[:1], e[1:])
     dist = torch.empty(nv, dtype=torch.int32, device="cuda")
