@@ -61,6 +61,9 @@ constexpr int32_t INF = 0x7fffffff;
 // in LDS (12 KB: vertices below 3 * 2^20).  A neighbour whose bitmap word is known
 // full was visited before the level, so its bitmap probe is skipped.
 constexpr int FULL_LDS_WORDS = 3072;
+#ifndef BFS_DIRECT_OPEN_PCT
+#define BFS_DIRECT_OPEN_PCT 25   // skip bitmap probes while more than this % of the adjacency volume is unreached
+#endif
 static_assert(UNIT_SLOTS % EXP_G == 0, "groups must not straddle a block start");
 
 __global__ void bfs_init_kernel(int32_t nv, const hn_vrec* __restrict__ rec, int32_t* __restrict__ dist,
@@ -87,7 +90,7 @@ __global__ void bfs_init_kernel(int32_t nv, const hn_vrec* __restrict__ rec, int
         P[0] = 0u;
         BfsCtl c{};
         c.fc[1] = ((uint64_t)1 << 32) | (uint64_t)ng;
-        c.scanned = (uint64_t)r.deg;
+        c.scanned[1] = (uint64_t)r.deg;   // [0]: nothing visited before level 1
         c.src_deg = (uint64_t)r.deg;
         *ctl = c;
     }
@@ -115,13 +118,23 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uin
                                                            uint8_t* __restrict__ state,
                                                            const uint32_t* __restrict__ bitmap,
                                                            const uint32_t* __restrict__ full, int64_t fwords,
-                                                           const uint64_t* __restrict__ part, int64_t nparts) {
+                                                           const uint64_t* __restrict__ part, int64_t nparts,
+                                                           int64_t ne) {
     typedef int32_t Vec __attribute__((ext_vector_type(G)));
     constexpr int CAP = ExpLb::CAP;
     __shared__ ExpShared sh;
     const uint64_t fcur = ctl->fc[L & 1];
     const int64_t F = (int64_t)(fcur >> 32);
     const int64_t W = F > 0 ? (int64_t)(uint32_t)fcur : 0;
+    // Skip the bitmap probes while most adjacency entries still lead to
+    // unreached vertices: the state bytes are cumulative (every vertex reached
+    // so far has its byte set), so the state probe alone decides, and the
+    // bitmap would only filter the few reached targets.  The share of entries
+    // leading to reached vertices is estimated by the adjacency volume reached
+    // before this level (written by the previous expansion) plus this
+    // frontier's (4 slots per group).
+    const uint64_t vol = ctl->scanned[(L + 1) & 1] + (uint64_t)W * G;
+    const bool direct = (uint64_t)(100 - BFS_DIRECT_OPEN_PCT) * (uint64_t)ne > 100 * vol;
     // level bookkeeping by workgroup 0: this level's collect counts into the
     // other counter (nothing else touches it during this launch), and the
     // degrees of every discovery so far, one partial per collect workgroup,
@@ -136,7 +149,7 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uin
         if (threadIdx.x == 0) {
             uint64_t t = ctl->src_deg;
             for (int k = 0; k < NWAVES; k++) t += red[k];
-            ctl->scanned = t;
+            ctl->scanned[L & 1] = t;
             ctl->levels = L;
             ctl->visited += (uint64_t)F;
         }
@@ -205,19 +218,27 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uin
                 const uint32_t f = sh.full[fw < (uint32_t)FULL_LDS_WORDS ? fw : 0u];
                 known |= (uint32_t)((fw < (uint32_t)FULL_LDS_WORDS) & ((f >> ((t >> 5) & 31)) & 1u)) << (j * G + e);
             }
-        uint32_t wb[EXP_IPT][G];
-#pragma unroll
-        for (int j = 0; j < EXP_IPT; j++)
-#pragma unroll
-            for (int e = 0; e < G; e++)
-                wb[j][e] = bitmap[((e < nval[j]) && !((known >> (j * G + e)) & 1u)) ? (v[j][e] >> 5) : 0];
         uint32_t cand = 0;
+        if (direct) {
 #pragma unroll
-        for (int j = 0; j < EXP_IPT; j++)
+            for (int j = 0; j < EXP_IPT; j++)
 #pragma unroll
-            for (int e = 0; e < G; e++)
-                cand |= (uint32_t)((e < nval[j]) & !((known >> (j * G + e)) & 1u) &
-                                   (((wb[j][e] >> (v[j][e] & 31)) & 1u) == 0)) << (j * G + e);
+                for (int e = 0; e < G; e++)
+                    cand |= (uint32_t)((e < nval[j]) & !((known >> (j * G + e)) & 1u)) << (j * G + e);
+        } else {
+            uint32_t wb[EXP_IPT][G];
+#pragma unroll
+            for (int j = 0; j < EXP_IPT; j++)
+#pragma unroll
+                for (int e = 0; e < G; e++)
+                    wb[j][e] = bitmap[((e < nval[j]) && !((known >> (j * G + e)) & 1u)) ? (v[j][e] >> 5) : 0];
+#pragma unroll
+            for (int j = 0; j < EXP_IPT; j++)
+#pragma unroll
+                for (int e = 0; e < G; e++)
+                    cand |= (uint32_t)((e < nval[j]) & !((known >> (j * G + e)) & 1u) &
+                                       (((wb[j][e] >> (v[j][e] & 31)) & 1u) == 0)) << (j * G + e);
+        }
         // phase 3b: candidates (not visited before this level) probe their
         // state byte; the rest probe state[0], one shared line
         uint8_t s[EXP_IPT][G];
@@ -469,7 +490,7 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
                                   (const uint32_t*)b.P, (const uint32_t*)b.tile_lo, (const int32_t*)b.qdeg,
                                   (const uint32_t*)b.qbase, b.ctl, (const int32_t*)g->arena, b.state,
                                   (const uint32_t*)b.bitmap, (const uint32_t*)b.full, (int64_t)((words + 31) / 32),
-                                  (const uint64_t*)b.part, (int64_t)ceil_div(words, BLOCK));
+                                  (const uint64_t*)b.part, (int64_t)ceil_div(words, BLOCK), (int64_t)g->ne);
             e = hipGetLastError();
             if (e == hipSuccess) {
                 bfs_collect_kernel<<<(unsigned)ceil_div(words, BLOCK), BLOCK, 0, st>>>(
@@ -510,7 +531,7 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
     if (stats) {
         stats->levels = (int32_t)c.levels;
         stats->visited = (int64_t)c.visited;
-        stats->edges_scanned = (int64_t)c.scanned;
+        stats->edges_scanned = (int64_t)c.scanned[c.levels & 1];
         stats->expand_ns = kns;
         stats->expand_launches = (int32_t)c.levels;
         stats->pad_ = 0;

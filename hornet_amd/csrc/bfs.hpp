@@ -14,9 +14,9 @@ struct BfsCtl {
                        // (collect L adds into fc[(L + 1) & 1], which expand L clears)
     uint32_t levels;   // last non-empty level = number of non-empty frontiers
     uint32_t pad_;
-    uint64_t visited;  // sum of the frontier sizes
-    uint64_t scanned;  // sum of the frontiers' degrees (m_C), as of the last expansion
-    uint64_t src_deg;  // the source's degree
+    uint64_t visited;     // sum of the frontier sizes
+    uint64_t scanned[2];  // sum of the frontiers' degrees (m_C) as of expansion L, in [L & 1]
+    uint64_t src_deg;     // the source's degree
 };
 
 // Scratch of one traversal, carved from one device buffer: the handle's
