@@ -54,13 +54,22 @@ namespace hn {
 namespace {
 
 constexpr int EXP_G = 4;                       // adjacency slots per work unit
-constexpr int EXP_IPT = 4;                     // work units per lane per tile
+#ifndef BFS_EXP_IPT
+#define BFS_EXP_IPT 4
+#endif
+#ifndef BFS_FULL_LDS_WORDS
+#define BFS_FULL_LDS_WORDS 3072
+#endif
+#ifndef BFS_GRID_PER_CU
+#define BFS_GRID_PER_CU 64
+#endif
+constexpr int EXP_IPT = BFS_EXP_IPT;           // work units per lane per tile
 constexpr int EXP_TILE = BLOCK * EXP_IPT;      // work units per tile
 constexpr int32_t INF = 0x7fffffff;
 // The expansion keeps the first FULL_LDS_WORDS words of the full-word summary
 // in LDS (12 KB: vertices below 3 * 2^20).  A neighbour whose bitmap word is known
 // full was visited before the level, so its bitmap probe is skipped.
-constexpr int FULL_LDS_WORDS = 3072;
+constexpr int FULL_LDS_WORDS = BFS_FULL_LDS_WORDS;
 #ifndef BFS_DIRECT_OPEN_PCT
 #define BFS_DIRECT_OPEN_PCT 25   // skip bitmap probes while more than this % of the adjacency volume is unreached
 #endif
@@ -459,7 +468,7 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
         g->nv, g->rec, dist, b.bitmap, b.full, reinterpret_cast<uint32_t*>(b.state), source, b.qdeg, b.qbase, b.P,
         b.tile_lo, b.part, b.ctl);
     HN_CHECK_LAUNCH();
-    const int64_t grid_cap = (int64_t)num_cus() * 64;
+    const int64_t grid_cap = (int64_t)num_cus() * BFS_GRID_PER_CU;
     const bool timed = may_time && stats && stats->time_kernels;
     // kernel timing: one event pair around each queued expansion of a chunk
     constexpr int MAX_CHUNK = hn_graph::BFS_EVENTS / 2;
