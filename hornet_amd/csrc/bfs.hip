@@ -58,17 +58,24 @@ constexpr int EXP_G = 4;                       // adjacency slots per work unit
 #define BFS_EXP_IPT 4
 #endif
 #ifndef BFS_FULL_LDS_WORDS
-#define BFS_FULL_LDS_WORDS 3072
+#define BFS_FULL_LDS_WORDS 7168
 #endif
 #ifndef BFS_GRID_PER_CU
-#define BFS_GRID_PER_CU 64
+#define BFS_GRID_PER_CU 64   // expansion grid, in 256-lane workgroups per CU
+#endif
+#ifndef BFS_EXP_BLOCK
+#define BFS_EXP_BLOCK 512
 #endif
 constexpr int EXP_IPT = BFS_EXP_IPT;           // work units per lane per tile
-constexpr int EXP_TILE = BLOCK * EXP_IPT;      // work units per tile
+constexpr int EXP_BLOCK = BFS_EXP_BLOCK;       // expansion workgroup size
+constexpr int EXP_WAVES = EXP_BLOCK / WAVE;
+constexpr int EXP_TILE = EXP_BLOCK * EXP_IPT;  // work units per tile
 constexpr int32_t INF = 0x7fffffff;
 // The expansion keeps the first FULL_LDS_WORDS words of the full-word summary
-// in LDS (12 KB: vertices below 3 * 2^20).  A neighbour whose bitmap word is known
-// full was visited before the level, so its bitmap probe is skipped.
+// in LDS (28 KB: vertices below 7 * 2^20).  A neighbour whose bitmap word is known
+// full was visited before the level, so its bitmap probe is skipped.  512-lane
+// workgroups share one copy per 8 waves: with the 24 KB load-balancing stage,
+// three workgroups fill a CU's 160 KB.
 constexpr int FULL_LDS_WORDS = BFS_FULL_LDS_WORDS;
 #ifndef BFS_DIRECT_OPEN_PCT
 #define BFS_DIRECT_OPEN_PCT 25   // skip bitmap probes while more than this % of the adjacency volume is unreached
@@ -105,9 +112,27 @@ __global__ void bfs_init_kernel(int32_t nv, const hn_vrec* __restrict__ rec, int
     }
 }
 
+// Probes go through buffer loads: a lane without a probe to make gets an
+// offset past the buffer's end, which the hardware's range check answers with
+// 0 and no cache access (a plain load of a shared dummy address would still
+// take a cache-tag lookup in every group of four lanes that holds one).
+constexpr uint32_t PROBE_NONE = 0xFFFFFFF0u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t probe_rsrc(const void* p, uint32_t bytes) {
+    // raw buffer (stride 0), gfx9-family descriptor word 3
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t probe_b32(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+}
+__device__ __forceinline__ uint8_t probe_u8(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return (uint8_t)__builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0);
+}
+
 // the frontier slice of a tile: group prefix, adjacency offset in 128-byte
 // units (every block starts on one), degree
-using ExpLb = LbShared<EXP_IPT, uint32_t, uint32_t>;
+// (LbShared sizes its stage for BLOCK threads: EXP_TILE + 1 entries)
+using ExpLb = LbShared<EXP_TILE / BLOCK, uint32_t, uint32_t>;
+static_assert(ExpLb::CAP == EXP_TILE + 1, "stage holds a tile's segments");
 struct ExpShared {
     ExpLb lb;
     int32_t deg[ExpLb::CAP];
@@ -118,7 +143,7 @@ struct ExpShared {
 // if not visited before this level, against state[]; unreached ones are
 // marked.  state[] is only ever set to 1 here, so concurrent stores agree.
 template <int G>
-__global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uint32_t* __restrict__ P,
+__global__ __launch_bounds__(EXP_BLOCK) void bfs_expand_kernel(uint32_t L, const uint32_t* __restrict__ P,
                                                            const uint32_t* __restrict__ tile_lo,
                                                            const int32_t* __restrict__ qdeg,
                                                            const uint32_t* __restrict__ qbase,
@@ -126,12 +151,15 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uin
                                                            const int32_t* __restrict__ arena,
                                                            uint8_t* __restrict__ state,
                                                            const uint32_t* __restrict__ bitmap,
-                                                           const uint32_t* __restrict__ full, int64_t fwords,
+                                                           const uint32_t* __restrict__ full, int64_t words,
+                                                           int64_t fwords,
                                                            const uint64_t* __restrict__ part, int64_t nparts,
                                                            int64_t ne) {
     typedef int32_t Vec __attribute__((ext_vector_type(G)));
     constexpr int CAP = ExpLb::CAP;
     __shared__ ExpShared sh;
+    const __amdgpu_buffer_rsrc_t rbm = probe_rsrc(bitmap, (uint32_t)(words * 4));
+    const __amdgpu_buffer_rsrc_t rst = probe_rsrc(state, (uint32_t)(words * 32));
     const uint64_t fcur = ctl->fc[L & 1];
     const int64_t F = (int64_t)(fcur >> 32);
     const int64_t W = F > 0 ? (int64_t)(uint32_t)fcur : 0;
@@ -149,15 +177,15 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uin
     // degrees of every discovery so far, one partial per collect workgroup,
     // join the source's in `scanned`
     if (blockIdx.x == 0 && F > 0) {
-        __shared__ uint64_t red[NWAVES];
+        __shared__ uint64_t red[EXP_WAVES];
         uint64_t x = 0;
-        for (int64_t i = threadIdx.x; i < nparts; i += BLOCK) x += part[i];
+        for (int64_t i = threadIdx.x; i < nparts; i += EXP_BLOCK) x += part[i];
         x = wave_reduce_sum(x);
         if (__lane_id() == 0) red[threadIdx.x / WAVE] = x;
         __syncthreads();
         if (threadIdx.x == 0) {
             uint64_t t = ctl->src_deg;
-            for (int k = 0; k < NWAVES; k++) t += red[k];
+            for (int k = 0; k < EXP_WAVES; k++) t += red[k];
             ctl->scanned[L & 1] = t;
             ctl->levels = L;
             ctl->visited += (uint64_t)F;
@@ -166,7 +194,7 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uin
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->fc[(L + 1) & 1] = 0;
     const int64_t ntiles = (W + EXP_TILE - 1) / EXP_TILE;
     if ((int64_t)blockIdx.x >= ntiles) return;
-    for (int i = threadIdx.x; i < FULL_LDS_WORDS; i += BLOCK) sh.full[i] = i < fwords ? full[i] : 0u;
+    for (int i = threadIdx.x; i < FULL_LDS_WORDS; i += EXP_BLOCK) sh.full[i] = i < fwords ? full[i] : 0u;
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int64_t lo = tile_lo[t];
         const int64_t hi = (t + 1 < ntiles) ? (int64_t)tile_lo[t + 1] : F - 1;
@@ -174,7 +202,7 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uin
         const int64_t span = hi - lo + 1;
         const bool staged = span <= CAP;
         if (staged) {
-            for (int i = threadIdx.x; i < (int)span; i += BLOCK) {
+            for (int i = threadIdx.x; i < (int)span; i += EXP_BLOCK) {
                 sh.lb.pref[i] = P[lo + i];
                 sh.lb.base[i] = qbase[lo + i];
                 sh.deg[i] = qdeg[lo + i];
@@ -187,15 +215,15 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uin
         if (staged) {
 #pragma unroll
             for (int j = 0; j < EXP_IPT; j++) {
-                const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
-                const int s = lb_locate<EXP_IPT>(sh.lb, (int)span, w);
+                const int64_t w = w0 + (int64_t)j * EXP_BLOCK + threadIdx.x;
+                const int s = lb_locate(sh.lb, (int)span, w);
                 const int64_t k = (w - (int64_t)sh.lb.pref[s]) * G;
                 addr[j] = (int64_t)sh.lb.base[s] * UNIT_SLOTS + k;
                 nval[j] = w < W ? (int)min((int64_t)G, (int64_t)sh.deg[s] - k) : 0;
             }
         } else {   // slice wider than LDS (runs of zero-degree frontier vertices)
             for (int j = 0; j < EXP_IPT; j++) {
-                const int64_t w = w0 + (int64_t)j * BLOCK + threadIdx.x;
+                const int64_t w = w0 + (int64_t)j * EXP_BLOCK + threadIdx.x;
                 addr[j] = 0;
                 nval[j] = 0;
                 if (w < W) {
@@ -240,7 +268,8 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uin
             for (int j = 0; j < EXP_IPT; j++)
 #pragma unroll
                 for (int e = 0; e < G; e++)
-                    wb[j][e] = bitmap[((e < nval[j]) && !((known >> (j * G + e)) & 1u)) ? (v[j][e] >> 5) : 0];
+                    wb[j][e] = probe_b32(rbm, ((e < nval[j]) && !((known >> (j * G + e)) & 1u))
+                                                   ? ((uint32_t)v[j][e] >> 5) * 4u : PROBE_NONE);
 #pragma unroll
             for (int j = 0; j < EXP_IPT; j++)
 #pragma unroll
@@ -254,7 +283,8 @@ __global__ __launch_bounds__(BLOCK) void bfs_expand_kernel(uint32_t L, const uin
 #pragma unroll
         for (int j = 0; j < EXP_IPT; j++)
 #pragma unroll
-            for (int e = 0; e < G; e++) s[j][e] = state[((cand >> (j * G + e)) & 1u) ? v[j][e] : 0];
+            for (int e = 0; e < G; e++)
+                s[j][e] = probe_u8(rst, ((cand >> (j * G + e)) & 1u) ? (uint32_t)v[j][e] : PROBE_NONE);
         uint32_t need = 0;
 #pragma unroll
         for (int j = 0; j < EXP_IPT; j++)
@@ -468,7 +498,7 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
         g->nv, g->rec, dist, b.bitmap, b.full, reinterpret_cast<uint32_t*>(b.state), source, b.qdeg, b.qbase, b.P,
         b.tile_lo, b.part, b.ctl);
     HN_CHECK_LAUNCH();
-    const int64_t grid_cap = (int64_t)num_cus() * BFS_GRID_PER_CU;
+    const int64_t grid_cap = (int64_t)num_cus() * BFS_GRID_PER_CU * BLOCK / EXP_BLOCK;
     const bool timed = may_time && stats && stats->time_kernels;
     // kernel timing: one event pair around each queued expansion of a chunk
     constexpr int MAX_CHUNK = hn_graph::BFS_EVENTS / 2;
@@ -494,11 +524,12 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
             L++;
             // timed: the events take the dispatch's own start and end
             // (hipExtLaunchKernel), with no marker between the launches
-            hipExtLaunchKernelGGL(bfs_expand_kernel<EXP_G>, dim3((unsigned)grid_cap), dim3(BLOCK), 0, st,
+            hipExtLaunchKernelGGL(bfs_expand_kernel<EXP_G>, dim3((unsigned)grid_cap), dim3(EXP_BLOCK), 0, st,
                                   timed ? ev[2 * k] : nullptr, timed ? ev[2 * k + 1] : nullptr, 0, L,
                                   (const uint32_t*)b.P, (const uint32_t*)b.tile_lo, (const int32_t*)b.qdeg,
                                   (const uint32_t*)b.qbase, b.ctl, (const int32_t*)g->arena, b.state,
-                                  (const uint32_t*)b.bitmap, (const uint32_t*)b.full, (int64_t)((words + 31) / 32),
+                                  (const uint32_t*)b.bitmap, (const uint32_t*)b.full, (int64_t)words,
+                                  (int64_t)((words + 31) / 32),
                                   (const uint64_t*)b.part, (int64_t)ceil_div(words, BLOCK), (int64_t)g->ne);
             e = hipGetLastError();
             if (e == hipSuccess) {
