@@ -704,6 +704,7 @@ def main() -> int:
         "alg_bytes_per_launch": round(alg_bytes / max(agg_l, 1)),
         "avg_launch_us": round(agg_ns / max(agg_l, 1) / 1e3, 2),
         "launches": agg_l,
+        "entries_per_launch": round(agg_m / max(agg_l, 1)),
     }
     if l2req and l2req.get("read_req_per_entry"):
         # the bound the random visited probes meet: L1 -> L2 read requests
