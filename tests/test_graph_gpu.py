@@ -173,6 +173,35 @@ def test_bfs_level_loop_depths(hn, rmat16):
     hp.release()
 
 
+def test_bfs_hubs_spanning_many_tiles(hn):
+    """Frontiers whose vertices' adjacency spans many expansion tiles (a tile
+    holds 2048 groups of 4 entries): two 100 000-leaf hubs joined through some
+    of the first hub's leaves, searched from a hub, from a leaf and from the
+    far side, symmetric and directed (where discovered vertices can have no
+    out-entries), each against the oracle."""
+    n = 200_000
+    a_leaf = np.arange(1, 100_001, dtype=np.int32)
+    b = 100_001
+    b_leaf = np.arange(100_002, n, dtype=np.int32)
+    bridge = a_leaf[::997]
+    s = np.concatenate([np.zeros(a_leaf.size, np.int32), np.full(b_leaf.size, b, np.int32), bridge,
+                        bridge[:5]])   # repeated bridge entries: a multigraph hub
+    d = np.concatenate([a_leaf, b_leaf, np.full(bridge.size, b, np.int32), np.full(5, b, np.int32)])
+    for sym in (True, False):
+        off, adj = O.coo_to_csr(n, s, d, symmetrize=sym, dedup=False)
+        h = hn.Hornet.from_csr(off, adj)
+        bfs = hn.BfsTopDown(h)
+        for src in (0, int(a_leaf[0]), b, int(b_leaf[-1]), int(bridge[3])):
+            print(f"case bfs hubs sym={sym} src={src}", flush=True)
+            exp, scanned = O.bfs(off, adj, src)
+            bfs.set_parameters(src)
+            bfs.run()
+            assert np.array_equal(bfs.distances().cpu().numpy(), exp), (sym, src)
+            assert bfs.stats.edges_scanned == scanned
+            assert bfs.stats.visited == int((exp != INF).sum())
+        h.release()
+
+
 def test_invalid_inputs_raise(hn):
     print("case invalid inputs", flush=True)
     from hornet_amd import HornetError
