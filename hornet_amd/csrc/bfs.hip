@@ -449,6 +449,12 @@ __global__ __launch_bounds__(BLOCK) void bfs_collect_kernel(uint32_t L, int64_t 
 
 }  // namespace
 
+// Resolves this file's code object (see preload_code_objects in graph.hip).
+void preload_bfs_code() {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&bfs_init_kernel)) != hipSuccess) (void)hipGetLastError();
+}
+
 // Lays the buffers of a traversal of a graph with nv vertices and ne edges
 // out from `base` (nullptr: sizes only); returns the bytes they take.
 int64_t bfs_layout(int64_t nv_in, int64_t ne, char* base, BfsBufs* b) {

@@ -81,6 +81,12 @@ __global__ void chung_lu_kernel(int32_t n, const uint64_t* __restrict__ cum, int
 
 }  // namespace
 
+// Resolves this file's code object (see preload_code_objects in graph.hip).
+void preload_gen_code() {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&rmat_kernel)) != hipSuccess) (void)hipGetLastError();
+}
+
 // cum[v] = sum_{i<v} floor(2^24 * (i+1)^(-1/(gamma-1))), computed on the host
 static std::vector<uint64_t> chung_lu_table(int32_t n, double gamma) {
     std::vector<uint64_t> cum((size_t)n + 1);

@@ -274,4 +274,14 @@ hn_status ms_window_clear(MsEngine* e, hipStream_t st);
 const unsigned long long* ms_engine_digest(const MsEngine* e);
 // device bytes one engine holds for a graph of nv vertices and ne edges
 int64_t ms_engine_bytes(int64_t nv, int64_t ne);
+
+// One per source file: resolves that file's code object (preload_code_objects).
+void preload_bfs_code();
+void preload_gen_code();
+void preload_graph_code();
+void preload_msbfs_code();
+void preload_prims_code();
+void preload_serve_code();
+void preload_tc_code();
+void preload_update_code();
 }  // namespace hn

@@ -829,6 +829,12 @@ int64_t ms_collect_grid(int64_t nv) {
 
 }  // namespace
 
+// Resolves this file's code object (see preload_code_objects in graph.hip).
+void preload_msbfs_code() {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&ms_init_kernel)) != hipSuccess) (void)hipGetLastError();
+}
+
 // Per-engine device state: masks, bitsets, work list and load-balancing
 // buffers, own stream and pinned readback buffer, captured level graph.
 struct MsEngine {

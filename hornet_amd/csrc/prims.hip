@@ -144,6 +144,12 @@ __global__ __launch_bounds__(BLOCK) void rs_scatter_kernel(const uint64_t* __res
 
 }  // namespace
 
+// Resolves this file's code object (see preload_code_objects in graph.hip).
+void preload_prims_code() {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&rs_hist_kernel)) != hipSuccess) (void)hipGetLastError();
+}
+
 size_t sort_temp_bytes(int64_t n) {
     int64_t tiles = ceil_div(n > 0 ? n : 1, RS_TILE);
     int64_t hn_ = tiles * RS_BUCKETS;

@@ -74,6 +74,12 @@ __global__ __launch_bounds__(BLOCK) void digest_kernel(int64_t nv, const int32_t
 
 }  // namespace
 
+// Resolves this file's code object (see preload_code_objects in graph.hip).
+void preload_serve_code() {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&digest_kernel)) != hipSuccess) (void)hipGetLastError();
+}
+
 struct ServeReq {
     int64_t id = 0;
     int32_t kind = 0;

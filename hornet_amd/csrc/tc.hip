@@ -416,6 +416,12 @@ struct U8In {
 
 }  // namespace
 
+// Resolves this file's code object (see preload_code_objects in graph.hip).
+void preload_tc_code() {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&tc_light_kernel)) != hipSuccess) (void)hipGetLastError();
+}
+
 static int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
 // bytes of tc_core's buffers for ne stored edges

@@ -869,6 +869,12 @@ __global__ void insert_batch_kernel(const uint32_t* __restrict__ nb, const uint6
 
 }  // namespace
 
+// Resolves this file's code object (see preload_code_objects in graph.hip).
+void preload_update_code() {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&pack_batch_kernel)) != hipSuccess) (void)hipGetLastError();
+}
+
 // scratch for a batch of n entries
 struct UpdBufs {
     uint64_t* ka;
