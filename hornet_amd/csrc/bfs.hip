@@ -235,8 +235,9 @@ __global__ __launch_bounds__(EXP_BLOCK) void bfs_expand_kernel(uint32_t L, const
             }
         }
         // Phases 2 and 3 load unconditionally (a unit without work reads
-        // arena[0..G), a masked slot probes state[0]) so that no load sits
-        // in a branch of its own, where the compiler waits for it at once.
+        // arena[0..G), a slot without a probe gets an out-of-range buffer
+        // offset) so that no load sits in a branch of its own, where the
+        // compiler waits for it at once.
         // phase 2: all adjacency groups in flight at once (streamed, not kept in L2)
         Vec v[EXP_IPT];
 #pragma unroll
@@ -244,7 +245,8 @@ __global__ __launch_bounds__(EXP_BLOCK) void bfs_expand_kernel(uint32_t L, const
             v[j] = __builtin_nontemporal_load(reinterpret_cast<const Vec*>(arena + (nval[j] > 0 ? addr[j] : 0)));
         // phase 3a: all visited-bitmap probes in flight at once (n/8 bytes,
         // L2 resident), folded branch-free into one bit per candidate slot
-        // (a neighbour in a word the LDS summary knows full probes word 0 instead)
+        // (a neighbour in a word the LDS summary knows full makes no probe); on
+        // direct levels the state probe alone decides
         uint32_t known = 0;
 #pragma unroll
         for (int j = 0; j < EXP_IPT; j++)
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(EXP_BLOCK) void bfs_expand_kernel(uint32_t L, const
                                        (((wb[j][e] >> (v[j][e] & 31)) & 1u) == 0)) << (j * G + e);
         }
         // phase 3b: candidates (not visited before this level) probe their
-        // state byte; the rest probe state[0], one shared line
+        // state byte; the rest make no probe
         uint8_t s[EXP_IPT][G];
 #pragma unroll
         for (int j = 0; j < EXP_IPT; j++)
