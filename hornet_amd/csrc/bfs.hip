@@ -9,7 +9,7 @@
 // may be enqueued more than once; the distances are still exact.
 //
 // Here, per level (no host round trip: counts live in a device control
-// block, and the host polls its done flag every few levels), two launches:
+// block, which the host reads every few levels), two launches:
 //   1. expand: the frontier arrives as (degree, adjacency offset, group
 //      prefix) triples plus, per tile of the expansion, the frontier index
 //      holding the tile's first group -- the partition that BinarySearchLB
@@ -19,12 +19,16 @@
 //      128-byte aligned, so a vertex's ceil(deg/4) groups lie inside its block
 //      and each is one aligned 16-byte load (slots past deg in the last group
 //      are masked).  Each tile stages its frontier slice in LDS and maps a
-//      group to its vertex with a fixed-step LDS search.  Every neighbour first
-//      probes the visited bitmap as of the start of the level (n/8 bytes,
-//      read-only here, so L2 resident); the survivors probe the byte-per-vertex
-//      state array (0 = never reached) and an unreached one is marked with an
-//      idempotent byte store of 1: probe and mark touch the same cache line,
-//      and marks made earlier in the level suppress later ones.  No atomics:
+//      group to its vertex with a fixed-step LDS search.  A neighbour in a
+//      bitmap word that the LDS summary knows full was reached before the
+//      level; every other one first probes the visited bitmap as of the start
+//      of the level (n/8 bytes, read-only here, so L2 resident), and the
+//      survivors probe the byte-per-vertex state array (0 = never reached) --
+//      on levels where most of the adjacency still leads to unreached vertices
+//      the bitmap probe is skipped and the state probe alone decides.  An
+//      unreached neighbour is marked with an idempotent byte store of 1: probe
+//      and mark touch the same cache line, and marks made earlier in the level
+//      suppress later ones.  No atomics:
 //      with eight XCDs, each holding its own L2, device-scope atomics are not
 //      served from cache (claiming vertices with atomicOr measured 7x slower,
 //      and even XCD-private mark bitmaps with workgroup-scope atomics measured
