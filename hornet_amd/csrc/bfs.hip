@@ -499,7 +499,7 @@ int64_t bfs_layout(int64_t nv_in, int64_t ne, char* base, BfsBufs* b) {
 // frontier of the next level to launch is empty.  Nothing here allocates (kernel timing,
 // which creates its events, only with `may_time`).
 hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats, hipStream_t st,
-                  const BfsBufs& b, bool may_time, uint64_t* host) {
+                  const BfsBufs& b, bool may_time, uint64_t* host, int32_t max_levels) {
     const int64_t nv = g->nv;
     const int64_t words = (nv + 31) / 32;
     // frontier adjacency offsets are kept in 128-byte units as 32-bit words
@@ -526,6 +526,7 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
             }
     const int last = g->bfs_last_levels.load(std::memory_order_relaxed);
     int64_t next = std::min<int64_t>(last > 0 ? last : 4, MAX_CHUNK);
+    if (max_levels > 0) next = std::min<int64_t>(next, max_levels);
     int64_t ramp = 1, kns = 0;
     uint32_t L = 0;   // levels launched
     uint32_t levels_seen = 0;
@@ -554,8 +555,9 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
         if (e == hipSuccess) e = hipMemcpyAsync(host, b.ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e == hipSuccess) std::memcpy(&c, host, sizeof(BfsCtl));
-        // done: the frontier of the next level to launch is empty
-        const bool done = (c.fc[(L + 1) & 1] >> 32) == 0;
+        // done: the frontier of the next level to launch is empty, or the
+        // level limit is reached
+        const bool done = (c.fc[(L + 1) & 1] >> 32) == 0 || (max_levels > 0 && (int64_t)L >= max_levels);
         if (timed && e == hipSuccess)
             // only the expansions of non-empty levels count
             for (uint32_t k = 0; k < c.levels - levels_seen && e == hipSuccess; k++) {
@@ -577,9 +579,11 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
         }
         next = ramp;
         ramp = std::min<int64_t>(2 * ramp, 4);
+        if (max_levels > 0) next = std::min<int64_t>(next, max_levels - (int64_t)L);
     }
     if (rc != HN_OK) return rc;
-    g->bfs_last_levels.store((int)levels_seen, std::memory_order_relaxed);
+    // the first poll of the next full traversal follows this one's depth
+    if (max_levels <= 0) g->bfs_last_levels.store((int)levels_seen, std::memory_order_relaxed);
     if (stats) {
         stats->levels = (int32_t)c.levels;
         stats->visited = (int64_t)c.visited;

@@ -43,9 +43,11 @@ struct BfsBufs {
 int64_t bfs_layout(int64_t nv, int64_t ne, char* base, BfsBufs* b);
 // BFS TopDown from `source` into dist[nv] over laid-out buffers, reading the
 // level counters back through `host` (pinned, at least sizeof(BfsCtl) bytes).
-// Allocates nothing, except the two timing events when `may_time` and
+// With max_levels > 0 the traversal stops after that many levels (vertices
+// farther than max_levels keep INT32_MAX; the 2-hop queries of the serving
+// loop).  Allocates nothing, except the timing events when `may_time` and
 // stats->time_kernels ask for kernel timing.
 hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stats, hipStream_t st,
-                  const BfsBufs& b, bool may_time, uint64_t* host);
+                  const BfsBufs& b, bool may_time, uint64_t* host, int32_t max_levels = 0);
 
 }  // namespace hn

@@ -161,19 +161,21 @@ static void worker_loop(hn_server* s, ServeSlot* slot) {
             srcs[i] = batch[i]->source;
             limits[i] = batch[i]->kind == HN_REQ_TWO_HOP ? 2 : 0;
         }
-        // One or two BFS queries run the single-source TopDown one after the
-        // other over the slot's own buffers (6.2 ms each on R-MAT s25 against
-        // 12.6 ms for a batched launch of one or two sources), each finishing
-        // when its own traversal does; a window's queries need the batched
-        // engine's version masks.
+        // One or two BFS or 2-hop queries run the single-source TopDown one
+        // after the other over the slot's own buffers (a 2-hop query stops
+        // after two levels; 6.2 ms per BFS on R-MAT s25 against 12.6 ms for a
+        // batched launch of one or two sources), each finishing when its own
+        // traversal does; a window's queries need the batched engine's version
+        // masks.
         BfsBufs bb;
         bool single = !window && k <= 2 && bfs_layout(g->nv, g->ne, nullptr, &bb) <= slot->bfs_scratch_bytes;
-        for (int32_t i = 0; i < k; i++) single = single && batch[i]->kind == HN_REQ_BFS;
+        for (int32_t i = 0; i < k; i++)
+            single = single && (batch[i]->kind == HN_REQ_BFS || batch[i]->kind == HN_REQ_TWO_HOP);
         if (single) {
             bfs_layout(g->nv, g->ne, slot->bfs_scratch, &bb);
             for (int32_t i = 0; i < k; i++) {
                 hn_status st = bfs_run(g, srcs[i], slot->dist, nullptr, slot->stream, bb, false,
-                                       reinterpret_cast<uint64_t*>(slot->h_digest));
+                                       reinterpret_cast<uint64_t*>(slot->h_digest), limits[i]);
                 if (st == HN_OK) {
                     hipError_t e = hipMemsetAsync(slot->digest, 0, sizeof(unsigned long long) * 3, slot->stream);
                     if (e == hipSuccess && nv > 0) {
