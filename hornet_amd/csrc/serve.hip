@@ -296,8 +296,13 @@ static Window scan_window(hn_server* s, int maxb) {
     return w;
 }
 
-// Runs window w (lock held on entry and exit; inflight == 0).  Returns false,
+// Runs window w (lock held on entry and exit; a slot is free).  Returns false,
 // with the queue untouched, when its updates are not symmetric batches.
+//
+// The window's entries are packed, copied in, sorted and looked up in the
+// current graph while launches already in flight still run: that set-up only
+// reads the graph, as they do.  The updates apply once every other launch
+// has finished.
 static bool run_window(hn_server* s, const Window& w, std::unique_lock<std::mutex>& lk) {
     hn_graph* g = s->g;
     ServeSlot* slot = free_slot(s);
@@ -315,7 +320,7 @@ static bool run_window(hn_server* s, const Window& w, std::unique_lock<std::mute
     const int nu = (int)us.size();
     const int64_t t0 = now_ns();
     const uint64_t base_version = g->version;
-    slot->busy = true;   // no other launch while the graph changes
+    slot->busy = true;   // the slot's engine holds the window
     s->inflight++;
     lk.unlock();
 
@@ -348,6 +353,16 @@ static bool run_window(hn_server* s, const Window& w, std::unique_lock<std::mute
         s->inflight--;
         return false;
     }
+    // the graph changes only once no other launch reads it
+    lk.lock();
+    s->cv_disp.wait(lk, [&] { return s->stop || s->inflight == 1; });
+    if (s->stop) {
+        (void)ms_window_clear(slot->eng, s->upd_stream);
+        slot->busy = false;
+        s->inflight--;
+        return true;
+    }
+    lk.unlock();
     // the updates, in arrival order
     std::vector<hn_status> ust((size_t)nu);
     std::vector<int64_t> uapp((size_t)nu), udone((size_t)nu);
@@ -414,7 +429,7 @@ static void dispatcher_loop(hn_server* s) {
         if (windows && barrier_updates == 0 && s->g->sym == 1) {
             Window w = scan_window(s, maxb);
             if (w.nq >= min_wq && w.nu > 0) {
-                s->cv_disp.wait(lk, [&] { return s->stop || s->inflight == 0; });
+                s->cv_disp.wait(lk, [&] { return s->stop || free_slot(s) != nullptr; });
                 if (s->stop) break;
                 w = scan_window(s, maxb);   // the queue may have grown
                 if (run_window(s, w, lk)) continue;
