@@ -258,13 +258,13 @@ hn_status ms_run_engine(MsEngine* e, const int32_t* sources, int32_t k, const in
 hn_status graph_is_symmetric(hn_graph* g, hipStream_t st, bool* yes);
 // Multi-version windows (one launch serving queries of several graph
 // versions).  begin: before the window's updates are applied, with every
-// update entry (h_tt[i] = update index 1..63 << 1 | 1 for an insert);
+// update entry on the device (d_tt[i] = update index 1..63 << 1 | 1 for an insert);
 // end: after them (failed = bit t for an update that changed nothing; sv[t] =
 // sources of window version t), builds the side records and synchronises;
 // clear: after the launch.
 // (*symmetric = false, and no window opened, when an entry lacks its reverse
 // in the same update.)
-hn_status ms_window_begin(MsEngine* e, const int32_t* h_src, const int32_t* h_dst, const uint8_t* h_tt, int64_t n,
+hn_status ms_window_begin(MsEngine* e, const int32_t* d_src, const int32_t* d_dst, const uint8_t* d_tt, int64_t n,
                           hipStream_t st, bool* symmetric);
 hn_status ms_window_end(MsEngine* e, int32_t m, uint64_t failed, const uint64_t* sv, hipStream_t st);
 hn_status ms_window_clear(MsEngine* e, hipStream_t st);

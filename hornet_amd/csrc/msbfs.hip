@@ -1228,7 +1228,7 @@ static hn_status ms_run_on(MsEngine* e, const int32_t* sources, int32_t k, const
 
 // ---- multi-version windows: called by the serving loop ----------------------
 
-hn_status ms_window_begin(MsEngine* e, const int32_t* h_src, const int32_t* h_dst, const uint8_t* h_tt, int64_t n,
+hn_status ms_window_begin(MsEngine* e, const int32_t* d_src, const int32_t* d_dst, const uint8_t* d_tt, int64_t n,
                           hipStream_t st, bool* symmetric) {
     hn_graph* g = e->g;
     const int sbits = std::max(1, ceil_log2_host((int64_t)g->nv + 1));
@@ -1239,18 +1239,10 @@ hn_status ms_window_begin(MsEngine* e, const int32_t* h_src, const int32_t* h_ds
     e->has_window = true;
     if (n == 0) return HN_OK;
     HN_TRY(e->w_keys.reserve((size_t)n * 8));
-    HN_TRY(e->w_alt.reserve((size_t)n * 9 + 512));
+    HN_TRY(e->w_alt.reserve((size_t)n * 8 + 512));
     HN_TRY(e->w_tmp.reserve(std::max(sort_temp_bytes(n), scan_temp_bytes(std::max<int64_t>(n, g->nv)))));
     HN_TRY(e->w_seg.reserve((size_t)(n + 1) * 8 + 256));
     HN_TRY(e->w_e0.reserve((size_t)n));
-    // entries staged in w_alt (the sort's second buffer once they are packed)
-    char* stg = e->w_alt.as<char>();
-    int32_t* d_src = reinterpret_cast<int32_t*>(stg);
-    int32_t* d_dst = reinterpret_cast<int32_t*>(stg + (size_t)n * 4);
-    uint8_t* d_tt = reinterpret_cast<uint8_t*>(stg + (size_t)n * 8);
-    HN_HIP(hipMemcpyAsync(d_src, h_src, (size_t)n * 4, hipMemcpyHostToDevice, st));
-    HN_HIP(hipMemcpyAsync(d_dst, h_dst, (size_t)n * 4, hipMemcpyHostToDevice, st));
-    HN_HIP(hipMemcpyAsync(d_tt, h_tt, (size_t)n, hipMemcpyHostToDevice, st));
     uint64_t* keys = e->w_keys.as<uint64_t>();
     uint64_t* alt = e->w_alt.as<uint64_t>();
     win_pack_kernel<<<(unsigned)ceil_div(n, BLOCK), BLOCK, 0, st>>>(n, d_src, d_dst, d_tt, sbits, keys);

@@ -119,6 +119,10 @@ struct hn_server {
     std::vector<std::unique_ptr<hn::ServeSlot>> slots;
     std::thread dispatcher;
     hipStream_t upd_stream = nullptr;
+    // a window's update entries on the device: src[cap], dst[cap], then one
+    // tag byte each (grown on demand, reused by every window)
+    int32_t* win_ent = nullptr;
+    int64_t win_cap = 0;
     int inflight = 0;
     int64_t completed = 0;
     bool stop = false;
@@ -251,6 +255,30 @@ static hn_status apply_update(hn_server* s, const ServeReq* r, int64_t* applied)
     return hn_graph_erase(s->g, r->src.data(), r->dst.data(), n, HN_DEDUP_BATCH, applied, s->upd_stream);
 }
 
+// apply one update of a window whose entries are already on the device
+// (validated, and within the budget, when the window was packed)
+static hn_status apply_update_dev(hn_server* s, const ServeReq* r, const int32_t* d_src, const int32_t* d_dst,
+                                  int64_t* applied) {
+    *applied = 0;
+    const int64_t n = (int64_t)r->src.size();
+    if (r->kind == HN_REQ_INSERT)
+        return hn_graph_insert(s->g, d_src, d_dst, n, HN_DEDUP_BATCH | HN_DEDUP_GRAPH | HN_DEVICE_PTRS, applied,
+                               s->upd_stream);
+    return hn_graph_erase(s->g, d_src, d_dst, n, HN_DEDUP_BATCH | HN_DEVICE_PTRS, applied, s->upd_stream);
+}
+
+// room for n window entries in win_ent
+static hn_status window_entries(hn_server* s, int64_t n) {
+    if (n <= s->win_cap) return HN_OK;
+    dfree(s->win_ent);
+    s->win_ent = nullptr;
+    s->win_cap = 0;
+    const int64_t cap = std::max<int64_t>(n, (int64_t)1 << 20);
+    HN_TRY(dalloc(&s->win_ent, (size_t)(2 * cap + (cap + 3) / 4)));
+    s->win_cap = cap;
+    return HN_OK;
+}
+
 // lock held
 static void record_update(hn_server* s, const ServeReq* r, hn_status st, int64_t applied, uint64_t version,
                           int64_t t_start, int64_t t_done) {
@@ -329,6 +357,7 @@ static bool run_window(hn_server* s, const Window& w, std::unique_lock<std::mute
     uint64_t failed = 0;
     std::vector<int32_t> es, ed;
     std::vector<uint8_t> et;
+    std::vector<int64_t> uoff((size_t)nu + 1, -1);   // update t's first entry in es (-1: it fails)
     for (int t = 1; t <= nu; t++) {
         const ServeReq* r = us[(size_t)t - 1];
         bool ok = (int64_t)r->src.size() * 160 <= s->update_budget;
@@ -339,13 +368,30 @@ static bool run_window(hn_server* s, const Window& w, std::unique_lock<std::mute
             continue;
         }
         const uint8_t tag = (uint8_t)((t << 1) | (r->kind == HN_REQ_INSERT ? 1 : 0));
+        uoff[(size_t)t] = (int64_t)es.size();
         es.insert(es.end(), r->src.begin(), r->src.end());
         ed.insert(ed.end(), r->dst.begin(), r->dst.end());
         et.insert(et.end(), r->src.size(), tag);
     }
+    // the entries go to the device once: the window's sort reads them there,
+    // and each update applies from its own range
+    const int64_t ne = (int64_t)es.size();
     bool symmetric = false;
-    hn_status st = ms_window_begin(slot->eng, es.data(), ed.data(), et.data(), (int64_t)es.size(), s->upd_stream,
-                                   &symmetric);
+    hn_status st = window_entries(s, ne);
+    int32_t* d_src = s->win_ent;
+    int32_t* d_dst = s->win_ent + s->win_cap;
+    uint8_t* d_tt = reinterpret_cast<uint8_t*>(s->win_ent + 2 * s->win_cap);
+    if (st == HN_OK && ne > 0 &&
+        (hipMemcpyAsync(d_src, es.data(), sizeof(int32_t) * (size_t)ne, hipMemcpyHostToDevice, s->upd_stream) !=
+             hipSuccess ||
+         hipMemcpyAsync(d_dst, ed.data(), sizeof(int32_t) * (size_t)ne, hipMemcpyHostToDevice, s->upd_stream) !=
+             hipSuccess ||
+         hipMemcpyAsync(d_tt, et.data(), (size_t)ne, hipMemcpyHostToDevice, s->upd_stream) != hipSuccess)) {
+        (void)hipGetLastError();
+        set_error("window entries: copy to the device");
+        st = HN_E_HIP;
+    }
+    if (st == HN_OK) st = ms_window_begin(slot->eng, d_src, d_dst, d_tt, ne, s->upd_stream, &symmetric);
     if (st != HN_OK || !symmetric) {
         (void)ms_window_clear(slot->eng, s->upd_stream);
         lk.lock();
@@ -369,7 +415,9 @@ static bool run_window(hn_server* s, const Window& w, std::unique_lock<std::mute
     std::vector<uint64_t> uver((size_t)nu);
     for (int t = 1; t <= nu; t++) {
         int64_t applied = 0;
-        hn_status us_st = apply_update(s, us[(size_t)t - 1], &applied);
+        const int64_t o = uoff[(size_t)t];
+        hn_status us_st = o >= 0 ? apply_update_dev(s, us[(size_t)t - 1], d_src + o, d_dst + o, &applied)
+                                 : apply_update(s, us[(size_t)t - 1], &applied);
         if (us_st != HN_OK) failed |= 1ull << t;
         ust[(size_t)t - 1] = us_st;
         uapp[(size_t)t - 1] = applied;
@@ -514,6 +562,7 @@ static void destroy_server(hn_server* s) {
         if (sl->h_digest) (void)hipHostFree(sl->h_digest);
         if (sl->stream) (void)hipStreamDestroy(sl->stream);
     }
+    dfree(s->win_ent);
     if (s->upd_stream) (void)hipStreamDestroy(s->upd_stream);
     delete s;
 }
