@@ -156,3 +156,34 @@ def test_window_spans_updates(hn):
     check(res, expected_results(off, adj, trace))
     q = res["kind"] <= 1
     assert res["batch"][q].max() >= 16
+
+
+@pytest.mark.parametrize("slots", [1, 2])
+def test_window_burst_with_failed_update(hn, slots):
+    # a burst spanning several windows (with two slots, a window's set-up runs
+    # while the previous launch is still on the device), one of whose updates
+    # names a vertex id out of range: that update fails as a whole and changes
+    # nothing, and every other result is the one its version gives alone
+    print(f"case serve window burst with a failed update slots={slots}", flush=True)
+    off, adj = O.rmat_graph(12, seed=31)
+    n = off.size - 1
+    h = hn.Hornet.from_csr(off, adj)
+    trace, _ = make_trace(off, adj, 400, seed=14)
+    upd = [i for i, (k, _) in enumerate(trace) if k >= 2]
+    bad = upd[len(upd) // 2]
+    a, b = trace[bad][1]
+    a = a.copy()
+    a[0] = n   # one id past the last vertex
+    trace[bad] = (trace[bad][0], (a, b))
+    srv = hn.Server(h, max_batch=64, slots=slots)
+    res = srv.replay(trace, np.zeros(len(trace), np.int64))
+    srv.close()
+    ok = [t for i, t in enumerate(trace) if i != bad]
+    exp = expected_results(off, adj, ok)
+    assert res["status"][bad] != 0
+    assert int(res["version"][bad]) == upd.index(bad)   # the updates before it all applied
+    keep = np.ones(len(trace), bool)
+    keep[bad] = False
+    check(res[keep], exp)
+    q = res["kind"] <= 1
+    assert res["batch"][q].max() >= 16
