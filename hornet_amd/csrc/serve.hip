@@ -123,6 +123,7 @@ struct hn_server {
     // tag byte each (grown on demand, reused by every window)
     int32_t* win_ent = nullptr;
     int64_t win_cap = 0;
+    int64_t window_max_entries = 0;   // update entries one window may span
     int inflight = 0;
     int64_t completed = 0;
     bool stop = false;
@@ -273,7 +274,7 @@ static hn_status window_entries(hn_server* s, int64_t n) {
     dfree(s->win_ent);
     s->win_ent = nullptr;
     s->win_cap = 0;
-    const int64_t cap = std::max<int64_t>(n, (int64_t)1 << 20);
+    const int64_t cap = std::max<int64_t>(n, std::min<int64_t>((int64_t)1 << 20, s->window_max_entries));
     HN_TRY(dalloc(&s->win_ent, (size_t)(2 * cap + (cap + 3) / 4)));
     s->win_cap = cap;
     return HN_OK;
@@ -293,10 +294,18 @@ static void record_update(hn_server* s, const ServeReq* r, hn_status st, int64_t
 }
 
 // A multi-version window: a queue prefix of up to max_batch queries with the
-// updates between them (at most 63, WINDOW_MAX_ENTRIES entries), ending at
+// updates between them (at most 63, window_max_entries entries), ending at
 // its last query.  One launch serves all its queries, each at the graph
 // version it arrived at.
+//
+// Device memory of a window's entries: the server's copy (WINDOW_SERVER_BYTES
+// each) and, in the engine of every slot that runs one, its sorted keys,
+// segments, existence masks and changed pairs (at most WINDOW_ENGINE_BYTES
+// each; an engine keeps them for the largest window it ran).  They are held to
+// half the update budget, so updates keep the other half.
 constexpr int64_t WINDOW_MAX_ENTRIES = 1 << 23;
+constexpr int64_t WINDOW_SERVER_BYTES = 9;
+constexpr int64_t WINDOW_ENGINE_BYTES = 96;
 struct Window {
     int nq = 0, nu = 0;
     size_t len = 0;
@@ -316,7 +325,7 @@ static Window scan_window(hn_server* s, int maxb) {
             w.len = i + 1;
         } else {
             const int64_t m = (int64_t)r->src.size();
-            if (nu == 63 || ent + m > WINDOW_MAX_ENTRIES) break;
+            if (nu == 63 || ent + m > s->window_max_entries) break;
             nu++;
             ent += m;
         }
@@ -606,6 +615,8 @@ hn_status hn_server_create(hn_graph* g, const hn_serve_config* cfg, hn_server** 
         return HN_E_OOM;
     }
     s->update_budget = std::max<int64_t>(budget - nslots * s->slot_bytes, 0);
+    s->window_max_entries = std::min<int64_t>(
+        WINDOW_MAX_ENTRIES, s->update_budget / (2 * (nslots * WINDOW_ENGINE_BYTES + WINDOW_SERVER_BYTES)));
     s->cfg.slots = nslots;
     if (hipStreamCreateWithFlags(&s->upd_stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();

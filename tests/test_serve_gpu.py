@@ -187,3 +187,19 @@ def test_window_burst_with_failed_update(hn, slots):
     check(res[keep], exp)
     q = res["kind"] <= 1
     assert res["batch"][q].max() >= 16
+
+
+def test_window_entries_capped_by_budget(hn):
+    # a budget whose update share holds windows to about 600 update entries:
+    # windows span fewer updates, and every result still matches its version
+    print("case serve windows capped by the update budget", flush=True)
+    off, adj = O.rmat_graph(12, seed=37)
+    h = hn.Hornet.from_csr(off, adj)
+    one_slot = hn.Server.slot_bytes(h, 64)
+    trace, _ = make_trace(off, adj, 300, seed=16)
+    srv = hn.Server(h, max_batch=64, slots=2, mem_budget_bytes=2 * one_slot + 2 * (2 * 96 + 9) * 600)
+    assert srv.slots() == 2
+    res = srv.replay(trace, np.zeros(len(trace), np.int64))
+    srv.close()
+    check(res, expected_results(off, adj, trace))
+    assert res["batch"][res["kind"] <= 1].max() > 1
