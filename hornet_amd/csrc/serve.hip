@@ -34,6 +34,12 @@
 #include "bfs.hpp"
 #include "graph.hpp"
 
+// launches of at most this many BFS / 2-hop queries (and nothing else) run
+// them as single-source TopDowns (overridable for sweeps)
+#ifndef SERVE_SINGLE_MAX
+#define SERVE_SINGLE_MAX 3
+#endif
+
 namespace hn {
 namespace {
 
@@ -166,14 +172,15 @@ static void worker_loop(hn_server* s, ServeSlot* slot) {
             srcs[i] = batch[i]->source;
             limits[i] = batch[i]->kind == HN_REQ_TWO_HOP ? 2 : 0;
         }
-        // One or two BFS or 2-hop queries run the single-source TopDown one
-        // after the other over the slot's own buffers (a 2-hop query stops
-        // after two levels; 6.2 ms per BFS on R-MAT s25 against 12.6 ms for a
-        // batched launch of one or two sources), each finishing when its own
-        // traversal does; a window's queries need the batched engine's version
-        // masks.
+        // Up to SERVE_SINGLE_MAX BFS or 2-hop queries run the single-source
+        // TopDown one after the other over the slot's own buffers (a 2-hop
+        // query stops after two levels; 6.2 ms per BFS on R-MAT s25 against
+        // 12.6 ms for a batched launch of one or two sources), each finishing
+        // when its own traversal does; a window's queries need the batched
+        // engine's version masks.
         BfsBufs bb;
-        bool single = !window && k <= 2 && bfs_layout(g->nv, g->ne, nullptr, &bb) <= slot->bfs_scratch_bytes;
+        bool single = !window && k <= SERVE_SINGLE_MAX &&
+                      bfs_layout(g->nv, g->ne, nullptr, &bb) <= slot->bfs_scratch_bytes;
         for (int32_t i = 0; i < k; i++)
             single = single && (batch[i]->kind == HN_REQ_BFS || batch[i]->kind == HN_REQ_TWO_HOP);
         if (single) {
