@@ -96,6 +96,25 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
     return v;
 }
 
+// The 64 x 64 bit matrix whose row l is lane l's word, transposed across the
+// wave: lane b returns column b (bit l = bit b of lane l's word).  Each step
+// swaps the off-diagonal j x j blocks of every 2j x 2j block: the upper lanes
+// of a pair keep their left columns and take the partner's left columns into
+// their right ones, the lower lanes the reverse.
+__device__ __forceinline__ uint64_t wave_transpose64(uint64_t x) {
+    const int lane = __lane_id();
+    constexpr uint64_t M[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                               0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int j = 32 >> k;
+        const uint64_t m = M[k];
+        const uint64_t p = __shfl_xor(x, j, WAVE);
+        x = (lane & j) ? ((x & ~m) | ((p & ~m) >> j)) : ((x & m) | ((p & m) << j));
+    }
+    return x;
+}
+
 // Bits that can still be found: sources whose frontier is non-empty (the
 // frontier is already cut by the depth limits).  A source that stops (depth
 // limit, or its component is exhausted) no longer keeps every vertex it did
@@ -562,15 +581,12 @@ __global__ __launch_bounds__(BLOCK) void ms_collect_kernel(int64_t nv, const int
         fany |= f;
         // distances, one source bit at a time so that lanes store consecutive
         // words; in digest mode one count per source bit and wave
-        uint64_t M = wave_or64(nb);
         if (digest) {
-            while (M) {
-                const int b = __ffsll((unsigned long long)M) - 1;
-                M &= M - 1;
-                const unsigned int c = (unsigned int)__popcll((unsigned long long)__ballot((nb >> b) & 1ull));
-                if (lane == 0) atomicAdd(&dcnt[b], c);
-            }
+            // lane b counts the wave's vertices newly reached from source b
+            const unsigned int c = (unsigned int)__popcll((unsigned long long)wave_transpose64(nb));
+            if (c) atomicAdd(&dcnt[lane], c);
         } else {
+            uint64_t M = wave_or64(nb);
             while (M) {
                 const int b = __ffsll((unsigned long long)M) - 1;
                 M &= M - 1;
