@@ -57,14 +57,39 @@ __global__ __launch_bounds__(BLOCK) void digest_kernel(int64_t nv, const int32_t
     const int32_t* row = dist + (int64_t)b * nv;
     unsigned long long cnt = 0, sum = 0;
     unsigned int mx = 0;
-    for (int64_t v = (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < nv; v += (int64_t)gridDim.x * BLOCK) {
-        const int32_t d = row[v];
+    auto acc = [&](int32_t d) {
         if (d != INF) {
             cnt++;
             sum += (unsigned long long)d;
             mx = d > (int32_t)mx ? (unsigned)d : mx;
         }
+    };
+    // 16-byte loads, four in flight per lane, over the aligned part of the
+    // row; the rest one entry at a time
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    const int64_t n4 = (reinterpret_cast<uintptr_t>(row) & 15) == 0 ? nv / 4 : 0;
+    const int4* r4 = reinterpret_cast<const int4*>(row);
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        int4 q[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) q[k] = r4[i + k * stride];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            acc(q[k].x);
+            acc(q[k].y);
+            acc(q[k].z);
+            acc(q[k].w);
+        }
     }
+    for (; i < n4; i += stride) {
+        const int4 q = r4[i];
+        acc(q.x);
+        acc(q.y);
+        acc(q.z);
+        acc(q.w);
+    }
+    for (int64_t v = 4 * n4 + (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < nv; v += stride) acc(row[v]);
     cnt = wave_reduce_sum(cnt);
     sum = wave_reduce_sum(sum);
     for (int d = WAVE / 2; d > 0; d >>= 1) {
