@@ -93,13 +93,40 @@ __global__ void bfs_init_kernel(int32_t nv, const hn_vrec* __restrict__ rec, int
                                 uint32_t* __restrict__ tile_lo, uint64_t* __restrict__ part,
                                 BfsCtl* __restrict__ ctl) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
     const int64_t words = ((int64_t)nv + 31) / 32;
-    if (i < (words + BLOCK - 1) / BLOCK) part[i] = 0u;
-    if (i < nv) dist[i] = (i == source) ? 0 : INF;
-    if (i < words) bitmap[i] = (i == (source >> 5)) ? (1u << (source & 31)) : 0u;
-    if (i < (words + 31) / 32) full[i] = 0u;
-    // state bytes (4 per word, words * 8 words): the source's alone set
-    if (i < words * 8) state4[i] = (i == (source >> 2)) ? (1u << (8 * (source & 3))) : 0u;
+    for (int64_t k = i; k < (words + BLOCK - 1) / BLOCK; k += nthr) part[k] = 0u;
+    for (int64_t k = i; k < (words + 31) / 32; k += nthr) full[k] = 0u;
+    for (int64_t k = i; k < words; k += nthr) bitmap[k] = (k == (source >> 5)) ? (1u << (source & 31)) : 0u;
+    // distances: 16-byte stores where the caller's array allows, the source's
+    // entry set within its own vector
+    const int64_t n4 = (reinterpret_cast<uintptr_t>(dist) & 15) == 0 ? (int64_t)nv / 4 : 0;
+    for (int64_t k = i; k < n4; k += nthr) {
+        int4 q = make_int4(INF, INF, INF, INF);
+        if (k == (source >> 2)) {
+            const int o = source & 3;
+            q.x = o == 0 ? 0 : q.x;
+            q.y = o == 1 ? 0 : q.y;
+            q.z = o == 2 ? 0 : q.z;
+            q.w = o == 3 ? 0 : q.w;
+        }
+        reinterpret_cast<int4*>(dist)[k] = q;
+    }
+    for (int64_t k = 4 * n4 + i; k < nv; k += nthr) dist[k] = (k == source) ? 0 : INF;
+    // state bytes (4 per word, words * 8 words, 256-byte aligned): the
+    // source's alone set
+    for (int64_t k = i; k < words * 2; k += nthr) {
+        uint4 q = make_uint4(0u, 0u, 0u, 0u);
+        if (k == (source >> 4)) {
+            const uint32_t b = 1u << (8 * (source & 3));
+            const int o = (source >> 2) & 3;
+            q.x = o == 0 ? b : 0u;
+            q.y = o == 1 ? b : 0u;
+            q.z = o == 2 ? b : 0u;
+            q.w = o == 3 ? b : 0u;
+        }
+        reinterpret_cast<uint4*>(state4)[k] = q;
+    }
     // the first frontier: the source alone, owning every tile of its groups
     const hn_vrec r = rec[source];
     const int64_t ng = ((int64_t)r.deg + EXP_G - 1) / EXP_G;
@@ -506,7 +533,7 @@ hn_status bfs_run(hn_graph* g, int32_t source, int32_t* dist, hn_bfs_stats* stat
     // (2^32 units = 512 GiB of arena, more than one device holds)
     HN_ARG(g->cap_units < ((int64_t)1 << 32), "edge arena beyond 2^32 128-byte units");
     // kernels only (no memset node), so a captured copy of this sequence replays correctly
-    bfs_init_kernel<<<(unsigned)ceil_div(nv, BLOCK), BLOCK, 0, st>>>(
+    bfs_init_kernel<<<(unsigned)std::min<int64_t>(ceil_div(nv, BLOCK * 8), 4096), BLOCK, 0, st>>>(
         g->nv, g->rec, dist, b.bitmap, b.full, reinterpret_cast<uint32_t*>(b.state), source, b.qdeg, b.qbase, b.P,
         b.tile_lo, b.part, b.ctl);
     HN_CHECK_LAUNCH();
