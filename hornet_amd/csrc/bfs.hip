@@ -220,6 +220,7 @@ __global__ __launch_bounds__(EXP_BLOCK) void bfs_expand_kernel(uint32_t L, const
             ctl->scanned[L & 1] = t;
             ctl->levels = L;
             ctl->visited += (uint64_t)F;
+            ctl->dsum += (uint64_t)F * (uint64_t)(L - 1);   // level L's frontier lies at distance L - 1
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->fc[(L + 1) & 1] = 0;

@@ -17,6 +17,7 @@ struct BfsCtl {
     uint64_t visited;     // sum of the frontier sizes
     uint64_t scanned[2];  // sum of the frontiers' degrees (m_C) as of expansion L, in [L & 1]
     uint64_t src_deg;     // the source's degree
+    uint64_t dsum;        // sum over the expanded frontiers of size x distance
 };
 
 // Scratch of one traversal, carved from one device buffer: the handle's

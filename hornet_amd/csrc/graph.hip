@@ -475,7 +475,6 @@ static void preload_code_objects() {
         preload_graph_code();
         preload_msbfs_code();
         preload_prims_code();
-        preload_serve_code();
         preload_tc_code();
         preload_update_code();
     });

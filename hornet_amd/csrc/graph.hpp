@@ -281,7 +281,6 @@ void preload_gen_code();
 void preload_graph_code();
 void preload_msbfs_code();
 void preload_prims_code();
-void preload_serve_code();
 void preload_tc_code();
 void preload_update_code();
 }  // namespace hn

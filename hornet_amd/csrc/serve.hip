@@ -18,8 +18,10 @@
 //   launches keep only per-source digests, never k x nv distances.  An update
 //   is refused (HN_E_OOM result) when its scratch estimate does not fit what
 //   the slots leave.
-// * Results are digests computed on the device (reached count, distance sum,
-//   eccentricity), so a batch returns 24 B per query instead of its vectors.
+// * Results are digests (reached count, distance sum, eccentricity), so a
+//   batch returns 24 B per query instead of its vectors: the batched engine
+//   counts them per source on the device; a single-source traversal's come
+//   from its level counters (frontier sizes and depths).
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -43,73 +45,12 @@
 namespace hn {
 namespace {
 
-constexpr int32_t INF = 0x7fffffff;
-
 int64_t now_ns() {
     return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
         .count();
 }
 
-// digest[b] = {reached, dist_sum, max_depth} of row b of a k x nv distance block
-__global__ __launch_bounds__(BLOCK) void digest_kernel(int64_t nv, const int32_t* __restrict__ dist,
-                                                       unsigned long long* __restrict__ digest) {
-    const int b = blockIdx.y;
-    const int32_t* row = dist + (int64_t)b * nv;
-    unsigned long long cnt = 0, sum = 0;
-    unsigned int mx = 0;
-    auto acc = [&](int32_t d) {
-        if (d != INF) {
-            cnt++;
-            sum += (unsigned long long)d;
-            mx = d > (int32_t)mx ? (unsigned)d : mx;
-        }
-    };
-    // 16-byte loads, four in flight per lane, over the aligned part of the
-    // row; the rest one entry at a time
-    const int64_t stride = (int64_t)gridDim.x * BLOCK;
-    const int64_t n4 = (reinterpret_cast<uintptr_t>(row) & 15) == 0 ? nv / 4 : 0;
-    const int4* r4 = reinterpret_cast<const int4*>(row);
-    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    for (; i + 3 * stride < n4; i += 4 * stride) {
-        int4 q[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) q[k] = r4[i + k * stride];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            acc(q[k].x);
-            acc(q[k].y);
-            acc(q[k].z);
-            acc(q[k].w);
-        }
-    }
-    for (; i < n4; i += stride) {
-        const int4 q = r4[i];
-        acc(q.x);
-        acc(q.y);
-        acc(q.z);
-        acc(q.w);
-    }
-    for (int64_t v = 4 * n4 + (int64_t)blockIdx.x * BLOCK + threadIdx.x; v < nv; v += stride) acc(row[v]);
-    cnt = wave_reduce_sum(cnt);
-    sum = wave_reduce_sum(sum);
-    for (int d = WAVE / 2; d > 0; d >>= 1) {
-        unsigned o = __shfl_xor(mx, d, WAVE);
-        mx = o > mx ? o : mx;
-    }
-    if (__lane_id() == 0 && cnt) {
-        atomicAdd(digest + 3 * b, cnt);
-        atomicAdd(digest + 3 * b + 1, sum);
-        atomicMax(digest + 3 * b + 2, (unsigned long long)mx);
-    }
-}
-
 }  // namespace
-
-// Resolves this file's code object (see preload_code_objects in graph.hip).
-void preload_serve_code() {
-    hipFuncAttributes a;
-    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&digest_kernel)) != hipSuccess) (void)hipGetLastError();
-}
 
 struct ServeReq {
     int64_t id = 0;
@@ -123,8 +64,7 @@ struct ServeSlot {
     int32_t* dist = nullptr;                 // nv: the single-source BFS's distances
     char* bfs_scratch = nullptr;             // the single-source BFS's level buffers
     int64_t bfs_scratch_bytes = 0;
-    unsigned long long* digest = nullptr;    // 64 x 3
-    unsigned long long* h_digest = nullptr;  // pinned; its first 16 bytes also serve the BFS level readback
+    unsigned long long* h_digest = nullptr;  // pinned: batched launches' digests, or the BFS level readback
     hipStream_t stream = nullptr;
     std::thread th;
     std::vector<ServeReq*> batch;
@@ -176,7 +116,6 @@ static void finish_query(hn_server* s, ServeReq* r, const unsigned long long* dg
 
 static void worker_loop(hn_server* s, ServeSlot* slot) {
     hn_graph* g = s->g;
-    const int64_t nv = g->nv;
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         slot->cv.wait(lk, [&] { return s->stop || slot->has_work; });
@@ -213,22 +152,27 @@ static void worker_loop(hn_server* s, ServeSlot* slot) {
             for (int32_t i = 0; i < k; i++) {
                 hn_status st = bfs_run(g, srcs[i], slot->dist, nullptr, slot->stream, bb, false,
                                        reinterpret_cast<uint64_t*>(slot->h_digest), limits[i]);
+                // the digest from the level counters bfs_run read back into
+                // h_digest: every frontier expanded adds its size and its
+                // distance sum; a depth-limited query's last discoveries
+                // (never expanded) lie at the limit
+                unsigned long long dg[3] = {0, 0, 0};
                 if (st == HN_OK) {
-                    hipError_t e = hipMemsetAsync(slot->digest, 0, sizeof(unsigned long long) * 3, slot->stream);
-                    if (e == hipSuccess && nv > 0) {
-                        dim3 grid((unsigned)std::min<int64_t>(ceil_div(nv, BLOCK), 1024), 1u);
-                        digest_kernel<<<grid, BLOCK, 0, slot->stream>>>(nv, slot->dist, slot->digest);
-                        e = hipGetLastError();
+                    BfsCtl c;
+                    std::memcpy(&c, slot->h_digest, sizeof(BfsCtl));
+                    dg[0] = c.visited;
+                    dg[1] = c.dsum;
+                    dg[2] = c.levels > 0 ? c.levels - 1 : 0;
+                    if (limits[i] > 0 && (int64_t)c.levels == (int64_t)limits[i]) {
+                        const uint64_t fn = c.fc[(c.levels + 1) & 1] >> 32;
+                        dg[0] += fn;
+                        dg[1] += fn * c.levels;
+                        if (fn) dg[2] = c.levels;
                     }
-                    if (e == hipSuccess)
-                        e = hipMemcpyAsync(slot->h_digest, slot->digest, sizeof(unsigned long long) * 3,
-                                           hipMemcpyDeviceToHost, slot->stream);
-                    if (e == hipSuccess) e = hipStreamSynchronize(slot->stream);
-                    if (e != hipSuccess) st = HN_E_HIP;
                 }
                 const int64_t t_done = now_ns();
                 lk.lock();
-                finish_query(s, batch[i], slot->h_digest, 1, ver, st, t_done);
+                finish_query(s, batch[i], dg, 1, ver, st, t_done);
                 s->completed += 1;
                 s->cv_done.notify_all();
                 lk.unlock();
@@ -599,7 +543,6 @@ static void destroy_server(hn_server* s) {
         ms_engine_destroy(sl->eng);
         dfree(sl->dist);
         dfree(sl->bfs_scratch);
-        dfree(sl->digest);
         if (sl->h_digest) (void)hipHostFree(sl->h_digest);
         if (sl->stream) (void)hipStreamDestroy(sl->stream);
     }
@@ -674,7 +617,6 @@ hn_status hn_server_create(hn_graph* g, const hn_serve_config* cfg, hn_server** 
             sl->bfs_scratch_bytes = slot_bfs_scratch(g);
             st = dalloc(&sl->bfs_scratch, (size_t)sl->bfs_scratch_bytes);
         }
-        if (st == HN_OK) st = dalloc(&sl->digest, 64 * 3);
         if (st == HN_OK && (hipHostMalloc(reinterpret_cast<void**>(&sl->h_digest), 64 * 3 * 8, 0) != hipSuccess ||
                             hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess)) {
             (void)hipGetLastError();
